@@ -1,0 +1,161 @@
+/*
+ * pisurf: MI355X (gfx950) implementation of the ParticleIsosurface field-evaluation and
+ * vertex-placement path, behind a plain C ABI.
+ *
+ * The reference has no FFI layer: its seams are the C++ classes HashGrid
+ * (include/hash_grid.h:9-45), Evaluator (include/evaluator.h:13-108) and SurfReconstructor
+ * (include/surface_reconstructor.h:21-93). Each entry point below replaces one of their
+ * methods (cited per function); the C++ mirror of those classes sits on top of this ABI
+ * (particleisosurface_amd/csrc/host_facade.h).
+ *
+ * Conventions
+ *  - every function returns PISURF_OK (0) or a negative PISURF_E_* code; no exception or
+ *    abort crosses the ABI; pisurf_last_error() gives the message of the last failure;
+ *  - host buffers are caller-owned, device buffers are context-owned;
+ *  - a context is single-threaded (one host thread, one HIP stream at default priority);
+ *  - positions are 3 packed floats per particle (layout of std::vector<Eigen::Vector3f>),
+ *    3x3 matrices are 9 floats/doubles column-major (layout of Eigen::Matrix3f);
+ *  - value outputs are double: FP32 results are exact in double, FP64 results are native.
+ */
+#ifndef PISURF_H
+#define PISURF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PISURF_OK 0
+#define PISURF_E_INVALID (-1)
+#define PISURF_E_OOM (-2)
+#define PISURF_E_HIP (-3)
+#define PISURF_E_UNSUPPORTED (-4)
+#define PISURF_E_STATE (-5)
+
+/* arithmetic of the field path */
+#define PISURF_FP32 0        /* float storage and sums */
+#define PISURF_FP32_KAHAN 1  /* float storage, compensated sums, FP64 eigensolve for clamped spectra */
+#define PISURF_FP64 2        /* double everywhere after the float positions */
+
+/* leaf vertex placement */
+#define PISURF_QEF_OFF 0      /* shipped behaviour: leaf vertex = centre (src/surface_reconstructor.cpp:258,270) */
+#define PISURF_QEF_REFERENCE 1/* TNode::NodeCalcNode: float 4x4 inverse (src/iso_method_ours.cpp:101-174) */
+#define PISURF_QEF_SVD64 2    /* FP64 truncated-SVD pseudo-inverse about the cell centre */
+
+typedef struct pisurf_ctx pisurf_ctx;
+
+typedef struct pisurf_params {
+    float radius;              /* constant particle radius r */
+    float smooth_factor;       /* box margin factor of checkEmptyAndCalcCurv (2.0) */
+    float iso_factor;          /* poly6 iso factor (1.9) */
+    int32_t use_ani;           /* USE_ANI (1) */
+    int32_t use_poly6;         /* USE_POLY6 (0: cubic B-spline) */
+    int32_t calc_normals;      /* CALC_P_NORMAL (1) */
+    int32_t oversample;        /* _OVERSAMPLE_QEF; only 2 is supported */
+    int32_t precision;         /* PISURF_FP32 / PISURF_FP32_KAHAN / PISURF_FP64 */
+    int32_t qef_mode;          /* PISURF_QEF_* */
+    float qef_svd_tol;         /* relative singular-value cutoff of PISURF_QEF_SVD64 */
+    int32_t depth_max_override;/* 0: derived D_max (resizeRootBoxConstR) */
+    int32_t reserved[5];
+} pisurf_params;
+
+typedef struct pisurf_grid_info {
+    float bounding[6];         /* HashGrid::Bounding */
+    uint32_t xyz_cell_num[3];  /* HashGrid::XYZCellNum */
+    int64_t cell_num;          /* HashGrid::CellNum */
+    float cell_size;           /* HashGrid::CellSize */
+    int64_t n_invalid;         /* particles whose cell hash is -1 */
+} pisurf_grid_info;
+
+typedef struct pisurf_tree_info {
+    int32_t depth_max, depth_min;
+    float root_center[3];
+    float root_half;
+    int64_t n_nodes;
+    int32_t n_levels;
+    int64_t n_leaves;
+} pisurf_tree_info;
+
+typedef struct pisurf_stats {
+    int64_t field_evals;       /* SingleEval-equivalents performed since the last reset */
+    int64_t grad_evals;        /* SingleEvalWithGrad-equivalents */
+    int64_t qef_solves;
+    int64_t candidates;        /* candidate particles visited by field evaluations */
+    double ms_grid, ms_aniso, ms_normals, ms_tree, ms_extract_host, ms_bisect, ms_total;
+    double ms_field_kernels;   /* summed device time of field-evaluation kernels */
+    int64_t n_dual_cells, n_verts, n_tris;
+} pisurf_stats;
+
+/* ---- context ---- */
+int pisurf_create(pisurf_ctx** out, int device);
+int pisurf_destroy(pisurf_ctx* ctx);
+const char* pisurf_last_error(const pisurf_ctx* ctx);
+int pisurf_default_params(pisurf_params* p);
+int pisurf_set_params(pisurf_ctx* ctx, const pisurf_params* p);
+/* SurfReconstructor ctor copies the particles (src/surface_reconstructor.cpp:13-28).
+ * on_device != 0: xyz is a device pointer (HBM-resident input); it is copied. */
+int pisurf_upload_particles(pisurf_ctx* ctx, const float* xyz, int64_t n, int32_t on_device);
+
+/* ---- H1-H5: HashGrid (src/hash_grid.cpp) ---- */
+/* HashGrid::HashGrid const-radius ctor (src/hash_grid.cpp:4-34). bbox NULL: the tight
+ * particle bbox of SurfReconstructor::loadRootBox (src/surface_reconstructor.cpp:30-49). */
+int pisurf_hashgrid_build(pisurf_ctx* ctx, const float* bbox, float inf_factor, pisurf_grid_info* out);
+/* IndexList (src/hash_grid.cpp:71-85) and the cell windows StartList/EndList
+ * (src/hash_grid.cpp:103-122) as dense arrays of cell_num entries (start = end for empty cells). */
+int pisurf_hashgrid_download(pisurf_ctx* ctx, int32_t* index_list, int32_t* cell_start, int32_t* cell_end);
+/* HashGrid::GetPIdxList for nq points (src/hash_grid.cpp:196-214): offsets[nq+1], ids[cap];
+ * *total receives the number of ids (ids beyond cap are not written). */
+int pisurf_neighbors(pisurf_ctx* ctx, const float* q, int64_t nq, int64_t* offsets, int32_t* ids, int64_t cap,
+                     int64_t* total);
+/* HashGrid::GetInBoxParticles (src/hash_grid.cpp:164-194) */
+int pisurf_box_particles(pisurf_ctx* ctx, const float* box1, const float* box2, int32_t* ids, int64_t cap,
+                         int64_t* total);
+
+/* ---- H6-H12: Evaluator (src/evaluator.cpp) ---- */
+/* Evaluator::compute_Gs_xMeans (src/evaluator.cpp:656-749) */
+int pisurf_anisotropy(pisurf_ctx* ctx);
+/* GlobalxMeans / GlobalGs / GlobalDeterminant (include/evaluator.h:72-74), particle order */
+int pisurf_download_aniso(pisurf_ctx* ctx, double* xmean, double* G9, double* det);
+/* CalculateMaxScalarConstR + RecommendIsoValueConstR (src/evaluator.cpp:272-291,323-335) */
+int pisurf_iso_constants(pisurf_ctx* ctx, double* max_scalar, double* iso_value);
+/* CalcParticlesNormal (src/evaluator.cpp:363-375); normals may be NULL */
+int pisurf_particle_normals(pisurf_ctx* ctx, double* normals);
+/* SingleEval (src/evaluator.cpp:60-87) at n points; grad != NULL gives SingleEvalWithGrad
+ * (src/evaluator.cpp:89-125) */
+int pisurf_eval_points(pisurf_ctx* ctx, const float* q, int64_t n, double* f, double* grad);
+
+/* ---- H13-H15: SurfReconstructor / TNode ---- */
+/* One batch of nodes of one level: checkEmptyAndCalcCurv + GenerateSampling + GridEval +
+ * classification (src/surface_reconstructor.cpp:152-273; src/iso_method_ours.cpp:52-99;
+ * src/evaluator.cpp:127-252). samples (n x 27 x 4) and grads (n x 27 x 3) may be NULL. */
+int pisurf_sweep_level(pisurf_ctx* ctx, const float* centers, const float* halfs, const int32_t* depths, int64_t n,
+                       uint8_t* type, double* f, float* curv, int32_t* signchange, double* samples, double* grads);
+/* genIsoOurs state 0 (src/surface_reconstructor.cpp:275-407) with resizeRootBoxConstR
+ * (:97-122): host level loop over batched device launches. */
+int pisurf_build_tree(pisurf_ctx* ctx, pisurf_tree_info* out);
+/* nodes in creation order (level by level, children of a parent consecutive) */
+int pisurf_tree_download(pisurf_ctx* ctx, float* center3, float* half, int32_t* depth, uint8_t* type, uint64_t* id,
+                         double* node4, int32_t* child);
+
+/* ---- H16: TNode::NodeCalcNode (src/iso_method_ours.cpp:101-174) for n nodes ---- */
+int pisurf_qef_solve(pisurf_ctx* ctx, const float* centers, const float* halfs, const double* samples,
+                     const double* grads, int64_t n, int32_t mode, double* node4);
+
+/* ---- H17: VisitorExtract::calc_vertices bisection (src/visitorextract.cpp:47-84) ----
+ * ends: n x 2 x 4 (x, y, z, f) edge endpoints; out_xyz n x 3; n_evals may be NULL. */
+int pisurf_bisect_edges(pisurf_ctx* ctx, const double* ends, int64_t n, double* out_xyz, int32_t* n_evals);
+
+/* ---- frame: extraction and mesh (genIsoOurs state 1, src/surface_reconstructor.cpp:290-324) ---- */
+int pisurf_extract(pisurf_ctx* ctx, int64_t* n_verts, int64_t* n_tris);
+int pisurf_mesh_download(pisurf_ctx* ctx, double* verts, int32_t* tris);
+/* SurfReconstructor::Run (src/surface_reconstructor.cpp:409-479) on the uploaded particles */
+int pisurf_run_frame(pisurf_ctx* ctx, pisurf_tree_info* tree);
+int pisurf_get_stats(pisurf_ctx* ctx, pisurf_stats* out);
+int pisurf_reset_stats(pisurf_ctx* ctx);
+/* device stream the library launches on (hipStream_t) */
+void* pisurf_stream(pisurf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1365 @@
+/*
+ * CPU oracle (TEST INFRASTRUCTURE ONLY; see pisurf_oracle.h).
+ *
+ * Clean-room restatement of the reference's CPU path. Each function cites the reference
+ * file:line whose behaviour it restates. Deliberate, documented deviations:
+ *  - the within-cell particle order uses std::stable_sort (the reference's std::sort is
+ *    unstable, so its within-cell order is implementation-defined: src/hash_grid.cpp:120);
+ *  - the 3x3 SVD of the (symmetric positive-definite) covariance is computed as a cyclic
+ *    Jacobi eigendecomposition (Golub & Van Loan, Matrix Computations, 4th ed., Alg. 8.5.2/8.5.3);
+ *    for SPD input U = V and the singular values are the eigenvalues, and G is a spectral
+ *    function of the covariance, so results agree with any SVD up to rounding;
+ *  - the 4x4 QEF inverse is the adjugate / determinant formula (cofactor expansion).
+ */
+#include "pisurf_oracle.h"
+#include "../particleisosurface_amd/csrc/dmc_tables.h"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cfloat>
+#include <cmath>
+#include <limits>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+struct V3f { float v[3]; float& operator[](int i) { return v[i]; } float operator[](int i) const { return v[i]; } };
+
+template <class T> struct V3 {
+    T v[3];
+    T& operator[](int i) { return v[i]; }
+    T operator[](int i) const { return v[i]; }
+};
+template <class T> struct M3 { T m[9]; /* column-major: (r,c) -> m[c*3+r] */
+    T& operator()(int r, int c) { return m[c * 3 + r]; }
+    T operator()(int r, int c) const { return m[c * 3 + r]; }
+};
+
+template <class T> inline T sqnorm3(T a, T b, T c) { return (a * a + b * b) + c * c; }
+
+// ---------------------------------------------------------------------------------
+// HashGrid: src/hash_grid.cpp:4-34 (ctor), :71-85 BuildTable, :87-101 CalcHashList,
+// :103-122 FindStartEnd, :124-139 CalcXYZIdx/CalcCellHash, :141-162 GetInCellList,
+// :164-194 GetInBoxParticles, :196-214 GetPIdxList.
+// ---------------------------------------------------------------------------------
+struct HashGrid {
+    const std::vector<V3f>* P = nullptr;
+    float Radius = 0, CellSize = 0;
+    float Bounding[6];
+    unsigned int XYZCellNum[3];
+    unsigned long long CellNum = 0;
+    std::vector<long long> HashList;
+    std::vector<int> IndexList;
+    std::map<long long, int> StartList, EndList;
+    int64_t n_invalid = 0;
+
+    void build(const std::vector<V3f>* particles, const float* bounding, float radius, float inf_factor) {
+        P = particles;
+        const size_t N = P->size();
+        Radius = radius;
+        CellSize = radius * inf_factor;
+        for (int i = 0; i < 3; i++) {
+            float length = std::ceil((bounding[i * 2 + 1] - bounding[i * 2]) / CellSize) * CellSize;
+            float center = (bounding[i * 2] + bounding[i * 2 + 1]) / 2;
+            Bounding[i * 2] = center - length / 2;
+            Bounding[i * 2 + 1] = center + length / 2;
+        }
+        for (int i = 0; i < 3; i++)
+            XYZCellNum[i] = (unsigned)std::max(int(std::ceil((Bounding[2 * i + 1] - Bounding[2 * i]) / CellSize)), 1);
+        CellNum = (unsigned long long)XYZCellNum[0] * XYZCellNum[1] * XYZCellNum[2];
+        HashList.assign(N, 0);
+        IndexList.assign(N, 0);
+        StartList.clear();
+        EndList.clear();
+        for (size_t i = 0; i < N; i++) {
+            int idx[3];
+            CalcXYZIdx((*P)[i], idx);
+            HashList[i] = CalcCellHash(idx);
+            IndexList[i] = (int)i;
+        }
+        std::stable_sort(IndexList.begin(), IndexList.end(),
+                         [&](int a, int b) { return HashList[a] < HashList[b]; });
+        std::vector<long long> tmp(HashList);
+        for (size_t i = 0; i < N; i++) HashList[i] = tmp[IndexList[i]];
+        // FindStartEnd: negative keys are skipped without advancing the running count.
+        int count = 0, previous = -1;
+        n_invalid = 0;
+        for (size_t i = 0; i < N; i++) {
+            int hash = (int)HashList[i];
+            if (hash < 0) { n_invalid++; continue; }
+            if (hash != previous) { StartList[hash] = count; previous = hash; }
+            count++;
+            EndList[hash] = count;
+        }
+        HashList.clear();
+    }
+    void CalcXYZIdx(const V3f& p, int idx[3]) const {
+        for (int i = 0; i < 3; i++) idx[i] = int((p[i] - Bounding[i * 2]) / CellSize);
+    }
+    long long CalcCellHash(const int idx[3]) const {
+        if (idx[0] < 0 || idx[0] >= (int)XYZCellNum[0] || idx[1] < 0 || idx[1] >= (int)XYZCellNum[1] ||
+            idx[2] < 0 || idx[2] >= (int)XYZCellNum[2])
+            return -1;
+        return (long long)idx[2] * (long long)XYZCellNum[0] * (long long)XYZCellNum[1] +
+               (long long)idx[1] * (long long)XYZCellNum[0] + (long long)idx[0];
+    }
+    void GetInCellList(long long hash, std::vector<int>& out) const {
+        auto s = StartList.find(hash);
+        auto e = EndList.find(hash);
+        if (s == StartList.end() || e == EndList.end()) return;
+        for (int k = s->second; k < e->second; k++) out.push_back(IndexList[k]);
+    }
+    void GetPIdxList(const V3f& pos, std::vector<int>& out) const {
+        int idx[3];
+        CalcXYZIdx(pos, idx);
+        for (int z = -1; z <= 1; z++)
+            for (int y = -1; y <= 1; y++)
+                for (int x = -1; x <= 1; x++) {
+                    int nb[3] = {idx[0] + x, idx[1] + y, idx[2] + z};
+                    long long h = CalcCellHash(nb);
+                    if (h < 0) continue;
+                    GetInCellList(h, out);
+                }
+    }
+    void GetInBoxParticles(V3f b1, V3f b2, std::vector<int>& out) const {
+        for (int i = 0; i < 3; i++) {
+            b1[i] = std::max(b1[i], Bounding[2 * i]);
+            b2[i] = std::min(b2[i], Bounding[2 * i + 1]);
+        }
+        int lo[3], hi[3];
+        CalcXYZIdx(b1, lo);
+        CalcXYZIdx(b2, hi);
+        for (int x = lo[0] - 1; x <= hi[0] + 1; x++)
+            for (int y = lo[1] - 1; y <= hi[1] + 1; y++)
+                for (int z = lo[2] - 1; z <= hi[2] + 1; z++) {
+                    int nb[3] = {x, y, z};
+                    long long h = CalcCellHash(nb);
+                    if (h < 0) continue;
+                    GetInCellList(h, out);
+                }
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// Scalar kernels. The reference computes these through pow() on float arguments, i.e. in
+// double, and narrows the result (src/evaluator.cpp:377-396, :751-766). For T=float the
+// narrowing is reproduced; for T=double nothing is narrowed.
+// ---------------------------------------------------------------------------------
+// wij: src/evaluator.cpp:751-766 (cubic spline normalised to w(0)=1)
+template <class T> inline T wij(T d, T h) {
+    if (d < h) {
+        const double dd = d, hh = h;
+        if (d < h / 2) return T(1 - (6 * std::pow(dd, 2) / std::pow(hh, 3) * double(h - d)));
+        return T(2 / std::pow(hh, 3) * std::pow(double(h - d), 3));
+    }
+    return T(0);
+}
+// Bspline_kernel: src/evaluator.cpp:383-396
+template <class T> inline T bspline(T ratio, T sigma) {
+    if (ratio < 2) {
+        const double q = ratio;
+        if (ratio <= 1) return T(double(sigma) * (1 - 1.5 * std::pow(q, 2) + 0.75 * std::pow(q, 3)));
+        return T(double(sigma) * (0.25 * std::pow(double(T(2) - ratio), 3)));
+    }
+    return T(0);
+}
+// poly6_kernel: src/evaluator.cpp:377-381
+template <class T> inline T poly6(T d2, T h2, T sigma) {
+    T p = (d2 > h2) ? T(0) : T(std::pow(double(h2 - d2), 3));
+    return p * sigma;
+}
+
+// ---------------------------------------------------------------------------------
+// Symmetric eigendecomposition (cyclic Jacobi, Golub & Van Loan Alg. 8.5.3).
+// ---------------------------------------------------------------------------------
+template <class T> void jacobi_eig3(const T a_in[3][3], T lam[3], T Q[3][3]) {
+    T a[3][3];
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) { a[i][j] = a_in[i][j]; Q[i][j] = (i == j) ? T(1) : T(0); }
+    const T eps = std::numeric_limits<T>::epsilon();
+    for (int sweep = 0; sweep < 32; sweep++) {
+        T off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        T dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (off <= eps * eps * dia * T(1e-2) || off == T(0)) break;
+        static const int PQ[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (int k = 0; k < 3; k++) {
+            int p = PQ[k][0], q = PQ[k][1];
+            T apq = a[p][q];
+            if (apq == T(0)) continue;
+            T tau = (a[q][q] - a[p][p]) / (2 * apq);
+            T t = (tau >= 0 ? T(1) : T(-1)) / (std::fabs(tau) + std::sqrt(T(1) + tau * tau));
+            T c = T(1) / std::sqrt(T(1) + t * t), s = t * c;
+            for (int r = 0; r < 3; r++) {  // A <- A J
+                T arp = a[r][p], arq = a[r][q];
+                a[r][p] = c * arp - s * arq;
+                a[r][q] = s * arp + c * arq;
+            }
+            for (int r = 0; r < 3; r++) {  // A <- J^T A
+                T apr = a[p][r], aqr = a[q][r];
+                a[p][r] = c * apr - s * aqr;
+                a[q][r] = s * apr + c * aqr;
+            }
+            a[p][q] = a[q][p] = T(0);
+            for (int r = 0; r < 3; r++) {  // Q <- Q J
+                T qrp = Q[r][p], qrq = Q[r][q];
+                Q[r][p] = c * qrp - s * qrq;
+                Q[r][q] = s * qrp + c * qrq;
+            }
+        }
+    }
+    for (int i = 0; i < 3; i++) lam[i] = a[i][i];
+    // descending order, as an SVD reports singular values
+    for (int i = 0; i < 2; i++)
+        for (int j = i + 1; j < 3; j++)
+            if (lam[j] > lam[i]) {
+                std::swap(lam[i], lam[j]);
+                for (int r = 0; r < 3; r++) std::swap(Q[r][i], Q[r][j]);
+            }
+}
+
+template <class T> void jacobi_eig4(const T a_in[4][4], T lam[4], T Q[4][4]) {
+    T a[4][4];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) { a[i][j] = a_in[i][j]; Q[i][j] = (i == j) ? T(1) : T(0); }
+    const T eps = std::numeric_limits<T>::epsilon();
+    for (int sweep = 0; sweep < 40; sweep++) {
+        T off = 0, dia = 0;
+        for (int i = 0; i < 4; i++) { dia += a[i][i] * a[i][i]; for (int j = i + 1; j < 4; j++) off += a[i][j] * a[i][j]; }
+        if (off <= eps * eps * dia * T(1e-2) || off == T(0)) break;
+        for (int p = 0; p < 3; p++)
+            for (int q = p + 1; q < 4; q++) {
+                T apq = a[p][q];
+                if (apq == T(0)) continue;
+                T tau = (a[q][q] - a[p][p]) / (2 * apq);
+                T t = (tau >= 0 ? T(1) : T(-1)) / (std::fabs(tau) + std::sqrt(T(1) + tau * tau));
+                T c = T(1) / std::sqrt(T(1) + t * t), s = t * c;
+                for (int r = 0; r < 4; r++) { T arp = a[r][p], arq = a[r][q]; a[r][p] = c * arp - s * arq; a[r][q] = s * arp + c * arq; }
+                for (int r = 0; r < 4; r++) { T apr = a[p][r], aqr = a[q][r]; a[p][r] = c * apr - s * aqr; a[q][r] = s * apr + c * aqr; }
+                a[p][q] = a[q][p] = T(0);
+                for (int r = 0; r < 4; r++) { T qrp = Q[r][p], qrq = Q[r][q]; Q[r][p] = c * qrp - s * qrq; Q[r][q] = s * qrp + c * qrq; }
+            }
+    }
+    for (int i = 0; i < 4; i++) lam[i] = a[i][i];
+}
+
+// 3x3 determinant by cofactor expansion along the first row
+template <class T> inline T det3(const M3<T>& g) {
+    return g(0, 0) * (g(1, 1) * g(2, 2) - g(1, 2) * g(2, 1)) - g(0, 1) * (g(1, 0) * g(2, 2) - g(1, 2) * g(2, 0)) +
+           g(0, 2) * (g(1, 0) * g(2, 1) - g(1, 1) * g(2, 0));
+}
+
+// 4x4 inverse by adjugate / determinant (returns inf/NaN entries for singular input, like a
+// plain cofactor inverse would)
+inline void inverse4(const float A[4][4], float inv[4][4]) {
+    auto minor3 = [&](int r, int c) {
+        float m[3][3];
+        int ri = 0;
+        for (int i = 0; i < 4; i++) {
+            if (i == r) continue;
+            int ci = 0;
+            for (int j = 0; j < 4; j++) { if (j == c) continue; m[ri][ci++] = A[i][j]; }
+            ri++;
+        }
+        return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+               m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    };
+    float cof[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) cof[i][j] = (((i + j) & 1) ? -1.0f : 1.0f) * minor3(i, j);
+    float det = A[0][0] * cof[0][0] + A[0][1] * cof[0][1] + A[0][2] * cof[0][2] + A[0][3] * cof[0][3];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) inv[i][j] = cof[j][i] / det;
+}
+
+enum : uint8_t { EMPTY = 0, INTERNAL = 1, LEAF = 2, UNCERTAIN = 3 };  // include/iso_method_ours.h:25-28
+
+template <class T> struct Node {
+    float c[3];
+    float half;
+    int depth;
+    uint8_t type;
+    uint64_t id;
+    T node[4];       // vertex xyz (centre unless moved by the QEF) and field value
+    int32_t child;   // index of first of 8 children, -1
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct OracleBase {
+    virtual ~OracleBase() {}
+};
+
+template <class T> struct Oracle : OracleBase {
+    orc_params prm;
+    std::vector<V3f> P;  // particle positions (float, as loaded: src/main.cpp:315-318)
+    HashGrid grid;
+    bool grid_built = false;
+    // Evaluator state (include/evaluator.h:56-74)
+    float Radius = 0, Radius2 = 0;
+    T Influnce2 = 0, Sigma = 0;
+    T ISO_VALUE = 0, MAX_SCALAR = -1;
+    const float NEIGHBOR_FACTOR = 4.0f;   // include/evaluator.h:21
+    float XMEAN_DELTA = 0.0f;             // include/evaluator.h:30, 0.9 in B-spline mode (evaluator.cpp:39-42)
+    std::vector<V3<T>> xMeans;
+    std::vector<M3<T>> Gs;
+    std::vector<T> dets;
+    std::vector<V3<T>> normals;
+    bool aniso_done = false, normals_done = false, consts_done = false;
+    // SurfReconstructor state
+    float BBox[6];
+    bool single_layer = false;
+    int D_MAX = 6, D_MIN = 5;
+    float RootHalf = 0, RootCenter[3] = {0, 0, 0};
+    std::vector<Node<T>> nodes;
+    bool tree_built = false;
+    // extraction
+    struct DualCell { int32_t n[8]; uint8_t sidx; };
+    std::vector<DualCell> duals;
+    std::vector<std::array<T, 3>> mesh_verts;
+    std::vector<std::array<int32_t, 3>> mesh_tris;
+    std::map<std::string, int> vmap, tmap;
+
+    explicit Oracle(const float* xyz, int64_t n, const orc_params& p) : prm(p) {
+        P.resize(n);
+        for (int64_t i = 0; i < n; i++) for (int k = 0; k < 3; k++) P[i][k] = xyz[3 * i + k];
+        // Evaluator ctor, const-radius branch (src/evaluator.cpp:31-37). The reference's
+        // constants use the *default* smooth factor 2.0 because setSmoothFactor runs later.
+        Radius = p.radius;
+        Radius2 = Radius * Radius;
+        const double h = double(Radius * 2.0f);
+        Influnce2 = T(std::pow(h, 2));
+        Sigma = T(1 / std::pow(h, 6));
+        if (!prm.use_poly6) XMEAN_DELTA = 0.9f;
+        loadRootBox();
+    }
+
+    // src/surface_reconstructor.cpp:30-49
+    void loadRootBox() {
+        BBox[0] = BBox[2] = BBox[4] = FLT_MAX;
+        BBox[1] = BBox[3] = BBox[5] = -FLT_MAX;
+        for (const V3f& p : P) {
+            for (int k = 0; k < 3; k++) {
+                if (p[k] < BBox[2 * k]) BBox[2 * k] = p[k];
+                if (p[k] > BBox[2 * k + 1]) BBox[2 * k + 1] = p[k];
+            }
+        }
+        single_layer = (BBox[0] == BBox[1] || BBox[2] == BBox[3] || BBox[4] == BBox[5]);
+    }
+
+    void build_grid(const float* bbox, float inf) {
+        grid.build(&P, bbox ? bbox : BBox, prm.radius, inf);
+        grid_built = true;
+    }
+
+    // ---- H10: AnisotropicInterpolate / Grad (src/evaluator.cpp:437-463), poly6 grad :408-415
+    inline T aniso_value(int j, const T d[3]) const {
+        const M3<T>& G = Gs[j];
+        T gd[3];
+        for (int a = 0; a < 3; a++) gd[a] = (G(a, 0) * d[0] + G(a, 1) * d[1]) + G(a, 2) * d[2];
+        T n2 = sqnorm3(gd[0], gd[1], gd[2]);
+        T k;
+        if (prm.use_poly6) k = poly6<T>(n2, Influnce2, Sigma);
+        else k = bspline<T>(T(std::sqrt(n2 / Influnce2)), Sigma);
+        return dets[j] * k;
+    }
+    inline void aniso_grad(int j, const T d[3], T g[3]) const {
+        const M3<T>& G = Gs[j];
+        T gd[3];
+        for (int a = 0; a < 3; a++) gd[a] = (G(a, 0) * d[0] + G(a, 1) * d[1]) + G(a, 2) * d[2];
+        T n2 = sqnorm3(gd[0], gd[1], gd[2]);
+        T w = (n2 > Influnce2) ? T(0) : ((Influnce2 - n2) * (Influnce2 - n2));
+        for (int a = 0; a < 3; a++) g[a] = dets[j] * ((Sigma * (T(-6) * gd[a])) * w);
+    }
+    inline T iso_value(const T d[3]) const { return poly6<T>(sqnorm3(d[0], d[1], d[2]), Influnce2, Sigma); }
+    inline void iso_grad(const T d[3], T g[3]) const {
+        T n2 = sqnorm3(d[0], d[1], d[2]);
+        T w = (n2 > Influnce2) ? T(0) : ((Influnce2 - n2) * (Influnce2 - n2));
+        for (int a = 0; a < 3; a++) g[a] = (Sigma * (T(-6) * d[a])) * w;
+    }
+
+    // ---- H11 SingleEval (src/evaluator.cpp:60-87). pos is evaluated in T; binning uses float.
+    T SingleEval(const T pos[3], const V3f& posf) const {
+        std::vector<int> nb;
+        grid.GetPIdxList(posf, nb);
+        T s = 0;
+        for (int j : nb) {
+            T d[3] = {pos[0] - xMeans[j][0], pos[1] - xMeans[j][1], pos[2] - xMeans[j][2]};
+            s += prm.use_ani ? aniso_value(j, d) : iso_value(d);
+        }
+        return ISO_VALUE - s;
+    }
+    T SingleEvalF(const float p[3]) const {
+        T pos[3] = {T(p[0]), T(p[1]), T(p[2])};
+        V3f pf{{p[0], p[1], p[2]}};
+        return SingleEval(pos, pf);
+    }
+    // ---- H12 SingleEvalWithGrad (src/evaluator.cpp:89-125)
+    void SingleEvalWithGrad(const V3f& p, T& scalar, T grad[3]) const {
+        if (MAX_SCALAR >= 0) { scalar = 0; grad[0] = grad[1] = grad[2] = 0; }
+        std::vector<int> nb;
+        grid.GetPIdxList(p, nb);
+        for (int j : nb) {
+            T d[3] = {T(p[0]) - xMeans[j][0], T(p[1]) - xMeans[j][1], T(p[2]) - xMeans[j][2]};
+            T g[3];
+            if (prm.use_ani) { scalar += aniso_value(j, d); aniso_grad(j, d, g); }
+            else { scalar += iso_value(d); iso_grad(d, g); }
+            for (int a = 0; a < 3; a++) grad[a] += g[a];
+        }
+        scalar = ISO_VALUE - scalar;
+    }
+
+    // ---- H8 compute_xMeans (src/evaluator.cpp:465-512)
+    void compute_xMean(int i, const std::vector<int>& cand, int& n_nb, V3<T>& xm) const {
+        const float nI = Radius * NEIGHBOR_FACTOR;
+        const float nI2 = nI * nI;
+        T wSum = 0;
+        xm = V3<T>{{0, 0, 0}};
+        n_nb = 0;
+        for (int j : cand) {
+            if (j == i) continue;
+            T dx = T(P[j][0]) - T(P[i][0]), dy = T(P[j][1]) - T(P[i][1]), dz = T(P[j][2]) - T(P[i][2]);
+            T d2 = sqnorm3(dx, dy, dz);
+            if (d2 >= T(nI2)) continue;
+            T d = std::sqrt(d2);
+            T w = wij<T>(d, T(nI));
+            wSum += w;
+            for (int k = 0; k < 3; k++) xm[k] += T(P[j][k]) * w;
+            n_nb++;
+        }
+        if (!prm.use_poly6) {
+            if (wSum > 0) {
+                for (int k = 0; k < 3; k++) xm[k] /= wSum;
+                const T a = T(1.0f - XMEAN_DELTA), b = T(XMEAN_DELTA);
+                for (int k = 0; k < 3; k++) xm[k] = T(P[i][k]) * a + xm[k] * b;
+            } else {
+                for (int k = 0; k < 3; k++) xm[k] = T(P[i][k]);
+            }
+        } else {
+            for (int k = 0; k < 3; k++) xm[k] = T(P[i][k]);
+        }
+    }
+
+    // ---- H9 compute_G_Yus (src/evaluator.cpp:585-654) and compute_G_ours (:514-583)
+    void compute_G(int i, const std::vector<int>& cand, M3<T>& G) const {
+        const float nI = Radius * NEIGHBOR_FACTOR;
+        const float nI2 = nI * nI;
+        T cov[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        T wSum = 0;
+        const V3<T>& xi = xMeans[i];
+        if (!prm.use_poly6) {
+            for (int a = 0; a < 3; a++) cov[a][a] = Influnce2;
+            for (int j : cand) {
+                const V3<T>& xj = xMeans[j];
+                T d2 = sqnorm3(xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]);
+                if (d2 >= T(nI2)) continue;
+                T d = std::sqrt(d2);
+                T w = wij<T>(d, T(nI));
+                wSum += w;
+                T wd[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) cov[a][b] += (wd[a] * wd[b]) * w;
+            }
+        } else {
+            const T invH = T(0.5);
+            for (int a = 0; a < 3; a++) cov[a][a] = T(1) + invH;
+            for (int j : cand) {
+                T d2 = sqnorm3(T(P[j][0]) - T(P[i][0]), T(P[j][1]) - T(P[i][1]), T(P[j][2]) - T(P[i][2]));
+                if (d2 >= T(nI2)) continue;
+                const V3<T>& xj = xMeans[j];
+                T d = std::sqrt(sqnorm3(T(P[i][0]) - xj[0], T(P[i][1]) - xj[1], T(P[i][2]) - xj[2]));
+                T w = wij<T>(d, T(nI));
+                wSum += w;
+                T wd[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) cov[a][b] += (wd[a] * wd[b]) * w;
+            }
+        }
+        for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) cov[a][b] /= wSum;
+        T lam[3], Q[3][3];
+        jacobi_eig3<T>(cov, lam, Q);
+        T w[3];
+        for (int k = 0; k < 3; k++) w[k] = std::fabs(lam[k]);
+        T mx = std::max(w[0], std::max(w[1], w[2]));
+        const T kr = T(4);
+        for (int k = 0; k < 3; k++) w[k] = std::max(w[k], mx / kr);
+        T inv[3] = {T(1) / w[0], T(1) / w[1], T(1) / w[2]};
+        T scale = T(std::pow(double((w[0] * w[1]) * w[2]), 1.0 / 3.0));
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                T s = ((Q[a][0] * inv[0]) * Q[b][0] + (Q[a][1] * inv[1]) * Q[b][1]) + (Q[a][2] * inv[2]) * Q[b][2];
+                G(a, b) = s * scale;
+                if (prm.use_poly6) G(a, b) = G(a, b) * T(0.5);
+            }
+    }
+
+    // ---- compute_Gs_xMeans (src/evaluator.cpp:656-749): two dependent passes
+    void anisotropy() {
+        const int N = (int)P.size();
+        xMeans.assign(N, V3<T>{{0, 0, 0}});
+        Gs.assign(N, M3<T>{});
+        dets.assign(N, T(1));
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int i = 0; i < N; i++) {
+            std::vector<int> cand;
+            grid.GetPIdxList(P[i], cand);
+            M3<T> I{};
+            for (int k = 0; k < 9; k++) I.m[k] = (k % 4 == 0) ? T(1) : T(0);
+            if (cand.size() <= 2) {
+                Gs[i] = I; dets[i] = det3(I);
+                for (int k = 0; k < 3; k++) xMeans[i][k] = T(P[i][k]);
+                continue;
+            }
+            int n_nb;
+            V3<T> xm;
+            compute_xMean(i, cand, n_nb, xm);
+            if (n_nb <= 2) { Gs[i] = I; dets[i] = det3(I); }
+            xMeans[i] = xm;
+        }
+        if (prm.use_ani) {
+#pragma omp parallel for schedule(dynamic, 256)
+            for (int i = 0; i < N; i++) {
+                std::vector<int> cand;
+                grid.GetPIdxList(P[i], cand);
+                M3<T> G;
+                compute_G(i, cand, G);
+                Gs[i] = G;
+                dets[i] = det3(G);
+            }
+        }
+        aniso_done = true;
+    }
+
+    // ---- H6 CalculateMaxScalarConstR (src/evaluator.cpp:272-291), RecommendIsoValueConstR (:323-335)
+    void iso_constants() {
+        const float sqrt2 = 1.4142135623730950488016887242097f, sqrt3 = 1.7320508075688772935274463415059f;
+        T k = 0;
+        if (prm.use_poly6) {
+            k += poly6<T>(T(0), Influnce2, Sigma);
+            k += poly6<T>(T(4 * Radius2), Influnce2, Sigma);
+            k += poly6<T>(T(8 * Radius2), Influnce2, Sigma);
+            k += poly6<T>(T(12 * Radius2), Influnce2, Sigma);
+        } else {
+            k += bspline<T>(T(0), Sigma);
+            k += bspline<T>(T(1), Sigma) * 6;
+            k += bspline<T>(T(sqrt2), Sigma) * 12;
+            k += bspline<T>(T(sqrt3), Sigma) * 8;
+        }
+        MAX_SCALAR = k;
+        if (prm.use_poly6) ISO_VALUE = poly6<T>(T(prm.iso_factor * Radius2), Influnce2, Sigma);
+        else ISO_VALUE = bspline<T>(T(0.01f), Sigma);
+        consts_done = true;
+    }
+
+    // ---- H12 CalcParticlesNormal (src/evaluator.cpp:363-375)
+    void particle_normals() {
+        const int N = (int)P.size();
+        normals.assign(N, V3<T>{{0, 0, 0}});
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int i = 0; i < N; i++) {
+            T s = 0, g[3] = {0, 0, 0};
+            SingleEvalWithGrad(P[i], s, g);
+            for (int k = 0; k < 3; k++) normals[i][k] = g[k];
+        }
+        normals_done = true;
+    }
+
+    // ---- H13 GenerateSampling (src/iso_method_ours.cpp:52-75) + GridEval (src/evaluator.cpp:127-252)
+    void gen_samples(const float c[3], float half, float* pts /*27*3 float*/) const {
+        const int os = prm.oversample;
+        float mn[3] = {c[0] - half, c[1] - half, c[2] - half};
+        float mx[3] = {c[0] + half, c[1] + half, c[2] + half};
+        for (int z = 0; z <= os; z++)
+            for (int y = 0; y <= os; y++)
+                for (int x = 0; x <= os; x++) {
+                    int idx = z * (os + 1) * (os + 1) + y * (os + 1) + x;
+                    int t[3] = {x, y, z};
+                    for (int a = 0; a < 3; a++) {
+                        float u = float(t[a]) / os;
+                        pts[idx * 3 + a] = (1 - u) * mn[a] + u * mx[a];
+                    }
+                }
+    }
+    void grid_eval(const float c[3], float half, T* samples /*n*4*/, T* grads /*n*3*/, bool& signchange) const {
+        const int os = prm.oversample, n1 = os + 1, ns = n1 * n1 * n1;
+        std::vector<float> pts(ns * 3);
+        gen_samples(c, half, pts.data());
+        const float cellsize = 2 * half;
+        const float step = cellsize / os;
+        signchange = false;
+        for (int i = 0; i < ns; i++) {
+            for (int a = 0; a < 3; a++) samples[i * 4 + a] = T(pts[i * 3 + a]);
+            samples[i * 4 + 3] = SingleEvalF(&pts[i * 3]);
+            bool origin = (samples[3] >= 0);
+            if (!signchange) signchange = origin ^ (samples[i * 4 + 3] >= 0);
+        }
+        // finite differences of the samples (negated gradient), one-sided at the faces
+        for (int z = 0; z <= os; z++)
+            for (int y = 0; y <= os; y++)
+                for (int x = 0; x <= os; x++) {
+                    int idx = z * n1 * n1 + y * n1 + x;
+                    int t[3] = {x, y, z};
+                    int strides[3] = {1, n1, n1 * n1};
+                    for (int a = 0; a < 3; a++) {
+                        T g;
+                        if (t[a] == 0) g = (samples[idx * 4 + 3] - samples[(idx + strides[a]) * 4 + 3]) / T(step);
+                        else if (t[a] == os) g = (samples[(idx - strides[a]) * 4 + 3] - samples[idx * 4 + 3]) / T(step);
+                        else g = (samples[(idx - strides[a]) * 4 + 3] - samples[(idx + strides[a]) * 4 + 3]) / T(step * 2);
+                        grads[idx * 3 + a] = g;
+                    }
+                }
+    }
+
+    // ---- H14 checkEmptyAndCalcCurv (src/surface_reconstructor.cpp:152-222)
+    void check_empty_curv(const float c[3], float half, bool& empty, float& curv) const {
+        V3f b1{{c[0] - half, c[1] - half, c[2] - half}};
+        V3f b2{{c[0] + half, c[1] + half, c[2] + half}};
+        std::vector<int> ins;
+        grid.GetInBoxParticles(b1, b2, ins);
+        empty = ins.empty();
+        float nrm[3] = {0, 0, 0};
+        float area = 0;
+        if (!empty) {
+            bool all_splash = true;
+            const float m = prm.radius * prm.smooth_factor;
+            for (int j : ins) {
+                const V3f& p = P[j];
+                if (p[0] > (b1[0] - m) && p[0] < (b2[0] + m) && p[1] > (b1[1] - m) && p[1] < (b2[1] + m) &&
+                    p[2] > (b1[2] - m) && p[2] < (b2[2] + m)) {
+                    if (prm.calc_normals) {
+                        float n[3] = {float(normals[j][0]), float(normals[j][1]), float(normals[j][2])};
+                        for (int a = 0; a < 3; a++) nrm[a] += n[a];
+                        area += std::sqrt(sqnorm3(n[0], n[1], n[2]));
+                    }
+                    all_splash = false;
+                }
+            }
+            empty = all_splash;
+        }
+        curv = (area == 0) ? 1.0f : (std::sqrt(sqnorm3(nrm[0], nrm[1], nrm[2])) / area);
+    }
+
+    // ---- H16 NodeCalcNode (src/iso_method_ours.cpp:101-174), calcErrorDMC (:40-50),
+    //      QEFNormal::combineSelf (include/qefnorm.h:18-33)
+    void qef_node(const float c[3], float half, const T* samples, const T* grads, int mode, T out[4]) const {
+        const int os = prm.oversample, ns = (os + 1) * (os + 1) * (os + 1);
+        const float cellsize = 2 * half;
+        const float border = 0.0f * cellsize;
+        float mn[3] = {c[0] - half + border, c[1] - half + border, c[2] - half + border};
+        float mxv[3] = {c[0] + half - border, c[1] + half - border, c[2] + half - border};
+        T pc[4] = {0, 0, 0, 0};
+        bool inside = false;
+        T err = T(1e30);
+        if (mode == 1) {
+            float q[15] = {0};
+            for (int i = 0; i < ns; i++) {
+                float pl[5];
+                pl[0] = float(grads[i * 3 + 0]); pl[1] = float(grads[i * 3 + 1]); pl[2] = float(grads[i * 3 + 2]);
+                pl[3] = -1;
+                float px = float(samples[i * 4]), py = float(samples[i * 4 + 1]), pz = float(samples[i * 4 + 2]);
+                pl[4] = -((px * pl[0] + py * pl[1]) + pz * pl[2]) + float(samples[i * 4 + 3]);
+                int k = 0;
+                for (int a = 0; a < 5; a++) for (int b = a; b < 5; b++) q[k++] += pl[a] * pl[b];
+            }
+            float A[4][4], B[4];
+            for (int a = 0; a < 4; a++) {
+                int idx = ((2 * 4 + 3 - a) * a) / 2;
+                for (int b = a; b < 4; b++) { A[a][b] = q[idx + b - a]; A[b][a] = A[a][b]; }
+                B[a] = -q[idx + 4 - a];
+            }
+            float inv[4][4];
+            inverse4(A, inv);
+            float r[4];
+            for (int a = 0; a < 4; a++) { r[a] = 0; for (int b = 0; b < 4; b++) r[a] += inv[b][a] * B[b]; }
+            for (int a = 0; a < 4; a++) pc[a] = T(r[a]);
+        } else {
+            // FP64 truncated-SVD pseudo-inverse about the mass point m = (centre, f at centre sample)
+            double q[5][5] = {{0}};
+            for (int i = 0; i < ns; i++) {
+                double pl[5];
+                pl[0] = double(grads[i * 3 + 0]); pl[1] = double(grads[i * 3 + 1]); pl[2] = double(grads[i * 3 + 2]);
+                pl[3] = -1;
+                pl[4] = -(double(samples[i * 4]) * pl[0] + double(samples[i * 4 + 1]) * pl[1] +
+                          double(samples[i * 4 + 2]) * pl[2]) + double(samples[i * 4 + 3]);
+                for (int a = 0; a < 5; a++) for (int b = 0; b < 5; b++) q[a][b] += pl[a] * pl[b];
+            }
+            double A[4][4], B[4];
+            for (int a = 0; a < 4; a++) { for (int b = 0; b < 4; b++) A[a][b] = q[a][b]; B[a] = -q[a][4]; }
+            int mid = ns / 2;
+            double m[4] = {double(c[0]), double(c[1]), double(c[2]), double(samples[mid * 4 + 3])};
+            double rhs[4];
+            for (int a = 0; a < 4; a++) { rhs[a] = B[a]; for (int b = 0; b < 4; b++) rhs[a] -= A[a][b] * m[b]; }
+            double lam[4], Q[4][4];
+            jacobi_eig4<double>(A, lam, Q);
+            double lmax = 0;
+            for (int a = 0; a < 4; a++) lmax = std::max(lmax, std::fabs(lam[a]));
+            double x[4] = {0, 0, 0, 0};
+            for (int k = 0; k < 4; k++) {
+                if (std::fabs(lam[k]) <= double(prm.qef_svd_tol) * lmax || lam[k] == 0) continue;
+                double dot = 0;
+                for (int a = 0; a < 4; a++) dot += Q[a][k] * rhs[a];
+                for (int a = 0; a < 4; a++) x[a] += Q[a][k] * dot / lam[k];
+            }
+            for (int a = 0; a < 4; a++) pc[a] = T(m[a] + x[a]);
+        }
+        T nd[4] = {T(c[0]), T(c[1]), T(c[2]), T(0)};
+        if (pc[0] >= T(mn[0]) && pc[0] <= T(mxv[0]) && pc[1] >= T(mn[1]) && pc[1] <= T(mxv[1]) &&
+            pc[2] >= T(mn[2]) && pc[2] <= T(mxv[2])) {
+            inside = true;
+            err = 0;
+            for (int i = 0; i < ns; i++) {
+                T g[3] = {grads[i * 3], grads[i * 3 + 1], grads[i * 3 + 2]};
+                T dp = (g[0] * (pc[0] - samples[i * 4]) + g[1] * (pc[1] - samples[i * 4 + 1])) + g[2] * (pc[2] - samples[i * 4 + 2]);
+                T e = dp - pc[3];
+                err += (e * e) / (T(1) + sqnorm3(g[0], g[1], g[2]));
+            }
+            for (int a = 0; a < 4; a++) nd[a] = pc[a];
+        }
+        (void)inside;
+        // the node value is always re-evaluated at the (possibly moved) vertex
+        T pos[3] = {nd[0], nd[1], nd[2]};
+        V3f pf{{float(nd[0]), float(nd[1]), float(nd[2])}};
+        nd[3] = SingleEval(pos, pf);
+        if (err / T(cellsize) < T(0.0001)) nd[3] = 0;
+        for (int a = 0; a < 4; a++) out[a] = nd[a];
+    }
+
+    // ---- H15 resizeRootBoxConstR (src/surface_reconstructor.cpp:97-122)
+    void resize_root() {
+        float r = prm.radius;
+        float maxLen = std::max({BBox[1] - BBox[0], BBox[3] - BBox[2], BBox[5] - BBox[4]});
+        int D = int(std::ceil(std::log2(std::ceil(double(maxLen / r)))));
+        float resizeLen = float(std::pow(2.0, D) * double(r));
+        while (resizeLen - maxLen < (NEIGHBOR_FACTOR * r * 2)) {
+            D++;
+            resizeLen = float(std::pow(2.0, D) * double(r));
+        }
+        if (prm.depth_max_override > 0) {
+            D = prm.depth_max_override;
+            resizeLen = float(std::pow(2.0, D) * double(r));
+        }
+        D_MAX = D;
+        RootHalf = resizeLen / 2;
+        for (int i = 0; i < 3; i++) RootCenter[i] = (BBox[2 * i] + BBox[2 * i + 1]) / 2;
+        D_MIN = D_MAX - (single_layer ? 1 : 3);
+    }
+
+    // classify one non-root node given its geometry (beforeSampleEval + afterSampleEval,
+    // src/surface_reconstructor.cpp:224-273); qef may move node xyz
+    void classify(Node<T>& nd, float* curv_out, int* sc_out) {
+        bool empty;
+        float curv;
+        check_empty_curv(nd.c, nd.half, empty, curv);
+        if (curv_out) *curv_out = curv;
+        if (sc_out) *sc_out = 0;
+        for (int a = 0; a < 3; a++) nd.node[a] = T(nd.c[a]);
+        if (empty) { nd.node[3] = ISO_VALUE; nd.type = EMPTY; return; }
+        const bool isbig = nd.depth < D_MIN;
+        bool signchange = false;
+        const float cellsize = 2 * nd.half;
+        const int ns = (prm.oversample + 1) * (prm.oversample + 1) * (prm.oversample + 1);
+        std::vector<T> samples(ns * 4), grads(ns * 3);
+        if (!isbig) grid_eval(nd.c, nd.half, samples.data(), grads.data(), signchange);
+        if (sc_out) *sc_out = signchange ? 1 : 0;
+        auto leaf_value = [&]() {
+            if (prm.qef_mode != 0 && !isbig) {
+                T out[4];
+                qef_node(nd.c, nd.half, samples.data(), grads.data(), prm.qef_mode, out);
+                for (int a = 0; a < 4; a++) nd.node[a] = out[a];
+            } else {
+                nd.node[3] = SingleEvalF(nd.c);
+            }
+        };
+        if (double(cellsize - prm.radius) < 1e-8) { nd.type = LEAF; leaf_value(); return; }
+        if (isbig || (signchange && curv < 0.99)) nd.type = INTERNAL;
+        else { nd.type = LEAF; leaf_value(); }
+    }
+
+    // ---- genIsoOurs state 0 (src/surface_reconstructor.cpp:275-407), level-synchronous
+    void build_tree() {
+        resize_root();
+        nodes.clear();
+        Node<T> root{};
+        for (int a = 0; a < 3; a++) { root.c[a] = RootCenter[a]; root.node[a] = T(RootCenter[a]); }
+        root.node[3] = 0;
+        root.half = RootHalf;
+        root.depth = 0;
+        root.type = UNCERTAIN;
+        root.id = 0;
+        root.child = -1;
+        nodes.push_back(root);
+        size_t lo = 0, hi = 1;
+        while (lo < hi) {
+#pragma omp parallel for schedule(dynamic, 16)
+            for (long long i = (long long)lo; i < (long long)hi; i++) classify(nodes[i], nullptr, nullptr);
+            for (size_t i = lo; i < hi; i++) {
+                if (nodes[i].type != INTERNAL) continue;
+                nodes[i].child = (int32_t)nodes.size();
+                for (int t = 0; t < 8; t++) {
+                    const Node<T>& par = nodes[i];
+                    Node<T> ch{};
+                    ch.depth = par.depth + 1;
+                    ch.half = par.half / 2;
+                    int bits[3] = {t & 1, (t >> 1) & 1, (t >> 2) & 1};
+                    for (int a = 0; a < 3; a++) {
+                        float s = bits[a] ? 1.0f : -1.0f;
+                        ch.c[a] = par.c[a] + s * ch.half;
+                        ch.node[a] = T(ch.c[a]);
+                    }
+                    ch.node[3] = 0;
+                    ch.type = UNCERTAIN;
+                    ch.id = par.id * 8 + (uint64_t)t;
+                    ch.child = -1;
+                    nodes.push_back(ch);
+                }
+            }
+            lo = hi;
+            hi = nodes.size();
+        }
+        tree_built = true;
+    }
+
+    // ---- Extraction: include/traverse.h:12-312 with VisitorExtract::on_* (src/visitorextract.cpp:229-452)
+    bool is_leaf(int n) const { return nodes[n].type == LEAF || nodes[n].type == EMPTY; }
+    int gen(int n, int i) const { return is_leaf(n) ? n : nodes[n].child + i; }
+    static int IX(int x, int y, int z) { return x | (y << 1) | (z << 2); }
+    int sgn(int n) const { return nodes[n].node[3] > 0 ? 1 : -1; }
+
+    void trav_vert(const int n[8]) {
+        bool all = true;
+        for (int i = 0; i < 8; i++) all = all && is_leaf(n[i]);
+        if (all) {
+            int index = 0;
+            for (int i = 0; i < 8; i++) if (sgn(n[i]) > 0) index += 1 << i;
+            uint8_t code = kDmcCase[index];
+            if (!(code == 0 || code == 255)) {
+                DualCell dc;
+                uint32_t perm = kDmcPerm[index];
+                for (int i = 0; i < 8; i++) dc.n[i] = n[(perm >> (4 * i)) & 15];
+                dc.sidx = (uint8_t)index;
+                duals.push_back(dc);
+            }
+            return;
+        }
+        int c[8];
+        for (int k = 0; k < 8; k++) c[k] = gen(n[k], k ^ 7);
+        trav_vert(c);
+    }
+    void vert8(const int* c) { int n[8]; for (int k = 0; k < 8; k++) n[k] = c[k]; trav_vert(n); }
+    void trav_edge_x(int n00, int n10, int n01, int n11) {
+        if (is_leaf(n00) && is_leaf(n10) && is_leaf(n01) && is_leaf(n11)) return;
+        int c[8];
+        for (int i = 0; i < 2; i++) {
+            c[IX(i, 0, 0)] = gen(n00, IX(i, 1, 1));
+            c[IX(i, 1, 0)] = gen(n10, IX(i, 0, 1));
+            c[IX(i, 0, 1)] = gen(n01, IX(i, 1, 0));
+            c[IX(i, 1, 1)] = gen(n11, IX(i, 0, 0));
+        }
+        for (int i = 0; i < 2; i++) trav_edge_x(c[IX(i, 0, 0)], c[IX(i, 1, 0)], c[IX(i, 0, 1)], c[IX(i, 1, 1)]);
+        vert8(c);
+    }
+    void trav_edge_y(int n00, int n10, int n01, int n11) {
+        if (is_leaf(n00) && is_leaf(n10) && is_leaf(n01) && is_leaf(n11)) return;
+        int c[8];
+        for (int i = 0; i < 2; i++) {
+            c[IX(0, i, 0)] = gen(n00, IX(1, i, 1));
+            c[IX(1, i, 0)] = gen(n10, IX(0, i, 1));
+            c[IX(0, i, 1)] = gen(n01, IX(1, i, 0));
+            c[IX(1, i, 1)] = gen(n11, IX(0, i, 0));
+        }
+        for (int i = 0; i < 2; i++) trav_edge_y(c[IX(0, i, 0)], c[IX(1, i, 0)], c[IX(0, i, 1)], c[IX(1, i, 1)]);
+        vert8(c);
+    }
+    void trav_edge_z(int n00, int n10, int n01, int n11) {
+        if (is_leaf(n00) && is_leaf(n10) && is_leaf(n01) && is_leaf(n11)) return;
+        int c[8];
+        for (int i = 0; i < 2; i++) {
+            c[IX(0, 0, i)] = gen(n00, IX(1, 1, i));
+            c[IX(1, 0, i)] = gen(n10, IX(0, 1, i));
+            c[IX(0, 1, i)] = gen(n01, IX(1, 0, i));
+            c[IX(1, 1, i)] = gen(n11, IX(0, 0, i));
+        }
+        for (int i = 0; i < 2; i++) trav_edge_z(c[IX(0, 0, i)], c[IX(1, 0, i)], c[IX(0, 1, i)], c[IX(1, 1, i)]);
+        vert8(c);
+    }
+    void trav_face_x(int n0, int n1) {
+        if (is_leaf(n0) && is_leaf(n1)) return;
+        int c[8];
+        for (int i = 0; i < 4; i++) {
+            int a = i & 1, b = (i >> 1) & 1;
+            c[IX(0, a, b)] = gen(n0, IX(1, a, b));
+            c[IX(1, a, b)] = gen(n1, IX(0, a, b));
+        }
+        for (int i = 0; i < 4; i++) { int a = i & 1, b = (i >> 1) & 1; trav_face_x(c[IX(0, a, b)], c[IX(1, a, b)]); }
+        for (int i = 0; i < 2; i++) {
+            trav_edge_y(c[IX(0, i, 0)], c[IX(1, i, 0)], c[IX(0, i, 1)], c[IX(1, i, 1)]);
+            trav_edge_z(c[IX(0, 0, i)], c[IX(1, 0, i)], c[IX(0, 1, i)], c[IX(1, 1, i)]);
+        }
+        vert8(c);
+    }
+    void trav_face_y(int n0, int n1) {
+        if (is_leaf(n0) && is_leaf(n1)) return;
+        int c[8];
+        for (int i = 0; i < 4; i++) {
+            int a = i & 1, b = (i >> 1) & 1;
+            c[IX(a, 0, b)] = gen(n0, IX(a, 1, b));
+            c[IX(a, 1, b)] = gen(n1, IX(a, 0, b));
+        }
+        for (int i = 0; i < 4; i++) { int a = i & 1, b = (i >> 1) & 1; trav_face_y(c[IX(a, 0, b)], c[IX(a, 1, b)]); }
+        for (int i = 0; i < 2; i++) {
+            trav_edge_x(c[IX(i, 0, 0)], c[IX(i, 1, 0)], c[IX(i, 0, 1)], c[IX(i, 1, 1)]);
+            trav_edge_z(c[IX(0, 0, i)], c[IX(1, 0, i)], c[IX(0, 1, i)], c[IX(1, 1, i)]);
+        }
+        vert8(c);
+    }
+    void trav_face_z(int n0, int n1) {
+        if (is_leaf(n0) && is_leaf(n1)) return;
+        int c[8];
+        for (int i = 0; i < 4; i++) {
+            int a = i & 1, b = (i >> 1) & 1;
+            c[IX(a, b, 0)] = gen(n0, IX(a, b, 1));
+            c[IX(a, b, 1)] = gen(n1, IX(a, b, 0));
+        }
+        for (int i = 0; i < 4; i++) { int a = i & 1, b = (i >> 1) & 1; trav_face_z(c[IX(a, b, 0)], c[IX(a, b, 1)]); }
+        for (int i = 0; i < 2; i++) {
+            trav_edge_x(c[IX(i, 0, 0)], c[IX(i, 1, 0)], c[IX(i, 0, 1)], c[IX(i, 1, 1)]);
+            trav_edge_y(c[IX(0, i, 0)], c[IX(1, i, 0)], c[IX(0, i, 1)], c[IX(1, i, 1)]);
+        }
+        vert8(c);
+    }
+    void trav_node(int n) {
+        if (is_leaf(n)) return;
+        int c[8];
+        for (int i = 0; i < 8; i++) { c[i] = gen(n, i); trav_node(c[i]); }
+        for (int i = 0; i < 4; i++) {
+            int a = i & 1, b = (i >> 1) & 1;
+            trav_face_x(c[IX(0, a, b)], c[IX(1, a, b)]);
+            trav_face_y(c[IX(a, 0, b)], c[IX(a, 1, b)]);
+            trav_face_z(c[IX(a, b, 0)], c[IX(a, b, 1)]);
+        }
+        for (int i = 0; i < 2; i++) {
+            trav_edge_x(c[IX(i, 0, 0)], c[IX(i, 1, 0)], c[IX(i, 0, 1)], c[IX(i, 1, 1)]);
+            trav_edge_y(c[IX(0, i, 0)], c[IX(1, i, 0)], c[IX(0, i, 1)], c[IX(1, i, 1)]);
+            trav_edge_z(c[IX(0, 0, i)], c[IX(1, 0, i)], c[IX(0, 1, i)], c[IX(1, 1, i)]);
+        }
+        vert8(c);
+    }
+
+    // ---- H17 bisection (src/visitorextract.cpp:47-84), one edge
+    void bisect(const T v1in[4], const T v2in[4], T out[3], int& nev) const {
+        T v1[4], v2[4], v[4] = {0, 0, 0, 0};
+        for (int a = 0; a < 4; a++) { v1[a] = v1in[a]; v2[a] = v2in[a]; }
+        T d = std::sqrt(sqnorm3(v1[0] - v2[0], v1[1] - v2[1], v1[2] - v2[2]));
+        const T r = T(prm.radius / 2);
+        nev = 0;
+        auto sg = [](const T* x) { return x[3] > 0 ? 1 : -1; };
+        while (d > r) {
+            for (int a = 0; a < 3; a++) v[a] = (v1[a] + v2[a]) / 2;
+            V3f pf{{float(v[0]), float(v[1]), float(v[2])}};
+            v[3] = SingleEval(v, pf);
+            nev++;
+            if (sg(v) == sg(v1)) { for (int a = 0; a < 4; a++) { v1[a] = v[a]; v[a] = 0; } }
+            else if (sg(v) == sg(v2)) { for (int a = 0; a < 4; a++) { v2[a] = v[a]; v[a] = 0; } }
+            else break;
+            d /= 2;
+        }
+        T ratio = (T(0) - v1[3]) / (v2[3] - v1[3]);
+        if (ratio < T(0.1)) for (int a = 0; a < 3; a++) out[a] = v1[a];
+        else if (ratio > T(0.9)) for (int a = 0; a < 3; a++) out[a] = v2[a];
+        else for (int a = 0; a < 3; a++) out[a] = v1[a] + (v2[a] - v1[a]) * ratio;
+    }
+
+    // ---- calc_vertices + generate_mesh (src/visitorextract.cpp:41-227), Mesh (src/global.cpp:13-53)
+    static std::string key_of(std::vector<unsigned long long> ids) {
+        std::sort(ids.begin(), ids.end());
+        std::string k;
+        for (auto v : ids) { k += std::to_string(v); k += ","; }
+        return k;
+    }
+    static std::string key_of_i(std::vector<int> ids) {
+        std::sort(ids.begin(), ids.end());
+        std::string k;
+        for (auto v : ids) { k += std::to_string(v); k += ","; }
+        return k;
+    }
+    int canon_slot(uint8_t code) const {
+        for (int k = 0; k < PISURF_DMC_NCASES; k++) if (kDmcCanon[k] == code) return k;
+        return -1;
+    }
+    void extract() {
+        duals.clear();
+        trav_node(0);
+        const size_t nd = duals.size();
+        std::vector<std::array<std::array<T, 3>, 12>> verts(nd);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long long i = 0; i < (long long)nd; i++) {
+            const DualCell& dc = duals[i];
+            for (int e = 0; e < 12; e++) {
+                const Node<T>& a = nodes[dc.n[kDmcEdge[e][0]]];
+                const Node<T>& b = nodes[dc.n[kDmcEdge[e][1]]];
+                int sa = a.node[3] > 0 ? 1 : -1, sb = b.node[3] > 0 ? 1 : -1;
+                if (sa != sb) {
+                    int nev;
+                    T out[3];
+                    bisect(a.node, b.node, out, nev);
+                    for (int k = 0; k < 3; k++) verts[i][e][k] = out[k];
+                }
+            }
+        }
+        mesh_verts.clear();
+        mesh_tris.clear();
+        vmap.clear();
+        tmap.clear();
+        for (size_t i = 0; i < nd; i++) {
+            const DualCell& dc = duals[i];
+            uint8_t code = kDmcCase[dc.sidx];
+            int slot = canon_slot(code);
+            if (slot < 0) continue;
+            for (int t = 0; t < kDmcNTri[slot]; t++) {
+                int vid[3];
+                for (int k = 0; k < 3; k++) {
+                    int e = kDmcTri[slot][t][k];
+                    uint64_t ida = nodes[dc.n[kDmcEdge[e][0]]].id, idb = nodes[dc.n[kDmcEdge[e][1]]].id;
+                    auto res = vmap.try_emplace(key_of({ida, idb}), (int)mesh_verts.size());
+                    if (res.second) mesh_verts.push_back(verts[i][e]);
+                    vid[k] = res.first->second + 1;
+                }
+                int t0, t1, t2;
+                if (kDmcMirror[dc.sidx]) { t0 = vid[0]; t1 = vid[1]; t2 = vid[2]; }
+                else { t0 = vid[2]; t1 = vid[1]; t2 = vid[0]; }
+                auto res = tmap.try_emplace(key_of_i({t0, t1, t2}), (int)mesh_tris.size());
+                if (res.second) mesh_tris.push_back({t0, t1, t2});
+            }
+        }
+    }
+};
+
+struct Ctx {
+    orc_params prm;
+    std::unique_ptr<Oracle<float>> f32;
+    std::unique_ptr<Oracle<double>> f64;
+};
+
+template <class F> auto dispatch(Ctx* c, F&& fn) {
+    if (c->f64) return fn(*c->f64);
+    return fn(*c->f32);
+}
+
+}  // namespace
+
+struct orc_ctx { Ctx impl; };
+
+extern "C" {
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+
+orc_ctx* orc_create(const float* xyz, int64_t n, const orc_params* p) {
+    if (!p || n < 0) return nullptr;
+    orc_ctx* c = new orc_ctx();
+    c->impl.prm = *p;
+    if (p->threads > 0) omp_set_num_threads(p->threads);
+    if (p->fp64) c->impl.f64.reset(new Oracle<double>(xyz, n, *p));
+    else c->impl.f32.reset(new Oracle<float>(xyz, n, *p));
+    return c;
+}
+void orc_destroy(orc_ctx* c) { delete c; }
+
+int orc_hashgrid_build(orc_ctx* c, const float* bbox, float inf, float* ob, uint32_t* ocn, int64_t* ocnt,
+                       float* ocs, int64_t* ninv) {
+    return dispatch(&c->impl, [&](auto& o) {
+        o.build_grid(bbox, inf);
+        if (ob) for (int i = 0; i < 6; i++) ob[i] = o.grid.Bounding[i];
+        if (ocn) for (int i = 0; i < 3; i++) ocn[i] = o.grid.XYZCellNum[i];
+        if (ocnt) *ocnt = (int64_t)o.grid.CellNum;
+        if (ocs) *ocs = o.grid.CellSize;
+        if (ninv) *ninv = o.grid.n_invalid;
+        return 0;
+    });
+}
+int orc_hashgrid_index_list(orc_ctx* c, int32_t* il) {
+    return dispatch(&c->impl, [&](auto& o) {
+        for (size_t i = 0; i < o.grid.IndexList.size(); i++) il[i] = o.grid.IndexList[i];
+        return 0;
+    });
+}
+int64_t orc_hashgrid_cells(orc_ctx* c, int64_t* keys, int32_t* start, int32_t* end) {
+    return dispatch(&c->impl, [&](auto& o) {
+        int64_t k = 0;
+        for (auto& kv : o.grid.StartList) {
+            if (keys) keys[k] = kv.first;
+            if (start) start[k] = kv.second;
+            if (end) end[k] = o.grid.EndList.at(kv.first);
+            k++;
+        }
+        return k;
+    });
+}
+int64_t orc_neighbors(orc_ctx* c, const float* q, int64_t nq, int64_t* off, int32_t* ids, int64_t cap) {
+    return dispatch(&c->impl, [&](auto& o) {
+        int64_t tot = 0;
+        std::vector<int> nb;
+        for (int64_t i = 0; i < nq; i++) {
+            nb.clear();
+            V3f p{{q[3 * i], q[3 * i + 1], q[3 * i + 2]}};
+            o.grid.GetPIdxList(p, nb);
+            if (off) off[i] = tot;
+            for (int j : nb) { if (ids && tot < cap) ids[tot] = j; tot++; }
+        }
+        if (off) off[nq] = tot;
+        return tot;
+    });
+}
+int64_t orc_box_particles(orc_ctx* c, const float* b1, const float* b2, int32_t* ids, int64_t cap) {
+    return dispatch(&c->impl, [&](auto& o) {
+        std::vector<int> v;
+        o.grid.GetInBoxParticles(V3f{{b1[0], b1[1], b1[2]}}, V3f{{b2[0], b2[1], b2[2]}}, v);
+        for (int64_t i = 0; i < (int64_t)v.size() && i < cap; i++) ids[i] = v[i];
+        return (int64_t)v.size();
+    });
+}
+int orc_anisotropy(orc_ctx* c) {
+    return dispatch(&c->impl, [&](auto& o) {
+        if (!o.grid_built) o.build_grid(nullptr, 4.0f);
+        o.anisotropy();
+        return 0;
+    });
+}
+int orc_get_aniso(orc_ctx* c, double* xm, double* G, double* det) {
+    return dispatch(&c->impl, [&](auto& o) {
+        if (!o.aniso_done) return -1;
+        for (size_t i = 0; i < o.P.size(); i++) {
+            for (int k = 0; k < 3; k++) if (xm) xm[3 * i + k] = double(o.xMeans[i][k]);
+            for (int k = 0; k < 9; k++) if (G) G[9 * i + k] = double(o.Gs[i].m[k]);
+            if (det) det[i] = double(o.dets[i]);
+        }
+        return 0;
+    });
+}
+int orc_set_aniso(orc_ctx* c, const double* xm, const double* G, const double* det) {
+    return dispatch(&c->impl, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.Influnce2)>::type;
+        const size_t N = o.P.size();
+        o.xMeans.resize(N); o.Gs.resize(N); o.dets.resize(N);
+        for (size_t i = 0; i < N; i++) {
+            for (int k = 0; k < 3; k++) o.xMeans[i][k] = T(xm[3 * i + k]);
+            for (int k = 0; k < 9; k++) o.Gs[i].m[k] = T(G[9 * i + k]);
+            o.dets[i] = T(det[i]);
+        }
+        o.aniso_done = true;
+        return 0;
+    });
+}
+int orc_iso_constants(orc_ctx* c, double* ms, double* iso) {
+    return dispatch(&c->impl, [&](auto& o) {
+        o.iso_constants();
+        if (ms) *ms = double(o.MAX_SCALAR);
+        if (iso) *iso = double(o.ISO_VALUE);
+        return 0;
+    });
+}
+int orc_get_constants(orc_ctx* c, double* i2, double* s) {
+    return dispatch(&c->impl, [&](auto& o) {
+        if (i2) *i2 = double(o.Influnce2);
+        if (s) *s = double(o.Sigma);
+        return 0;
+    });
+}
+int orc_particle_normals(orc_ctx* c, double* nrm) {
+    return dispatch(&c->impl, [&](auto& o) {
+        if (!o.aniso_done || !o.consts_done) return -1;
+        o.particle_normals();
+        if (nrm) for (size_t i = 0; i < o.P.size(); i++) for (int k = 0; k < 3; k++) nrm[3 * i + k] = double(o.normals[i][k]);
+        return 0;
+    });
+}
+int orc_eval_points(orc_ctx* c, const float* q, int64_t n, double* f, double* grad) {
+    return dispatch(&c->impl, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.Influnce2)>::type;
+        if (!o.aniso_done) return -1;
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; i++) {
+            if (grad) {
+                T s = 0, g[3] = {0, 0, 0};
+                V3f p{{q[3 * i], q[3 * i + 1], q[3 * i + 2]}};
+                o.SingleEvalWithGrad(p, s, g);
+                f[i] = double(s);
+                for (int k = 0; k < 3; k++) grad[3 * i + k] = double(g[k]);
+            } else {
+                f[i] = double(o.SingleEvalF(&q[3 * i]));
+            }
+        }
+        return 0;
+    });
+}
+int orc_grid_eval(orc_ctx* c, const float* center, float half, double* samples, double* grads, int* sc) {
+    return dispatch(&c->impl, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.Influnce2)>::type;
+        if (!o.aniso_done) return -1;
+        const int ns = (o.prm.oversample + 1) * (o.prm.oversample + 1) * (o.prm.oversample + 1);
+        std::vector<T> s(ns * 4), g(ns * 3);
+        bool schg;
+        o.grid_eval(center, half, s.data(), g.data(), schg);
+        for (int i = 0; i < ns * 4; i++) samples[i] = double(s[i]);
+        if (grads) for (int i = 0; i < ns * 3; i++) grads[i] = double(g[i]);
+        if (sc) *sc = schg ? 1 : 0;
+        return 0;
+    });
+}
+int orc_classify_nodes(orc_ctx* c, const float* centers, const float* halfs, const int32_t* depths, int64_t n,
+                       uint8_t* type, double* f, float* curv, int32_t* sc) {
+    return dispatch(&c->impl, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.Influnce2)>::type;
+        if (!o.aniso_done || !o.consts_done || (o.prm.calc_normals && !o.normals_done)) return -1;
+        if (!o.tree_built) o.resize_root();
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < n; i++) {
+            Node<T> nd{};
+            for (int a = 0; a < 3; a++) nd.c[a] = centers[3 * i + a];
+            nd.half = halfs[i];
+            nd.depth = depths[i];
+            float cv;
+            int s;
+            o.classify(nd, &cv, &s);
+            if (type) type[i] = nd.type;
+            if (f) f[i] = double(nd.node[3]);
+            if (curv) curv[i] = cv;
+            if (sc) sc[i] = s;
+        }
+        return 0;
+    });
+}
+int orc_build_tree(orc_ctx* c) {
+    return dispatch(&c->impl, [&](auto& o) {
+        if (!o.aniso_done || !o.consts_done || (o.prm.calc_normals && !o.normals_done)) return -1;
+        o.build_tree();
+        return 0;
+    });
+}
+int orc_tree_info(orc_ctx* c, int* dmax, int* dmin, float* rc, float* rh, int64_t* nn) {
+    return dispatch(&c->impl, [&](auto& o) {
+        if (!o.tree_built) o.resize_root();
+        if (dmax) *dmax = o.D_MAX;
+        if (dmin) *dmin = o.D_MIN;
+        if (rc) for (int a = 0; a < 3; a++) rc[a] = o.RootCenter[a];
+        if (rh) *rh = o.RootHalf;
+        if (nn) *nn = (int64_t)o.nodes.size();
+        return 0;
+    });
+}
+int orc_tree_nodes(orc_ctx* c, float* cen, float* half, int32_t* depth, uint8_t* type, uint64_t* id, double* node4,
+                   int32_t* child) {
+    return dispatch(&c->impl, [&](auto& o) {
+        for (size_t i = 0; i < o.nodes.size(); i++) {
+            const auto& nd = o.nodes[i];
+            if (cen) for (int a = 0; a < 3; a++) cen[3 * i + a] = nd.c[a];
+            if (half) half[i] = nd.half;
+            if (depth) depth[i] = nd.depth;
+            if (type) type[i] = nd.type;
+            if (id) id[i] = nd.id;
+            if (node4) for (int a = 0; a < 4; a++) node4[4 * i + a] = double(nd.node[a]);
+            if (child) child[i] = nd.child;
+        }
+        return 0;
+    });
+}
+int orc_qef(orc_ctx* c, const float* center, float half, const double* samples, const double* grads, int mode,
+            double* node4) {
+    return dispatch(&c->impl, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.Influnce2)>::type;
+        const int ns = (o.prm.oversample + 1) * (o.prm.oversample + 1) * (o.prm.oversample + 1);
+        std::vector<T> s(ns * 4), g(ns * 3);
+        for (int i = 0; i < ns * 4; i++) s[i] = T(samples[i]);
+        for (int i = 0; i < ns * 3; i++) g[i] = T(grads[i]);
+        T out[4];
+        o.qef_node(center, half, s.data(), g.data(), mode, out);
+        for (int a = 0; a < 4; a++) node4[a] = double(out[a]);
+        return 0;
+    });
+}
+int orc_bisect_edges(orc_ctx* c, const double* ends, int64_t n, double* out, int32_t* nev) {
+    return dispatch(&c->impl, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.Influnce2)>::type;
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; i++) {
+            T a[4], b[4], r[3];
+            for (int k = 0; k < 4; k++) { a[k] = T(ends[8 * i + k]); b[k] = T(ends[8 * i + 4 + k]); }
+            int ne;
+            o.bisect(a, b, r, ne);
+            for (int k = 0; k < 3; k++) out[3 * i + k] = double(r[k]);
+            if (nev) nev[i] = ne;
+        }
+        return 0;
+    });
+}
+int orc_extract(orc_ctx* c) {
+    return dispatch(&c->impl, [&](auto& o) {
+        if (!o.tree_built) return -1;
+        o.extract();
+        return 0;
+    });
+}
+int orc_mesh_counts(orc_ctx* c, int64_t* nv, int64_t* nt, int64_t* nd) {
+    return dispatch(&c->impl, [&](auto& o) {
+        if (nv) *nv = (int64_t)o.mesh_verts.size();
+        if (nt) *nt = (int64_t)o.mesh_tris.size();
+        if (nd) *nd = (int64_t)o.duals.size();
+        return 0;
+    });
+}
+int orc_mesh_get(orc_ctx* c, double* v, int32_t* t) {
+    return dispatch(&c->impl, [&](auto& o) {
+        for (size_t i = 0; i < o.mesh_verts.size(); i++) for (int k = 0; k < 3; k++) v[3 * i + k] = double(o.mesh_verts[i][k]);
+        for (size_t i = 0; i < o.mesh_tris.size(); i++) for (int k = 0; k < 3; k++) t[3 * i + k] = o.mesh_tris[i][k];
+        return 0;
+    });
+}
+int orc_dual_cells(orc_ctx* c, int32_t* n8, uint8_t* sidx) {
+    return dispatch(&c->impl, [&](auto& o) {
+        for (size_t i = 0; i < o.duals.size(); i++) {
+            for (int k = 0; k < 8; k++) n8[8 * i + k] = o.duals[i].n[k];
+            if (sidx) sidx[i] = o.duals[i].sidx;
+        }
+        return 0;
+    });
+}
+// writeObjFile (src/main.cpp:54-67)
+int orc_write_obj(orc_ctx* c, const char* path) {
+    return dispatch(&c->impl, [&](auto& o) {
+        FILE* f = std::fopen(path, "w");
+        if (!f) return -1;
+        for (auto& p : o.mesh_verts) std::fprintf(f, "v %f %f %f\n", double(p[0]), double(p[1]), double(p[2]));
+        for (auto& t : o.mesh_tris) std::fprintf(f, "f %d %d %d\n", t[0], t[1], t[2]);
+        std::fclose(f);
+        return 0;
+    });
+}
+// SurfReconstructor::Run (src/surface_reconstructor.cpp:409-479)
+int orc_run(orc_ctx* c, double* pt) {
+    return dispatch(&c->impl, [&](auto& o) {
+        double t0 = now_s();
+        o.build_grid(nullptr, 4.0f);
+        double t1 = now_s();
+        o.anisotropy();
+        double t2 = now_s();
+        o.resize_root();
+        o.iso_constants();
+        double t3 = now_s();
+        if (o.prm.calc_normals) o.particle_normals();
+        double t4 = now_s();
+        o.build_tree();
+        double t5 = now_s();
+        o.extract();
+        double t6 = now_s();
+        if (pt) { pt[0] = t1 - t0; pt[1] = t2 - t1; pt[2] = t3 - t2; pt[3] = t4 - t3; pt[4] = t5 - t4; pt[5] = t6 - t5; pt[6] = t6 - t0; }
+        return 0;
+    });
+}
+int64_t orc_eval_count(orc_ctx* c) { (void)c; return 0; }
+
+}  // extern "C"

@@ -1,0 +1,1308 @@
+// pisurf C ABI: context, device buffers, the host level loop of the octree build, and the
+// extraction driver. All field arithmetic runs in the HIP kernels (k_*.hip); the host only sets
+// up grid geometry, drives launches level by level, and assembles the mesh topology.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pisurf.h"
+#include "dmc_tables.h"
+#include "pisurf_internal.h"
+
+using namespace pisurf;
+
+namespace {
+
+struct DBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+struct LevelBuf {
+    int64_t n = 0;
+    DBuf c, depth, id, type, empty, curv, node, flag, child, scanned;
+    hipError_t ensure(int64_t m) {
+        hipError_t e;
+        if ((e = c.ensure(m * sizeof(float4))) != hipSuccess) return e;
+        if ((e = depth.ensure(m)) != hipSuccess) return e;
+        if ((e = id.ensure(m * 8)) != hipSuccess) return e;
+        if ((e = type.ensure(m)) != hipSuccess) return e;
+        if ((e = empty.ensure(m)) != hipSuccess) return e;
+        if ((e = curv.ensure(m * 4)) != hipSuccess) return e;
+        if ((e = node.ensure(m * sizeof(double4))) != hipSuccess) return e;
+        if ((e = flag.ensure((m + 1) * 4)) != hipSuccess) return e;
+        if ((e = child.ensure(m * 4)) != hipSuccess) return e;
+        if ((e = scanned.ensure((m + 1) * 4)) != hipSuccess) return e;
+        return hipSuccess;
+    }
+    NodeLevel view() const {
+        NodeLevel v;
+        v.c = c.as<float4>();
+        v.depth = depth.as<int8_t>();
+        v.id = id.as<uint64_t>();
+        v.type = type.as<uint8_t>();
+        v.empty = empty.as<uint8_t>();
+        v.curv = curv.as<float>();
+        v.node = node.as<double4>();
+        v.flag = flag.as<int>();
+        v.child = child.as<int>();
+        v.n = n;
+        return v;
+    }
+};
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// open-addressing map from a 128-bit key to an int (mesh vertex / triangle dedupe)
+struct KeyMap {
+    std::vector<uint64_t> ka, kb;
+    std::vector<int32_t> val;
+    size_t mask = 0, count = 0;
+    void reset(size_t expect) {
+        size_t cap = 1024;
+        while (cap < expect * 2) cap <<= 1;
+        ka.assign(cap, 0);
+        kb.assign(cap, 0);
+        val.assign(cap, -1);
+        mask = cap - 1;
+        count = 0;
+    }
+    static uint64_t mix(uint64_t x) {
+        x ^= x >> 33;
+        x *= 0xff51afd7ed558ccdULL;
+        x ^= x >> 33;
+        x *= 0xc4ceb9fe1a85ec53ULL;
+        x ^= x >> 33;
+        return x;
+    }
+    void grow() {
+        std::vector<uint64_t> oa, ob;
+        std::vector<int32_t> ov;
+        oa.swap(ka);
+        ob.swap(kb);
+        ov.swap(val);
+        const size_t cap = oa.size() * 2;
+        ka.assign(cap, 0);
+        kb.assign(cap, 0);
+        val.assign(cap, -1);
+        mask = cap - 1;
+        for (size_t i = 0; i < oa.size(); i++)
+            if (ov[i] >= 0) {
+                size_t h = mix(oa[i] * 0x9E3779B97F4A7C15ULL ^ ob[i]) & mask;
+                while (val[h] >= 0) h = (h + 1) & mask;
+                ka[h] = oa[i];
+                kb[h] = ob[i];
+                val[h] = ov[i];
+            }
+    }
+    // returns (value, inserted)
+    std::pair<int32_t, bool> try_emplace(uint64_t a, uint64_t b, int32_t v) {
+        if ((count + 1) * 2 > ka.size()) grow();
+        size_t h = mix(a * 0x9E3779B97F4A7C15ULL ^ b) & mask;
+        while (val[h] >= 0) {
+            if (ka[h] == a && kb[h] == b) return {val[h], false};
+            h = (h + 1) & mask;
+        }
+        ka[h] = a;
+        kb[h] = b;
+        val[h] = v;
+        count++;
+        return {v, true};
+    }
+};
+
+}  // namespace
+
+struct pisurf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    pisurf_params prm;
+    std::string err;
+    // particles
+    int64_t n = 0;
+    DBuf xyz;
+    float bbox[6] = {0, 0, 0, 0, 0, 0};
+    bool single_layer = false;
+    bool bbox_done = false;
+    // grid
+    bool grid_done = false;
+    pisurf_grid_info ginfo;
+    GridDev g;
+    DBuf keys, vals, keys2, vals2, cellcount, off, tmp, spos, part, small;
+    // anisotropy (records are for prec_rec)
+    bool aniso_done = false;
+    int prec_rec = 0;
+    DBuf xm, rec_a, rec_g;
+    // constants
+    bool consts_done = false;
+    double influnce2 = 0, sigma = 0, iso = 0, max_scalar = 0;
+    // normals
+    bool normals_done = false;
+    DBuf nrm;
+    // tree
+    bool tree_done = false;
+    int dmax = 0, dmin = 0;
+    float root_c[3] = {0, 0, 0}, root_half = 0;
+    std::vector<LevelBuf*> levels;
+    DBuf samples;
+    // scratch for API calls
+    DBuf q, out1, out2, out3;
+    // mesh
+    std::vector<double> mesh_v;
+    std::vector<int32_t> mesh_t;
+    // stats
+    DBuf counters;
+    pisurf_stats st;
+    unsigned long long n_fp64_eig = 0;
+
+    ~pisurf_ctx() {
+        for (auto* l : levels) delete l;
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+    int hipfail(hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return PISURF_E_HIP;
+    }
+    Counters* ctr() { return counters.as<Counters>(); }
+};
+
+#define HIPCHK(ctx, call)                                   \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return (ctx)->hipfail(e_, #call); \
+    } while (0)
+
+#define API_BEGIN try {
+#define API_END(ctx)                                                          \
+    }                                                                         \
+    catch (const std::bad_alloc&) { return (ctx)->fail(PISURF_E_OOM, "host allocation failed"); } \
+    catch (...) { return (ctx)->fail(PISURF_E_INVALID, "unexpected host exception"); }
+
+namespace {
+
+int prec_of(const pisurf_ctx* c) { return c->prm.precision; }
+
+FieldConst field_const(const pisurf_ctx* c) {
+    FieldConst fc;
+    fc.h2 = c->influnce2;
+    fc.inv_h2 = 1.0 / c->influnce2;
+    fc.sigma = c->sigma;
+    fc.iso = c->iso;
+    fc.use_ani = c->prm.use_ani;
+    fc.use_poly6 = c->prm.use_poly6;
+    if (prec_of(c) != PISURF_FP64) fc.inv_h2 = double(1.0f / float(c->influnce2));
+    return fc;
+}
+
+// Cubic B-spline and poly6 of the reference's constants (src/evaluator.cpp:377-396), evaluated
+// in double; FP32 modes narrow the result like the reference does.
+double bspline_d(double q, double sigma) {
+    if (q < 2) {
+        if (q <= 1) return sigma * (1 - 1.5 * std::pow(q, 2) + 0.75 * std::pow(q, 3));
+        return sigma * (0.25 * std::pow(2 - q, 3));
+    }
+    return 0.0;
+}
+
+// Evaluator ctor constants (src/evaluator.cpp:31-37) and CalculateMaxScalarConstR /
+// RecommendIsoValueConstR (src/evaluator.cpp:272-291,323-335).
+void compute_constants(pisurf_ctx* c) {
+    const float r = c->prm.radius;
+    const double h = double(r * 2.0f);
+    const bool f32 = prec_of(c) != PISURF_FP64;
+    auto nar = [&](double v) { return f32 ? double(float(v)) : v; };
+    c->influnce2 = nar(std::pow(h, 2));
+    c->sigma = nar(1 / std::pow(h, 6));
+    const float r2 = r * r;
+    if (c->prm.use_poly6) {
+        auto p6 = [&](double d2) {
+            double pd = d2 > c->influnce2 ? 0.0 : nar(std::pow(c->influnce2 - d2, 3));
+            return nar(pd * c->sigma);
+        };
+        double k = 0;
+        k = nar(k + p6(0.0));
+        k = nar(k + p6(nar(4 * r2)));
+        k = nar(k + p6(nar(8 * r2)));
+        k = nar(k + p6(nar(12 * r2)));
+        c->max_scalar = k;
+        c->iso = p6(nar(double(c->prm.iso_factor * r2)));
+    } else {
+        const float sqrt2 = 1.4142135623730950488016887242097f, sqrt3 = 1.7320508075688772935274463415059f;
+        double k = 0;
+        k = nar(k + nar(bspline_d(0.0, c->sigma)));
+        k = nar(k + nar(nar(bspline_d(1.0, c->sigma)) * 6));
+        k = nar(k + nar(nar(bspline_d(double(sqrt2), c->sigma)) * 12));
+        k = nar(k + nar(nar(bspline_d(double(sqrt3), c->sigma)) * 8));
+        c->max_scalar = k;
+        c->iso = nar(bspline_d(double(0.01f), c->sigma));
+    }
+    c->consts_done = true;
+}
+
+// HashGrid geometry (src/hash_grid.cpp:4-34), float arithmetic as in the reference
+void grid_geometry(const float* bounding, float radius, float inf, float out_b[6], uint32_t n3[3], int64_t& ncell,
+                   float& cell) {
+    cell = radius * inf;
+    for (int i = 0; i < 3; i++) {
+        const float length = std::ceil((bounding[2 * i + 1] - bounding[2 * i]) / cell) * cell;
+        const float center = (bounding[2 * i] + bounding[2 * i + 1]) / 2;
+        out_b[2 * i] = center - length / 2;
+        out_b[2 * i + 1] = center + length / 2;
+    }
+    for (int i = 0; i < 3; i++)
+        n3[i] = (uint32_t)std::max(int(std::ceil((out_b[2 * i + 1] - out_b[2 * i]) / cell)), 1);
+    ncell = (int64_t)n3[0] * n3[1] * n3[2];
+}
+
+int host_cell_coord(float p, float b0, float cell, int n) {
+    const float v = (p - b0) / cell;
+    if (!(v > -8.0f && v < float(n) + 8.0f)) return (v > 0.0f) ? n + 8 : -8;
+    return (int)v;
+}
+
+int ensure_bbox(pisurf_ctx* c) {
+    if (c->bbox_done) return PISURF_OK;
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    HIPCHK(c, c->part.ensure(6 * 1024 * sizeof(float) + 64));
+    HIPCHK(c, c->small.ensure(256));
+    HIPCHK(c, hipMemsetAsync(c->small.as<char>(), 0, 256, c->stream));
+    float* out6 = (float*)(c->small.as<char>() + 64);
+    int* nanc = c->small.as<int>();
+    launch_bbox(c->xyz.as<float>(), c->n, c->part.as<float>(), out6, nanc, c->stream);
+    HIPCHK(c, hipGetLastError());
+    int nans = 0;
+    HIPCHK(c, hipMemcpyAsync(c->bbox, out6, 6 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&nans, nanc, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (nans) return c->fail(PISURF_E_INVALID, "particle positions contain NaN");
+    for (int a = 0; a < 6; a++)
+        if (!std::isfinite(c->bbox[a])) return c->fail(PISURF_E_INVALID, "particle positions are not finite");
+    c->single_layer = (c->bbox[0] == c->bbox[1] || c->bbox[2] == c->bbox[3] || c->bbox[4] == c->bbox[5]);
+    c->bbox_done = true;
+    return PISURF_OK;
+}
+
+int do_grid(pisurf_ctx* c, const float* bbox, float inf) {
+    int rc = ensure_bbox(c);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    if (!(c->prm.radius > 0.0f) || !std::isfinite(c->prm.radius)) return c->fail(PISURF_E_INVALID, "radius must be > 0");
+    if (!(inf > 0.0f)) return c->fail(PISURF_E_INVALID, "inf_factor must be > 0");
+    float b[6];
+    uint32_t n3[3];
+    int64_t ncell;
+    float cell;
+    grid_geometry(bbox ? bbox : c->bbox, c->prm.radius, inf, b, n3, ncell, cell);
+    for (int a = 0; a < 6; a++)
+        if (!std::isfinite(b[a])) return c->fail(PISURF_E_INVALID, "grid bounding box is not finite");
+    if (ncell >= (int64_t)INT32_MAX - 1) return c->fail(PISURF_E_UNSUPPORTED, "cell count exceeds 2^31 - 2");
+    const int64_t n = c->n;
+    HIPCHK(c, c->keys.ensure(n * 4));
+    HIPCHK(c, c->vals.ensure(n * 4));
+    HIPCHK(c, c->keys2.ensure(n * 4));
+    HIPCHK(c, c->vals2.ensure(n * 4));
+    HIPCHK(c, c->cellcount.ensure((ncell + 1) * 4));
+    HIPCHK(c, c->off.ensure((ncell + 1) * 4));
+    HIPCHK(c, c->spos.ensure(n * sizeof(float4)));
+    const int64_t tmp_ints = std::max(radix_tmp_ints(n), scan_tmp_ints(ncell + 1)) + 64;
+    HIPCHK(c, c->tmp.ensure(tmp_ints * 4));
+    HIPCHK(c, c->small.ensure(256));
+    HIPCHK(c, hipMemsetAsync(c->cellcount.p, 0, (ncell + 1) * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->small.p, 0, 64, c->stream));
+    GridDev g;
+    for (int a = 0; a < 3; a++) {
+        g.b0[a] = b[2 * a];
+        g.n[a] = (int)n3[a];
+    }
+    g.cell = cell;
+    g.ncell = (int)ncell;
+    g.off = c->off.as<int>();
+    int* ninv_d = c->small.as<int>();
+    launch_keys(c->xyz.as<float>(), n, g, c->keys.as<uint32_t>(), c->vals.as<int>(), c->cellcount.as<int>(), ninv_d,
+                c->stream);
+    int bits = 1;
+    while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)ncell) bits++;
+    radix_sort_pairs(c->keys.as<uint32_t>(), c->vals.as<int>(), c->keys2.as<uint32_t>(), c->vals2.as<int>(), n, bits,
+                     c->tmp.as<int>(), c->stream);
+    exclusive_scan_i32(c->cellcount.as<int>(), c->off.as<int>(), ncell + 1, c->tmp.as<int>(), c->stream);
+    launch_gather_pos(c->xyz.as<float>(), c->vals.as<int>(), n, c->spos.as<float4>(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    int ninv = 0;
+    HIPCHK(c, hipMemcpyAsync(&ninv, ninv_d, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->g = g;
+    for (int a = 0; a < 6; a++) c->ginfo.bounding[a] = b[a];
+    for (int a = 0; a < 3; a++) c->ginfo.xyz_cell_num[a] = n3[a];
+    c->ginfo.cell_num = ncell;
+    c->ginfo.cell_size = cell;
+    c->ginfo.n_invalid = ninv;
+    c->grid_done = true;
+    c->aniso_done = c->normals_done = c->tree_done = false;
+    c->st.ms_grid += now_ms() - t0;
+    return PISURF_OK;
+}
+
+int do_aniso(pisurf_ctx* c) {
+    if (!c->grid_done) {
+        int rc = do_grid(c, nullptr, 4.0f);
+        if (rc) return rc;
+    }
+    const double t0 = now_ms();
+    const int prec = prec_of(c);
+    const size_t rb = (prec == PISURF_FP64) ? sizeof(double4) : sizeof(float4);
+    HIPCHK(c, c->xm.ensure(c->n * rb));
+    HIPCHK(c, c->rec_a.ensure(c->n * rb));
+    HIPCHK(c, c->rec_g.ensure(2 * c->n * rb));
+    HIPCHK(c, c->small.ensure(256));
+    HIPCHK(c, hipMemsetAsync(c->small.as<char>() + 128, 0, 8, c->stream));
+    AnisoParams ap;
+    ap.radius = c->prm.radius;
+    ap.nI = c->prm.radius * 4.0f;
+    ap.nI2 = ap.nI * ap.nI;
+    const double h = double(c->prm.radius * 2.0f);
+    ap.h2 = (prec == PISURF_FP64) ? std::pow(h, 2) : double(float(std::pow(h, 2)));
+    ap.xmean_delta = 0.9f;
+    ap.use_poly6 = c->prm.use_poly6;
+    ap.use_ani = c->prm.use_ani;
+    launch_xmean(prec, c->g, c->spos.as<float4>(), c->n, ap, c->xm.p, c->stream);
+    unsigned long long* nfp = (unsigned long long*)(c->small.as<char>() + 128);
+    launch_gmat(prec, c->g, c->spos.as<float4>(), c->n, ap, c->xm.p, c->rec_a.p, c->rec_g.p, nfp, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&c->n_fp64_eig, nfp, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->prec_rec = prec;
+    c->aniso_done = true;
+    c->normals_done = c->tree_done = false;
+    c->st.ms_aniso += now_ms() - t0;
+    return PISURF_OK;
+}
+
+int need_aniso(pisurf_ctx* c) {
+    if (!c->aniso_done || c->prec_rec != prec_of(c)) return do_aniso(c);
+    return PISURF_OK;
+}
+
+int do_normals(pisurf_ctx* c) {
+    int rc = need_aniso(c);
+    if (rc) return rc;
+    if (!c->consts_done) compute_constants(c);
+    const double t0 = now_ms();
+    HIPCHK(c, c->nrm.ensure(c->n * sizeof(float4)));
+    launch_normals(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->spos.as<float4>(), c->n,
+                   c->nrm.as<float4>(), c->ctr(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->normals_done = true;
+    c->st.ms_normals += now_ms() - t0;
+    return PISURF_OK;
+}
+
+// resizeRootBoxConstR (src/surface_reconstructor.cpp:97-122)
+void resize_root(pisurf_ctx* c) {
+    const float r = c->prm.radius;
+    const float* B = c->bbox;
+    const float maxLen = std::max({B[1] - B[0], B[3] - B[2], B[5] - B[4]});
+    int D = int(std::ceil(std::log2(std::ceil(double(maxLen / r)))));
+    float resizeLen = float(std::pow(2.0, D) * double(r));
+    while (resizeLen - maxLen < (4.0f * r * 2)) {
+        D++;
+        resizeLen = float(std::pow(2.0, D) * double(r));
+    }
+    if (c->prm.depth_max_override > 0) {
+        D = c->prm.depth_max_override;
+        resizeLen = float(std::pow(2.0, D) * double(r));
+    }
+    c->dmax = D;
+    c->root_half = resizeLen / 2;
+    for (int i = 0; i < 3; i++) c->root_c[i] = (B[2 * i] + B[2 * i + 1]) / 2;
+    c->dmin = c->dmax - (c->single_layer ? 1 : 3);
+}
+
+TreeParams tree_params(const pisurf_ctx* c) {
+    TreeParams tp;
+    tp.radius = c->prm.radius;
+    tp.margin = c->prm.radius * c->prm.smooth_factor;
+    for (int a = 0; a < 3; a++) {
+        tp.b_lo[a] = c->ginfo.bounding[2 * a];
+        tp.b_hi[a] = c->ginfo.bounding[2 * a + 1];
+    }
+    tp.depth_min = c->dmin;
+    tp.depth_max = c->dmax;
+    tp.calc_normals = c->prm.calc_normals;
+    tp.qef_mode = c->prm.qef_mode;
+    tp.qef_svd_tol = c->prm.qef_svd_tol;
+    return tp;
+}
+
+int frame_ready(pisurf_ctx* c) {
+    int rc = need_aniso(c);
+    if (rc) return rc;
+    if (!c->consts_done) compute_constants(c);
+    if (c->prm.calc_normals && !c->normals_done) {
+        rc = do_normals(c);
+        if (rc) return rc;
+    }
+    if (!c->normals_done) {  // curvature reads normals; without CALC_P_NORMAL they stay zero
+        HIPCHK(c, c->nrm.ensure(c->n * sizeof(float4)));
+        HIPCHK(c, hipMemsetAsync(c->nrm.p, 0, c->n * sizeof(float4), c->stream));
+    }
+    return PISURF_OK;
+}
+
+// run the three level kernels on `lv` (nodes already in lv.c / depth / id)
+int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev) {
+    const int prec = prec_of(c);
+    const size_t rsz = (prec == PISURF_FP64) ? 8 : 4;
+    HIPCHK(c, c->samples.ensure(L->n * 27 * rsz));
+    const TreeParams tp = tree_params(c);
+    const FieldConst fc = field_const(c);
+    const NodeLevel v = L->view();
+    launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, c->stream);
+    launch_samples(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, c->ctr(), c->stream);
+    launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, sc_dev, c->ctr(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    return PISURF_OK;
+}
+
+int do_tree(pisurf_ctx* c) {
+    int rc = frame_ready(c);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    resize_root(c);
+    for (auto* l : c->levels) l->n = 0;
+    size_t li = 0;
+    auto level = [&](size_t i) {
+        while (c->levels.size() <= i) c->levels.push_back(new LevelBuf());
+        return c->levels[i];
+    };
+    LevelBuf* L = level(0);
+    L->n = 1;
+    HIPCHK(c, L->ensure(1));
+    {
+        const float4 rc4 = make_float4(c->root_c[0], c->root_c[1], c->root_c[2], c->root_half);
+        const int8_t d0 = 0;
+        const uint64_t id0 = 0;
+        HIPCHK(c, hipMemcpyAsync(L->c.p, &rc4, sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(L->depth.p, &d0, 1, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(L->id.p, &id0, 8, hipMemcpyHostToDevice, c->stream));
+    }
+    while (true) {
+        L = level(li);
+        rc = sweep(c, L, nullptr);
+        if (rc) return rc;
+        HIPCHK(c, hipMemsetAsync(L->flag.as<int>() + L->n, 0, 4, c->stream));
+        exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), L->n + 1, c->tmp.as<int>(), c->stream);
+        int n_int = 0;
+        HIPCHK(c, hipMemcpyAsync(&n_int, L->scanned.as<int>() + L->n, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (n_int == 0) break;
+        if ((int64_t)n_int * 8 > (int64_t)INT32_MAX / 2) return c->fail(PISURF_E_UNSUPPORTED, "octree level too large");
+        LevelBuf* N = level(li + 1);
+        N->n = (int64_t)n_int * 8;
+        HIPCHK(c, N->ensure(N->n));
+        const int64_t need_tmp = scan_tmp_ints(N->n + 1) + 64;
+        if ((size_t)need_tmp * 4 > c->tmp.cap) {
+            // grow the scan workspace without losing the grid (tmp holds no persistent data)
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, c->tmp.ensure(need_tmp * 4));
+        }
+        launch_children(L->view(), L->scanned.as<int>(), N->view(), c->stream);
+        HIPCHK(c, hipGetLastError());
+        li++;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->tree_done = true;
+    c->st.ms_tree += now_ms() - t0;
+    return PISURF_OK;
+}
+
+struct HostTree {
+    std::vector<uint8_t> type;
+    std::vector<double4> node;
+    std::vector<uint64_t> id;
+    std::vector<int32_t> child;  // global index of the first child, -1
+    std::vector<float4> cen;
+    std::vector<int8_t> depth;
+    std::vector<int64_t> level_off;
+};
+
+int download_tree(pisurf_ctx* c, HostTree& t, bool geometry) {
+    int64_t tot = 0;
+    t.level_off.clear();
+    for (auto* l : c->levels) {
+        if (l->n == 0) break;
+        t.level_off.push_back(tot);
+        tot += l->n;
+    }
+    t.level_off.push_back(tot);
+    t.type.resize(tot);
+    t.node.resize(tot);
+    t.id.resize(tot);
+    t.child.resize(tot);
+    if (geometry) {
+        t.cen.resize(tot);
+        t.depth.resize(tot);
+    }
+    for (size_t li = 0; li + 1 < t.level_off.size(); li++) {
+        LevelBuf* l = c->levels[li];
+        const int64_t o = t.level_off[li];
+        HIPCHK(c, hipMemcpyAsync(&t.type[o], l->type.p, l->n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&t.node[o], l->node.p, l->n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&t.id[o], l->id.p, l->n * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&t.child[o], l->child.p, l->n * 4, hipMemcpyDeviceToHost, c->stream));
+        if (geometry) {
+            HIPCHK(c, hipMemcpyAsync(&t.cen[o], l->c.p, l->n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(&t.depth[o], l->depth.p, l->n, hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t li = 0; li + 1 < t.level_off.size(); li++) {
+        const int64_t next = t.level_off[li + 1];
+        for (int64_t i = t.level_off[li]; i < next; i++)
+            if (t.child[i] >= 0) t.child[i] = (int32_t)(next + t.child[i]);
+    }
+    return PISURF_OK;
+}
+
+// Dual-cell enumeration of the octree (include/traverse.h:12-312 with the VisitorExtract
+// callbacks, src/visitorextract.cpp:229-452). The recursion order fixes the order of the dual
+// cells and therefore the mesh numbering.
+struct Extractor {
+    const HostTree& t;
+    struct Dual {
+        int32_t n[8];
+        uint8_t sidx;
+    };
+    std::vector<Dual> duals;
+    explicit Extractor(const HostTree& tr) : t(tr) {}
+    bool leaf(int n) const { return t.type[n] == 2 || t.type[n] == 0; }
+    int gen(int n, int i) const { return leaf(n) ? n : t.child[n] + i; }
+    static int IX(int x, int y, int z) { return x | (y << 1) | (z << 2); }
+    void vert(const int* n) {
+        bool all = true;
+        for (int i = 0; i < 8; i++) all = all && leaf(n[i]);
+        if (all) {
+            int index = 0;
+            for (int i = 0; i < 8; i++)
+                if (t.node[n[i]].w > 0) index |= 1 << i;
+            const uint8_t code = kDmcCase[index];
+            if (code != 0 && code != 255) {
+                Dual d;
+                const uint32_t perm = kDmcPerm[index];
+                for (int i = 0; i < 8; i++) d.n[i] = n[(perm >> (4 * i)) & 15];
+                d.sidx = (uint8_t)index;
+                duals.push_back(d);
+            }
+            return;
+        }
+        int c[8];
+        for (int k = 0; k < 8; k++) c[k] = gen(n[k], k ^ 7);
+        vert(c);
+    }
+    void edge(int axis, int n00, int n10, int n01, int n11) {
+        if (leaf(n00) && leaf(n10) && leaf(n01) && leaf(n11)) return;
+        int c[8];
+        // slot (i, a, b) along `axis`, the other two axes (a, b) in x, y, z order
+        auto slot = [&](int i, int a, int b) {
+            if (axis == 0) return IX(i, a, b);
+            if (axis == 1) return IX(a, i, b);
+            return IX(a, b, i);
+        };
+        for (int i = 0; i < 2; i++) {
+            c[slot(i, 0, 0)] = gen(n00, slot(i, 1, 1));
+            c[slot(i, 1, 0)] = gen(n10, slot(i, 0, 1));
+            c[slot(i, 0, 1)] = gen(n01, slot(i, 1, 0));
+            c[slot(i, 1, 1)] = gen(n11, slot(i, 0, 0));
+        }
+        for (int i = 0; i < 2; i++) edge(axis, c[slot(i, 0, 0)], c[slot(i, 1, 0)], c[slot(i, 0, 1)], c[slot(i, 1, 1)]);
+        vert(c);
+    }
+    void face(int axis, int n0, int n1) {
+        if (leaf(n0) && leaf(n1)) return;
+        int c[8];
+        auto slot = [&](int s, int a, int b) {
+            if (axis == 0) return IX(s, a, b);
+            if (axis == 1) return IX(a, s, b);
+            return IX(a, b, s);
+        };
+        for (int i = 0; i < 4; i++) {
+            const int a = i & 1, b = (i >> 1) & 1;
+            c[slot(0, a, b)] = gen(n0, slot(1, a, b));
+            c[slot(1, a, b)] = gen(n1, slot(0, a, b));
+        }
+        for (int i = 0; i < 4; i++) {
+            const int a = i & 1, b = (i >> 1) & 1;
+            face(axis, c[slot(0, a, b)], c[slot(1, a, b)]);
+        }
+        // the two edge directions perpendicular to `axis`, in x, y, z order
+        for (int i = 0; i < 2; i++) {
+            for (int e = 0; e < 3; e++) {
+                if (e == axis) continue;
+                if (e == 0) edge(0, c[IX(i, 0, 0)], c[IX(i, 1, 0)], c[IX(i, 0, 1)], c[IX(i, 1, 1)]);
+                else if (e == 1) edge(1, c[IX(0, i, 0)], c[IX(1, i, 0)], c[IX(0, i, 1)], c[IX(1, i, 1)]);
+                else edge(2, c[IX(0, 0, i)], c[IX(1, 0, i)], c[IX(0, 1, i)], c[IX(1, 1, i)]);
+            }
+        }
+        vert(c);
+    }
+    void node(int n) {
+        if (leaf(n)) return;
+        int c[8];
+        for (int i = 0; i < 8; i++) {
+            c[i] = gen(n, i);
+            node(c[i]);
+        }
+        for (int i = 0; i < 4; i++) {
+            const int a = i & 1, b = (i >> 1) & 1;
+            face(0, c[IX(0, a, b)], c[IX(1, a, b)]);
+            face(1, c[IX(a, 0, b)], c[IX(a, 1, b)]);
+            face(2, c[IX(a, b, 0)], c[IX(a, b, 1)]);
+        }
+        for (int i = 0; i < 2; i++) {
+            edge(0, c[IX(i, 0, 0)], c[IX(i, 1, 0)], c[IX(i, 0, 1)], c[IX(i, 1, 1)]);
+            edge(1, c[IX(0, i, 0)], c[IX(1, i, 0)], c[IX(0, i, 1)], c[IX(1, i, 1)]);
+            edge(2, c[IX(0, 0, i)], c[IX(1, 0, i)], c[IX(0, 1, i)], c[IX(1, 1, i)]);
+        }
+        vert(c);
+    }
+};
+
+int do_extract(pisurf_ctx* c) {
+    if (!c->tree_done) return c->fail(PISURF_E_STATE, "octree not built");
+    double t0 = now_ms();
+    HostTree t;
+    int rc = download_tree(c, t, false);
+    if (rc) return rc;
+    Extractor ex(t);
+    if (!t.type.empty()) ex.node(0);
+    // mesh topology (VisitorExtract::generate_mesh, src/visitorextract.cpp:203-227; Mesh::insert_vert /
+    // insert_tri, src/global.cpp:13-53): a vertex is keyed by the unordered pair of leaf ids of its
+    // edge, the first dual cell to use it fixes the edge orientation; triangles are keyed by their
+    // sorted vertex triple.
+    std::vector<int32_t> vend;  // ordered node pair per vertex
+    std::vector<int32_t>& tris = c->mesh_t;
+    tris.clear();
+    KeyMap vmap, tmap;
+    vmap.reset(ex.duals.size() * 2 + 16);
+    tmap.reset(ex.duals.size() * 3 + 16);
+    int canon_slot[256];
+    for (int i = 0; i < 256; i++) canon_slot[i] = -1;
+    for (int k = 0; k < PISURF_DMC_NCASES; k++) canon_slot[kDmcCanon[k]] = k;
+    for (const auto& d : ex.duals) {
+        const int slot = canon_slot[kDmcCase[d.sidx]];
+        if (slot < 0) continue;
+        for (int tt = 0; tt < kDmcNTri[slot]; tt++) {
+            int vid[3];
+            for (int k = 0; k < 3; k++) {
+                const int e = kDmcTri[slot][tt][k];
+                const int na = d.n[kDmcEdge[e][0]], nb = d.n[kDmcEdge[e][1]];
+                uint64_t ia = t.id[na], ib = t.id[nb];
+                if (ia > ib) std::swap(ia, ib);
+                auto r = vmap.try_emplace(ia, ib, (int32_t)(vend.size() / 2));
+                if (r.second) {
+                    vend.push_back(na);
+                    vend.push_back(nb);
+                }
+                vid[k] = r.first + 1;
+            }
+            int t0v, t1v, t2v;
+            if (kDmcMirror[d.sidx]) { t0v = vid[0]; t1v = vid[1]; t2v = vid[2]; }
+            else { t0v = vid[2]; t1v = vid[1]; t2v = vid[0]; }
+            int s[3] = {t0v, t1v, t2v};
+            std::sort(s, s + 3);
+            const uint64_t ka = ((uint64_t)(uint32_t)s[0] << 32) | (uint32_t)s[1];
+            auto r = tmap.try_emplace(ka, (uint64_t)(uint32_t)s[2], (int32_t)(tris.size() / 3));
+            if (r.second) {
+                tris.push_back(t0v);
+                tris.push_back(t1v);
+                tris.push_back(t2v);
+            }
+        }
+    }
+    const int64_t nv = (int64_t)vend.size() / 2;
+    std::vector<double> ends(nv * 8);
+    for (int64_t i = 0; i < nv; i++) {
+        const double4 a = t.node[vend[2 * i]], b = t.node[vend[2 * i + 1]];
+        double* e = &ends[8 * i];
+        e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+        e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
+    }
+    c->st.n_dual_cells = (int64_t)ex.duals.size();
+    c->st.ms_extract_host += now_ms() - t0;
+    t0 = now_ms();
+    c->mesh_v.assign(nv * 3, 0.0);
+    if (nv > 0) {
+        HIPCHK(c, c->q.ensure(nv * 8 * sizeof(double)));
+        HIPCHK(c, c->out1.ensure(nv * 3 * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(c->q.p, ends.data(), nv * 8 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        launch_bisect(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->prm.radius / 2, c->q.as<double>(), nv,
+                      c->out1.as<double>(), nullptr, c->ctr(), c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(c->mesh_v.data(), c->out1.p, nv * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->st.ms_bisect += now_ms() - t0;
+    c->st.n_verts = nv;
+    c->st.n_tris = (int64_t)tris.size() / 3;
+    return PISURF_OK;
+}
+
+int sync_counters(pisurf_ctx* c) {
+    Counters h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->st.field_evals = (int64_t)h.field_evals;
+    c->st.grad_evals = (int64_t)h.grad_evals;
+    c->st.candidates = (int64_t)h.candidates;
+    c->st.qef_solves = (int64_t)h.qef_solves;
+    return PISURF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pisurf_default_params(pisurf_params* p) {
+    if (!p) return PISURF_E_INVALID;
+    std::memset(p, 0, sizeof(*p));
+    p->radius = 0.0f;
+    p->smooth_factor = 2.0f;
+    p->iso_factor = 1.9f;
+    p->use_ani = 1;
+    p->use_poly6 = 0;
+    p->calc_normals = 1;
+    p->oversample = 2;
+    p->precision = PISURF_FP32;
+    p->qef_mode = PISURF_QEF_OFF;
+    p->qef_svd_tol = 1e-3f;
+    p->depth_max_override = 0;
+    return PISURF_OK;
+}
+
+int pisurf_create(pisurf_ctx** out, int device) {
+    if (!out) return PISURF_E_INVALID;
+    *out = nullptr;
+    pisurf_ctx* c = new (std::nothrow) pisurf_ctx();
+    if (!c) return PISURF_E_OOM;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        delete c;
+        return PISURF_E_HIP;
+    }
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PISURF_E_HIP;
+    }
+    pisurf_default_params(&c->prm);
+    std::memset(&c->st, 0, sizeof(c->st));
+    if (c->counters.ensure(sizeof(Counters)) != hipSuccess ||
+        hipMemset(c->counters.p, 0, sizeof(Counters)) != hipSuccess) {
+        delete c;
+        return PISURF_E_HIP;
+    }
+    *out = c;
+    return PISURF_OK;
+}
+
+int pisurf_destroy(pisurf_ctx* c) {
+    if (!c) return PISURF_E_INVALID;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete c;
+    return PISURF_OK;
+}
+
+const char* pisurf_last_error(const pisurf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* pisurf_stream(pisurf_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int pisurf_set_params(pisurf_ctx* c, const pisurf_params* p) {
+    if (!c || !p) return PISURF_E_INVALID;
+    if (!(p->radius > 0.0f) || !std::isfinite(p->radius)) return c->fail(PISURF_E_INVALID, "radius must be > 0");
+    if (p->oversample != 2) return c->fail(PISURF_E_UNSUPPORTED, "only oversample = 2 (27 samples) is supported");
+    if (p->precision < 0 || p->precision > 2) return c->fail(PISURF_E_INVALID, "bad precision");
+    if (p->qef_mode < 0 || p->qef_mode > 2) return c->fail(PISURF_E_INVALID, "bad qef_mode");
+    if (p->depth_max_override < 0 || p->depth_max_override > 24) return c->fail(PISURF_E_INVALID, "bad depth override");
+    const bool geo = p->radius != c->prm.radius || p->use_ani != c->prm.use_ani || p->use_poly6 != c->prm.use_poly6 ||
+                     p->precision != c->prm.precision;
+    c->prm = *p;
+    if (geo) {
+        c->grid_done = false;
+        c->aniso_done = c->normals_done = c->tree_done = c->consts_done = false;
+    }
+    c->tree_done = false;
+    c->consts_done = false;
+    return PISURF_OK;
+}
+
+int pisurf_upload_particles(pisurf_ctx* c, const float* xyz, int64_t n, int32_t on_device) {
+    if (!c) return PISURF_E_INVALID;
+    if (n < 0 || (n > 0 && !xyz)) return c->fail(PISURF_E_INVALID, "bad particle buffer");
+    if (n >= (int64_t)INT32_MAX / 2) return c->fail(PISURF_E_UNSUPPORTED, "particle count exceeds 2^30");
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, c->xyz.ensure(std::max<int64_t>(n, 1) * 3 * sizeof(float)));
+    if (n > 0)
+        HIPCHK(c, hipMemcpyAsync(c->xyz.p, xyz, n * 3 * sizeof(float),
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n = n;
+    c->bbox_done = c->grid_done = c->aniso_done = c->normals_done = c->tree_done = c->consts_done = false;
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_hashgrid_build(pisurf_ctx* c, const float* bbox, float inf, pisurf_grid_info* out) {
+    if (!c) return PISURF_E_INVALID;
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    int rc = do_grid(c, bbox, inf);
+    if (rc) return rc;
+    if (out) *out = c->ginfo;
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_hashgrid_download(pisurf_ctx* c, int32_t* index_list, int32_t* cs, int32_t* ce) {
+    if (!c) return PISURF_E_INVALID;
+    if (!c->grid_done) return c->fail(PISURF_E_STATE, "hash grid not built");
+    API_BEGIN
+    if (index_list)
+        HIPCHK(c, hipMemcpyAsync(index_list, c->vals.p, c->n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (cs || ce) {
+        const int64_t nc = c->ginfo.cell_num;
+        HIPCHK(c, c->out1.ensure(nc * 4));
+        HIPCHK(c, c->out2.ensure(nc * 4));
+        launch_cell_window(c->g, c->out1.as<int>(), c->out2.as<int>(), c->stream);
+        HIPCHK(c, hipGetLastError());
+        if (cs) HIPCHK(c, hipMemcpyAsync(cs, c->out1.p, nc * 4, hipMemcpyDeviceToHost, c->stream));
+        if (ce) HIPCHK(c, hipMemcpyAsync(ce, c->out2.p, nc * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_neighbors(pisurf_ctx* c, const float* q, int64_t nq, int64_t* offsets, int32_t* ids, int64_t cap,
+                     int64_t* total) {
+    if (!c) return PISURF_E_INVALID;
+    if (!c->grid_done) return c->fail(PISURF_E_STATE, "hash grid not built");
+    if (nq < 0 || (nq > 0 && !q)) return c->fail(PISURF_E_INVALID, "bad query buffer");
+    API_BEGIN
+    HIPCHK(c, c->q.ensure(std::max<int64_t>(nq, 1) * 3 * 4));
+    HIPCHK(c, c->out1.ensure((nq + 1) * 4));
+    HIPCHK(c, c->out2.ensure((nq + 1) * 4));
+    HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, (scan_tmp_ints(nq + 1) + 64) * 4)));
+    if (nq > 0) HIPCHK(c, hipMemcpyAsync(c->q.p, q, nq * 12, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->out1.as<int>() + nq, 0, 4, c->stream));
+    launch_neighbor_count(c->g, c->q.as<float>(), nq, c->out1.as<int>(), c->stream);
+    exclusive_scan_i32(c->out1.as<int>(), c->out2.as<int>(), nq + 1, c->tmp.as<int>(), c->stream);
+    std::vector<int> off32(nq + 1);
+    HIPCHK(c, hipMemcpyAsync(off32.data(), c->out2.p, (nq + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t tot = off32[nq];
+    if (total) *total = tot;
+    if (offsets)
+        for (int64_t i = 0; i <= nq; i++) offsets[i] = off32[i];
+    if (ids && tot > 0) {
+        HIPCHK(c, c->out3.ensure(tot * 4));
+        launch_neighbor_write(c->g, c->q.as<float>(), nq, c->out2.as<int>(), c->vals.as<int>(), c->out3.as<int>(),
+                              c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(ids, c->out3.p, std::min(tot, cap) * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_box_particles(pisurf_ctx* c, const float* box1, const float* box2, int32_t* ids, int64_t cap,
+                         int64_t* total) {
+    if (!c || !box1 || !box2) return PISURF_E_INVALID;
+    if (!c->grid_done) return c->fail(PISURF_E_STATE, "hash grid not built");
+    API_BEGIN
+    int lo[3], n3[3];
+    int64_t ncells = 1;
+    for (int a = 0; a < 3; a++) {
+        const float b1 = std::max(box1[a], c->ginfo.bounding[2 * a]);
+        const float b2 = std::min(box2[a], c->ginfo.bounding[2 * a + 1]);
+        const int l = host_cell_coord(b1, c->g.b0[a], c->g.cell, c->g.n[a]);
+        const int h = host_cell_coord(b2, c->g.b0[a], c->g.cell, c->g.n[a]);
+        lo[a] = l - 1;
+        n3[a] = std::max(0, (h + 1) - (l - 1) + 1);
+        ncells *= n3[a];
+    }
+    int geo[6] = {lo[0], lo[1], lo[2], n3[0], n3[1], n3[2]};
+    HIPCHK(c, c->q.ensure(64));
+    HIPCHK(c, hipMemcpyAsync(c->q.p, geo, sizeof(geo), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, c->out1.ensure((ncells + 1) * 4));
+    HIPCHK(c, c->out2.ensure((ncells + 1) * 4));
+    HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, (scan_tmp_ints(ncells + 1) + 64) * 4)));
+    HIPCHK(c, hipMemsetAsync(c->out1.as<int>() + ncells, 0, 4, c->stream));
+    launch_box_count(c->g, c->q.as<int>(), c->q.as<int>() + 3, ncells, c->out1.as<int>(), c->stream);
+    exclusive_scan_i32(c->out1.as<int>(), c->out2.as<int>(), ncells + 1, c->tmp.as<int>(), c->stream);
+    int tot = 0;
+    HIPCHK(c, hipMemcpyAsync(&tot, c->out2.as<int>() + ncells, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (total) *total = tot;
+    if (ids && tot > 0) {
+        HIPCHK(c, c->out3.ensure((int64_t)tot * 4));
+        launch_box_write(c->g, c->q.as<int>(), c->q.as<int>() + 3, ncells, c->out2.as<int>(), c->vals.as<int>(),
+                         c->out3.as<int>(), c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(ids, c->out3.p, std::min<int64_t>(tot, cap) * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_anisotropy(pisurf_ctx* c) {
+    if (!c) return PISURF_E_INVALID;
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    return do_aniso(c);
+    API_END(c)
+}
+
+int pisurf_download_aniso(pisurf_ctx* c, double* xmean, double* G9, double* det) {
+    if (!c) return PISURF_E_INVALID;
+    if (!c->aniso_done) return c->fail(PISURF_E_STATE, "anisotropy not computed");
+    API_BEGIN
+    const int64_t n = c->n;
+    const bool f64 = c->prec_rec == PISURF_FP64;
+    const size_t rb = f64 ? sizeof(double4) : sizeof(float4);
+    std::vector<char> a(n * rb), g(2 * n * rb);
+    std::vector<int> sidx(n);
+    HIPCHK(c, hipMemcpyAsync(a.data(), c->rec_a.p, n * rb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g.data(), c->rec_g.p, 2 * n * rb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sidx.data(), c->vals.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto rd = [&](const std::vector<char>& b, int64_t k) {
+        return f64 ? ((const double*)b.data())[k] : double(((const float*)b.data())[k]);
+    };
+    for (int64_t p = 0; p < n; p++) {
+        const int64_t i = sidx[p];
+        if (xmean)
+            for (int k = 0; k < 3; k++) xmean[3 * i + k] = rd(a, 4 * p + k);
+        const double G6[6] = {rd(g, 8 * p), rd(g, 8 * p + 1), rd(g, 8 * p + 2), rd(g, 8 * p + 3), rd(g, 8 * p + 4),
+                              rd(g, 8 * p + 5)};
+        if (G9) {
+            // column-major 3x3 from the symmetric (00, 01, 02, 11, 12, 22)
+            const double m[9] = {G6[0], G6[1], G6[2], G6[1], G6[3], G6[4], G6[2], G6[4], G6[5]};
+            for (int k = 0; k < 9; k++) G9[9 * i + k] = m[k];
+        }
+        if (det) det[i] = rd(g, 8 * p + 6);
+    }
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_iso_constants(pisurf_ctx* c, double* ms, double* iso) {
+    if (!c) return PISURF_E_INVALID;
+    if (!(c->prm.radius > 0.0f)) return c->fail(PISURF_E_STATE, "parameters not set");
+    compute_constants(c);
+    if (ms) *ms = c->max_scalar;
+    if (iso) *iso = c->iso;
+    return PISURF_OK;
+}
+
+int pisurf_particle_normals(pisurf_ctx* c, double* normals) {
+    if (!c) return PISURF_E_INVALID;
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    int rc = do_normals(c);
+    if (rc) return rc;
+    if (normals) {
+        std::vector<float4> h(c->n);
+        std::vector<int> sidx(c->n);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->nrm.p, c->n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(sidx.data(), c->vals.p, c->n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t p = 0; p < c->n; p++) {
+            normals[3 * sidx[p]] = h[p].x;
+            normals[3 * sidx[p] + 1] = h[p].y;
+            normals[3 * sidx[p] + 2] = h[p].z;
+        }
+    }
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_eval_points(pisurf_ctx* c, const float* q, int64_t n, double* f, double* grad) {
+    if (!c) return PISURF_E_INVALID;
+    if (n < 0 || (n > 0 && (!q || !f))) return c->fail(PISURF_E_INVALID, "bad buffers");
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    int rc = need_aniso(c);
+    if (rc) return rc;
+    if (!c->consts_done) compute_constants(c);
+    if (n == 0) return PISURF_OK;
+    HIPCHK(c, c->q.ensure(n * 12));
+    HIPCHK(c, c->out1.ensure(n * 8));
+    if (grad) HIPCHK(c, c->out2.ensure(n * 24));
+    HIPCHK(c, hipMemcpyAsync(c->q.p, q, n * 12, hipMemcpyHostToDevice, c->stream));
+    launch_eval_points(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->q.as<float>(), n,
+                       c->out1.as<double>(), grad ? c->out2.as<double>() : nullptr, c->ctr(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(f, c->out1.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->out2.p, n * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_sweep_level(pisurf_ctx* c, const float* centers, const float* halfs, const int32_t* depths, int64_t n,
+                       uint8_t* type, double* f, float* curv, int32_t* signchange, double* samples, double* grads) {
+    if (!c) return PISURF_E_INVALID;
+    if (n < 0 || (n > 0 && (!centers || !halfs || !depths))) return c->fail(PISURF_E_INVALID, "bad node buffers");
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    int rc = frame_ready(c);
+    if (rc) return rc;
+    resize_root(c);
+    if (n == 0) return PISURF_OK;
+    LevelBuf L;
+    L.n = n;
+    HIPCHK(c, L.ensure(n));
+    std::vector<float4> c4(n);
+    std::vector<int8_t> d8(n);
+    std::vector<uint64_t> ids(n, 0);
+    for (int64_t i = 0; i < n; i++) {
+        c4[i] = make_float4(centers[3 * i], centers[3 * i + 1], centers[3 * i + 2], halfs[i]);
+        d8[i] = (int8_t)depths[i];
+    }
+    HIPCHK(c, hipMemcpyAsync(L.c.p, c4.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(L.depth.p, d8.data(), n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(L.id.p, ids.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, c->out3.ensure(n * 4));
+    rc = sweep(c, &L, c->out3.as<int32_t>());
+    if (rc) return rc;
+    if (samples || grads) {
+        HIPCHK(c, c->out1.ensure(n * 27 * 4 * 8));
+        HIPCHK(c, c->out2.ensure(n * 27 * 3 * 8));
+        launch_sample_grads(prec_of(c), tree_params(c), L.view(), c->samples.p, c->out1.as<double>(),
+                            c->out2.as<double>(), c->stream);
+        HIPCHK(c, hipGetLastError());
+        if (samples) HIPCHK(c, hipMemcpyAsync(samples, c->out1.p, n * 27 * 4 * 8, hipMemcpyDeviceToHost, c->stream));
+        if (grads) HIPCHK(c, hipMemcpyAsync(grads, c->out2.p, n * 27 * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    std::vector<double4> nd(n);
+    if (type) HIPCHK(c, hipMemcpyAsync(type, L.type.p, n, hipMemcpyDeviceToHost, c->stream));
+    if (curv) HIPCHK(c, hipMemcpyAsync(curv, L.curv.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (signchange) HIPCHK(c, hipMemcpyAsync(signchange, c->out3.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(nd.data(), L.node.p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (f)
+        for (int64_t i = 0; i < n; i++) f[i] = nd[i].w;
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_build_tree(pisurf_ctx* c, pisurf_tree_info* out) {
+    if (!c) return PISURF_E_INVALID;
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    int rc = do_tree(c);
+    if (rc) return rc;
+    if (out) {
+        out->depth_max = c->dmax;
+        out->depth_min = c->dmin;
+        for (int a = 0; a < 3; a++) out->root_center[a] = c->root_c[a];
+        out->root_half = c->root_half;
+        int64_t tot = 0;
+        int nl = 0;
+        for (auto* l : c->levels) {
+            if (l->n == 0) break;
+            tot += l->n;
+            nl++;
+        }
+        out->n_nodes = tot;
+        out->n_levels = nl;
+        out->n_leaves = 0;
+    }
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_tree_download(pisurf_ctx* c, float* center3, float* half, int32_t* depth, uint8_t* type, uint64_t* id,
+                         double* node4, int32_t* child) {
+    if (!c) return PISURF_E_INVALID;
+    if (!c->tree_done) return c->fail(PISURF_E_STATE, "octree not built");
+    API_BEGIN
+    HostTree t;
+    int rc = download_tree(c, t, true);
+    if (rc) return rc;
+    const size_t n = t.type.size();
+    for (size_t i = 0; i < n; i++) {
+        if (center3) {
+            center3[3 * i] = t.cen[i].x;
+            center3[3 * i + 1] = t.cen[i].y;
+            center3[3 * i + 2] = t.cen[i].z;
+        }
+        if (half) half[i] = t.cen[i].w;
+        if (depth) depth[i] = t.depth[i];
+        if (type) type[i] = t.type[i];
+        if (id) id[i] = t.id[i];
+        if (node4) {
+            node4[4 * i] = t.node[i].x;
+            node4[4 * i + 1] = t.node[i].y;
+            node4[4 * i + 2] = t.node[i].z;
+            node4[4 * i + 3] = t.node[i].w;
+        }
+        if (child) child[i] = t.child[i];
+    }
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_qef_solve(pisurf_ctx* c, const float* centers, const float* halfs, const double* samples,
+                     const double* grads, int64_t n, int32_t mode, double* node4) {
+    if (!c) return PISURF_E_INVALID;
+    if (mode != 1 && mode != 2) return c->fail(PISURF_E_INVALID, "mode must be 1 or 2");
+    if (n < 0 || (n > 0 && (!centers || !halfs || !samples || !grads || !node4)))
+        return c->fail(PISURF_E_INVALID, "bad buffers");
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    int rc = need_aniso(c);
+    if (rc) return rc;
+    if (!c->consts_done) compute_constants(c);
+    if (n == 0) return PISURF_OK;
+    std::vector<float4> c4(n);
+    for (int64_t i = 0; i < n; i++) c4[i] = make_float4(centers[3 * i], centers[3 * i + 1], centers[3 * i + 2], halfs[i]);
+    HIPCHK(c, c->q.ensure(n * sizeof(float4)));
+    HIPCHK(c, c->out1.ensure(n * 27 * 4 * 8));
+    HIPCHK(c, c->out2.ensure(n * 27 * 3 * 8));
+    HIPCHK(c, c->out3.ensure(n * sizeof(double4)));
+    HIPCHK(c, hipMemcpyAsync(c->q.p, c4.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->out1.p, samples, n * 27 * 4 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->out2.p, grads, n * 27 * 3 * 8, hipMemcpyHostToDevice, c->stream));
+    TreeParams tp = tree_params(c);
+    launch_qef(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), tp, c->q.as<float4>(), c->out1.as<double>(),
+               c->out2.as<double>(), n, mode, c->out3.as<double4>(), c->ctr(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(node4, c->out3.p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_bisect_edges(pisurf_ctx* c, const double* ends, int64_t n, double* out_xyz, int32_t* n_evals) {
+    if (!c) return PISURF_E_INVALID;
+    if (n < 0 || (n > 0 && (!ends || !out_xyz))) return c->fail(PISURF_E_INVALID, "bad buffers");
+    for (int64_t i = 0; i < 8 * n; i++)
+        if (!std::isfinite(ends[i])) return c->fail(PISURF_E_INVALID, "edge endpoints must be finite");
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    int rc = need_aniso(c);
+    if (rc) return rc;
+    if (!c->consts_done) compute_constants(c);
+    if (n == 0) return PISURF_OK;
+    HIPCHK(c, c->q.ensure(n * 64));
+    HIPCHK(c, c->out1.ensure(n * 24));
+    HIPCHK(c, c->out2.ensure(n * 4));
+    HIPCHK(c, hipMemcpyAsync(c->q.p, ends, n * 64, hipMemcpyHostToDevice, c->stream));
+    launch_bisect(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->prm.radius / 2, c->q.as<double>(), n,
+                  c->out1.as<double>(), c->out2.as<int>(), c->ctr(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out_xyz, c->out1.p, n * 24, hipMemcpyDeviceToHost, c->stream));
+    if (n_evals) HIPCHK(c, hipMemcpyAsync(n_evals, c->out2.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_extract(pisurf_ctx* c, int64_t* nv, int64_t* nt) {
+    if (!c) return PISURF_E_INVALID;
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    int rc = do_extract(c);
+    if (rc) return rc;
+    if (nv) *nv = (int64_t)c->mesh_v.size() / 3;
+    if (nt) *nt = (int64_t)c->mesh_t.size() / 3;
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_mesh_download(pisurf_ctx* c, double* verts, int32_t* tris) {
+    if (!c) return PISURF_E_INVALID;
+    if (verts) std::memcpy(verts, c->mesh_v.data(), c->mesh_v.size() * sizeof(double));
+    if (tris) std::memcpy(tris, c->mesh_t.data(), c->mesh_t.size() * sizeof(int32_t));
+    return PISURF_OK;
+}
+
+int pisurf_run_frame(pisurf_ctx* c, pisurf_tree_info* tree) {
+    if (!c) return PISURF_E_INVALID;
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    const double t0 = now_ms();
+    c->bbox_done = c->grid_done = c->aniso_done = c->normals_done = c->tree_done = c->consts_done = false;
+    int rc = do_grid(c, nullptr, 4.0f);
+    if (!rc) rc = do_aniso(c);
+    if (!rc) {
+        compute_constants(c);
+        rc = frame_ready(c);
+    }
+    if (!rc) rc = pisurf_build_tree(c, tree);
+    if (!rc) rc = do_extract(c);
+    if (rc) return rc;
+    c->st.ms_total += now_ms() - t0;
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_get_stats(pisurf_ctx* c, pisurf_stats* out) {
+    if (!c || !out) return PISURF_E_INVALID;
+    int rc = sync_counters(c);
+    if (rc) return rc;
+    *out = c->st;
+    return PISURF_OK;
+}
+
+int pisurf_reset_stats(pisurf_ctx* c) {
+    if (!c) return PISURF_E_INVALID;
+    std::memset(&c->st, 0, sizeof(c->st));
+    HIPCHK(c, hipMemsetAsync(c->counters.p, 0, sizeof(Counters), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PISURF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Field evaluation at arbitrary points (Evaluator::SingleEval / SingleEvalWithGrad,
+// src/evaluator.cpp:60-125) and particle normals (CalcParticlesNormal, :363-375).
+#include "pisurf_internal.h"
+
+namespace pisurf {
+
+// counter reduction: one 64-bit atomic per wave
+__device__ __forceinline__ void wave_count(unsigned long long* ctr, unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
+}
+
+template <class R, bool K, bool GRAD>
+__global__ void __launch_bounds__(256) k_eval_points(GridDev g, Recs<R> rec, FieldConst fc, const float* __restrict__ q,
+                                                     int64_t n, double* __restrict__ f, double* __restrict__ grad,
+                                                     Counters* ctr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vis = 0;
+    if (i < n) {
+        const float x = q[3 * i], y = q[3 * i + 1], z = q[3 * i + 2];
+        R s, gr[3];
+        vis = field_sum<R, K, GRAD>(g, rec, fc, x, y, z, R(x), R(y), R(z), s, gr);
+        const R sigma = R(fc.sigma);
+        f[i] = double(R(fc.iso) - sigma * s);
+        if (GRAD) {
+            grad[3 * i] = double(sigma * gr[0]);
+            grad[3 * i + 1] = double(sigma * gr[1]);
+            grad[3 * i + 2] = double(sigma * gr[2]);
+        }
+    }
+    wave_count(GRAD ? &ctr->grad_evals : &ctr->field_evals, i < n ? 1ull : 0ull);
+    wave_count(&ctr->candidates, (unsigned long long)vis);
+}
+
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_normals(GridDev g, Recs<R> rec, FieldConst fc, const float4* __restrict__ spos,
+                                                 int64_t n, float4* __restrict__ nrm, Counters* ctr) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vis = 0;
+    if (p < n) {
+        const float4 pp = spos[p];
+        R s, gr[3];
+        vis = field_sum<R, K, true>(g, rec, fc, pp.x, pp.y, pp.z, R(pp.x), R(pp.y), R(pp.z), s, gr);
+        const R sigma = R(fc.sigma);
+        nrm[p] = make_float4(float(sigma * gr[0]), float(sigma * gr[1]), float(sigma * gr[2]), 0.0f);
+    }
+    wave_count(&ctr->grad_evals, p < n ? 1ull : 0ull);
+    wave_count(&ctr->candidates, (unsigned long long)vis);
+}
+
+void launch_eval_points(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, const float* q, int64_t n,
+                        double* f, double* grad, Counters* ctr, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (prec == 2) {
+        Recs<double> r{(const double4*)ra, (const double4*)rg};
+        if (grad) k_eval_points<double, false, true><<<nb, 256, 0, s>>>(g, r, fc, q, n, f, grad, ctr);
+        else k_eval_points<double, false, false><<<nb, 256, 0, s>>>(g, r, fc, q, n, f, grad, ctr);
+    } else {
+        Recs<float> r{(const float4*)ra, (const float4*)rg};
+        if (prec == 1) {
+            if (grad) k_eval_points<float, true, true><<<nb, 256, 0, s>>>(g, r, fc, q, n, f, grad, ctr);
+            else k_eval_points<float, true, false><<<nb, 256, 0, s>>>(g, r, fc, q, n, f, grad, ctr);
+        } else {
+            if (grad) k_eval_points<float, false, true><<<nb, 256, 0, s>>>(g, r, fc, q, n, f, grad, ctr);
+            else k_eval_points<float, false, false><<<nb, 256, 0, s>>>(g, r, fc, q, n, f, grad, ctr);
+        }
+    }
+}
+
+void launch_normals(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, const float4* spos, int64_t n,
+                    float4* nrm, Counters* ctr, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (prec == 2) {
+        Recs<double> r{(const double4*)ra, (const double4*)rg};
+        k_normals<double, false><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
+    } else {
+        Recs<float> r{(const float4*)ra, (const float4*)rg};
+        if (prec == 1) k_normals<float, true><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
+        else k_normals<float, false><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
+    }
+}
+
+}  // namespace pisurf

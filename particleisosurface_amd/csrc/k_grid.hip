@@ -1,0 +1,369 @@
+// Hash grid on the device: bounding box, cell keys, stable LSD radix sort, CSR cell offsets,
+// and the neighbour / box queries of HashGrid (src/hash_grid.cpp).
+#include <cfloat>
+
+#include "pisurf_internal.h"
+
+namespace pisurf {
+
+// ---------------------------------------------------------------------------------------------
+// Exclusive scan (int32): tile of 2048 = 256 threads x 8 consecutive items, wave64 shuffles.
+// ---------------------------------------------------------------------------------------------
+constexpr int kScanT = 256, kScanI = 8, kScanTile = kScanT * kScanI;
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(kScanT) k_scan_tile(const int* __restrict__ in, int* __restrict__ out, int64_t n,
+                                                      int* __restrict__ sums) {
+    __shared__ int wsum[kScanT / kWave];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)t * kScanI;
+    int v[kScanI];
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanI; k++) {
+        const int64_t i = base + k;
+        v[k] = (i < n) ? in[i] : 0;
+        acc += v[k];
+    }
+    const int incl = wave_incl_scan(acc, lane);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int k = 0; k < w; k++) woff += wsum[k];
+    int run = woff + incl - acc;
+#pragma unroll
+    for (int k = 0; k < kScanI; k++) {
+        const int64_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (t == kScanT - 1) sums[blockIdx.x] = woff + incl;
+}
+
+__global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restrict__ add) {
+    const int64_t i = (int64_t)blockIdx.x * kScanTile + threadIdx.x;
+    const int a = add[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanI; k++) {
+        const int64_t j = i + (int64_t)k * kScanT;
+        if (j < n) out[j] += a;
+    }
+}
+
+int64_t scan_tmp_ints(int64_t n) {
+    int64_t tot = 0;
+    while (n > 1) {
+        n = (n + kScanTile - 1) / kScanTile;
+        tot += 2 * n + 2;
+    }
+    return tot + 4;
+}
+
+void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    int* sums = tmp;
+    int* sums_scanned = tmp + nb + 1;
+    k_scan_tile<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(in, out, n, sums);
+    if (nb > 1) {
+        exclusive_scan_i32(sums, sums_scanned, nb, tmp + 2 * nb + 2, s);
+        k_scan_add<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums_scanned);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits. Per tile (16 rounds of 256 keys): digit histogram, a
+// global digit-major exclusive scan of the per-tile histograms, then a stable scatter where the
+// rank of a key inside its round comes from wave64 ballots (lanes with equal digit) and a
+// per-digit prefix over the 4 waves.
+// ---------------------------------------------------------------------------------------------
+constexpr int kRsT = 256, kRsRounds = 16, kRsTile = kRsT * kRsRounds;
+
+__global__ void __launch_bounds__(kRsT) k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                  int* __restrict__ bh, int nblocks) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsRounds; r++) {
+        const int64_t i = base + r * kRsT + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    bh[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kRsT) k_rs_scatter(const uint32_t* __restrict__ kin, const int* __restrict__ vin,
+                                                     uint32_t* __restrict__ kout, int* __restrict__ vout, int64_t n,
+                                                     int shift, const int* __restrict__ bscan, int nblocks) {
+    __shared__ int base[256];
+    __shared__ int wc[4][256];
+    __shared__ int tot[256];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    base[t] = bscan[(int64_t)t * nblocks + blockIdx.x];
+    wc[0][t] = wc[1][t] = wc[2][t] = wc[3][t] = 0;
+    __syncthreads();
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int r = 0; r < kRsRounds; r++) {
+        const int64_t i = (int64_t)blockIdx.x * kRsTile + r * kRsT + t;
+        const bool valid = i < n;
+        const uint32_t k = valid ? kin[i] : 0u;
+        const int v = valid ? vin[i] : 0;
+        const int d = (int)((k >> shift) & 255u);
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const unsigned long long bb = __ballot((d >> b) & 1);
+            m &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        const int rank = __popcll(m & lt);
+        if (valid && rank == 0) wc[w][d] = __popcll(m);
+        __syncthreads();
+        {
+            int sacc = 0;
+#pragma unroll
+            for (int ww = 0; ww < 4; ww++) {
+                const int c = wc[ww][t];
+                wc[ww][t] = sacc;
+                sacc += c;
+            }
+            tot[t] = sacc;
+        }
+        __syncthreads();
+        if (valid) {
+            const int pos = base[d] + wc[w][d] + rank;
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        base[t] += tot[t];
+        wc[0][t] = wc[1][t] = wc[2][t] = wc[3][t] = 0;
+        __syncthreads();
+    }
+}
+
+int64_t radix_tmp_ints(int64_t n) {
+    const int64_t nb = (n + kRsTile - 1) / kRsTile;
+    return 2 * 256 * nb + scan_tmp_ints(256 * nb) + 16;
+}
+
+void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
+    if (n <= 1) return;
+    const int nb = (int)((n + kRsTile - 1) / kRsTile);
+    int* bh = tmp;
+    int* bscan = tmp + 256 * (int64_t)nb;
+    int* stmp = bscan + 256 * (int64_t)nb;
+    const int passes = (bits + 7) / 8;
+    uint32_t *ks = k0, *kd = k1;
+    int *vs = v0, *vd = v1;
+    for (int p = 0; p < passes; p++) {
+        const int shift = 8 * p;
+        k_rs_hist<<<nb, kRsT, 0, s>>>(ks, n, shift, bh, nb);
+        exclusive_scan_i32(bh, bscan, 256 * (int64_t)nb, stmp, s);
+        k_rs_scatter<<<nb, kRsT, 0, s>>>(ks, vs, kd, vd, n, shift, bscan, nb);
+        uint32_t* tk = ks; ks = kd; kd = tk;
+        int* tv = vs; vs = vd; vd = tv;
+    }
+    if (ks != k0) {
+        (void)hipMemcpyAsync(k0, ks, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(v0, vs, n * sizeof(int), hipMemcpyDeviceToDevice, s);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Tight bounding box (SurfReconstructor::loadRootBox, src/surface_reconstructor.cpp:30-49):
+// min/max are exact, so the reduction order does not matter. NaN coordinates are counted.
+// ---------------------------------------------------------------------------------------------
+constexpr int kBbT = 256, kBbBlocks = 1024;
+
+__global__ void __launch_bounds__(kBbT) k_bbox_partial(const float* __restrict__ xyz, int64_t n,
+                                                       float* __restrict__ part, int* __restrict__ nan_count) {
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    int nans = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBbT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBbT) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float v = xyz[3 * i + a];
+            if (v != v) { nans++; continue; }
+            mn[a] = fminf(mn[a], v);
+            mx[a] = fmaxf(mx[a], v);
+        }
+    }
+    __shared__ float smn[3][kBbT], smx[3][kBbT];
+    __shared__ int sn[kBbT];
+    for (int a = 0; a < 3; a++) { smn[a][threadIdx.x] = mn[a]; smx[a][threadIdx.x] = mx[a]; }
+    sn[threadIdx.x] = nans;
+    __syncthreads();
+    for (int o = kBbT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            for (int a = 0; a < 3; a++) {
+                smn[a][threadIdx.x] = fminf(smn[a][threadIdx.x], smn[a][threadIdx.x + o]);
+                smx[a][threadIdx.x] = fmaxf(smx[a][threadIdx.x], smx[a][threadIdx.x + o]);
+            }
+            sn[threadIdx.x] += sn[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; a++) { part[6 * blockIdx.x + 2 * a] = smn[a][0]; part[6 * blockIdx.x + 2 * a + 1] = smx[a][0]; }
+        if (sn[0]) atomicAdd(nan_count, sn[0]);
+    }
+}
+
+__global__ void __launch_bounds__(kBbT) k_bbox_final(const float* __restrict__ part, int nb, float* __restrict__ out) {
+    __shared__ float s[6][kBbT];
+    for (int a = 0; a < 3; a++) { s[2 * a][threadIdx.x] = FLT_MAX; s[2 * a + 1][threadIdx.x] = -FLT_MAX; }
+    for (int b = threadIdx.x; b < nb; b += kBbT)
+        for (int a = 0; a < 3; a++) {
+            s[2 * a][threadIdx.x] = fminf(s[2 * a][threadIdx.x], part[6 * b + 2 * a]);
+            s[2 * a + 1][threadIdx.x] = fmaxf(s[2 * a + 1][threadIdx.x], part[6 * b + 2 * a + 1]);
+        }
+    __syncthreads();
+    for (int o = kBbT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int a = 0; a < 3; a++) {
+                s[2 * a][threadIdx.x] = fminf(s[2 * a][threadIdx.x], s[2 * a][threadIdx.x + o]);
+                s[2 * a + 1][threadIdx.x] = fmaxf(s[2 * a + 1][threadIdx.x], s[2 * a + 1][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+void launch_bbox(const float* xyz, int64_t n, float* partial, float* out6, int* nan_count, hipStream_t s) {
+    int nb = (int)std::min<int64_t>(kBbBlocks, (n + kBbT - 1) / kBbT);
+    if (nb < 1) nb = 1;
+    k_bbox_partial<<<nb, kBbT, 0, s>>>(xyz, n, partial, nan_count);
+    k_bbox_final<<<1, kBbT, 0, s>>>(partial, nb, out6);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cell keys (CalcHashList, src/hash_grid.cpp:87-101). key+1 is sorted as an unsigned value so the
+// out-of-range key -1 sorts first, as the signed keys do in the reference.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_keys(const float* __restrict__ xyz, int64_t n, GridDev g, uint32_t* __restrict__ keys,
+                       int* __restrict__ vals, int* __restrict__ cell_count, int* __restrict__ n_invalid) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ix, iy, iz;
+    cell_of(g, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], ix, iy, iz);
+    const bool valid = ix >= 0 && ix < g.n[0] && iy >= 0 && iy < g.n[1] && iz >= 0 && iz < g.n[2];
+    const int key = valid ? (iz * g.n[1] + iy) * g.n[0] + ix : -1;
+    keys[i] = (uint32_t)(key + 1);
+    vals[i] = (int)i;
+    if (valid) atomicAdd(&cell_count[key], 1);
+    else atomicAdd(n_invalid, 1);
+}
+
+void launch_keys(const float* xyz, int64_t n, GridDev g, uint32_t* keys, int* vals, int* cell_count, int* n_invalid,
+                 hipStream_t s) {
+    if (n <= 0) return;
+    k_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(xyz, n, g, keys, vals, cell_count, n_invalid);
+}
+
+__global__ void k_gather_pos(const float* __restrict__ xyz, const int* __restrict__ sidx, int64_t n,
+                             float4* __restrict__ spos) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = sidx[p];
+    spos[p] = make_float4(xyz[3 * (int64_t)i], xyz[3 * (int64_t)i + 1], xyz[3 * (int64_t)i + 2], 0.0f);
+}
+
+void launch_gather_pos(const float* xyz, const int* sidx, int64_t n, float4* spos, hipStream_t s) {
+    if (n <= 0) return;
+    k_gather_pos<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(xyz, sidx, n, spos);
+}
+
+// dense StartList/EndList view: start[c] = off[c], end[c] = off[c+1]
+__global__ void k_cell_window(GridDev g, int* __restrict__ start, int* __restrict__ end) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.ncell) return;
+    start[c] = g.off[c];
+    end[c] = g.off[c + 1];
+}
+void launch_cell_window(GridDev g, int* start, int* end, hipStream_t s) {
+    if (g.ncell <= 0) return;
+    k_cell_window<<<(unsigned)((g.ncell + 255) / 256), 256, 0, s>>>(g, start, end);
+}
+
+// ---------------------------------------------------------------------------------------------
+// GetPIdxList for query points: count pass, then write pass (ids are particle indices).
+// ---------------------------------------------------------------------------------------------
+__global__ void k_nb_count(GridDev g, const float* __restrict__ q, int64_t nq, int* __restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    int ix, iy, iz;
+    cell_of(g, q[3 * i], q[3 * i + 1], q[3 * i + 2], ix, iy, iz);
+    int c = 0;
+    for (int r = 0; r < 9; r++) {
+        int lo, hi;
+        if (row_range(g, ix, iy, iz, r, lo, hi)) c += hi - lo;
+    }
+    counts[i] = c;
+}
+__global__ void k_nb_write(GridDev g, const float* __restrict__ q, int64_t nq, const int* __restrict__ offsets,
+                           const int* __restrict__ sidx, int* __restrict__ ids) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    int ix, iy, iz;
+    cell_of(g, q[3 * i], q[3 * i + 1], q[3 * i + 2], ix, iy, iz);
+    int o = offsets[i];
+    for (int r = 0; r < 9; r++) {
+        int lo, hi;
+        if (!row_range(g, ix, iy, iz, r, lo, hi)) continue;
+        for (int j = lo; j < hi; j++) ids[o++] = sidx[j];
+    }
+}
+void launch_neighbor_count(GridDev g, const float* q, int64_t nq, int* counts, hipStream_t s) {
+    if (nq <= 0) return;
+    k_nb_count<<<(unsigned)((nq + 255) / 256), 256, 0, s>>>(g, q, nq, counts);
+}
+void launch_neighbor_write(GridDev g, const float* q, int64_t nq, const int* offsets, const int* sidx, int* ids,
+                           hipStream_t s) {
+    if (nq <= 0) return;
+    k_nb_write<<<(unsigned)((nq + 255) / 256), 256, 0, s>>>(g, q, nq, offsets, sidx, ids);
+}
+
+// GetInBoxParticles (src/hash_grid.cpp:164-194): cells enumerated x-outer, y, z-inner over
+// [lo, lo + n) per axis; out-of-range cells contribute nothing.
+__device__ __forceinline__ int box_cell(const GridDev& g, const int* lo3, const int* n3, int64_t t) {
+    const int z = lo3[2] + (int)(t % n3[2]);
+    const int y = lo3[1] + (int)((t / n3[2]) % n3[1]);
+    const int x = lo3[0] + (int)(t / ((int64_t)n3[2] * n3[1]));
+    if (x < 0 || x >= g.n[0] || y < 0 || y >= g.n[1] || z < 0 || z >= g.n[2]) return -1;
+    return (z * g.n[1] + y) * g.n[0] + x;
+}
+__global__ void k_box_count(GridDev g, const int* lo3, const int* n3, int64_t ncells, int* counts) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    const int c = box_cell(g, lo3, n3, t);
+    counts[t] = (c < 0) ? 0 : g.off[c + 1] - g.off[c];
+}
+__global__ void k_box_write(GridDev g, const int* lo3, const int* n3, int64_t ncells, const int* offsets,
+                            const int* sidx, int* ids) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    const int c = box_cell(g, lo3, n3, t);
+    if (c < 0) return;
+    int o = offsets[t];
+    for (int j = g.off[c]; j < g.off[c + 1]; j++) ids[o++] = sidx[j];
+}
+void launch_box_count(GridDev g, const int* lo3, const int* n3, int64_t ncells, int* counts, hipStream_t s) {
+    if (ncells <= 0) return;
+    k_box_count<<<(unsigned)((ncells + 255) / 256), 256, 0, s>>>(g, lo3, n3, ncells, counts);
+}
+void launch_box_write(GridDev g, const int* lo3, const int* n3, int64_t ncells, const int* offsets, const int* sidx,
+                      int* ids, hipStream_t s) {
+    if (ncells <= 0) return;
+    k_box_write<<<(unsigned)((ncells + 255) / 256), 256, 0, s>>>(g, lo3, n3, ncells, offsets, sidx, ids);
+}
+
+}  // namespace pisurf

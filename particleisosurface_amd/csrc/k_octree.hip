@@ -1,0 +1,296 @@
+// Octree level sweep: node emptiness and curvature, 27-point sampling, classification and
+// child generation (SurfReconstructor::genIsoOurs state 0, src/surface_reconstructor.cpp:152-407;
+// TNode, src/iso_method_ours.cpp:16-99). One launch of each kernel covers all nodes of a level.
+#include "pisurf_internal.h"
+#include "qef_device.h"
+
+namespace pisurf {
+
+enum : uint8_t { kEmpty = 0, kInternal = 1, kLeaf = 2 };  // include/iso_method_ours.h:25-27
+
+template <class R> struct R4t { typedef typename Vec4<R>::T T; };
+
+__device__ __forceinline__ void wave_add(unsigned long long* ctr, unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
+}
+
+// leaf test of afterSampleEval: (cellsize - min_radius) < 1e-8 in double
+__device__ __forceinline__ bool leaf_depth(float half, float r) {
+    const float cellsize = __fmul_rn(2.0f, half);
+    return double(__fsub_rn(cellsize, r)) < 1e-8;
+}
+
+// checkEmptyAndCalcCurv (src/surface_reconstructor.cpp:152-222): one wave per node. The box
+// query enumerates cells [lo-1, hi+1] of the clamped box (GetInBoxParticles); each lane walks
+// whole (y,z) rows of that range (one contiguous window of sorted positions per row). A
+// particle counts when it lies strictly inside the unclamped box grown by r * smooth. Nodes
+// above D_min only need emptiness and stop at the first hit.
+__global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __restrict__ spos,
+                                                    const float4* __restrict__ nrm, TreeParams tp, NodeLevel lv) {
+    const int64_t node = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (node >= lv.n) return;
+    const float4 c = lv.c[node];
+    const int depth = lv.depth[node];
+    const float cc[3] = {c.x, c.y, c.z};
+    float b1[3], b2[3], bc1[3], bc2[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        b1[a] = __fsub_rn(cc[a], c.w);
+        b2[a] = __fadd_rn(cc[a], c.w);
+        bc1[a] = fmaxf(b1[a], tp.b_lo[a]);
+        bc2[a] = fminf(b2[a], tp.b_hi[a]);
+    }
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int l = cell_coord(bc1[a], g.b0[a], g.cell, g.n[a]);
+        const int h = cell_coord(bc2[a], g.b0[a], g.cell, g.n[a]);
+        lo[a] = max(l - 1, 0);
+        hi[a] = min(h + 1, g.n[a] - 1);
+    }
+    const bool need_curv = tp.calc_normals && depth >= tp.depth_min;
+    float lx[3], hx[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lx[a] = __fsub_rn(b1[a], tp.margin);
+        hx[a] = __fadd_rn(b2[a], tp.margin);
+    }
+    bool found = false;
+    float sx = 0.f, sy = 0.f, sz = 0.f, area = 0.f;
+    const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    const int nrows = (hi[0] >= lo[0] && ny > 0 && nz > 0) ? ny * nz : 0;
+    for (int r0 = 0; r0 < nrows; r0 += kWave) {
+        const int row = r0 + lane;
+        if (row < nrows) {
+            const int y = lo[1] + row % ny, z = lo[2] + row / ny;
+            const int base = (z * g.n[1] + y) * g.n[0];
+            const int j0 = g.off[base + lo[0]], j1 = g.off[base + hi[0] + 1];
+            for (int j = j0; j < j1; j++) {
+                const float4 p = spos[j];
+                if (p.x > lx[0] && p.x < hx[0] && p.y > lx[1] && p.y < hx[1] && p.z > lx[2] && p.z < hx[2]) {
+                    found = true;
+                    if (!need_curv) break;
+                    const float4 n = nrm[j];
+                    sx += n.x;
+                    sy += n.y;
+                    sz += n.z;
+                    area += sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+                }
+            }
+        }
+        if (!need_curv && __any(found)) break;
+    }
+    const bool any = __any(found);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, kWave);
+        sy += __shfl_xor(sy, o, kWave);
+        sz += __shfl_xor(sz, o, kWave);
+        area += __shfl_xor(area, o, kWave);
+    }
+    if (lane == 0) {
+        lv.empty[node] = any ? 0 : 1;
+        lv.curv[node] = (area == 0.f) ? 1.0f : sqrtf(sx * sx + sy * sy + sz * sz) / area;
+    }
+}
+
+// nodes whose 27 samples are needed: non-empty, depth >= D_min, and not at leaf depth unless the
+// QEF consumes them (the shipped build samples leaf-depth nodes but never uses the result)
+__device__ __forceinline__ bool wants_samples(const NodeLevel& lv, const TreeParams& tp, int64_t node) {
+    if (lv.empty[node]) return false;
+    if (lv.depth[node] < tp.depth_min) return false;
+    if (tp.qef_mode == 0 && leaf_depth(lv.c[node].w, tp.radius)) return false;
+    return true;
+}
+
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_samples(GridDev g, Recs<R> rec, FieldConst fc, TreeParams tp, NodeLevel lv,
+                                                 R* __restrict__ samples, Counters* ctr) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t node = t / 27;
+    const int s = (int)(t - node * 27);
+    int vis = 0;
+    bool did = false;
+    if (node < lv.n && wants_samples(lv, tp, node)) {
+        float p[3];
+        sample_point(lv.c[node], s, p);
+        R sum, gr[3];
+        vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), sum, gr);
+        samples[t] = R(fc.iso) - R(fc.sigma) * sum;
+        did = true;
+    }
+    wave_add(&ctr->field_evals, did ? 1ull : 0ull);
+    wave_add(&ctr->candidates, (unsigned long long)vis);
+}
+
+// afterSampleEval classification (src/surface_reconstructor.cpp:237-273) + leaf values
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldConst fc, TreeParams tp, NodeLevel lv,
+                                                  const R* __restrict__ samples, int32_t* __restrict__ sc_out,
+                                                  Counters* ctr) {
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vis = 0;
+    unsigned long long evals = 0, qefs = 0;
+    if (node < lv.n) {
+        const float4 c = lv.c[node];
+        const int depth = lv.depth[node];
+        uint8_t type;
+        R fval = R(fc.iso);
+        R vx = R(c.x), vy = R(c.y), vz = R(c.z);
+        bool sc = false;
+        if (lv.empty[node]) {
+            type = kEmpty;
+        } else {
+            const bool isbig = depth < tp.depth_min;
+            const bool has_samples = wants_samples(lv, tp, node);
+            if (has_samples) {
+                const R* f = samples + node * 27;
+                const bool s0 = f[0] >= R(0);
+                for (int s = 1; s < 27; s++) sc = sc || ((f[s] >= R(0)) != s0);
+            }
+            const bool leafd = leaf_depth(c.w, tp.radius);
+            if (!leafd && (isbig || (sc && double(lv.curv[node]) < 0.99))) {
+                type = kInternal;
+            } else {
+                type = kLeaf;
+                if (tp.qef_mode != 0 && has_samples) {
+                    R f[27], gr[27][3];
+                    float pts[27][3];
+                    for (int s = 0; s < 27; s++) {
+                        f[s] = samples[node * 27 + s];
+                        sample_point(c, s, pts[s]);
+                    }
+                    fd_grads27<R>(f, __fmul_rn(2.0f, c.w), gr);
+                    R pc[4], err;
+                    if (qef_solve<R>(c, pts, f, gr, tp.qef_mode, tp.qef_svd_tol, pc, err)) {
+                        vx = pc[0];
+                        vy = pc[1];
+                        vz = pc[2];
+                    }
+                    R sum, g3[3];
+                    vis += field_sum<R, K, false>(g, rec, fc, float(vx), float(vy), float(vz), vx, vy, vz, sum, g3);
+                    evals++;
+                    qefs++;
+                    fval = R(fc.iso) - R(fc.sigma) * sum;
+                    if (err / R(__fmul_rn(2.0f, c.w)) < R(0.0001)) fval = R(0);
+                } else {
+                    float p13[3];
+                    sample_point(c, 13, p13);
+                    if (has_samples && p13[0] == c.x && p13[1] == c.y && p13[2] == c.z) {
+                        fval = samples[node * 27 + 13];  // the centre sample is the node centre
+                    } else {
+                        R sum, g3[3];
+                        vis += field_sum<R, K, false>(g, rec, fc, c.x, c.y, c.z, R(c.x), R(c.y), R(c.z), sum, g3);
+                        evals++;
+                        fval = R(fc.iso) - R(fc.sigma) * sum;
+                    }
+                }
+            }
+        }
+        lv.type[node] = type;
+        lv.flag[node] = (type == kInternal) ? 1 : 0;
+        lv.node[node] = make_double4(double(vx), double(vy), double(vz), double(fval));
+        lv.child[node] = -1;
+        if (sc_out) sc_out[node] = sc ? 1 : 0;
+    }
+    wave_add(&ctr->field_evals, evals);
+    wave_add(&ctr->qef_solves, qefs);
+    wave_add(&ctr->candidates, (unsigned long long)vis);
+}
+
+// Children of INTERNAL nodes (TNode ctor, src/iso_method_ours.cpp:25-38): child t has
+// centre parent +/- half_child per axis (bit0 = x, bit1 = y, bit2 = z) and id = parent * 8 + t.
+__global__ void k_children(NodeLevel par, const int* __restrict__ scanned, NodeLevel nxt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= par.n || !par.flag[i]) return;
+    const int64_t base = (int64_t)scanned[i] * 8;
+    par.child[i] = (int)base;
+    const float4 c = par.c[i];
+    const float hc = c.w / 2.0f;
+    const uint64_t id = par.id[i];
+    const int8_t d = par.depth[i] + 1;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const float sx = (t & 1) ? hc : -hc, sy = (t & 2) ? hc : -hc, sz = (t & 4) ? hc : -hc;
+        nxt.c[base + t] = make_float4(__fadd_rn(c.x, sx), __fadd_rn(c.y, sy), __fadd_rn(c.z, sz), hc);
+        nxt.id[base + t] = id * 8ull + (uint64_t)t;
+        nxt.depth[base + t] = d;
+    }
+}
+
+// samples and their finite-difference gradients for the sweep API
+template <class R>
+__global__ void k_sample_grads(TreeParams tp, NodeLevel lv, const R* __restrict__ samples, double* __restrict__ os,
+                               double* __restrict__ og) {
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= lv.n) return;
+    const float4 c = lv.c[node];
+    const bool has = wants_samples(lv, tp, node);
+    R f[27], gr[27][3];
+    for (int s = 0; s < 27; s++) f[s] = has ? samples[node * 27 + s] : R(0);
+    fd_grads27<R>(f, __fmul_rn(2.0f, c.w), gr);
+    for (int s = 0; s < 27; s++) {
+        float p[3];
+        sample_point(c, s, p);
+        if (os) {
+            os[(node * 27 + s) * 4 + 0] = p[0];
+            os[(node * 27 + s) * 4 + 1] = p[1];
+            os[(node * 27 + s) * 4 + 2] = p[2];
+            os[(node * 27 + s) * 4 + 3] = double(f[s]);
+        }
+        if (og)
+            for (int a = 0; a < 3; a++) og[(node * 27 + s) * 3 + a] = has ? double(gr[s][a]) : 0.0;
+    }
+}
+
+void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, hipStream_t s) {
+    if (lv.n <= 0) return;
+    const int64_t threads = lv.n * kWave;
+    k_empty_curv<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(g, spos, nrm, tp, lv);
+}
+
+void launch_samples(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, TreeParams tp, NodeLevel lv,
+                    void* samples, Counters* ctr, hipStream_t s) {
+    if (lv.n <= 0) return;
+    const unsigned nb = (unsigned)((lv.n * 27 + 255) / 256);
+    if (prec == 2) {
+        Recs<double> r{(const double4*)ra, (const double4*)rg};
+        k_samples<double, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (double*)samples, ctr);
+    } else {
+        Recs<float> r{(const float4*)ra, (const float4*)rg};
+        if (prec == 1) k_samples<float, true><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (float*)samples, ctr);
+        else k_samples<float, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (float*)samples, ctr);
+    }
+}
+
+void launch_classify(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, TreeParams tp, NodeLevel lv,
+                     const void* samples, int32_t* sc, Counters* ctr, hipStream_t s) {
+    if (lv.n <= 0) return;
+    const unsigned nb = (unsigned)((lv.n + 255) / 256);
+    if (prec == 2) {
+        Recs<double> r{(const double4*)ra, (const double4*)rg};
+        k_classify<double, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const double*)samples, sc, ctr);
+    } else {
+        Recs<float> r{(const float4*)ra, (const float4*)rg};
+        if (prec == 1) k_classify<float, true><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const float*)samples, sc, ctr);
+        else k_classify<float, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const float*)samples, sc, ctr);
+    }
+}
+
+void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s) {
+    if (parent.n <= 0) return;
+    k_children<<<(unsigned)((parent.n + 255) / 256), 256, 0, s>>>(parent, scanned, next);
+}
+
+void launch_sample_grads(int prec, TreeParams tp, NodeLevel lv, const void* samples, double* os, double* og,
+                         hipStream_t s) {
+    if (lv.n <= 0) return;
+    const unsigned nb = (unsigned)((lv.n + 63) / 64);
+    if (prec == 2) k_sample_grads<double><<<nb, 64, 0, s>>>(tp, lv, (const double*)samples, os, og);
+    else k_sample_grads<float><<<nb, 64, 0, s>>>(tp, lv, (const float*)samples, os, og);
+}
+
+}  // namespace pisurf

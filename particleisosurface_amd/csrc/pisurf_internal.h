@@ -1,0 +1,95 @@
+// Internal declarations shared by the pisurf host code and kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.h"
+
+namespace pisurf {
+
+// ---- scan / sort workspace ----
+// device exclusive scan of n int32 values; tmp must hold scan_tmp_ints(n) ints
+int64_t scan_tmp_ints(int64_t n);
+void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
+// stable LSD radix sort of (key, value) by the low `bits` bits of key; results in k0/v0.
+// k1/v1 are ping-pong buffers; tmp must hold radix_tmp_ints(n) ints.
+int64_t radix_tmp_ints(int64_t n);
+void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s);
+
+// ---- grid ----
+void launch_bbox(const float* xyz, int64_t n, float* partial /*>= 6*1024*/, float* out6, int* nan_count,
+                 hipStream_t s);
+void launch_keys(const float* xyz, int64_t n, GridDev g, uint32_t* keys, int* vals, int* cell_count, int* n_invalid,
+                 hipStream_t s);
+void launch_gather_pos(const float* xyz, const int* sidx, int64_t n, float4* spos, hipStream_t s);
+void launch_neighbor_count(GridDev g, const float* q, int64_t nq, int* counts, hipStream_t s);
+void launch_neighbor_write(GridDev g, const float* q, int64_t nq, const int* offsets, const int* sidx, int* ids,
+                           hipStream_t s);
+void launch_box_count(GridDev g, const int* lo3, const int* n3, int64_t ncells, int* counts, hipStream_t s);
+void launch_box_write(GridDev g, const int* lo3, const int* n3, int64_t ncells, const int* offsets, const int* sidx,
+                      int* ids, hipStream_t s);
+void launch_cell_window(GridDev g, int* start, int* end, hipStream_t s);
+
+// ---- anisotropy ----
+struct AnisoParams {
+    float radius;          // r
+    float nI;              // neighbour radius r * 4
+    float nI2;             // nI * nI (float, as compared in the reference)
+    double h2;             // Influnce2
+    float xmean_delta;     // 0.9 (B-spline) / unused in poly6 mode
+    int use_poly6;
+    int use_ani;
+};
+void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, void* xm /*R x4*/,
+                  hipStream_t s);
+void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, const void* xm, void* rec_a,
+                 void* rec_g, unsigned long long* n_fp64_eig, hipStream_t s);
+
+// ---- field ----
+struct Counters {
+    unsigned long long field_evals, grad_evals, candidates, qef_solves;
+};
+void launch_eval_points(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float* q,
+                        int64_t n, double* f, double* grad, Counters* ctr, hipStream_t s);
+void launch_normals(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float4* spos,
+                    int64_t n, float4* nrm, Counters* ctr, hipStream_t s);
+
+// ---- octree levels ----
+struct NodeLevel {
+    float4* c;         // centre xyz, half
+    int8_t* depth;
+    uint64_t* id;
+    uint8_t* type;
+    uint8_t* empty;
+    float* curv;
+    double4* node;     // vertex xyz, field value
+    int* flag;         // INTERNAL -> 1 (scanned into child offsets)
+    int* child;        // level-local index of the first child in the next level, -1
+    int64_t n;
+};
+struct TreeParams {
+    float radius;
+    float margin;      // r * smooth_factor (box margin of checkEmptyAndCalcCurv)
+    float b_lo[3], b_hi[3];  // HashGrid::Bounding
+    int depth_min, depth_max;
+    int calc_normals;
+    int qef_mode;
+    float qef_svd_tol;
+};
+void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, hipStream_t s);
+void launch_samples(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
+                    NodeLevel lv, void* samples /*n*27 R*/, Counters* ctr, hipStream_t s);
+void launch_classify(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
+                     NodeLevel lv, const void* samples, int32_t* signchange, Counters* ctr, hipStream_t s);
+void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s);
+void launch_sample_grads(int prec, TreeParams tp, NodeLevel lv, const void* samples, double* out_samples,
+                         double* out_grads, hipStream_t s);
+
+// ---- vertex placement ----
+void launch_bisect(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, float half_r,
+                   const double* ends, int64_t n, double* out, int* nev, Counters* ctr, hipStream_t s);
+void launch_qef(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
+                const float4* cen, const double* samples, const double* grads, int64_t n, int mode, double4* out,
+                Counters* ctr, hipStream_t s);
+
+}  // namespace pisurf
