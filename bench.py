@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: field evals/s and frame mesh time of the MI355X path on the crown-splash frame.
+
+One step = one full frame through the hot path (hash grid, anisotropy, iso constants, particle
+normals, octree level sweeps, dual-cell extraction, bisection vertex placement) on particles
+already resident in HBM, in the FP32-compensated mode (Kahan sums, FP64 eigensolve for clamped
+spectra). The same frame is also meshed by the all-FP64 GPU path, and the two are compared.
+
+Prints one JSON line (see the keys below). The CPU baseline is the oracle (the reference's CPU
+algorithm with its std::map cell tables and OpenMP loops) evaluating a bounded sample of the
+frame's own octree sample points on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+from particleisosurface_amd import capi, synth  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CAND = {capi.FP32: 52, capi.FP32_KAHAN: 52, capi.FP64: 104}
+BYTES_FIXED = 16 + 27 * 8
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def frame(ctx, steps, warmup):
+    for _ in range(warmup):
+        ctx.run_frame()
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = ctx.run_frame()
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    return info, st, dt
+
+
+def sample_points_of_tree(ctx, info, r, limit, seed=3):
+    """Octree sample points (27 per sampled node) of the last frame, as the sweep evaluates them."""
+    nd = ctx.tree_nodes(info["n_nodes"])
+    cw = (2.0 * nd["half"]).astype(np.float32)
+    leafd = (cw - np.float32(r)).astype(np.float64) < 1e-8
+    sel = (nd["type"] != capi.EMPTY) & (nd["depth"] >= info["depth_min"]) & ~leafd
+    c = nd["center"][sel]
+    h = nd["half"][sel][:, None]
+    mn = (c - h).astype(np.float32)
+    mx = (c + h).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(len(c) * 27, size=min(limit, len(c) * 27), replace=False)
+    node, s = pick // 27, pick % 27
+    t = np.stack([s % 3, (s // 3) % 3, s // 9], 1).astype(np.float32) / np.float32(2)
+    pts = ((np.float32(1) - t) * mn[node] + t * mx[node]).astype(np.float32)
+    return pts
+
+
+def mesh_error(v1, t1, v2, t2, r):
+    """Vertex distance / r between two meshes of the same frame."""
+    if len(v1) == 0 or len(v2) == 0:
+        return None
+    if v1.shape == v2.shape and np.array_equal(t1, t2):
+        d = np.linalg.norm(v1 - v2, axis=1) / r
+        how = "same topology"
+    else:
+        from scipy.spatial import cKDTree
+        d, _ = cKDTree(v2).query(v1)
+        d = d / r
+        how = "nearest vertex"
+    return dict(match=how, p50=float(np.median(d)), p99=float(np.quantile(d, 0.99)), max=float(d.max()),
+                frac_within_1e3r=float(np.mean(d <= 1e-3)))
+
+
+def cpu_baseline(pts_all, xyz, r, ctx, budget_s, log_every=True):
+    from oracle import pyoracle
+    o = pyoracle.Oracle(xyz, r, fp64=False)
+    t = time.perf_counter()
+    o.hashgrid_build()
+    xm, G, d = ctx.aniso()  # FP32 records of the same frame seed the oracle's evaluator
+    o.set_aniso(xm, G, d)
+    o.iso_constants()
+    setup = time.perf_counter() - t
+    done, spent = 0, 0.0
+    chunk = 250_000
+    while done < len(pts_all) and spent < budget_s:
+        q = pts_all[done:done + chunk]
+        t = time.perf_counter()
+        o.eval_points(q)
+        spent += time.perf_counter() - t
+        done += len(q)
+    return dict(value=done / spent, unit="field evals/s", cores=pyoracle.num_threads(), kind="port",
+                sample="%d octree sample points of the same frame (SingleEval with std::map cell tables, "
+                       "OpenMP over points), %.1f s; setup %.1f s not timed" % (done, spent, setup))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pool-layers", type=int, default=42, help="42: ~4.0 M particles, 20: geometry as first stated")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--precision", default="fp32k", choices=["fp32", "fp32k", "fp64"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    t = time.perf_counter()
+    xyz, r = synth.crown_splash(pool_layers=args.pool_layers)
+    log("workload: crown splash, %d particles, r=%g (%.1f s to generate)" % (len(xyz), r, time.perf_counter() - t))
+
+    prec = {"fp32": capi.FP32, "fp32k": capi.FP32_KAHAN, "fp64": capi.FP64}[args.precision]
+    ctx = capi.Context(0)
+    ctx.set_params(r, precision=prec)
+    ctx.upload(xyz)  # resident in HBM before the timed region
+    info, st, dt = frame(ctx, args.steps, args.warmup)
+    ms = 1e3 * dt / args.steps
+    evals_per_frame = st["field_evals"] + st["grad_evals"]
+    log("frame %.2f ms  nodes %d  evals/frame %d  phases(ms/frame): grid %.2f aniso %.2f normals %.2f tree %.2f "
+        "extract-host %.2f bisect %.2f" % (ms, info["n_nodes"], evals_per_frame / args.steps,
+                                            st["ms_grid"] / args.steps, st["ms_aniso"] / args.steps,
+                                            st["ms_normals"] / args.steps, st["ms_tree"] / args.steps,
+                                            st["ms_extract_host"] / args.steps, st["ms_bisect"] / args.steps))
+    value = evals_per_frame / (dt / args.steps) / args.steps
+    # roofline of the node-sampling kernel (the dominant field-evaluation kernel)
+    k_ms = st["ms_sample_kernel"] / max(st["sample_launches"], 1)
+    alg_bytes = st["sample_evals"] * BYTES_FIXED + st["sample_candidates"] * BYTES_PER_CAND[prec]
+    achieved = alg_bytes / max(st["sample_launches"], 1) / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_samples", launches_per_frame=st["sample_launches"] / args.steps,
+                avg_launch_ms=round(k_ms, 4), evals_per_launch=st["sample_evals"] / max(st["sample_launches"], 1),
+                candidates_per_eval=st["sample_candidates"] / max(st["sample_evals"], 1),
+                sample_kernel_evals_per_s=st["sample_evals"] / (st["ms_sample_kernel"] * 1e-3)
+                if st["ms_sample_kernel"] > 0 else None)
+    v32, t32 = ctx.mesh()
+    out = {
+        "metric": "field evals/s + frame mesh time @4M particles, FP32-comp vs FP64 at stated err",
+        "value": round(value, 1), "unit": "field evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp32": "f32", "fp32k": "f32 (compensated; f64 eigensolve when clamped)", "fp64": "f64"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "crown splash (config 3), %d particles, r=0.001, pool %d layers"
+                               % (len(xyz), args.pool_layers), "precision": args.precision,
+                   "depth_max": info["depth_max"], "depth_min": info["depth_min"]},
+        "frame_ms": round(ms, 3),
+        "field_evals_per_frame": evals_per_frame // args.steps,
+        "phases_ms": {k[3:]: round(st[k] / args.steps, 3) for k in
+                      ["ms_grid", "ms_aniso", "ms_normals", "ms_tree", "ms_extract_host", "ms_bisect"]},
+        "mesh": {"verts": len(v32), "tris": len(t32), "nodes": info["n_nodes"]},
+        "fp64_eigensolves": st["fp64_eigensolves"],
+        "roofline": roof,
+    }
+
+    # FP64 comparison on the same frame
+    if not args.no_fp64 and prec != capi.FP64:
+        rng = np.random.default_rng(1)
+        qi = rng.choice(len(xyz), 200_000, replace=False)
+        q = (xyz[qi] + rng.uniform(-2 * r, 2 * r, (len(qi), 3))).astype(np.float32)
+        f32 = ctx.eval_points(q)
+        n32 = ctx.particle_normals()
+        iso = ctx.iso_constants()[1]
+        ctx.set_params(r, precision=capi.FP64)
+        info64, st64, dt64 = frame(ctx, 1, 1)
+        ms64 = 1e3 * dt64
+        v64, t64 = ctx.mesh()
+        f64 = ctx.eval_points(q)
+        n64 = ctx.particle_normals()
+        S32, S64 = iso - f32, iso - f64
+        mask = S64 > 1e-3 * iso
+        rel = np.abs(S32 - S64)[mask] / S64[mask]
+        nn = np.linalg.norm(n32, axis=1) * np.linalg.norm(n64, axis=1)
+        ok = nn > 0
+        cosang = np.clip(np.sum(n32 * n64, axis=1)[ok] / nn[ok], -1, 1)
+        ang = np.degrees(np.arccos(cosang))
+        out["fp64_frame_ms"] = round(ms64, 3)
+        out["speedup_vs_fp64"] = round(ms64 / ms, 3)
+        out["error_vs_fp64"] = {
+            "field_rel_p99": float(np.quantile(rel, 0.99)), "field_rel_max": float(rel.max()),
+            "field_points": int(mask.sum()),
+            "sign_mismatch_outside_2e-5_iso": int(np.sum(((f32 > 0) != (f64 > 0)) & (np.abs(f64) > 2e-5 * iso))),
+            "normal_angle_deg_p99": float(np.quantile(ang, 0.99)), "normal_angle_deg_max": float(ang.max()),
+            "vertex_over_r": mesh_error(v32, t32, v64, t64, r),
+            "same_tree": bool(info64["n_nodes"] == info["n_nodes"]),
+        }
+        log("fp64 frame %.2f ms, speedup %.2fx" % (ms64, ms64 / ms))
+        ctx.set_params(r, precision=prec)
+        info, st, dt = frame(ctx, 1, 0)
+
+    if not args.no_cpu:
+        pts = sample_points_of_tree(ctx, info, r, limit=20_000_000)
+        out["cpu_baseline"] = cpu_baseline(pts, xyz, r, ctx, args.cpu_seconds)
+        log("cpu baseline %.3g evals/s on %d threads" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"]))
+    else:
+        out["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

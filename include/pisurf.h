@@ -80,10 +80,14 @@ typedef struct pisurf_stats {
     int64_t field_evals;       /* SingleEval-equivalents performed since the last reset */
     int64_t grad_evals;        /* SingleEvalWithGrad-equivalents */
     int64_t qef_solves;
-    int64_t candidates;        /* candidate particles visited by field evaluations */
+    int64_t candidates;        /* candidate particles visited by all field evaluations */
+    int64_t sample_evals;      /* evaluations of the node-sampling kernel (27 per sampled node) */
+    int64_t sample_candidates; /* candidates visited by the node-sampling kernel */
+    int64_t sample_launches;   /* launches of the node-sampling kernel */
+    int64_t fp64_eigensolves;  /* clamped spectra re-solved in FP64 (PISURF_FP32_KAHAN) */
     double ms_grid, ms_aniso, ms_normals, ms_tree, ms_extract_host, ms_bisect, ms_total;
-    double ms_field_kernels;   /* summed device time of field-evaluation kernels */
-    int64_t n_dual_cells, n_verts, n_tris;
+    double ms_sample_kernel;   /* summed device time of the node-sampling kernel (HIP events) */
+    int64_t n_dual_cells, n_verts, n_tris, n_nodes;
 } pisurf_stats;
 
 /* ---- context ---- */

@@ -46,10 +46,12 @@ class TreeInfo(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("field_evals", C.c_int64), ("grad_evals", C.c_int64), ("qef_solves", C.c_int64),
-                ("candidates", C.c_int64), ("ms_grid", C.c_double), ("ms_aniso", C.c_double),
+                ("candidates", C.c_int64), ("sample_evals", C.c_int64), ("sample_candidates", C.c_int64),
+                ("sample_launches", C.c_int64), ("fp64_eigensolves", C.c_int64),
+                ("ms_grid", C.c_double), ("ms_aniso", C.c_double),
                 ("ms_normals", C.c_double), ("ms_tree", C.c_double), ("ms_extract_host", C.c_double),
-                ("ms_bisect", C.c_double), ("ms_total", C.c_double), ("ms_field_kernels", C.c_double),
-                ("n_dual_cells", C.c_int64), ("n_verts", C.c_int64), ("n_tris", C.c_int64)]
+                ("ms_bisect", C.c_double), ("ms_total", C.c_double), ("ms_sample_kernel", C.c_double),
+                ("n_dual_cells", C.c_int64), ("n_verts", C.c_int64), ("n_tris", C.c_int64), ("n_nodes", C.c_int64)]
 
 
 _lib = None

@@ -178,10 +178,34 @@ struct pisurf_ctx {
     DBuf counters;
     pisurf_stats st;
     unsigned long long n_fp64_eig = 0;
+    std::vector<hipEvent_t> ev;  // start/stop pairs around node-sampling launches
+    size_t ev_used = 0;
 
     ~pisurf_ctx() {
         for (auto* l : levels) delete l;
+        for (auto e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+    hipError_t event_pair(hipEvent_t& a, hipEvent_t& b) {
+        while (ev.size() < ev_used + 2) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ev.push_back(e);
+        }
+        a = ev[ev_used];
+        b = ev[ev_used + 1];
+        ev_used += 2;
+        return hipSuccess;
+    }
+    // sum the recorded sampling-kernel intervals (stream already synchronized)
+    void harvest_events() {
+        for (size_t i = 0; i + 1 < ev_used; i += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) st.ms_sample_kernel += ms;
+            st.sample_launches++;
+        }
+        ev_used = 0;
     }
     int fail(int code, const std::string& m) {
         err = m;
@@ -486,7 +510,11 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev) {
     const FieldConst fc = field_const(c);
     const NodeLevel v = L->view();
     launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, c->stream);
+    hipEvent_t e0, e1;
+    HIPCHK(c, c->event_pair(e0, e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
     launch_samples(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, c->ctr(), c->stream);
+    HIPCHK(c, hipEventRecord(e1, c->stream));
     launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, sc_dev, c->ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
     return PISURF_OK;
@@ -523,6 +551,7 @@ int do_tree(pisurf_ctx* c) {
         int n_int = 0;
         HIPCHK(c, hipMemcpyAsync(&n_int, L->scanned.as<int>() + L->n, 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->harvest_events();
         if (n_int == 0) break;
         if ((int64_t)n_int * 8 > (int64_t)INT32_MAX / 2) return c->fail(PISURF_E_UNSUPPORTED, "octree level too large");
         LevelBuf* N = level(li + 1);
@@ -540,6 +569,8 @@ int do_tree(pisurf_ctx* c) {
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->tree_done = true;
+    c->st.n_nodes = 0;
+    for (auto* l : c->levels) c->st.n_nodes += l->n;
     c->st.ms_tree += now_ms() - t0;
     return PISURF_OK;
 }
@@ -783,6 +814,9 @@ int sync_counters(pisurf_ctx* c) {
     c->st.grad_evals = (int64_t)h.grad_evals;
     c->st.candidates = (int64_t)h.candidates;
     c->st.qef_solves = (int64_t)h.qef_solves;
+    c->st.sample_evals = (int64_t)h.sample_evals;
+    c->st.sample_candidates = (int64_t)h.sample_candidates;
+    c->st.fp64_eigensolves = (int64_t)c->n_fp64_eig;
     return PISURF_OK;
 }
 
@@ -1127,6 +1161,7 @@ int pisurf_sweep_level(pisurf_ctx* c, const float* centers, const float* halfs, 
     if (signchange) HIPCHK(c, hipMemcpyAsync(signchange, c->out3.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(nd.data(), L.node.p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->harvest_events();
     if (f)
         for (int64_t i = 0; i < n; i++) f[i] = nd[i].w;
     return PISURF_OK;

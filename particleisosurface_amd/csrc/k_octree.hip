@@ -124,6 +124,8 @@ __global__ void __launch_bounds__(256) k_samples(GridDev g, Recs<R> rec, FieldCo
     }
     wave_add(&ctr->field_evals, did ? 1ull : 0ull);
     wave_add(&ctr->candidates, (unsigned long long)vis);
+    wave_add(&ctr->sample_evals, did ? 1ull : 0ull);
+    wave_add(&ctr->sample_candidates, (unsigned long long)vis);
 }
 
 // afterSampleEval classification (src/surface_reconstructor.cpp:237-273) + leaf values

@@ -47,7 +47,7 @@ void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams
 
 // ---- field ----
 struct Counters {
-    unsigned long long field_evals, grad_evals, candidates, qef_solves;
+    unsigned long long field_evals, grad_evals, candidates, qef_solves, sample_evals, sample_candidates;
 };
 void launch_eval_points(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float* q,
                         int64_t n, double* f, double* grad, Counters* ctr, hipStream_t s);
