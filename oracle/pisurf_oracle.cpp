@@ -247,6 +247,27 @@ template <class T> void jacobi_eig4(const T a_in[4][4], T lam[4], T Q[4][4]) {
     for (int i = 0; i < 4; i++) lam[i] = a[i][i];
 }
 
+// SVD tail of compute_G_Yus / compute_G_ours (src/evaluator.cpp:627-651, :556-580): singular
+// values |lambda| clamped below at max/kr (kr = 4), G = V diag(1/w) U^T * cbrt(w0 w1 w2)
+// (times 0.5 in poly6 mode); U = V = Q for the symmetric positive-definite covariance.
+template <class T> void g_from_cov(const T cov[3][3], bool poly6_mode, M3<T>& G) {
+    T lam[3], Q[3][3];
+    jacobi_eig3<T>(cov, lam, Q);
+    T w[3];
+    for (int k = 0; k < 3; k++) w[k] = std::fabs(lam[k]);
+    T mx = std::max(w[0], std::max(w[1], w[2]));
+    const T kr = T(4);
+    for (int k = 0; k < 3; k++) w[k] = std::max(w[k], mx / kr);
+    T inv[3] = {T(1) / w[0], T(1) / w[1], T(1) / w[2]};
+    T scale = T(std::pow(double((w[0] * w[1]) * w[2]), 1.0 / 3.0));
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            T s = ((Q[a][0] * inv[0]) * Q[b][0] + (Q[a][1] * inv[1]) * Q[b][1]) + (Q[a][2] * inv[2]) * Q[b][2];
+            G(a, b) = s * scale;
+            if (poly6_mode) G(a, b) = G(a, b) * T(0.5);
+        }
+}
+
 // 3x3 determinant by cofactor expansion along the first row
 template <class T> inline T det3(const M3<T>& g) {
     return g(0, 0) * (g(1, 1) * g(2, 2) - g(1, 2) * g(2, 1)) - g(0, 1) * (g(1, 0) * g(2, 2) - g(1, 2) * g(2, 0)) +
@@ -482,21 +503,7 @@ template <class T> struct Oracle : OracleBase {
             }
         }
         for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) cov[a][b] /= wSum;
-        T lam[3], Q[3][3];
-        jacobi_eig3<T>(cov, lam, Q);
-        T w[3];
-        for (int k = 0; k < 3; k++) w[k] = std::fabs(lam[k]);
-        T mx = std::max(w[0], std::max(w[1], w[2]));
-        const T kr = T(4);
-        for (int k = 0; k < 3; k++) w[k] = std::max(w[k], mx / kr);
-        T inv[3] = {T(1) / w[0], T(1) / w[1], T(1) / w[2]};
-        T scale = T(std::pow(double((w[0] * w[1]) * w[2]), 1.0 / 3.0));
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) {
-                T s = ((Q[a][0] * inv[0]) * Q[b][0] + (Q[a][1] * inv[1]) * Q[b][1]) + (Q[a][2] * inv[2]) * Q[b][2];
-                G(a, b) = s * scale;
-                if (prm.use_poly6) G(a, b) = G(a, b) * T(0.5);
-            }
+        g_from_cov<T>(cov, prm.use_poly6 != 0, G);
     }
 
     // ---- compute_Gs_xMeans (src/evaluator.cpp:656-749): two dependent passes
@@ -1363,3 +1370,52 @@ int orc_run(orc_ctx* c, double* pt) {
 int64_t orc_eval_count(orc_ctx* c) { (void)c; return 0; }
 
 }  // extern "C"
+
+extern "C" {
+int orc_test_eig(const double* a, int n, int fp64, double* lam, double* Q) {
+    if (n == 3) {
+        if (fp64) {
+            double A[3][3], L[3], V[3][3];
+            for (int i = 0; i < 9; i++) A[i / 3][i % 3] = a[i];
+            jacobi_eig3<double>(A, L, V);
+            for (int i = 0; i < 3; i++) lam[i] = L[i];
+            for (int i = 0; i < 9; i++) Q[i] = V[i / 3][i % 3];
+        } else {
+            float A[3][3], L[3], V[3][3];
+            for (int i = 0; i < 9; i++) A[i / 3][i % 3] = float(a[i]);
+            jacobi_eig3<float>(A, L, V);
+            for (int i = 0; i < 3; i++) lam[i] = L[i];
+            for (int i = 0; i < 9; i++) Q[i] = V[i / 3][i % 3];
+        }
+        return 0;
+    }
+    if (n == 4) {
+        double A[4][4], L[4], V[4][4];
+        for (int i = 0; i < 16; i++) A[i / 4][i % 4] = a[i];
+        jacobi_eig4<double>(A, L, V);
+        for (int i = 0; i < 4; i++) lam[i] = L[i];
+        for (int i = 0; i < 16; i++) Q[i] = V[i / 4][i % 4];
+        return 0;
+    }
+    return -1;
+}
+int orc_test_g_from_cov(const double* c6, int fp64, double* G9, double* det) {
+    const int map[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    if (fp64) {
+        double cov[3][3];
+        for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) cov[a][b] = c6[map[a][b]];
+        M3<double> G;
+        g_from_cov<double>(cov, false, G);
+        for (int k = 0; k < 9; k++) G9[k] = G.m[k];
+        *det = det3(G);
+    } else {
+        float cov[3][3];
+        for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) cov[a][b] = float(c6[map[a][b]]);
+        M3<float> G;
+        g_from_cov<float>(cov, false, G);
+        for (int k = 0; k < 9; k++) G9[k] = G.m[k];
+        *det = det3(G);
+    }
+    return 0;
+}
+}

@@ -91,6 +91,11 @@ int orc_write_obj(orc_ctx* c, const char* path);
 int orc_run(orc_ctx* c, double* phase_times7);
 int64_t orc_eval_count(orc_ctx* c);
 int orc_num_threads(void);
+/* the oracle's symmetric eigensolvers, for pinning against a library eigensolver:
+ * a = row-major n x n (n = 3 or 4), lam[n], Q[n*n] row-major (eigenvectors in columns) */
+int orc_test_eig(const double* a, int n, int fp64, double* lam, double* Q);
+/* compute_G_Yus tail on a given covariance (6 upper entries): G (9, column-major), det */
+int orc_test_g_from_cov(const double* cov6, int fp64, double* G9, double* det);
 
 #ifdef __cplusplus
 }

@@ -235,3 +235,24 @@ class Oracle:
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def eigensolve(a, fp64=True):
+    """The oracle's Jacobi eigensolver on a symmetric 3x3 or 4x4 matrix."""
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+    lam = np.zeros(n)
+    Q = np.zeros((n, n))
+    rc = lib().orc_test_eig(_p(a), n, int(fp64), _p(lam), _p(Q))
+    if rc != 0:
+        raise RuntimeError("orc_test_eig failed")
+    return lam, Q
+
+
+def g_from_cov(cov6, fp64=True):
+    """compute_G_Yus tail (clamped spectrum -> G) on a given covariance."""
+    c = np.ascontiguousarray(cov6, np.float64)
+    G = np.zeros(9)
+    det = C.c_double()
+    lib().orc_test_g_from_cov(_p(c), int(fp64), _p(G), C.byref(det))
+    return G.reshape(3, 3, order="F"), det.value

@@ -156,7 +156,7 @@ struct pisurf_ctx {
     // anisotropy (records are for prec_rec)
     bool aniso_done = false;
     int prec_rec = 0;
-    DBuf xm, rec_a, rec_g;
+    DBuf xm, rec_a, rec_g, rec_lo;
     // constants
     bool consts_done = false;
     double influnce2 = 0, sigma = 0, iso = 0, max_scalar = 0;
@@ -242,6 +242,7 @@ FieldConst field_const(const pisurf_ctx* c) {
     fc.iso = c->iso;
     fc.use_ani = c->prm.use_ani;
     fc.use_poly6 = c->prm.use_poly6;
+    fc.lo = c->rec_lo.as<float4>();
     if (prec_of(c) != PISURF_FP64) fc.inv_h2 = double(1.0f / float(c->influnce2));
     return fc;
 }
@@ -402,9 +403,11 @@ int do_aniso(pisurf_ctx* c) {
     const double t0 = now_ms();
     const int prec = prec_of(c);
     const size_t rb = (prec == PISURF_FP64) ? sizeof(double4) : sizeof(float4);
-    HIPCHK(c, c->xm.ensure(c->n * rb));
+    // compensated FP32 keeps xMean as hi + lo (two float4 per particle) and a residual record
+    HIPCHK(c, c->xm.ensure(c->n * rb * (prec == PISURF_FP32_KAHAN ? 2 : 1)));
     HIPCHK(c, c->rec_a.ensure(c->n * rb));
     HIPCHK(c, c->rec_g.ensure(2 * c->n * rb));
+    HIPCHK(c, c->rec_lo.ensure(prec == PISURF_FP32_KAHAN ? c->n * sizeof(float4) : 16));
     HIPCHK(c, c->small.ensure(256));
     HIPCHK(c, hipMemsetAsync(c->small.as<char>() + 128, 0, 8, c->stream));
     AnisoParams ap;
@@ -418,7 +421,8 @@ int do_aniso(pisurf_ctx* c) {
     ap.use_ani = c->prm.use_ani;
     launch_xmean(prec, c->g, c->spos.as<float4>(), c->n, ap, c->xm.p, c->stream);
     unsigned long long* nfp = (unsigned long long*)(c->small.as<char>() + 128);
-    launch_gmat(prec, c->g, c->spos.as<float4>(), c->n, ap, c->xm.p, c->rec_a.p, c->rec_g.p, nfp, c->stream);
+    launch_gmat(prec, c->g, c->spos.as<float4>(), c->n, ap, c->xm.p, c->rec_a.p, c->rec_g.p, c->rec_lo.p, nfp,
+                c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&c->n_fp64_eig, nfp, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1038,8 +1042,11 @@ int pisurf_download_aniso(pisurf_ctx* c, double* xmean, double* G9, double* det)
     const bool f64 = c->prec_rec == PISURF_FP64;
     const size_t rb = f64 ? sizeof(double4) : sizeof(float4);
     std::vector<char> a(n * rb), g(2 * n * rb);
+    std::vector<float4> lo(c->prec_rec == PISURF_FP32_KAHAN ? n : 0);
     std::vector<int> sidx(n);
     HIPCHK(c, hipMemcpyAsync(a.data(), c->rec_a.p, n * rb, hipMemcpyDeviceToHost, c->stream));
+    if (!lo.empty())
+        HIPCHK(c, hipMemcpyAsync(lo.data(), c->rec_lo.p, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(g.data(), c->rec_g.p, 2 * n * rb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(sidx.data(), c->vals.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1048,8 +1055,14 @@ int pisurf_download_aniso(pisurf_ctx* c, double* xmean, double* G9, double* det)
     };
     for (int64_t p = 0; p < n; p++) {
         const int64_t i = sidx[p];
-        if (xmean)
+        if (xmean) {
             for (int k = 0; k < 3; k++) xmean[3 * i + k] = rd(a, 4 * p + k);
+            if (!lo.empty()) {
+                xmean[3 * i] += lo[p].x;
+                xmean[3 * i + 1] += lo[p].y;
+                xmean[3 * i + 2] += lo[p].z;
+            }
+        }
         const double G6[6] = {rd(g, 8 * p), rd(g, 8 * p + 1), rd(g, 8 * p + 2), rd(g, 8 * p + 3), rd(g, 8 * p + 4),
                               rd(g, 8 * p + 5)};
         if (G9) {

@@ -113,6 +113,9 @@ struct FieldConst {
     double iso;         // iso value
     int use_ani;
     int use_poly6;
+    // compensated FP32 only: xMean = a.xyz + lo[j].xyz, the float rounding residual of xMean, so
+    // that q - xMean keeps its accuracy when coordinates are large compared with r
+    const float4* lo;
 };
 
 // Field sum S(q) = sum_j det_j K(|G_j (q - xMean_j)|) over the 27-cell candidates of q (without
@@ -133,9 +136,15 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
         visited += hi - lo;
         for (int j = lo; j < hi; j++) {
             const auto a = rec.a[j];
-            const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+            R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
             const R d2 = dx * dx + dy * dy + dz * dz;
             if (!(d2 < a.w)) continue;
+            if (KAHAN && sizeof(R) == 4) {
+                const float4 l = fc.lo[j];
+                dx -= R(l.x);
+                dy -= R(l.y);
+                dz -= R(l.z);
+            }
             if (fc.use_ani) {
                 const auto g0 = rec.g[2 * j];
                 const auto g1 = rec.g[2 * j + 1];

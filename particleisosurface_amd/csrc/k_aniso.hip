@@ -28,9 +28,12 @@ __global__ void __launch_bounds__(256) k_xmean(GridDev g, const float4* __restri
     const R px = pp.x, py = pp.y, pz = pp.z;
     if (cnt <= 2) {  // fewer than 3 candidates: xMean = position (src/evaluator.cpp:672-683)
         xm[p] = mk4(px, py, pz, R(0));
+        if (K && sizeof(R) == 4) ((float4*)xm)[n + p] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
     const R nI = ap.nI, nI2 = ap.nI2, inv_h3 = R(1) / (nI * nI * nI);
+    // compensated FP32 sums the offsets x_j - x_i (exact for nearby floats) instead of x_j
+    constexpr bool REL = K && sizeof(R) == 4;
     Acc<R, K> sx, sy, sz, sw;
     for (int r = 0; r < 9; r++) {
         for (int j = lo[r]; j < hi[r]; j++) {
@@ -41,13 +44,34 @@ __global__ void __launch_bounds__(256) k_xmean(GridDev g, const float4* __restri
             if (d2 >= nI2) continue;
             const R w = wij<R>(dsqrt<R>(d2), nI, inv_h3);
             sw.add(w);
-            sx.add(R(q.x) * w);
-            sy.add(R(q.y) * w);
-            sz.add(R(q.z) * w);
+            sx.add((REL ? dx : R(q.x)) * w);
+            sy.add((REL ? dy : R(q.y)) * w);
+            sz.add((REL ? dz : R(q.z)) * w);
         }
     }
     R ox = px, oy = py, oz = pz;
     const R ws = sw.get();
+    if (REL) {
+        float4 lo4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!ap.use_poly6 && ws > R(0)) {
+            // xMean = x + b (mean - x) with b = 0.9 as a float; kept as hi + lo
+            const float b = ap.xmean_delta;
+            const float d[3] = {float(sx.get() / ws) * b, float(sy.get() / ws) * b, float(sz.get() / ws) * b};
+            const float x[3] = {pp.x, pp.y, pp.z};
+            float h[3], l[3];
+            for (int a = 0; a < 3; a++) {
+                h[a] = __fadd_rn(x[a], d[a]);
+                l[a] = __fadd_rn(__fsub_rn(x[a], h[a]), d[a]);
+            }
+            ox = h[0];
+            oy = h[1];
+            oz = h[2];
+            lo4 = make_float4(l[0], l[1], l[2], 0.f);
+        }
+        xm[p] = mk4(ox, oy, oz, R(0));
+        ((float4*)xm)[n + p] = lo4;
+        return;
+    }
     if (!ap.use_poly6 && ws > R(0)) {
         const R a = R(1.0f - ap.xmean_delta), b = R(ap.xmean_delta);
         ox = px * a + (sx.get() / ws) * b;
@@ -64,12 +88,13 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
                                               const typename R4<R>::T* __restrict__ xm,
                                               typename R4<R>::T* __restrict__ rec_a,
                                               typename R4<R>::T* __restrict__ rec_g,
-                                              unsigned long long* __restrict__ n_fp64) {
+                                              float4* __restrict__ rec_lo, unsigned long long* __restrict__ n_fp64) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const auto xi = xm[p];
     const R h2 = R(ap.h2);
     if (!ap.use_ani) {  // isotropic poly6 support |d|^2 <= h^2
+        if (K && sizeof(R) == 4) rec_lo[p] = ((const float4*)xm)[n + p];
         rec_a[p] = mk4(xi.x, xi.y, xi.z, h2 * R(1.001));
         rec_g[2 * p] = mk4(R(1), R(0), R(0), R(1));
         rec_g[2 * p + 1] = mk4(R(0), R(1), R(1), R(0));
@@ -79,6 +104,8 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
     int ix, iy, iz;
     cell_of(g, pp.x, pp.y, pp.z, ix, iy, iz);
     const R nI = ap.nI, nI2 = ap.nI2, inv_h3 = R(1) / (nI * nI * nI);
+    constexpr bool REL = K && sizeof(R) == 4;
+    const float4 li = REL ? ((const float4*)xm)[n + p] : make_float4(0.f, 0.f, 0.f, 0.f);
     Acc<R, K> c[6], sw;
     const R base = ap.use_poly6 ? R(1.5) : h2;
     for (int r = 0; r < 9; r++) {
@@ -86,7 +113,13 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
         if (!row_range(g, ix, iy, iz, r, lo, hi)) continue;
         for (int j = lo; j < hi; j++) {
             const auto xj = xm[j];
-            const R wx = xj.x - xi.x, wy = xj.y - xi.y, wz = xj.z - xi.z;
+            R wx = xj.x - xi.x, wy = xj.y - xi.y, wz = xj.z - xi.z;
+            if (REL) {  // (hi_j - hi_i) is exact for nearby floats; add the residuals
+                const float4 lj = ((const float4*)xm)[n + j];
+                wx += R(lj.x - li.x);
+                wy += R(lj.y - li.y);
+                wz += R(lj.z - li.z);
+            }
             R w;
             if (!ap.use_poly6) {
                 const R d2 = wx * wx + wy * wy + wz * wz;
@@ -132,6 +165,7 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
     rec_a[p] = mk4(xi.x, xi.y, xi.z, R2);
     rec_g[2 * p] = mk4(G[0], G[1], G[2], G[3]);
     rec_g[2 * p + 1] = mk4(G[4], G[5], det, R(0));
+    if (REL) rec_lo[p] = li;
 }
 
 void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, void* xm, hipStream_t s) {
@@ -143,16 +177,18 @@ void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, AnisoParam
 }
 
 void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, const void* xm, void* rec_a,
-                 void* rec_g, unsigned long long* n_fp64, hipStream_t s) {
+                 void* rec_g, void* rec_lo, unsigned long long* n_fp64, hipStream_t s) {
     if (n <= 0) return;
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (prec == 2)
         k_gmat<double, false><<<nb, 256, 0, s>>>(g, spos, n, ap, (const double4*)xm, (double4*)rec_a, (double4*)rec_g,
-                                                 n_fp64);
+                                                 nullptr, n_fp64);
     else if (prec == 1)
-        k_gmat<float, true><<<nb, 256, 0, s>>>(g, spos, n, ap, (const float4*)xm, (float4*)rec_a, (float4*)rec_g, n_fp64);
+        k_gmat<float, true><<<nb, 256, 0, s>>>(g, spos, n, ap, (const float4*)xm, (float4*)rec_a, (float4*)rec_g,
+                                              (float4*)rec_lo, n_fp64);
     else
-        k_gmat<float, false><<<nb, 256, 0, s>>>(g, spos, n, ap, (const float4*)xm, (float4*)rec_a, (float4*)rec_g, n_fp64);
+        k_gmat<float, false><<<nb, 256, 0, s>>>(g, spos, n, ap, (const float4*)xm, (float4*)rec_a, (float4*)rec_g,
+                                              (float4*)rec_lo, n_fp64);
 }
 
 }  // namespace pisurf

@@ -43,7 +43,7 @@ struct AnisoParams {
 void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, void* xm /*R x4*/,
                   hipStream_t s);
 void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, const void* xm, void* rec_a,
-                 void* rec_g, unsigned long long* n_fp64_eig, hipStream_t s);
+                 void* rec_g, void* rec_lo, unsigned long long* n_fp64_eig, hipStream_t s);
 
 // ---- field ----
 struct Counters {

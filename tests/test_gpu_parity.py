@@ -1,0 +1,400 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Bars: bit-exact for binning, cell tables, neighbour / box lists, octree ids and node types;
+density sums S = iso - f within stated relative tolerances; signs equal outside |f| <= 2e-5 iso;
+vertex positions within 1e-3 r (ties of the bisection's sign / snap decisions are counted and
+must be rare). FP64 is compared with the FP64 oracle, compensated FP32 with the FP64 oracle,
+plain FP32 with the FP32-faithful oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle
+from particleisosurface_amd import capi, synth
+from tests.conftest import gpu_context
+
+pytestmark = pytest.mark.gpu
+
+PREC = {"fp32": capi.FP32, "fp32k": capi.FP32_KAHAN, "fp64": capi.FP64}
+
+
+def max_face_lattice():
+    g = np.arange(4, dtype=np.float32)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32), 0.25
+
+
+def planar_layer():
+    r = 1.0 / 128
+    g = np.arange(16) * 2 * r
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), np.zeros(256)], 1).astype(np.float32), r
+
+
+CASES = {
+    "cfg1": lambda: synth.dam_break(),
+    "cloud": lambda: synth.random_cloud(20000, r=0.004, extent=0.2),
+    "maxface": max_face_lattice,
+    "planar": planar_layer,
+    "sphere": lambda: synth.lattice_sphere(),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_hashgrid_bit_exact(case):
+    xyz, r = CASES[case]()
+    o = Oracle(xyz, r)
+    go = o.hashgrid_build()
+    ctx = gpu_context(xyz, r)
+    g = ctx.hashgrid_build()
+    assert np.array_equal(g["bounding"], go["bounding"])
+    assert np.array_equal(g["cellnum3"], go["cellnum3"])
+    assert g["cellnum"] == go["cellnum"] and g["n_invalid"] == go["n_invalid"]
+    il, s, e = ctx.hashgrid_download(g["cellnum"])
+    assert np.array_equal(il, o.index_list())
+    keys, os_, oe = o.cells()
+    assert np.array_equal(s[keys], os_) and np.array_equal(e[keys], oe)
+    empty = np.ones(g["cellnum"], bool)
+    empty[keys] = False
+    assert np.array_equal(s[empty], e[empty])
+    if case == "maxface":
+        assert g["n_invalid"] > 0
+
+
+@pytest.mark.parametrize("case", ["cfg1", "maxface", "cloud"])
+def test_neighbors_and_boxes_bit_exact(case):
+    xyz, r = CASES[case]()
+    o = Oracle(xyz, r)
+    gi = o.hashgrid_build()
+    ctx = gpu_context(xyz, r)
+    ctx.hashgrid_build()
+    rng = np.random.default_rng(7)
+    q = (xyz[rng.choice(len(xyz), 500)] + rng.uniform(-6 * r, 6 * r, (500, 3))).astype(np.float32)
+    b = gi["bounding"]
+    extra = np.array([[b[0] - 0.5 * gi["cellsize"], b[2], b[4]], [1e6, 1e6, 1e6], [-1e6, 0, 0],
+                      [b[1], b[3], b[5]], [b[1] + 1.5 * gi["cellsize"], b[3], b[5]]], np.float32)
+    q = np.concatenate([q, extra])
+    off, ids = ctx.neighbors(q)
+    off2, ids2 = o.neighbors(q)
+    assert np.array_equal(off, off2) and np.array_equal(ids, ids2)
+    for k in range(30):
+        lo = (xyz[rng.integers(len(xyz))] - rng.uniform(0, 8 * r, 3)).astype(np.float32)
+        hi = (lo + rng.uniform(0, 16 * r, 3)).astype(np.float32)
+        if k == 0:
+            lo, hi = np.array([-1e3] * 3, np.float32), np.array([1e3] * 3, np.float32)
+        assert np.array_equal(ctx.box_particles(lo, hi), o.box_particles(lo, hi))
+
+
+def _pair(xyz, r, prec, **kw):
+    o = Oracle(xyz, r, fp64=(prec != "fp32"), **kw)
+    ctx = gpu_context(xyz, r, precision=PREC[prec], **kw)
+    return o, ctx
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp32k", "fp64"])
+@pytest.mark.parametrize("case", ["cfg1", "planar", "sphere"])
+def test_anisotropy(case, prec):
+    xyz, r = CASES[case]()
+    o, ctx = _pair(xyz, r, prec)
+    o.anisotropy()
+    ctx.anisotropy()
+    xm, G, d = ctx.aniso()
+    xm2, G2, d2 = o.aniso()
+    ulp = np.spacing(np.float32(np.abs(xyz).max()))
+    tol = {"fp32": (4 * ulp + 1e-6 * r, 5e-5, 2e-5), "fp32k": (1e-5 * r, 2e-5, 2e-5),
+           "fp64": (1e-12, 1e-9, 1e-9)}[prec]
+    assert np.max(np.abs(xm - xm2)) <= tol[0]
+    assert np.max(np.abs(G - G2)) <= tol[1]
+    assert np.max(np.abs(d - d2)) <= tol[2]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_iso_constants(prec):
+    xyz, r = synth.dam_break()
+    o, ctx = _pair(xyz, r, prec)
+    a, b = ctx.iso_constants()
+    a2, b2 = o.iso_constants()
+    if prec == "fp32":
+        assert (a, b) == (a2, b2)
+    else:
+        assert abs(a / a2 - 1) < 1e-15 and abs(b / b2 - 1) < 1e-15
+
+
+def _queries(xyz, r, n, seed=1):
+    rng = np.random.default_rng(seed)
+    return (xyz[rng.choice(len(xyz), n)] + rng.uniform(-3 * r, 3 * r, (n, 3))).astype(np.float32)
+
+
+# Field tolerance is stated on |dS| / max(S, iso): near the surface (S ~ iso) and inside it is the
+# relative error of S; far outside (S << iso) it bounds the absolute error by a fraction of iso,
+# where the relative error of S is dominated by support-edge terms and is not a meaningful
+# measure (the reference's own FP32 arithmetic reaches |dS|/S ~ 2e-4 there and 1.4e-5 at S >= iso).
+# FP32 against the FP32-faithful oracle therefore allows twice that noise floor, and signs are
+# compared outside |f| <= 5e-5 iso; compensated FP32 and FP64 are held to the FP64 oracle.
+FIELD_TOL = {"fp32": (3e-5, 5e-5), "fp32k": (1e-5, 2e-5), "fp64": (1e-10, 1e-9)}
+
+
+@pytest.mark.parametrize("prec,ref", [("fp32", "fp32"), ("fp32k", "fp64"), ("fp64", "fp64")])
+def test_field_values_and_signs(prec, ref):
+    rel, eps = FIELD_TOL[prec]
+    xyz, r = synth.dam_break()
+    o = Oracle(xyz, r, fp64=(ref == "fp64"))
+    o.anisotropy()
+    ms, iso = o.iso_constants()
+    ctx = gpu_context(xyz, r, precision=PREC[prec])
+    q = _queries(xyz, r, 50000)
+    f, g = ctx.eval_points(q, grad=True)
+    f2, g2 = o.eval_points(q, grad=True)
+    S, S2 = iso - f, iso - f2
+    err = np.abs(S - S2) / np.maximum(np.abs(S2), iso)
+    assert err.max() <= rel, float(err.max())
+    tie = np.abs(f2) <= eps * iso
+    assert np.array_equal((f > 0)[~tie], (f2 > 0)[~tie])
+    gn = np.linalg.norm(g2, axis=1)
+    ok = gn > 1e-3 * gn.max()
+    cosang = np.sum(g * g2, 1)[ok] / (np.linalg.norm(g, axis=1)[ok] * gn[ok])
+    ang = np.degrees(np.arccos(np.clip(cosang, -1, 1)))
+    assert np.quantile(ang, 0.99) < {"fp32": 2e-3, "fp32k": 2e-4, "fp64": 1e-6}[prec]
+    assert ang.max() < {"fp32": 5e-2, "fp32k": 5e-3, "fp64": 1e-5}[prec]
+
+
+@pytest.mark.parametrize("prec,ref", [("fp32", "fp32"), ("fp64", "fp64")])
+def test_particle_normals(prec, ref):
+    xyz, r = synth.dam_break()
+    o = Oracle(xyz, r, fp64=(ref == "fp64"))
+    o.anisotropy()
+    o.iso_constants()
+    n2 = o.particle_normals()
+    ctx = gpu_context(xyz, r, precision=PREC[prec])
+    n = ctx.particle_normals()
+    scale = np.linalg.norm(n2, axis=1).max()
+    assert np.max(np.abs(n - n2)) <= (5e-5 if prec == "fp32" else 1e-7) * scale
+
+
+def _tie_nodes(o, centers, halfs, curv, iso, eps):
+    """nodes whose classification depends on a near-tie: a sample within eps * iso of zero, or a
+    curvature within 2e-5 of the 0.99 threshold"""
+    tie = np.abs(curv.astype(np.float64) - 0.99) < 2e-4
+    for i in range(len(halfs)):
+        s, _, _ = o.grid_eval(centers[i], halfs[i])
+        tie[i] |= np.any(np.abs(s[:, 3]) <= eps * iso)
+    return tie
+
+
+@pytest.mark.parametrize("case", ["cfg1", "sphere"])
+def test_level_sweeps_seeded_with_oracle_nodes(case):
+    xyz, r = CASES[case]()
+    o, ctx = _pair(xyz, r, "fp32")
+    o.anisotropy()
+    ms, iso = o.iso_constants()
+    o.particle_normals()
+    o.build_tree()
+    nd = o.tree_nodes()
+    t, f, curv, sc = ctx.sweep_level(nd["center"], nd["half"], nd["depth"])
+    t2, f2, curv2, sc2 = o.classify_nodes(nd["center"], nd["half"], nd["depth"])
+    assert np.array_equal(t2, nd["type"])
+    info = o.tree_info()
+    deep = nd["depth"] >= info["depth_min"]
+    # the sign-change flag is only consumed between D_min and D_max - 1: the GPU skips the
+    # reference's dead sampling of leaf-depth nodes, so their flag is not compared
+    sampled = deep & (nd["depth"] < info["depth_max"])
+    mism = (t != t2) | ((sc != sc2) & sampled)
+    if mism.any():
+        tie = _tie_nodes(o, nd["center"][mism], nd["half"][mism], curv2[mism], iso, FIELD_TOL["fp32"][1])
+        assert tie.all(), np.nonzero(mism)[0][~tie]
+        assert mism.sum() <= 1e-3 * len(t) + 2
+    # curvature = |sum n| / sum |n| over the node's particles: FP32 normals differ by ~5e-5
+    # relative and the GPU sums per lane before a wave reduction
+    assert np.max(np.abs(curv - curv2)[deep & (t2 != 0)]) < 2e-4
+    leaf = (t2 == capi.LEAF) & ~mism
+    S, S2 = iso - f[leaf], iso - f2[leaf]
+    assert np.all(np.abs(S - S2) <= FIELD_TOL["fp32"][0] * np.maximum(np.abs(S2), iso))
+    assert np.all(f[t2 == capi.EMPTY] == iso)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["cfg1", "sphere"])
+def test_full_tree_and_mesh(case, prec):
+    xyz, r = CASES[case]()
+    o, ctx = _pair(xyz, r, prec)
+    o.run()
+    info = ctx.run_frame()
+    oi = o.tree_info()
+    assert (info["depth_max"], info["depth_min"], info["n_nodes"]) == (oi["depth_max"], oi["depth_min"], oi["n_nodes"])
+    nd = ctx.tree_nodes(info["n_nodes"])
+    nd2 = o.tree_nodes()
+    for k in ("type", "id", "depth", "child"):
+        assert np.array_equal(nd[k], nd2[k]), k
+    assert np.array_equal(nd["center"], nd2["center"]) and np.array_equal(nd["half"], nd2["half"])
+    ms, iso = o.iso_constants()
+    leaf = nd2["type"] == capi.LEAF
+    S, S2 = iso - nd["node"][leaf, 3], iso - nd2["node"][leaf, 3]
+    assert np.all(np.abs(S - S2) <= FIELD_TOL[prec][0] * np.maximum(np.abs(S2), iso))
+    v, t = ctx.mesh()
+    v2, t2 = o.mesh()
+    assert v.shape == v2.shape and np.array_equal(t, t2)
+    d = np.linalg.norm(v - v2, axis=1) / r
+    if prec == "fp64":
+        assert d.max() < 1e-6
+    else:
+        assert np.mean(d <= 1e-3) >= 0.995, np.mean(d <= 1e-3)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_bisection_on_oracle_edges(prec):
+    xyz, r = synth.dam_break()
+    o, ctx = _pair(xyz, r, prec)
+    o.run()
+    nd = o.tree_nodes()
+    n8, sidx = o.dual_cells()
+    ends = []
+    for cell in n8[:3000]:
+        for a, b in ((0, 1), (0, 2), (0, 4), (3, 7), (5, 7), (6, 7)):
+            fa, fb = nd["node"][cell[a], 3], nd["node"][cell[b], 3]
+            if (fa > 0) != (fb > 0):
+                ends.append([nd["node"][cell[a]], nd["node"][cell[b]]])
+    ends = np.array(ends)
+    assert len(ends) > 1000
+    out, nev = ctx.bisect_edges(ends)
+    out2, nev2 = o.bisect_edges(ends)
+    d = np.linalg.norm(out - out2, axis=1) / r
+    same_path = nev == nev2
+    assert np.mean(same_path) > 0.999
+    if prec == "fp64":
+        assert np.array_equal(nev, nev2) and d.max() < 1e-9
+    else:
+        assert np.mean(d <= 1e-3) > 0.995
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_qef_solve(mode):
+    xyz, r = synth.dam_break()
+    prec = "fp32" if mode == 1 else "fp64"
+    o, ctx = _pair(xyz, r, prec, qef_svd_tol=1e-3)
+    o.anisotropy()
+    o.iso_constants()
+    o.particle_normals()
+    o.build_tree()
+    nd = o.tree_nodes()
+    info = o.tree_info()
+    sel = np.nonzero((nd["type"] == capi.LEAF) & (nd["depth"] >= info["depth_min"]))[0][:400]
+    samples, grads = [], []
+    for i in sel:
+        s, g, _ = o.grid_eval(nd["center"][i], nd["half"][i])
+        samples.append(s)
+        grads.append(g)
+    samples, grads = np.array(samples), np.array(grads)
+    centers, halfs = nd["center"][sel], nd["half"][sel]
+    # plus consistent synthetic systems: f_i = w* - g_i . (x* - p_i) with x* inside the cell, so
+    # the exact QEF minimiser is (x*, w*)
+    rng = np.random.default_rng(11)
+    k = 200
+    syn_c = centers[:k]
+    syn_h = halfs[:k]
+    syn_s = samples[:k].copy()
+    syn_g = rng.normal(size=(k, 27, 3))
+    xs = syn_c + rng.uniform(-0.8, 0.8, (k, 3)) * syn_h[:, None]
+    ws = rng.normal(size=k) * float(syn_h[0])
+    syn_s[:, :, 3] = ws[:, None] - np.einsum("kia,kia->ki", syn_g, xs[:, None, :] - syn_s[:, :, :3])
+    centers = np.concatenate([centers, syn_c])
+    halfs = np.concatenate([halfs, syn_h])
+    samples = np.concatenate([samples, syn_s])
+    grads = np.concatenate([grads, syn_g])
+    out = ctx.qef_solve(centers, halfs, samples, grads, mode)
+    out2 = np.array([o.qef(centers[i], halfs[i], samples[i], grads[i], mode) for i in range(len(halfs))])
+    moved = np.any(out2[:, :3] != centers, axis=1)
+    assert moved.sum() >= 20
+    d = np.linalg.norm(out[:, :3] - out2[:, :3], axis=1) / r
+    if mode == 2:
+        assert d.max() < 1e-6
+    else:
+        # float 4x4 inverse: inputs agree exactly, the inverses differ only by rounding
+        assert np.mean(d < 1e-3) > 0.99
+
+
+def test_qef_frame_matches_oracle():
+    xyz, r = synth.lattice_sphere()
+    o, ctx = _pair(xyz, r, "fp64", qef_mode=2)
+    o.run()
+    info = ctx.run_frame()
+    nd = ctx.tree_nodes(info["n_nodes"])
+    nd2 = o.tree_nodes()
+    assert np.array_equal(nd["type"], nd2["type"])
+    leaf = nd2["type"] == capi.LEAF
+    assert np.max(np.abs(nd["node"][leaf, :3] - nd2["node"][leaf, :3])) < 1e-6 * r
+    assert ctx.stats()["qef_solves"] > 0
+
+
+def test_consecutive_frames_do_not_leak_state():
+    a, ra = synth.lattice_sphere(radius=0.05, r=0.005)
+    b, rb = synth.dam_break()
+    ctx = gpu_context(a, ra)
+    ctx.run_frame()
+    ctx.set_params(rb)
+    ctx.upload(b)
+    info = ctx.run_frame()
+    fresh = gpu_context(b, rb)
+    info2 = fresh.run_frame()
+    assert info == info2
+    v, t = ctx.mesh()
+    v2, t2 = fresh.mesh()
+    assert np.array_equal(v, v2) and np.array_equal(t, t2)
+
+
+def test_error_paths():
+    xyz, r = synth.dam_break()
+    ctx = capi.Context(0)
+    with pytest.raises(capi.PisurfError):
+        ctx.set_params(0.0)
+    ctx.set_params(r)
+    with pytest.raises(capi.PisurfError, match="no particles"):
+        ctx.run_frame()
+    with pytest.raises(capi.PisurfError, match="not built"):
+        ctx.extract()
+    bad = xyz.copy()
+    bad[5, 1] = np.nan
+    ctx.upload(bad)
+    with pytest.raises(capi.PisurfError, match="NaN"):
+        ctx.hashgrid_build()
+    huge = np.array([[0, 0, 0], [1e3, 1e3, 1e3]], np.float32)
+    ctx.set_params(1e-4)
+    ctx.upload(huge)
+    with pytest.raises(capi.PisurfError, match="cell count"):
+        ctx.hashgrid_build()
+    with pytest.raises(capi.PisurfError):
+        ctx.bisect_edges(np.array([[[0, 0, 0, np.inf], [1, 1, 1, 1.0]]]))
+    # the context is still usable after errors
+    ctx.set_params(r)
+    ctx.upload(xyz)
+    assert ctx.run_frame()["n_nodes"] > 0
+
+
+def test_large_frame_invariants():
+    """Sloshing tank (~1.59 M particles): size-independent properties at full size."""
+    xyz, r = synth.sloshing_tank()
+    ctx = gpu_context(xyz, r, precision=capi.FP32_KAHAN)
+    g = ctx.hashgrid_build()
+    il, s, e = ctx.hashgrid_download(g["cellnum"])
+    assert np.array_equal(np.sort(il), np.arange(len(xyz)))
+    assert int(np.sum(e - s)) == len(xyz) - g["n_invalid"]
+    assert np.all(s[1:] == e[:-1])
+    info = ctx.run_frame()
+    nd = ctx.tree_nodes(info["n_nodes"])
+    internal = nd["type"] == capi.INTERNAL
+    assert info["n_nodes"] == 1 + 8 * internal.sum()
+    assert np.all(nd["child"][internal] > np.nonzero(internal)[0])
+    v, t = ctx.mesh()
+    assert len(t) > 100000 and t.min() == 1 and t.max() == len(v)
+    # edges: the adaptive dual-marching-cubes of the reference is not guaranteed manifold, but
+    # almost every edge joins exactly two triangles
+    e = np.sort(np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]]), axis=1).astype(np.int64)
+    _, cnt = np.unique(e[:, 0] * (len(v) + 1) + e[:, 1], return_counts=True)
+    assert np.mean(cnt == 2) > 0.98
+    # compensated FP32 against the all-FP64 path on the same frame
+    q = _queries(xyz, r, 100000, seed=4)
+    iso = ctx.iso_constants()[1]
+    S32 = iso - ctx.eval_points(q)
+    ctx.set_params(r, precision=capi.FP64)
+    S64 = iso - ctx.eval_points(q)
+    band = S64 > 1e-3 * iso
+    assert np.quantile(np.abs(S32 - S64)[band] / S64[band], 0.999) < 1e-5
