@@ -56,7 +56,9 @@ typedef struct pisurf_params {
     int32_t qef_mode;          /* PISURF_QEF_* */
     float qef_svd_tol;         /* relative singular-value cutoff of PISURF_QEF_SVD64 */
     int32_t depth_max_override;/* 0: derived D_max (resizeRootBoxConstR) */
-    int32_t reserved[5];
+    int32_t mesh_order;        /* 0: extraction on the device, dual cells in point-key order;
+                                  1: host traversal in the reference's order (byte-identical OBJ) */
+    int32_t reserved[4];
 } pisurf_params;
 
 typedef struct pisurf_grid_info {

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libpisurf.so")
-SOURCES = ["k_grid.hip", "k_aniso.hip", "k_field.hip", "k_octree.hip", "k_vertex.hip", "capi.cpp"]
+SOURCES = ["k_grid.hip", "k_aniso.hip", "k_field.hip", "k_octree.hip", "k_vertex.hip", "k_extract.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PISURF_ARCH", "gfx950")
 FLAGS = [
@@ -44,6 +44,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(LIB + ".tmp", LIB)
+    # the C++ mirror of the reference classes, exercised by a small driver program
+    demo = os.path.join(HERE, "pisurf_facade_demo")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "facade_demo.cpp"), "-o", demo + ".tmp",
+           "-L" + HERE, "-lpisurf", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("facade demo build failed:\n" + r.stdout + r.stderr)
+    os.replace(demo + ".tmp", demo)
     return LIB
 
 

@@ -30,7 +30,7 @@ class Params(C.Structure):
         ("radius", C.c_float), ("smooth_factor", C.c_float), ("iso_factor", C.c_float),
         ("use_ani", C.c_int32), ("use_poly6", C.c_int32), ("calc_normals", C.c_int32), ("oversample", C.c_int32),
         ("precision", C.c_int32), ("qef_mode", C.c_int32), ("qef_svd_tol", C.c_float),
-        ("depth_max_override", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("depth_max_override", C.c_int32), ("mesh_order", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -107,10 +107,11 @@ class Context:
             raise PisurfError("%s failed (%d): %s" % (what, rc, self.L.pisurf_last_error(self.h).decode()))
 
     def set_params(self, radius, precision=FP32, use_ani=True, use_poly6=False, calc_normals=True, qef_mode=QEF_OFF,
-                   qef_svd_tol=1e-3, depth_max_override=0, smooth_factor=2.0, iso_factor=1.9):
+                   qef_svd_tol=1e-3, depth_max_override=0, smooth_factor=2.0, iso_factor=1.9, mesh_order=0):
         p = Params()
         self.L.pisurf_default_params(C.byref(p))
         p.radius = radius
+        p.mesh_order = mesh_order
         p.precision = precision
         p.use_ani = int(use_ani)
         p.use_poly6 = int(use_poly6)

@@ -42,10 +42,11 @@ struct DBuf {
 
 struct LevelBuf {
     int64_t n = 0;
-    DBuf c, depth, id, type, empty, curv, node, flag, child, scanned;
+    DBuf c, depth, id, type, empty, curv, node, flag, child, scanned, ip;
     hipError_t ensure(int64_t m) {
         hipError_t e;
         if ((e = c.ensure(m * sizeof(float4))) != hipSuccess) return e;
+        if ((e = ip.ensure(m * sizeof(int4))) != hipSuccess) return e;
         if ((e = depth.ensure(m)) != hipSuccess) return e;
         if ((e = id.ensure(m * 8)) != hipSuccess) return e;
         if ((e = type.ensure(m)) != hipSuccess) return e;
@@ -68,6 +69,7 @@ struct LevelBuf {
         v.node = node.as<double4>();
         v.flag = flag.as<int>();
         v.child = child.as<int>();
+        v.ip = ip.as<int4>();
         v.n = n;
         return v;
     }
@@ -171,6 +173,8 @@ struct pisurf_ctx {
     DBuf samples;
     // scratch for API calls
     DBuf q, out1, out2, out3;
+    // device extraction workspace
+    DBuf eb[24];
     // mesh
     std::vector<double> mesh_v;
     std::vector<int32_t> mesh_t;
@@ -542,6 +546,8 @@ int do_tree(pisurf_ctx* c) {
         const float4 rc4 = make_float4(c->root_c[0], c->root_c[1], c->root_c[2], c->root_half);
         const int8_t d0 = 0;
         const uint64_t id0 = 0;
+        const int4 ip0 = make_int4(0, 0, 0, 0);
+        HIPCHK(c, hipMemcpyAsync(L->ip.p, &ip0, sizeof(int4), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(L->c.p, &rc4, sizeof(float4), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(L->depth.p, &d0, 1, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(L->id.p, &id0, 8, hipMemcpyHostToDevice, c->stream));
@@ -730,8 +736,180 @@ struct Extractor {
     }
 };
 
+int bits_for(int64_t v) {
+    int b = 1;
+    while (b < 63 && ((int64_t)1 << b) <= v) b++;
+    return b;
+}
+
+int read_int(pisurf_ctx* c, const int* dptr, int64_t& out) {
+    int v = 0;
+    HIPCHK(c, hipMemcpyAsync(&v, dptr, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out = v;
+    return PISURF_OK;
+}
+
+int ensure_tmp(pisurf_ctx* c, int64_t n) {
+    const int64_t need = std::max(radix_tmp_ints(n), scan_tmp_ints(n + 1)) + 64;
+    if ((size_t)need * 4 > c->tmp.cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, c->tmp.ensure(need * 4));
+    }
+    return PISURF_OK;
+}
+
+// Extraction on the device (k_extract.hip): leaf corners -> sorted points -> dual cells ->
+// triangle corners -> vertices keyed by leaf pairs -> bisection -> triangle dedupe.
+int do_extract_device(pisurf_ctx* c) {
+    const double t0 = now_ms();
+    hipStream_t s = c->stream;
+    TreeView tv;
+    std::memset(&tv, 0, sizeof(tv));
+    int64_t T = 0;
+    for (auto* l : c->levels) {
+        if (l->n == 0) break;
+        if (tv.nlev >= kMaxLevels) return c->fail(PISURF_E_UNSUPPORTED, "too many octree levels");
+        tv.off[tv.nlev] = T;
+        tv.type[tv.nlev] = l->type.as<uint8_t>();
+        tv.child[tv.nlev] = l->child.as<int>();
+        T += l->n;
+        tv.nlev++;
+    }
+    tv.dp = tv.nlev - 1;
+    if (tv.dp > 21) return c->fail(PISURF_E_UNSUPPORTED, "octree deeper than 21 levels");
+    DBuf* E = c->eb;
+    int rc;
+    // leaves
+    HIPCHK(c, E[0].ensure((T + 1) * 4));
+    HIPCHK(c, E[1].ensure((T + 1) * 4));
+    for (int L = 0; L < tv.nlev; L++) launch_leaf_flag(c->levels[L]->view(), tv.off[L], E[0].as<int>(), s);
+    HIPCHK(c, hipMemsetAsync(E[0].as<int>() + T, 0, 4, s));
+    if ((rc = ensure_tmp(c, T + 1))) return rc;
+    exclusive_scan_i32(E[0].as<int>(), E[1].as<int>(), T + 1, c->tmp.as<int>(), s);
+    int64_t nleaf;
+    if ((rc = read_int(c, E[1].as<int>() + T, nleaf))) return rc;
+    const int* lidx = E[1].as<int>();
+    // corner keys
+    const int64_t nent_all = 8 * nleaf;
+    HIPCHK(c, E[2].ensure(std::max<int64_t>(nleaf, 1) * sizeof(double4)));
+    HIPCHK(c, E[3].ensure(std::max<int64_t>(nent_all, 1) * 8));
+    HIPCHK(c, E[4].ensure(std::max<int64_t>(nent_all, 1) * 4));
+    HIPCHK(c, E[5].ensure(std::max<int64_t>(nent_all, 1) * 8));
+    HIPCHK(c, E[6].ensure(std::max<int64_t>(nent_all, 1) * 4));
+    HIPCHK(c, E[7].ensure(64));
+    HIPCHK(c, hipMemsetAsync(E[7].p, 0, 64, s));
+    double4* leaf_node = E[2].as<double4>();
+    for (int L = 0; L < tv.nlev; L++)
+        launch_leaf_emit(c->levels[L]->view(), tv.off[L], lidx, tv.dp, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(),
+                         E[7].as<int>(), s);
+    if ((rc = ensure_tmp(c, nent_all + 1))) return rc;
+    radix_sort_pairs64(E[3].as<uint64_t>(), E[4].as<int>(), E[5].as<uint64_t>(), E[6].as<int>(), nent_all,
+                       std::min(64, 3 * tv.dp + 1), c->tmp.as<int>(), s);
+    HIPCHK(c, hipGetLastError());
+    int64_t nent;
+    if ((rc = read_int(c, E[7].as<int>(), nent))) return rc;
+    // points -> dual cells
+    HIPCHK(c, E[8].ensure((nent + 1) * 4));
+    HIPCHK(c, E[9].ensure((nent + 1) * 4));
+    launch_points(tv, lidx, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(), nent, E[8].as<int>(), nullptr, nullptr, s);
+    HIPCHK(c, hipMemsetAsync(E[8].as<int>() + nent, 0, 4, s));
+    exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nent + 1, c->tmp.as<int>(), s);
+    int64_t nd;
+    if ((rc = read_int(c, E[9].as<int>() + nent, nd))) return rc;
+    HIPCHK(c, E[10].ensure(std::max<int64_t>(nd, 1) * 9 * 4));
+    launch_points(tv, lidx, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(), nent, E[8].as<int>(), E[9].as<int>(),
+                  E[10].as<int>(), s);
+    // triangles
+    HIPCHK(c, E[11].ensure((nd + 1) * 4));
+    HIPCHK(c, E[12].ensure((nd + 1) * 4));
+    launch_tri_count(E[10].as<int>(), nd, E[11].as<int>(), s);
+    HIPCHK(c, hipMemsetAsync(E[11].as<int>() + nd, 0, 4, s));
+    exclusive_scan_i32(E[11].as<int>(), E[12].as<int>(), nd + 1, c->tmp.as<int>(), s);
+    int64_t nt;
+    if ((rc = read_int(c, E[12].as<int>() + nd, nt))) return rc;
+    const int64_t nc = 3 * nt;
+    HIPCHK(c, E[13].ensure(std::max<int64_t>(nc, 1) * 8));
+    HIPCHK(c, E[14].ensure(std::max<int64_t>(nc, 1) * 4));
+    HIPCHK(c, E[15].ensure(std::max<int64_t>(nc, 1) * 8));
+    HIPCHK(c, E[16].ensure(std::max<int64_t>(nc, 1) * 4));
+    HIPCHK(c, E[17].ensure(std::max<int64_t>(nc, 1) * 8));
+    launch_tri_corners(E[10].as<int>(), nd, E[12].as<int>(), E[13].as<uint64_t>(), E[14].as<int>(), E[17].as<int>(), s);
+    if ((rc = ensure_tmp(c, nc + 1))) return rc;
+    radix_sort_pairs64(E[13].as<uint64_t>(), E[14].as<int>(), E[15].as<uint64_t>(), E[16].as<int>(), nc,
+                       32 + bits_for(nleaf), c->tmp.as<int>(), s);
+    // vertices
+    HIPCHK(c, E[18].ensure((nc + 1) * 4));
+    HIPCHK(c, E[19].ensure((nc + 1) * 4));
+    launch_run_heads(E[13].as<uint64_t>(), nc, E[18].as<int>(), s);
+    HIPCHK(c, hipMemsetAsync(E[18].as<int>() + nc, 0, 4, s));
+    exclusive_scan_i32(E[18].as<int>(), E[19].as<int>(), nc + 1, c->tmp.as<int>(), s);
+    int64_t nv;
+    if ((rc = read_int(c, E[19].as<int>() + nc, nv))) return rc;
+    HIPCHK(c, E[20].ensure(std::max<int64_t>(nc, 1) * 4));
+    HIPCHK(c, E[21].ensure(std::max<int64_t>(nv, 1) * 8 * 8));
+    int* cvid = E[20].as<int>();
+    launch_vertex_ids(E[13].as<uint64_t>(), E[14].as<int>(), nc, E[19].as<int>(), E[17].as<int>(), leaf_node, cvid,
+                      E[21].as<double>(), s);
+    HIPCHK(c, hipGetLastError());
+    c->st.ms_extract_host += now_ms() - t0;  // device extraction time (topology)
+    const double t1 = now_ms();
+    HIPCHK(c, E[22].ensure(std::max<int64_t>(nv, 1) * 3 * 8));
+    launch_bisect(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->prm.radius / 2, E[21].as<double>(), nv,
+                  E[22].as<double>(), nullptr, c->ctr(), s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->st.ms_bisect += now_ms() - t1;
+    const double t2 = now_ms();
+    // triangle dedupe by sorted vertex triple; the first triangle of a repeated triple stays
+    uint32_t* tk = (uint32_t*)E[3].p;  // reuse the corner-key buffers (>= 8 * nleaf entries)
+    int64_t ntk = nt;
+    HIPCHK(c, E[3].ensure(std::max<int64_t>(ntk, 1) * 8));
+    HIPCHK(c, E[4].ensure(std::max<int64_t>(ntk, 1) * 4));
+    HIPCHK(c, E[5].ensure(std::max<int64_t>(ntk, 1) * 8));
+    HIPCHK(c, E[6].ensure(std::max<int64_t>(ntk, 1) * 4));
+    HIPCHK(c, E[8].ensure((ntk + 1) * 4));
+    HIPCHK(c, E[9].ensure((ntk + 1) * 4));
+    tk = (uint32_t*)E[3].p;
+    launch_tri_key_hi(cvid, nt, tk, E[4].as<int>(), s);
+    radix_sort_pairs(tk, E[4].as<int>(), (uint32_t*)E[5].p, E[6].as<int>(), nt, bits_for(nv), c->tmp.as<int>(), s);
+    // second pass keys go to E[5]/E[6] (E[4] holds the first permutation)
+    HIPCHK(c, E[16].ensure(std::max<int64_t>(ntk, 1) * 8));
+    HIPCHK(c, E[15].ensure(std::max<int64_t>(ntk, 1) * 8));
+    launch_tri_key_lo(cvid, E[4].as<int>(), nt, E[5].as<uint64_t>(), E[6].as<int>(), s);
+    radix_sort_pairs64(E[5].as<uint64_t>(), E[6].as<int>(), E[15].as<uint64_t>(), E[16].as<int>(), nt,
+                       32 + bits_for(nv), c->tmp.as<int>(), s);
+    launch_tri_keep(cvid, E[6].as<int>(), nt, E[8].as<int>(), s);
+    HIPCHK(c, hipMemsetAsync(E[8].as<int>() + nt, 0, 4, s));
+    exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nt + 1, c->tmp.as<int>(), s);
+    int64_t ntf;
+    if ((rc = read_int(c, E[9].as<int>() + nt, ntf))) return rc;
+    HIPCHK(c, E[23].ensure(std::max<int64_t>(ntf, 1) * 3 * 4));
+    launch_tri_out(cvid, E[8].as<int>(), E[9].as<int>(), nt, E[23].as<int>(), s);
+    HIPCHK(c, hipGetLastError());
+    c->mesh_v.resize(nv * 3);
+    c->mesh_t.resize(ntf * 3);
+    if (nv) HIPCHK(c, hipMemcpyAsync(c->mesh_v.data(), E[22].p, nv * 3 * 8, hipMemcpyDeviceToHost, s));
+    if (ntf) HIPCHK(c, hipMemcpyAsync(c->mesh_t.data(), E[23].p, ntf * 3 * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->st.ms_extract_host += now_ms() - t2;
+    c->st.n_dual_cells = nd;
+    c->st.n_verts = nv;
+    c->st.n_tris = ntf;
+    return PISURF_OK;
+}
+
+int do_extract(pisurf_ctx* c);
+
+int do_extract_host(pisurf_ctx* c);
+
 int do_extract(pisurf_ctx* c) {
     if (!c->tree_done) return c->fail(PISURF_E_STATE, "octree not built");
+    if (c->prm.mesh_order == 0) return do_extract_device(c);
+    return do_extract_host(c);
+}
+
+int do_extract_host(pisurf_ctx* c) {
     double t0 = now_ms();
     HostTree t;
     int rc = download_tree(c, t, false);
@@ -889,7 +1067,8 @@ int pisurf_set_params(pisurf_ctx* c, const pisurf_params* p) {
     if (p->oversample != 2) return c->fail(PISURF_E_UNSUPPORTED, "only oversample = 2 (27 samples) is supported");
     if (p->precision < 0 || p->precision > 2) return c->fail(PISURF_E_INVALID, "bad precision");
     if (p->qef_mode < 0 || p->qef_mode > 2) return c->fail(PISURF_E_INVALID, "bad qef_mode");
-    if (p->depth_max_override < 0 || p->depth_max_override > 24) return c->fail(PISURF_E_INVALID, "bad depth override");
+    if (p->depth_max_override < 0 || p->depth_max_override > 21) return c->fail(PISURF_E_INVALID, "bad depth override");
+    if (p->mesh_order < 0 || p->mesh_order > 1) return c->fail(PISURF_E_INVALID, "bad mesh_order");
     const bool geo = p->radius != c->prm.radius || p->use_ani != c->prm.use_ani || p->use_poly6 != c->prm.use_poly6 ||
                      p->precision != c->prm.precision;
     c->prm = *p;

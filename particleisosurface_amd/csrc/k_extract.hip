@@ -1,0 +1,278 @@
+// Dual-cell enumeration and mesh assembly on the device (genIsoOurs state 1,
+// src/surface_reconstructor.cpp:290-324; traversal include/traverse.h:12-312; VisitorExtract,
+// src/visitorextract.cpp:41-257; Mesh::insert_vert / insert_tri, src/global.cpp:13-53).
+//
+// The recursive traversal of the reference visits every octree vertex P interior to the root that
+// is a corner of some leaf, and emits the 8 leaves around P. Here every leaf emits its 8 corners as
+// integer keys at the finest level; a radix sort groups equal points; the leaves of octants not
+// covered by a corner entry (P on a face or edge of a larger leaf) are found by point location.
+// Dual cells come out in point-key order instead of traversal order; vertices are keyed by the
+// unordered pair of leaves of their edge and triangles are deduplicated by sorted vertex triple.
+#include "pisurf_internal.h"
+
+#define PISURF_DMC_QUAL __constant__ static const
+#define PISURF_DMC_NAME(x) d_##x
+#include "dmc_tables.h"
+
+namespace pisurf {
+
+__device__ __forceinline__ bool is_leaf_type(uint8_t t) { return t == 2 || t == 0; }
+
+__global__ void k_leaf_flag(NodeLevel lv, int64_t off, int* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= lv.n) return;
+    flag[off + i] = is_leaf_type(lv.type[i]) ? 1 : 0;
+}
+
+// leaf records and the 8 corner keys of every leaf (corners on the root boundary get the
+// sentinel key ~0 and sort last)
+__global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp,
+                            double4* __restrict__ leaf_node, uint64_t* __restrict__ ckey, int* __restrict__ cval,
+                            int* __restrict__ n_valid) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int valid = 0;
+    if (i < lv.n && is_leaf_type(lv.type[i])) {
+        const int L = lidx[off + i];
+        leaf_node[L] = lv.node[i];
+        const int4 p = lv.ip[i];
+        const int sh = dp - p.w;
+        const int64_t hi = ((int64_t)1 << dp) - 1;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int64_t X = (int64_t)(p.x + (c & 1)) << sh, Y = (int64_t)(p.y + ((c >> 1) & 1)) << sh,
+                          Z = (int64_t)(p.z + ((c >> 2) & 1)) << sh;
+            const bool in = X >= 1 && X <= hi && Y >= 1 && Y <= hi && Z >= 1 && Z <= hi;
+            ckey[8 * (int64_t)L + c] = in ? ((uint64_t)Z << (2 * dp)) | ((uint64_t)Y << dp) | (uint64_t)X : ~0ull;
+            cval[8 * (int64_t)L + c] = 8 * L + c;
+            valid += in;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) valid += __shfl_xor(valid, o, kWave);
+    if ((threadIdx.x & 63) == 0 && valid) atomicAdd(n_valid, valid);
+}
+
+// leaf containing the finest cell q (descent from the root)
+__device__ __forceinline__ int locate(const TreeView& tv, const int* lidx, int qx, int qy, int qz) {
+    int L = 0;
+    int64_t i = 0;
+    for (int guard = 0; guard < tv.nlev; guard++) {
+        const uint8_t t = tv.type[L][i];
+        if (is_leaf_type(t)) return lidx[tv.off[L] + i];
+        const int sh = tv.dp - L - 1;
+        const int b = ((qx >> sh) & 1) | (((qy >> sh) & 1) << 1) | (((qz >> sh) & 1) << 2);
+        i = (int64_t)tv.child[L][i] + b;
+        L++;
+    }
+    return -1;
+}
+
+// the 8 leaves around the point of run head j; returns false if j is not a run head
+__device__ __forceinline__ bool gather_point(const TreeView& tv, const int* lidx, const uint64_t* skey,
+                                             const int* sval, int64_t nent, int64_t j, int n[8]) {
+    const uint64_t key = skey[j];
+    if (j > 0 && skey[j - 1] == key) return false;
+#pragma unroll
+    for (int o = 0; o < 8; o++) n[o] = -1;
+    for (int64_t k = j; k < nent && k < j + 8 && skey[k] == key; k++) {
+        const int v = sval[k];
+        n[(v & 7) ^ 7] = v >> 3;  // a leaf with P as corner c lies in octant c ^ 7 of P
+    }
+    const uint64_t m = ((uint64_t)1 << tv.dp) - 1;
+    const int px = (int)(key & m), py = (int)((key >> tv.dp) & m), pz = (int)(key >> (2 * tv.dp));
+    int any = -1;
+    for (int o = 0; o < 8; o++)
+        if (n[o] < 0) n[o] = locate(tv, lidx, px - 1 + (o & 1), py - 1 + ((o >> 1) & 1), pz - 1 + ((o >> 2) & 1));
+    for (int o = 0; o < 8; o++)
+        if (n[o] >= 0) any = n[o];
+    for (int o = 0; o < 8; o++)
+        if (n[o] < 0) n[o] = any;  // unreachable for a consistent tree
+    return true;
+}
+
+__device__ __forceinline__ int sign_index(const double4* leaf_node, const int n[8]) {
+    int s = 0;
+#pragma unroll
+    for (int o = 0; o < 8; o++)
+        if (leaf_node[n[o]].w > 0.0) s |= 1 << o;
+    return s;
+}
+
+// pass 1: flag points whose dual cell is non-trivial; pass 2: write the dual cells
+__global__ void k_points(TreeView tv, const int* __restrict__ lidx, const double4* __restrict__ leaf_node,
+                         const uint64_t* __restrict__ skey, const int* __restrict__ sval, int64_t nent,
+                         int* __restrict__ dflag, const int* __restrict__ dpos, int* __restrict__ duals) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nent) return;
+    int n[8];
+    if (!gather_point(tv, lidx, skey, sval, nent, j, n)) {
+        if (!duals) dflag[j] = 0;
+        return;
+    }
+    const int s = sign_index(leaf_node, n);
+    const uint8_t code = d_kDmcCase[s];
+    const bool keep = code != 0 && code != 255;
+    if (!duals) {
+        dflag[j] = keep ? 1 : 0;
+        return;
+    }
+    if (!keep) return;
+    int* d = duals + 9 * (int64_t)dpos[j];
+    const uint32_t perm = d_kDmcPerm[s];
+#pragma unroll
+    for (int k = 0; k < 8; k++) d[k] = n[(perm >> (4 * k)) & 15];
+    d[8] = s;
+}
+
+__device__ __forceinline__ int canon_slot(uint8_t code) {
+    for (int k = 0; k < PISURF_DMC_NCASES; k++)
+        if (d_kDmcCanon[k] == code) return k;
+    return -1;
+}
+
+__global__ void k_tri_count(const int* __restrict__ duals, int64_t nd, int* __restrict__ ntri) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    const int slot = canon_slot(d_kDmcCase[duals[9 * d + 8]]);
+    ntri[d] = slot < 0 ? 0 : d_kDmcNTri[slot];
+}
+
+// triangle corners in output winding (generate_mesh: mirrored cases keep the table order, the
+// others are reversed); vertex key = unordered leaf pair of the corner's dual edge
+__global__ void k_tri_corners(const int* __restrict__ duals, int64_t nd, const int* __restrict__ toff,
+                              uint64_t* __restrict__ vkey, int* __restrict__ vval, int* __restrict__ vend) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    const int* n8 = duals + 9 * d;
+    const int s = n8[8];
+    const int slot = canon_slot(d_kDmcCase[s]);
+    if (slot < 0) return;
+    const bool mirror = d_kDmcMirror[s] != 0;
+    for (int t = 0; t < d_kDmcNTri[slot]; t++)
+        for (int kk = 0; kk < 3; kk++) {
+            const int k = mirror ? kk : 2 - kk;
+            const int e = d_kDmcTri[slot][t][k];
+            const int la = n8[d_kDmcEdge[e][0]], lb = n8[d_kDmcEdge[e][1]];
+            const int64_t idx = 3 * ((int64_t)toff[d] + t) + kk;
+            const uint64_t a = (uint64_t)min(la, lb), b = (uint64_t)max(la, lb);
+            vkey[idx] = (a << 32) | b;
+            vval[idx] = (int)idx;
+            vend[2 * idx] = la;
+            vend[2 * idx + 1] = lb;
+        }
+}
+
+__global__ void k_run_heads(const uint64_t* __restrict__ key, int64_t n, int* __restrict__ head) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    head[j] = (j == 0 || key[j] != key[j - 1]) ? 1 : 0;
+}
+
+// vertex ids of triangle corners and the bisection edge of each vertex (first occurrence)
+__global__ void k_vertex_ids(const uint64_t* __restrict__ key, const int* __restrict__ sval, int64_t n,
+                             const int* __restrict__ head_scan, const int* __restrict__ vend,
+                             const double4* __restrict__ leaf_node, int* __restrict__ corner_vid,
+                             double* __restrict__ ends) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const bool head = (j == 0 || key[j] != key[j - 1]);
+    const int vid = head_scan[j] + (head ? 1 : 0) - 1;
+    const int c = sval[j];
+    corner_vid[c] = vid;
+    if (head) {
+        const double4 a = leaf_node[vend[2 * (int64_t)c]], b = leaf_node[vend[2 * (int64_t)c + 1]];
+        double* e = ends + 8 * (int64_t)vid;
+        e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+        e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
+    }
+}
+
+// triangle dedupe keys: sorted vertex triple (first pass by the largest id, second by the pair)
+__global__ void k_tri_key_hi(const int* __restrict__ cv, int64_t nt, uint32_t* __restrict__ k, int* __restrict__ v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int a = cv[3 * t], b = cv[3 * t + 1], c = cv[3 * t + 2];
+    k[t] = (uint32_t)max(a, max(b, c));
+    v[t] = (int)t;
+}
+__global__ void k_tri_key_lo(const int* __restrict__ cv, const int* __restrict__ perm, int64_t nt,
+                             uint64_t* __restrict__ k, int* __restrict__ v) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    const int t = perm[j];
+    const int a = cv[3 * (int64_t)t], b = cv[3 * (int64_t)t + 1], c = cv[3 * (int64_t)t + 2];
+    const int lo = min(a, min(b, c)), hi = max(a, max(b, c)), mid = a + b + c - lo - hi;
+    k[j] = ((uint64_t)(uint32_t)lo << 32) | (uint32_t)mid;
+    v[j] = t;
+}
+__device__ __forceinline__ void sorted3(const int* cv, int64_t t, int& lo, int& mid, int& hi) {
+    const int a = cv[3 * t], b = cv[3 * t + 1], c = cv[3 * t + 2];
+    lo = min(a, min(b, c));
+    hi = max(a, max(b, c));
+    mid = a + b + c - lo - hi;
+}
+__global__ void k_tri_keep(const int* __restrict__ cv, const int* __restrict__ order, int64_t nt,
+                           int* __restrict__ keep) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    const int t = order[j];
+    int k = 1;
+    if (j > 0) {
+        int a0, a1, a2, b0, b1, b2;
+        sorted3(cv, t, a0, a1, a2);
+        sorted3(cv, order[j - 1], b0, b1, b2);
+        k = (a0 == b0 && a1 == b1 && a2 == b2) ? 0 : 1;  // equal triples: the earlier triangle stays
+    }
+    keep[t] = k;
+}
+__global__ void k_tri_out(const int* __restrict__ cv, const int* __restrict__ keep, const int* __restrict__ kpos,
+                          int64_t nt, int* __restrict__ tris) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt || !keep[t]) return;
+    const int64_t o = kpos[t];
+    tris[3 * o] = cv[3 * t] + 1;
+    tris[3 * o + 1] = cv[3 * t + 1] + 1;
+    tris[3 * o + 2] = cv[3 * t + 2] + 1;
+}
+
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s) {
+    if (lv.n > 0) k_leaf_flag<<<nblk(lv.n), 256, 0, s>>>(lv, off, flag);
+}
+void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double4* leaf_node, uint64_t* ckey,
+                      int* cval, int* n_valid, hipStream_t s) {
+    if (lv.n > 0) k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, leaf_node, ckey, cval, n_valid);
+}
+void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, const uint64_t* skey,
+                   const int* sval, int64_t nent, int* dflag, const int* dpos, int* duals, hipStream_t s) {
+    if (nent > 0) k_points<<<nblk(nent), 256, 0, s>>>(tv, lidx, leaf_node, skey, sval, nent, dflag, dpos, duals);
+}
+void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s) {
+    if (nd > 0) k_tri_count<<<nblk(nd), 256, 0, s>>>(duals, nd, ntri);
+}
+void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t* vkey, int* vval, int* vend,
+                        hipStream_t s) {
+    if (nd > 0) k_tri_corners<<<nblk(nd), 256, 0, s>>>(duals, nd, toff, vkey, vval, vend);
+}
+void launch_run_heads(const uint64_t* key, int64_t n, int* head, hipStream_t s) {
+    if (n > 0) k_run_heads<<<nblk(n), 256, 0, s>>>(key, n, head);
+}
+void launch_vertex_ids(const uint64_t* key, const int* sval, int64_t n, const int* head_scan, const int* vend,
+                       const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s) {
+    if (n > 0) k_vertex_ids<<<nblk(n), 256, 0, s>>>(key, sval, n, head_scan, vend, leaf_node, corner_vid, ends);
+}
+void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s) {
+    if (nt > 0) k_tri_key_hi<<<nblk(nt), 256, 0, s>>>(cv, nt, k, v);
+}
+void launch_tri_key_lo(const int* cv, const int* perm, int64_t nt, uint64_t* k, int* v, hipStream_t s) {
+    if (nt > 0) k_tri_key_lo<<<nblk(nt), 256, 0, s>>>(cv, perm, nt, k, v);
+}
+void launch_tri_keep(const int* cv, const int* order, int64_t nt, int* keep, hipStream_t s) {
+    if (nt > 0) k_tri_keep<<<nblk(nt), 256, 0, s>>>(cv, order, nt, keep);
+}
+void launch_tri_out(const int* cv, const int* keep, const int* kpos, int64_t nt, int* tris, hipStream_t s) {
+    if (nt > 0) k_tri_out<<<nblk(nt), 256, 0, s>>>(cv, keep, kpos, nt, tris);
+}
+
+}  // namespace pisurf

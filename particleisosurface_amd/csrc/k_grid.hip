@@ -87,7 +87,8 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_
 // ---------------------------------------------------------------------------------------------
 constexpr int kRsT = 256, kRsRounds = 16, kRsTile = kRsT * kRsRounds;
 
-__global__ void __launch_bounds__(kRsT) k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+template <class KT>
+__global__ void __launch_bounds__(kRsT) k_rs_hist(const KT* __restrict__ keys, int64_t n, int shift,
                                                   int* __restrict__ bh, int nblocks) {
     __shared__ int h[256];
     h[threadIdx.x] = 0;
@@ -95,14 +96,15 @@ __global__ void __launch_bounds__(kRsT) k_rs_hist(const uint32_t* __restrict__ k
     const int64_t base = (int64_t)blockIdx.x * kRsTile;
     for (int r = 0; r < kRsRounds; r++) {
         const int64_t i = base + r * kRsT + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1);
+        if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255u)], 1);
     }
     __syncthreads();
     bh[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(kRsT) k_rs_scatter(const uint32_t* __restrict__ kin, const int* __restrict__ vin,
-                                                     uint32_t* __restrict__ kout, int* __restrict__ vout, int64_t n,
+template <class KT>
+__global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin, const int* __restrict__ vin,
+                                                     KT* __restrict__ kout, int* __restrict__ vout, int64_t n,
                                                      int shift, const int* __restrict__ bscan, int nblocks) {
     __shared__ int base[256];
     __shared__ int wc[4][256];
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__(kRsT) k_rs_scatter(const uint32_t* __restrict_
     for (int r = 0; r < kRsRounds; r++) {
         const int64_t i = (int64_t)blockIdx.x * kRsTile + r * kRsT + t;
         const bool valid = i < n;
-        const uint32_t k = valid ? kin[i] : 0u;
+        const KT k = valid ? kin[i] : KT(0);
         const int v = valid ? vin[i] : 0;
         const int d = (int)((k >> shift) & 255u);
         unsigned long long m = __ballot(valid);
@@ -155,27 +157,36 @@ int64_t radix_tmp_ints(int64_t n) {
     return 2 * 256 * nb + scan_tmp_ints(256 * nb) + 16;
 }
 
-void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
+template <class KT>
+static void radix_sort_impl(KT* k0, int* v0, KT* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
     if (n <= 1) return;
     const int nb = (int)((n + kRsTile - 1) / kRsTile);
     int* bh = tmp;
     int* bscan = tmp + 256 * (int64_t)nb;
     int* stmp = bscan + 256 * (int64_t)nb;
     const int passes = (bits + 7) / 8;
-    uint32_t *ks = k0, *kd = k1;
+    KT *ks = k0, *kd = k1;
     int *vs = v0, *vd = v1;
     for (int p = 0; p < passes; p++) {
         const int shift = 8 * p;
-        k_rs_hist<<<nb, kRsT, 0, s>>>(ks, n, shift, bh, nb);
+        k_rs_hist<KT><<<nb, kRsT, 0, s>>>(ks, n, shift, bh, nb);
         exclusive_scan_i32(bh, bscan, 256 * (int64_t)nb, stmp, s);
-        k_rs_scatter<<<nb, kRsT, 0, s>>>(ks, vs, kd, vd, n, shift, bscan, nb);
-        uint32_t* tk = ks; ks = kd; kd = tk;
+        k_rs_scatter<KT><<<nb, kRsT, 0, s>>>(ks, vs, kd, vd, n, shift, bscan, nb);
+        KT* tk = ks; ks = kd; kd = tk;
         int* tv = vs; vs = vd; vd = tv;
     }
     if (ks != k0) {
-        (void)hipMemcpyAsync(k0, ks, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(k0, ks, n * sizeof(KT), hipMemcpyDeviceToDevice, s);
         (void)hipMemcpyAsync(v0, vs, n * sizeof(int), hipMemcpyDeviceToDevice, s);
     }
+}
+
+void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
+    radix_sort_impl<uint32_t>(k0, v0, k1, v1, n, bits, tmp, s);
+}
+
+void radix_sort_pairs64(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
+    radix_sort_impl<uint64_t>(k0, v0, k1, v1, n, bits, tmp, s);
 }
 
 // ---------------------------------------------------------------------------------------------

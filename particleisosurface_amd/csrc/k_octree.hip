@@ -214,12 +214,14 @@ __global__ void k_children(NodeLevel par, const int* __restrict__ scanned, NodeL
     const float hc = c.w / 2.0f;
     const uint64_t id = par.id[i];
     const int8_t d = par.depth[i] + 1;
+    const int4 pi = par.ip[i];
 #pragma unroll
     for (int t = 0; t < 8; t++) {
         const float sx = (t & 1) ? hc : -hc, sy = (t & 2) ? hc : -hc, sz = (t & 4) ? hc : -hc;
         nxt.c[base + t] = make_float4(__fadd_rn(c.x, sx), __fadd_rn(c.y, sy), __fadd_rn(c.z, sz), hc);
         nxt.id[base + t] = id * 8ull + (uint64_t)t;
         nxt.depth[base + t] = d;
+        nxt.ip[base + t] = make_int4(2 * pi.x + (t & 1), 2 * pi.y + ((t >> 1) & 1), 2 * pi.z + ((t >> 2) & 1), d);
     }
 }
 

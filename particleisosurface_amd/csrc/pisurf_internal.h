@@ -15,6 +15,7 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_
 // k1/v1 are ping-pong buffers; tmp must hold radix_tmp_ints(n) ints.
 int64_t radix_tmp_ints(int64_t n);
 void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s);
+void radix_sort_pairs64(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s);
 
 // ---- grid ----
 void launch_bbox(const float* xyz, int64_t n, float* partial /*>= 6*1024*/, float* out6, int* nan_count,
@@ -65,6 +66,7 @@ struct NodeLevel {
     double4* node;     // vertex xyz, field value
     int* flag;         // INTERNAL -> 1 (scanned into child offsets)
     int* child;        // level-local index of the first child in the next level, -1
+    int4* ip;          // integer cell coordinates at the node's depth (x, y, z, depth)
     int64_t n;
 };
 struct TreeParams {
@@ -84,6 +86,31 @@ void launch_classify(int prec, GridDev g, const void* rec_a, const void* rec_g, 
 void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s);
 void launch_sample_grads(int prec, TreeParams tp, NodeLevel lv, const void* samples, double* out_samples,
                          double* out_grads, hipStream_t s);
+
+// ---- device extraction ----
+constexpr int kMaxLevels = 32;
+struct TreeView {
+    int nlev;                          // levels present
+    int dp;                            // deepest level (finest integer resolution 2^dp)
+    int64_t off[kMaxLevels];           // global node offset of each level
+    const uint8_t* type[kMaxLevels];
+    const int* child[kMaxLevels];
+};
+void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
+void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double4* leaf_node, uint64_t* ckey,
+                      int* cval, int* n_valid, hipStream_t s);
+void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, const uint64_t* skey,
+                   const int* sval, int64_t nent, int* dflag, const int* dpos, int* duals, hipStream_t s);
+void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s);
+void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t* vkey, int* vval, int* vend,
+                        hipStream_t s);
+void launch_run_heads(const uint64_t* key, int64_t n, int* head, hipStream_t s);
+void launch_vertex_ids(const uint64_t* key, const int* sval, int64_t n, const int* head_scan, const int* vend,
+                       const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s);
+void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s);
+void launch_tri_key_lo(const int* cv, const int* perm, int64_t nt, uint64_t* k, int* v, hipStream_t s);
+void launch_tri_keep(const int* cv, const int* order, int64_t nt, int* keep, hipStream_t s);
+void launch_tri_out(const int* cv, const int* keep, const int* kpos, int64_t nt, int* tris, hipStream_t s);
 
 // ---- vertex placement ----
 void launch_bisect(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, float half_r,

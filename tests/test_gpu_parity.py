@@ -212,11 +212,75 @@ def test_level_sweeps_seeded_with_oracle_nodes(case):
     assert np.all(f[t2 == capi.EMPTY] == iso)
 
 
+def mesh_correspondence(v, t, v2, t2, r):
+    """Match mesh (v, t) to the oracle mesh (v2, t2) by vertex position; returns per-vertex
+    distances / r and whether the triangle sets coincide under the matching."""
+    from scipy.spatial import cKDTree
+    d, idx = cKDTree(v2).query(v)
+    bij = len(np.unique(idx)) == len(v) == len(v2)
+    a = {tuple(x) for x in np.sort(idx[t - 1], axis=1)}
+    b = {tuple(x) for x in np.sort(t2 - 1, axis=1)}
+    return d / r, bij and a == b and len(t) == len(t2)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["cfg1", "sphere", "cloud"])
+def test_device_extraction_matches_oracle_mesh(case, prec):
+    """Dual cells, triangles and vertices built on the device (point-key order) are the same mesh
+    as the reference traversal's, up to vertex numbering."""
+    xyz, r = CASES[case]()
+    o, ctx = _pair(xyz, r, prec)
+    o.run()
+    ctx.run_frame()
+    v, t = ctx.mesh()
+    v2, t2 = o.mesh()
+    d, same = mesh_correspondence(v, t, v2, t2, r)
+    assert same
+    assert ctx.stats()["n_dual_cells"] == len(o.dual_cells()[1])
+    if prec == "fp64":
+        assert d.max() < 1e-6
+    else:
+        assert np.mean(d <= 1e-3) >= 0.995
+
+
+def test_facade_driver(tmp_path):
+    """The C++ mirror of SurfReconstructor / HashGrid / Evaluator drives a frame and writes the same
+    OBJ as the reference-order extraction of the C ABI."""
+    import json
+    import os
+    import subprocess
+    xyz, r = synth.dam_break()
+    pf = tmp_path / "p.f32"
+    xyz.astype(np.float32).tofile(pf)
+    exe = os.path.join(os.path.dirname(capi.LIB_PATH), "pisurf_facade_demo")
+    out = tmp_path / "m.obj"
+    res = subprocess.run([exe, str(pf), repr(float(np.float32(r))), str(out), "0", "1"], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    info = json.loads(res.stdout)
+    o = Oracle(xyz, r)
+    o.run()
+    ref = tmp_path / "ref.obj"
+    o.write_obj(str(ref))
+    v2, t2 = o.mesh()
+    assert (info["verts"], info["tris"]) == (len(v2), len(t2))
+    assert (info["depth_max"], info["depth_min"]) == (7, 4)
+    lines = out.read_text().splitlines()
+    ref_lines = ref.read_text().splitlines()
+    assert len(lines) == len(ref_lines)
+    assert [l for l in lines if l.startswith("f")] == [l for l in ref_lines if l.startswith("f")]
+    va = np.array([[float(x) for x in l.split()[1:]] for l in lines if l.startswith("v")])
+    vb = np.array([[float(x) for x in l.split()[1:]] for l in ref_lines if l.startswith("v")])
+    assert np.mean(np.linalg.norm(va - vb, axis=1) <= 1e-3 * r + 2e-6) > 0.995
+    assert info["neighbors0"] > 0 and abs(info["iso"] - o.iso_constants()[1]) <= 1e-6 * info["iso"]
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("case", ["cfg1", "sphere"])
 def test_full_tree_and_mesh(case, prec):
     xyz, r = CASES[case]()
-    o, ctx = _pair(xyz, r, prec)
+    o = Oracle(xyz, r, fp64=(prec != "fp32"))
+    ctx = gpu_context(xyz, r, precision=PREC[prec], mesh_order=1)  # reference traversal order
     o.run()
     info = ctx.run_frame()
     oi = o.tree_info()

@@ -425,7 +425,7 @@ def _dmc_tables():
     src = open(os.path.join(os.path.dirname(__file__), "..", "particleisosurface_amd", "csrc", "dmc_tables.h")).read()
 
     def arr(name):
-        body = re.search(name + r"\[[^\]]*\](?:\[[^\]]*\])*\s*=\s*\{(.*?)\};", src, re.S).group(1)
+        body = re.search(name + r"\)?\[[^\]]*\](?:\[[^\]]*\])*\s*=\s*\{(.*?)\};", src, re.S).group(1)
         return [int(v, 0) for v in re.findall(r"0x[0-9a-f]+|\d+", body)]
     perm = arr("kDmcPerm")
     case = arr("kDmcCase")
