@@ -132,16 +132,19 @@ def main():
                                             st["ms_normals"] / args.steps, st["ms_tree"] / args.steps,
                                             st["ms_extract_host"] / args.steps, st["ms_bisect"] / args.steps))
     value = evals_per_frame / (dt / args.steps) / args.steps
-    # roofline of the node-sampling kernel (the dominant field-evaluation kernel)
+    # roofline of the node-sampling kernel (the dominant field-evaluation kernel): algorithmic bytes
+    # of the evaluations it performs (one per distinct sample point when deduplicated)
+    performed = st["sample_unique"] if st["sample_unique"] > 0 else st["sample_evals"]
     k_ms = st["ms_sample_kernel"] / max(st["sample_launches"], 1)
-    alg_bytes = st["sample_evals"] * BYTES_FIXED + st["sample_candidates"] * BYTES_PER_CAND[prec]
+    alg_bytes = performed * BYTES_FIXED + st["sample_candidates"] * BYTES_PER_CAND[prec]
     achieved = alg_bytes / max(st["sample_launches"], 1) / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_samples", launches_per_frame=st["sample_launches"] / args.steps,
-                avg_launch_ms=round(k_ms, 4), evals_per_launch=st["sample_evals"] / max(st["sample_launches"], 1),
-                candidates_per_eval=st["sample_candidates"] / max(st["sample_evals"], 1),
-                sample_kernel_evals_per_s=st["sample_evals"] / (st["ms_sample_kernel"] * 1e-3)
+                kernel="k_sample_eval" if st["sample_unique"] > 0 else "k_samples",
+                launches_per_frame=st["sample_launches"] / args.steps,
+                avg_launch_ms=round(k_ms, 4), evals_per_launch=performed / max(st["sample_launches"], 1),
+                candidates_per_eval=st["sample_candidates"] / max(performed, 1),
+                sample_kernel_evals_per_s=performed / (st["ms_sample_kernel"] * 1e-3)
                 if st["ms_sample_kernel"] > 0 else None)
     v32, t32 = ctx.mesh()
     out = {
@@ -155,6 +158,7 @@ def main():
                    "depth_max": info["depth_max"], "depth_min": info["depth_min"]},
         "frame_ms": round(ms, 3),
         "field_evals_per_frame": evals_per_frame // args.steps,
+        "field_evals_performed_per_frame": (evals_per_frame - st["sample_evals"] + performed) // args.steps,
         "phases_ms": {k[3:]: round(st[k] / args.steps, 3) for k in
                       ["ms_grid", "ms_aniso", "ms_normals", "ms_tree", "ms_extract_host", "ms_bisect"]},
         "mesh": {"verts": len(v32), "tris": len(t32), "nodes": info["n_nodes"]},

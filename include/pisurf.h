@@ -58,7 +58,9 @@ typedef struct pisurf_params {
     int32_t depth_max_override;/* 0: derived D_max (resizeRootBoxConstR) */
     int32_t mesh_order;        /* 0: extraction on the device, dual cells in point-key order;
                                   1: host traversal in the reference's order (byte-identical OBJ) */
-    int32_t reserved[4];
+    int32_t sample_dedupe;     /* 1 (default): node samples shared by neighbouring nodes of a level
+                                  (bitwise-identical coordinates) are evaluated once; 0: 27 per node */
+    int32_t reserved[3];
 } pisurf_params;
 
 typedef struct pisurf_grid_info {
@@ -79,7 +81,8 @@ typedef struct pisurf_tree_info {
 } pisurf_tree_info;
 
 typedef struct pisurf_stats {
-    int64_t field_evals;       /* SingleEval-equivalents performed since the last reset */
+    int64_t field_evals;       /* SingleEval-equivalents since the last reset (the reference's count:
+                                  deduplicated node samples count once per node) */
     int64_t grad_evals;        /* SingleEvalWithGrad-equivalents */
     int64_t qef_solves;
     int64_t candidates;        /* candidate particles visited by all field evaluations */
@@ -90,6 +93,7 @@ typedef struct pisurf_stats {
     double ms_grid, ms_aniso, ms_normals, ms_tree, ms_extract_host, ms_bisect, ms_total;
     double ms_sample_kernel;   /* summed device time of the node-sampling kernel (HIP events) */
     int64_t n_dual_cells, n_verts, n_tris, n_nodes;
+    int64_t sample_unique;     /* sample evaluations performed after deduplication (<= sample_evals) */
 } pisurf_stats;
 
 /* ---- context ---- */

@@ -30,7 +30,8 @@ class Params(C.Structure):
         ("radius", C.c_float), ("smooth_factor", C.c_float), ("iso_factor", C.c_float),
         ("use_ani", C.c_int32), ("use_poly6", C.c_int32), ("calc_normals", C.c_int32), ("oversample", C.c_int32),
         ("precision", C.c_int32), ("qef_mode", C.c_int32), ("qef_svd_tol", C.c_float),
-        ("depth_max_override", C.c_int32), ("mesh_order", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("depth_max_override", C.c_int32), ("mesh_order", C.c_int32), ("sample_dedupe", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -51,7 +52,8 @@ class Stats(C.Structure):
                 ("ms_grid", C.c_double), ("ms_aniso", C.c_double),
                 ("ms_normals", C.c_double), ("ms_tree", C.c_double), ("ms_extract_host", C.c_double),
                 ("ms_bisect", C.c_double), ("ms_total", C.c_double), ("ms_sample_kernel", C.c_double),
-                ("n_dual_cells", C.c_int64), ("n_verts", C.c_int64), ("n_tris", C.c_int64), ("n_nodes", C.c_int64)]
+                ("n_dual_cells", C.c_int64), ("n_verts", C.c_int64), ("n_tris", C.c_int64), ("n_nodes", C.c_int64),
+                ("sample_unique", C.c_int64)]
 
 
 _lib = None
@@ -107,11 +109,12 @@ class Context:
             raise PisurfError("%s failed (%d): %s" % (what, rc, self.L.pisurf_last_error(self.h).decode()))
 
     def set_params(self, radius, precision=FP32, use_ani=True, use_poly6=False, calc_normals=True, qef_mode=QEF_OFF,
-                   qef_svd_tol=1e-3, depth_max_override=0, smooth_factor=2.0, iso_factor=1.9, mesh_order=0):
+                   qef_svd_tol=1e-3, depth_max_override=0, smooth_factor=2.0, iso_factor=1.9, mesh_order=0, sample_dedupe=True):
         p = Params()
         self.L.pisurf_default_params(C.byref(p))
         p.radius = radius
         p.mesh_order = mesh_order
+        p.sample_dedupe = int(sample_dedupe)
         p.precision = precision
         p.use_ani = int(use_ani)
         p.use_poly6 = int(use_poly6)

@@ -170,7 +170,10 @@ struct pisurf_ctx {
     int dmax = 0, dmin = 0;
     float root_c[3] = {0, 0, 0}, root_half = 0;
     std::vector<LevelBuf*> levels;
-    DBuf samples;
+    DBuf samples, cl_flag, cl_pos, cl_list;
+    // sample deduplication: keys / entries (ping-pong), heads, scanned heads, representatives,
+    // head list, per-point values
+    DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals;
     // scratch for API calls
     DBuf q, out1, out2, out3;
     // device extraction workspace
@@ -510,19 +513,72 @@ int frame_ready(pisurf_ctx* c) {
 }
 
 // run the three level kernels on `lv` (nodes already in lv.c / depth / id)
-int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev) {
+int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     const int prec = prec_of(c);
     const size_t rsz = (prec == PISURF_FP64) ? 8 : 4;
     HIPCHK(c, c->samples.ensure(L->n * 27 * rsz));
     const TreeParams tp = tree_params(c);
     const FieldConst fc = field_const(c);
     const NodeLevel v = L->view();
-    launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, c->stream);
-    hipEvent_t e0, e1;
-    HIPCHK(c, c->event_pair(e0, e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
-    launch_samples(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, c->ctr(), c->stream);
-    HIPCHK(c, hipEventRecord(e1, c->stream));
+    const int64_t need_tmp = scan_tmp_ints(L->n + 1) + 64;
+    if ((size_t)need_tmp * 4 > c->tmp.cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, c->tmp.ensure(need_tmp * 4));
+    }
+    // deep levels: node boxes span a few cells, so 16 lanes per node
+    launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v,
+                      level_depth >= 0 && level_depth >= c->dmin - 1, c->stream);
+    // the level's flag / scanned / child arrays serve as the sample list until classification
+    HIPCHK(c, c->cl_flag.ensure((L->n + 1) * 4));
+    HIPCHK(c, c->cl_pos.ensure((L->n + 1) * 4));
+    HIPCHK(c, c->cl_list.ensure((L->n + 1) * 4));
+    launch_sample_list(tp, v, L->flag.as<int>(), L->scanned.as<int>(), L->child.as<int>(), c->cl_flag.as<int>(),
+                       c->cl_pos.as<int>(), c->cl_list.as<int>(), c->tmp.as<int>(), c->stream);
+    // Deduplicated sampling needs one node size per level (lattice keys) and 21-bit lattice
+    // coordinates: 2^(depth + 1) half-widths span the root box.
+    const bool dedupe = c->prm.sample_dedupe && level_depth >= 0 && level_depth <= 19;
+    if (dedupe) {
+        int k = 0;
+        HIPCHK(c, hipMemcpyAsync(&k, L->scanned.as<int>() + L->n, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int64_t nent = (int64_t)k * 27;
+        if (nent >= INT32_MAX) return c->fail(PISURF_E_UNSUPPORTED, "octree level has too many sample points");
+        if (nent > 0) {
+            HIPCHK(c, c->sd_key.ensure(nent * 8));
+            HIPCHK(c, c->sd_key2.ensure(nent * 8));
+            HIPCHK(c, c->sd_val.ensure(nent * 4));
+            HIPCHK(c, c->sd_val2.ensure(nent * 4));
+            HIPCHK(c, c->sd_head.ensure((nent + 1) * 4));
+            HIPCHK(c, c->sd_hpos.ensure((nent + 1) * 4));
+            HIPCHK(c, c->sd_rep.ensure(nent * 4));
+            HIPCHK(c, c->sd_hlist.ensure(nent * 4));
+            HIPCHK(c, c->sd_vals.ensure(nent * rsz));
+            const int64_t need = std::max(radix_tmp_ints(nent), scan_tmp_ints(nent + 1)) + 64;
+            HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, need * 4)));
+            const float origin[3] = {c->root_c[0] - c->root_half, c->root_c[1] - c->root_half,
+                                     c->root_c[2] - c->root_half};
+            launch_sample_dedupe_keys(v, L->child.as<int>(), nent, origin, c->sd_key.as<uint64_t>(),
+                                      c->sd_val.as<int>(), c->stream);
+            radix_sort_pairs64(c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
+                               c->sd_val2.as<int>(), nent, 3 * (level_depth + 2), c->tmp.as<int>(), c->stream);
+            hipEvent_t e0, e1;
+            HIPCHK(c, c->event_pair(e0, e1));
+            int64_t nheads = 0;
+            launch_sample_dedupe_eval(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, L->child.as<int>(),
+                                      c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), nent, c->sd_head.as<int>(),
+                                      c->sd_hpos.as<int>(), c->sd_rep.as<int>(), c->sd_hlist.as<int>(), &nheads,
+                                      c->sd_vals.p, c->samples.p, c->tmp.as<int>(), c->ctr(), e0, e1, c->stream);
+        }
+    } else {
+        hipEvent_t e0, e1;
+        HIPCHK(c, c->event_pair(e0, e1));
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        launch_samples(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, L->child.as<int>(),
+                       L->scanned.as<int>() + L->n, c->samples.p, c->ctr(), c->stream);
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+    }
+    launch_centers(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), c->cl_pos.as<int>() + L->n,
+                   c->samples.p, c->ctr(), c->stream);
     launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, sc_dev, c->ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
     return PISURF_OK;
@@ -554,7 +610,7 @@ int do_tree(pisurf_ctx* c) {
     }
     while (true) {
         L = level(li);
-        rc = sweep(c, L, nullptr);
+        rc = sweep(c, L, nullptr, (int)li);
         if (rc) return rc;
         HIPCHK(c, hipMemsetAsync(L->flag.as<int>() + L->n, 0, 4, c->stream));
         exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), L->n + 1, c->tmp.as<int>(), c->stream);
@@ -998,6 +1054,7 @@ int sync_counters(pisurf_ctx* c) {
     c->st.qef_solves = (int64_t)h.qef_solves;
     c->st.sample_evals = (int64_t)h.sample_evals;
     c->st.sample_candidates = (int64_t)h.sample_candidates;
+    c->st.sample_unique = (int64_t)h.sample_unique;
     c->st.fp64_eigensolves = (int64_t)c->n_fp64_eig;
     return PISURF_OK;
 }
@@ -1020,6 +1077,7 @@ int pisurf_default_params(pisurf_params* p) {
     p->qef_mode = PISURF_QEF_OFF;
     p->qef_svd_tol = 1e-3f;
     p->depth_max_override = 0;
+    p->sample_dedupe = 1;
     return PISURF_OK;
 }
 
@@ -1069,6 +1127,7 @@ int pisurf_set_params(pisurf_ctx* c, const pisurf_params* p) {
     if (p->qef_mode < 0 || p->qef_mode > 2) return c->fail(PISURF_E_INVALID, "bad qef_mode");
     if (p->depth_max_override < 0 || p->depth_max_override > 21) return c->fail(PISURF_E_INVALID, "bad depth override");
     if (p->mesh_order < 0 || p->mesh_order > 1) return c->fail(PISURF_E_INVALID, "bad mesh_order");
+    if (p->sample_dedupe < 0 || p->sample_dedupe > 1) return c->fail(PISURF_E_INVALID, "bad sample_dedupe");
     const bool geo = p->radius != c->prm.radius || p->use_ani != c->prm.use_ani || p->use_poly6 != c->prm.use_poly6 ||
                      p->precision != c->prm.precision;
     c->prm = *p;
@@ -1336,7 +1395,7 @@ int pisurf_sweep_level(pisurf_ctx* c, const float* centers, const float* halfs, 
     HIPCHK(c, hipMemcpyAsync(L.depth.p, d8.data(), n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(L.id.p, ids.data(), n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, c->out3.ensure(n * 4));
-    rc = sweep(c, &L, c->out3.as<int32_t>());
+    rc = sweep(c, &L, c->out3.as<int32_t>(), -1);
     if (rc) return rc;
     if (samples || grads) {
         HIPCHK(c, c->out1.ensure(n * 27 * 4 * 8));

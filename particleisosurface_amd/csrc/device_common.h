@@ -129,16 +129,16 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
     Acc<R, KAHAN> acc, ga[3];
     const R inv_h2 = R(fc.inv_h2);
     const R h2 = R(fc.h2);
+    // all 9 row windows first (independent loads in flight together)
+    int rlo[9], rhi[9];
     int visited = 0;
+#pragma unroll
     for (int r = 0; r < 9; r++) {
-        int lo, hi;
-        if (!row_range(g, ix, iy, iz, r, lo, hi)) continue;
-        visited += hi - lo;
-        for (int j = lo; j < hi; j++) {
-            const auto a = rec.a[j];
-            R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-            const R d2 = dx * dx + dy * dy + dz * dz;
-            if (!(d2 < a.w)) continue;
+        if (!row_range(g, ix, iy, iz, r, rlo[r], rhi[r])) rlo[r] = rhi[r] = 0;
+        visited += rhi[r] - rlo[r];
+    }
+    auto contrib = [&](int j, R dx, R dy, R dz, R d2) {
+        {
             if (KAHAN && sizeof(R) == 4) {
                 const float4 l = fc.lo[j];
                 dx -= R(l.x);
@@ -177,6 +177,30 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
                     }
                 }
             }
+        }
+    };
+    for (int r = 0; r < 9; r++) {
+        int j = rlo[r];
+        const int e = rhi[r];
+        // four candidates per step: their records are requested before any is used
+        for (; j + 4 <= e; j += 4) {
+            const auto a0 = rec.a[j], a1 = rec.a[j + 1], a2 = rec.a[j + 2], a3 = rec.a[j + 3];
+            const R x0 = px - a0.x, y0 = py - a0.y, z0 = pz - a0.z;
+            const R x1 = px - a1.x, y1 = py - a1.y, z1 = pz - a1.z;
+            const R x2 = px - a2.x, y2 = py - a2.y, z2 = pz - a2.z;
+            const R x3 = px - a3.x, y3 = py - a3.y, z3 = pz - a3.z;
+            const R q0 = x0 * x0 + y0 * y0 + z0 * z0, q1 = x1 * x1 + y1 * y1 + z1 * z1;
+            const R q2 = x2 * x2 + y2 * y2 + z2 * z2, q3 = x3 * x3 + y3 * y3 + z3 * z3;
+            if (q0 < a0.w) contrib(j, x0, y0, z0, q0);
+            if (q1 < a1.w) contrib(j + 1, x1, y1, z1, q1);
+            if (q2 < a2.w) contrib(j + 2, x2, y2, z2, q2);
+            if (q3 < a3.w) contrib(j + 3, x3, y3, z3, q3);
+        }
+        for (; j < e; j++) {
+            const auto a = rec.a[j];
+            const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+            const R d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < a.w) contrib(j, dx, dy, dz, d2);
         }
     }
     sum = acc.get();

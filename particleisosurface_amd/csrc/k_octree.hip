@@ -22,15 +22,18 @@ __device__ __forceinline__ bool leaf_depth(float half, float r) {
     return double(__fsub_rn(cellsize, r)) < 1e-8;
 }
 
-// checkEmptyAndCalcCurv (src/surface_reconstructor.cpp:152-222): one wave per node. The box
-// query enumerates cells [lo-1, hi+1] of the clamped box (GetInBoxParticles); each lane walks
-// whole (y,z) rows of that range (one contiguous window of sorted positions per row). A
-// particle counts when it lies strictly inside the unclamped box grown by r * smooth. Nodes
-// above D_min only need emptiness and stop at the first hit.
+// checkEmptyAndCalcCurv (src/surface_reconstructor.cpp:152-222): a group of G lanes per node
+// (G = 64 for large boxes, 16 for the small boxes of the deep levels). The box query enumerates
+// cells [lo-1, hi+1] of the clamped box (GetInBoxParticles); each lane walks whole (y,z) rows of
+// that range (one contiguous window of sorted positions per row). A particle counts when it lies
+// strictly inside the unclamped box grown by r * smooth. Nodes above D_min only need emptiness and
+// stop at the first hit.
+template <int G>
 __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __restrict__ spos,
                                                     const float4* __restrict__ nrm, TreeParams tp, NodeLevel lv) {
-    const int64_t node = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
+    const int64_t node = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    const int gbase = (threadIdx.x & 63) - lane;  // first lane of this group within the wave
     if (node >= lv.n) return;
     const float4 c = lv.c[node];
     const int depth = lv.depth[node];
@@ -62,7 +65,8 @@ __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __r
     float sx = 0.f, sy = 0.f, sz = 0.f, area = 0.f;
     const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
     const int nrows = (hi[0] >= lo[0] && ny > 0 && nz > 0) ? ny * nz : 0;
-    for (int r0 = 0; r0 < nrows; r0 += kWave) {
+    const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << (G % 64)) - 1) << gbase);
+    for (int r0 = 0; r0 < nrows; r0 += G) {
         const int row = r0 + lane;
         if (row < nrows) {
             const int y = lo[1] + row % ny, z = lo[2] + row / ny;
@@ -81,11 +85,11 @@ __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __r
                 }
             }
         }
-        if (!need_curv && __any(found)) break;
+        if (!need_curv && (__ballot(found) & gmask)) break;
     }
-    const bool any = __any(found);
+    const bool any = (__ballot(found) & gmask) != 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = G / 2; o > 0; o >>= 1) {
         sx += __shfl_xor(sx, o, kWave);
         sy += __shfl_xor(sy, o, kWave);
         sz += __shfl_xor(sz, o, kWave);
@@ -106,20 +110,177 @@ __device__ __forceinline__ bool wants_samples(const NodeLevel& lv, const TreePar
     return true;
 }
 
-template <class R, bool K>
-__global__ void __launch_bounds__(256) k_samples(GridDev g, Recs<R> rec, FieldConst fc, TreeParams tp, NodeLevel lv,
-                                                 R* __restrict__ samples, Counters* ctr) {
+// leaf-depth nodes that need their centre value and have no samples (QEF off)
+__device__ __forceinline__ bool wants_center(const NodeLevel& lv, const TreeParams& tp, int64_t node) {
+    return !lv.empty[node] && tp.qef_mode == 0 && leaf_depth(lv.c[node].w, tp.radius) &&
+           lv.depth[node] >= tp.depth_min;
+}
+
+// compaction of the nodes to evaluate: flag, then (after a scan) the list. A listed node gets its
+// 27 samples, or only its centre value (stored in sample slot 13) when it sits at leaf depth.
+__global__ void k_sample_flag(TreeParams tp, NodeLevel lv, int* __restrict__ flag, int* __restrict__ cflag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < lv.n) {
+        flag[i] = wants_samples(lv, tp, i) ? 1 : 0;
+        cflag[i] = wants_center(lv, tp, i) ? 1 : 0;
+    }
+    if (i == lv.n) flag[i] = cflag[i] = 0;
+}
+
+// ---- per-level sample deduplication --------------------------------------------------------
+// Neighbouring nodes share sample points on their faces, edges and corners. Entries (listed node,
+// sample) are keyed by the Morton code of their lattice position round((p - origin) / half), so a
+// shared point's copies sort next to each other and evaluation runs in space-filling order; only
+// copies with bitwise-identical float coordinates are merged, so every value is exactly what the
+// node's own evaluation would produce.
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1fffff;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ void k_sample_keys(NodeLevel lv, const int* __restrict__ list, int64_t nent, float ox, float oy, float oz,
+                              uint64_t* __restrict__ key, int* __restrict__ val) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t node = t / 27;
-    const int s = (int)(t - node * 27);
+    if (t >= nent) return;
+    const int node = list[t / 27];
+    const float4 c = lv.c[node];
+    float p[3];
+    sample_point(c, (int)(t % 27), p);
+    const float o[3] = {ox, oy, oz};
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float u = rintf((p[a] - o[a]) / c.w);
+        q[a] = (uint32_t)fminf(fmaxf(u, 0.0f), 2097151.0f);
+    }
+    key[t] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    val[t] = (int)t;
+}
+
+__device__ __forceinline__ void entry_point(const NodeLevel& lv, const int* list, int e, float p[3]) {
+    sample_point(lv.c[list[e / 27]], e % 27, p);
+}
+
+// head[j] = 1 when no earlier entry of the same key run has the same coordinates; rep[j] = sorted
+// position of the first entry with j's coordinates
+__global__ void k_sample_heads(NodeLevel lv, const int* __restrict__ list, const uint64_t* __restrict__ key,
+                               const int* __restrict__ val, int64_t nent, int* __restrict__ head,
+                               int* __restrict__ rep) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > nent) return;
+    if (j == nent) {
+        head[j] = 0;
+        return;
+    }
+    int64_t s0 = j;
+    while (s0 > 0 && j - s0 < 64 && key[s0 - 1] == key[j]) s0--;
+    float p[3];
+    entry_point(lv, list, val[j], p);
+    int64_t r = j;
+    for (int64_t k = s0; k < j; k++) {
+        float q[3];
+        entry_point(lv, list, val[k], q);
+        if (q[0] == p[0] && q[1] == p[1] && q[2] == p[2] && __float_as_uint(q[0]) == __float_as_uint(p[0]) &&
+            __float_as_uint(q[1]) == __float_as_uint(p[1]) && __float_as_uint(q[2]) == __float_as_uint(p[2])) {
+            r = k;
+            break;
+        }
+    }
+    head[j] = (r == j) ? 1 : 0;
+    rep[j] = (int)r;
+}
+
+__global__ void k_head_list(const int* __restrict__ head, const int* __restrict__ hpos, int64_t nent,
+                            int* __restrict__ hlist) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nent && head[j]) hlist[hpos[j]] = (int)j;
+}
+
+// one evaluation per distinct point, in Morton order
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_sample_eval(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
+                                                     const int* __restrict__ list, const int* __restrict__ val,
+                                                     const int* __restrict__ hlist, int64_t nheads, R* __restrict__ out,
+                                                     Counters* ctr) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vis = 0;
+    if (h < nheads) {
+        float p[3];
+        entry_point(lv, list, val[hlist[h]], p);
+        R sum, gr[3];
+        vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), sum, gr);
+        out[h] = R(fc.iso) - R(fc.sigma) * sum;
+    }
+    wave_add(&ctr->candidates, (unsigned long long)vis);
+    wave_add(&ctr->sample_unique, h < nheads ? 1ull : 0ull);
+    wave_add(&ctr->sample_candidates, (unsigned long long)vis);
+}
+
+// every entry receives its point's value; entries are what the reference evaluates (27 per
+// sampled node), so they are what field_evals / sample_evals count
+__global__ void k_sample_scatter(const int* __restrict__ list, const int* __restrict__ val,
+                                 const int* __restrict__ rep, const int* __restrict__ hpos, int64_t nent,
+                                 const void* __restrict__ vals, int rsz, void* __restrict__ samples, Counters* ctr) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nent) {
+        const int e = val[j];
+        const int64_t dst = (int64_t)list[e / 27] * 27 + e % 27;
+        const int64_t src = hpos[rep[j]];
+        if (rsz == 8) ((double*)samples)[dst] = ((const double*)vals)[src];
+        else ((float*)samples)[dst] = ((const float*)vals)[src];
+    }
+    wave_add(&ctr->field_evals, j < nent ? 1ull : 0ull);
+    wave_add(&ctr->sample_evals, j < nent ? 1ull : 0ull);
+}
+
+// centre values of leaf-depth nodes (sample slot 13), one thread per listed node
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_centers(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
+                                                 const int* __restrict__ list, const int* __restrict__ count,
+                                                 R* __restrict__ samples, Counters* ctr) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int vis = 0;
     bool did = false;
-    if (node < lv.n && wants_samples(lv, tp, node)) {
+    if (k < *count) {
+        const int64_t node = list[k];
+        const float4 c = lv.c[node];
+        R sum, gr[3];
+        vis = field_sum<R, K, false>(g, rec, fc, c.x, c.y, c.z, R(c.x), R(c.y), R(c.z), sum, gr);
+        samples[node * 27 + 13] = R(fc.iso) - R(fc.sigma) * sum;
+        did = true;
+    }
+    wave_add(&ctr->field_evals, did ? 1ull : 0ull);
+    wave_add(&ctr->candidates, (unsigned long long)vis);
+}
+__global__ void k_sample_list(NodeLevel lv, const int* __restrict__ flag, const int* __restrict__ pos,
+                              int* __restrict__ list) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < lv.n && flag[i]) list[pos[i]] = (int)i;
+}
+
+// 27 samples of each listed node; thread t handles sample t % 27 of list entry t / 27, so a wave
+// holds samples of 2-3 spatially adjacent nodes and no idle lanes for skipped nodes
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_samples(GridDev g, Recs<R> rec, FieldConst fc, TreeParams tp, NodeLevel lv,
+                                                 const int* __restrict__ list, const int* __restrict__ count,
+                                                 R* __restrict__ samples, Counters* ctr) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = t / 27;
+    const int s = (int)(t - k * 27);
+    int vis = 0;
+    bool did = false;
+    if (k < *count) {
+        const int64_t node = list[k];
         float p[3];
         sample_point(lv.c[node], s, p);
         R sum, gr[3];
         vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), sum, gr);
-        samples[t] = R(fc.iso) - R(fc.sigma) * sum;
+        samples[node * 27 + s] = R(fc.iso) - R(fc.sigma) * sum;
         did = true;
     }
     wave_add(&ctr->field_evals, did ? 1ull : 0ull);
@@ -181,7 +342,9 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
                 } else {
                     float p13[3];
                     sample_point(c, 13, p13);
-                    if (has_samples && p13[0] == c.x && p13[1] == c.y && p13[2] == c.z) {
+                    if (wants_center(lv, tp, node)) {
+                        fval = samples[node * 27 + 13];  // evaluated by k_centers
+                    } else if (has_samples && p13[0] == c.x && p13[1] == c.y && p13[2] == c.z) {
                         fval = samples[node * 27 + 13];  // the centre sample is the node centre
                     } else {
                         R sum, g3[3];
@@ -250,23 +413,93 @@ __global__ void k_sample_grads(TreeParams tp, NodeLevel lv, const R* __restrict_
     }
 }
 
-void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, hipStream_t s) {
+void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, bool small,
+                       hipStream_t s) {
     if (lv.n <= 0) return;
-    const int64_t threads = lv.n * kWave;
-    k_empty_curv<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(g, spos, nrm, tp, lv);
+    if (small) {
+        const int64_t threads = lv.n * 16;
+        k_empty_curv<16><<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(g, spos, nrm, tp, lv);
+    } else {
+        const int64_t threads = lv.n * kWave;
+        k_empty_curv<64><<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(g, spos, nrm, tp, lv);
+    }
+}
+
+void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,
+                        int* clist, int* tmp, hipStream_t s) {
+    if (lv.n <= 0) return;
+    k_sample_flag<<<(unsigned)((lv.n + 1 + 255) / 256), 256, 0, s>>>(tp, lv, flag, cflag);
+    exclusive_scan_i32(flag, pos, lv.n + 1, tmp, s);
+    k_sample_list<<<(unsigned)((lv.n + 255) / 256), 256, 0, s>>>(lv, flag, pos, list);
+    exclusive_scan_i32(cflag, cpos, lv.n + 1, tmp, s);
+    k_sample_list<<<(unsigned)((lv.n + 255) / 256), 256, 0, s>>>(lv, cflag, cpos, clist);
+}
+
+void launch_sample_dedupe_keys(NodeLevel lv, const int* list, int64_t nent, const float origin[3], uint64_t* key,
+                               int* val, hipStream_t s) {
+    if (nent <= 0) return;
+    k_sample_keys<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(lv, list, nent, origin[0], origin[1], origin[2], key,
+                                                                 val);
+}
+
+void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
+                               const int* list, const uint64_t* key, const int* val, int64_t nent, int* head, int* hpos,
+                               int* rep, int* hlist, int64_t* nheads_out, void* vals, void* samples, int* tmp,
+                               Counters* ctr, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+    *nheads_out = 0;
+    if (nent <= 0) return;
+    const unsigned nb = (unsigned)((nent + 1 + 255) / 256);
+    k_sample_heads<<<nb, 256, 0, s>>>(lv, list, key, val, nent, head, rep);
+    exclusive_scan_i32(head, hpos, nent + 1, tmp, s);
+    k_head_list<<<nb, 256, 0, s>>>(head, hpos, nent, hlist);
+    int nh = 0;
+    (void)hipMemcpyAsync(&nh, hpos + nent, 4, hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    *nheads_out = nh;
+    const unsigned ne = (unsigned)((nh + 255) / 256);
+    (void)hipEventRecord(e0, s);
+    if (nh > 0) {
+        if (prec == 2) {
+            Recs<double> r{(const double4*)ra, (const double4*)rg};
+            k_sample_eval<double, false><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (double*)vals, ctr);
+        } else {
+            Recs<float> r{(const float4*)ra, (const float4*)rg};
+            if (prec == 1)
+                k_sample_eval<float, true><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (float*)vals, ctr);
+            else
+                k_sample_eval<float, false><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (float*)vals, ctr);
+        }
+    }
+    (void)hipEventRecord(e1, s);
+    k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, val, rep, hpos, nent, vals,
+                                                                    prec == 2 ? 8 : 4, samples, ctr);
+}
+
+void launch_centers(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv, const int* list,
+                    const int* count, void* samples, Counters* ctr, hipStream_t s) {
+    if (lv.n <= 0) return;
+    const unsigned nb = (unsigned)((lv.n + 255) / 256);
+    if (prec == 2) {
+        Recs<double> r{(const double4*)ra, (const double4*)rg};
+        k_centers<double, false><<<nb, 256, 0, s>>>(g, r, fc, lv, list, count, (double*)samples, ctr);
+    } else {
+        Recs<float> r{(const float4*)ra, (const float4*)rg};
+        if (prec == 1) k_centers<float, true><<<nb, 256, 0, s>>>(g, r, fc, lv, list, count, (float*)samples, ctr);
+        else k_centers<float, false><<<nb, 256, 0, s>>>(g, r, fc, lv, list, count, (float*)samples, ctr);
+    }
 }
 
 void launch_samples(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, TreeParams tp, NodeLevel lv,
-                    void* samples, Counters* ctr, hipStream_t s) {
+                    const int* list, const int* count, void* samples, Counters* ctr, hipStream_t s) {
     if (lv.n <= 0) return;
     const unsigned nb = (unsigned)((lv.n * 27 + 255) / 256);
     if (prec == 2) {
         Recs<double> r{(const double4*)ra, (const double4*)rg};
-        k_samples<double, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (double*)samples, ctr);
+        k_samples<double, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, list, count, (double*)samples, ctr);
     } else {
         Recs<float> r{(const float4*)ra, (const float4*)rg};
-        if (prec == 1) k_samples<float, true><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (float*)samples, ctr);
-        else k_samples<float, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (float*)samples, ctr);
+        if (prec == 1) k_samples<float, true><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, list, count, (float*)samples, ctr);
+        else k_samples<float, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, list, count, (float*)samples, ctr);
     }
 }
 

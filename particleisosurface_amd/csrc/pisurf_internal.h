@@ -48,7 +48,8 @@ void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams
 
 // ---- field ----
 struct Counters {
-    unsigned long long field_evals, grad_evals, candidates, qef_solves, sample_evals, sample_candidates;
+    unsigned long long field_evals, grad_evals, candidates, qef_solves, sample_evals, sample_candidates,
+        sample_unique;
 };
 void launch_eval_points(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float* q,
                         int64_t n, double* f, double* grad, Counters* ctr, hipStream_t s);
@@ -78,9 +79,26 @@ struct TreeParams {
     int qef_mode;
     float qef_svd_tol;
 };
-void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, hipStream_t s);
+// small: every node box spans few cells (16 lanes per node instead of a wave)
+void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, bool small,
+                       hipStream_t s);
+// flag[n+1], pos[n+1] (pos[n] = number of listed nodes), list[n]: nodes to sample; cflag / cpos /
+// clist: leaf-depth nodes that need only their centre value
+void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,
+                        int* clist, int* tmp, hipStream_t s);
+void launch_centers(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
+                    const int* list, const int* count, void* samples, Counters* ctr, hipStream_t s);
+// sample deduplication: keys for the nent = 27 * listed entries, then (after a 64-bit key sort)
+// heads, one evaluation per distinct point between events e0 / e1, and the scatter to samples
+void launch_sample_dedupe_keys(NodeLevel lv, const int* list, int64_t nent, const float origin[3], uint64_t* key,
+                               int* val, hipStream_t s);
+void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
+                               const int* list, const uint64_t* key, const int* val, int64_t nent, int* head, int* hpos,
+                               int* rep, int* hlist, int64_t* nheads_out, void* vals, void* samples, int* tmp,
+                               Counters* ctr, hipEvent_t e0, hipEvent_t e1, hipStream_t s);
 void launch_samples(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
-                    NodeLevel lv, void* samples /*n*27 R*/, Counters* ctr, hipStream_t s);
+                    NodeLevel lv, const int* list, const int* count, void* samples /*n*27 R*/, Counters* ctr,
+                    hipStream_t s);
 void launch_classify(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
                      NodeLevel lv, const void* samples, int32_t* signchange, Counters* ctr, hipStream_t s);
 void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s);

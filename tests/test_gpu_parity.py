@@ -462,3 +462,25 @@ def test_large_frame_invariants():
     S64 = iso - ctx.eval_points(q)
     band = S64 > 1e-3 * iso
     assert np.quantile(np.abs(S32 - S64)[band] / S64[band], 0.999) < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp32k", "fp64"])
+@pytest.mark.parametrize("case", ["cfg1", "cloud", "sphere"])
+def test_sample_dedupe_is_bit_identical(case, prec):
+    """Shared node samples evaluated once give the same tree, values and mesh as 27 per node."""
+    xyz, r = CASES[case]()
+    a = gpu_context(xyz, r, precision=PREC[prec], sample_dedupe=True)
+    b = gpu_context(xyz, r, precision=PREC[prec], sample_dedupe=False)
+    ia, ib = a.run_frame(), b.run_frame()
+    assert ia == ib
+    na, nb = a.tree_nodes(ia["n_nodes"]), b.tree_nodes(ib["n_nodes"])
+    for k in na:
+        assert np.array_equal(na[k], nb[k]), k
+    va, ta = a.mesh()
+    vb, tb = b.mesh()
+    assert np.array_equal(va, vb) and np.array_equal(ta, tb)
+    sa, sb = a.stats(), b.stats()
+    # both count the reference's 27 evaluations per sampled node; dedupe performs fewer
+    assert sa["sample_evals"] == sb["sample_evals"] and sa["field_evals"] == sb["field_evals"]
+    assert sb["sample_unique"] == 0
+    assert 0 < sa["sample_unique"] < sa["sample_evals"]
