@@ -100,6 +100,57 @@ def cpu_baseline(pts_all, xyz, r, ctx, budget_s, log_every=True):
                        "OpenMP over points), %.1f s; setup %.1f s not timed" % (done, spent, setup))
 
 
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
+                      "launches": 5}), flush=True)
+
+
+def batch_leg(prec, dedupe):
+    """24 frames of uneven size: one persistent run_frames call vs. a loop of the plain path (a
+    fresh context per frame, as the reference builds a new SurfReconstructor per frame)."""
+    t = time.perf_counter()
+    frames = synth.batch_frames()
+    log("batch: %d frames, %d particles (%.1f s to generate)" % (len(frames), sum(len(f) for f, _ in frames),
+                                                                 time.perf_counter() - t))
+    xs, rs = [f for f, _ in frames], [r for _, r in frames]
+    tris = [0]
+
+    def on_mesh(k, v, t_, tree):
+        tris[0] += len(t_)
+
+    ctx = capi.Context(0)
+    ctx.set_params(rs[0], precision=prec, sample_dedupe=dedupe)
+    ctx.run_frames(xs[:2], rs[:2])  # warm-up: buffers, code objects
+    tris[0] = 0
+    t = time.perf_counter()
+    ctx.run_frames(xs, rs, on_mesh)
+    dt_persistent = time.perf_counter() - t
+    ctx.close()
+    tris_p = tris[0]
+    t = time.perf_counter()
+    tris_l = 0
+    for x, r in zip(xs, rs):
+        c = capi.Context(0)
+        c.set_params(r, precision=prec, sample_dedupe=dedupe)
+        c.upload(x)
+        c.run_frame()
+        tris_l += len(c.mesh()[1])
+        c.close()
+    dt_loop = time.perf_counter() - t
+    return dict(frames=len(xs), particles=int(sum(len(x) for x in xs)),
+                persistent_frames_per_s=round(len(xs) / dt_persistent, 3),
+                loop_frames_per_s=round(len(xs) / dt_loop, 3), speedup=round(dt_loop / dt_persistent, 3),
+                same_triangle_count=bool(tris_p == tris_l), host_input=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,7 +161,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--precision", default="fp32k", choices=["fp32", "fp32k", "fp64"])
+    ap.add_argument("--no-dedupe", action="store_true", help="evaluate all 27 samples of every node")
+    ap.add_argument("--batch", action="store_true",
+                    help="also mesh the 24-frame batch (50 k - 3 M particles per frame, radii by frame) "
+                         "through the persistent multi-frame call and through a loop of fresh contexts")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read + write) 5 times, to scale "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings, and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -121,7 +182,7 @@ def main():
 
     prec = {"fp32": capi.FP32, "fp32k": capi.FP32_KAHAN, "fp64": capi.FP64}[args.precision]
     ctx = capi.Context(0)
-    ctx.set_params(r, precision=prec)
+    ctx.set_params(r, precision=prec, sample_dedupe=not args.no_dedupe)
     ctx.upload(xyz)  # resident in HBM before the timed region
     info, st, dt = frame(ctx, args.steps, args.warmup)
     ms = 1e3 * dt / args.steps
@@ -174,7 +235,7 @@ def main():
         f32 = ctx.eval_points(q)
         n32 = ctx.particle_normals()
         iso = ctx.iso_constants()[1]
-        ctx.set_params(r, precision=capi.FP64)
+        ctx.set_params(r, precision=capi.FP64, sample_dedupe=not args.no_dedupe)
         info64, st64, dt64 = frame(ctx, 1, 1)
         ms64 = 1e3 * dt64
         v64, t64 = ctx.mesh()
@@ -198,8 +259,12 @@ def main():
             "same_tree": bool(info64["n_nodes"] == info["n_nodes"]),
         }
         log("fp64 frame %.2f ms, speedup %.2fx" % (ms64, ms64 / ms))
-        ctx.set_params(r, precision=prec)
+        ctx.set_params(r, precision=prec, sample_dedupe=not args.no_dedupe)
         info, st, dt = frame(ctx, 1, 0)
+
+    if args.batch:
+        out["batch"] = batch_leg(prec, not args.no_dedupe)
+        log("batch: %s" % out["batch"])
 
     if not args.no_cpu:
         pts = sample_points_of_tree(ctx, info, r, limit=20_000_000)

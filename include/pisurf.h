@@ -147,6 +147,13 @@ int pisurf_build_tree(pisurf_ctx* ctx, pisurf_tree_info* out);
 int pisurf_tree_download(pisurf_ctx* ctx, float* center3, float* half, int32_t* depth, uint8_t* type, uint64_t* id,
                          double* node4, int32_t* child);
 
+/* ---- UniformGrid::Run (src/marching.cpp:93-186): the field sampled on a dense lattice ----
+ * Box: the tight particle box, per axis resized to ceil(len / r) * r + 4 r about its centre
+ * (resizeRootBoxConstR, src/marching.cpp:41-56); dims[a] = int(len' / r) + 1 points, spacing r.
+ * scalars (NULL: geometry only): dims[0] * dims[1] * dims[2] SingleEval values, x fastest
+ * (gridSampling, src/marching.cpp:72-91). dims / box / spacing may each be NULL. */
+int pisurf_uniform_grid(pisurf_ctx* ctx, int32_t dims[3], float box[6], float* spacing, float* scalars);
+
 /* ---- H16: TNode::NodeCalcNode (src/iso_method_ours.cpp:101-174) for n nodes ---- */
 int pisurf_qef_solve(pisurf_ctx* ctx, const float* centers, const float* halfs, const double* samples,
                      const double* grads, int64_t n, int32_t mode, double* node4);
@@ -160,6 +167,19 @@ int pisurf_extract(pisurf_ctx* ctx, int64_t* n_verts, int64_t* n_tris);
 int pisurf_mesh_download(pisurf_ctx* ctx, double* verts, int32_t* tris);
 /* SurfReconstructor::Run (src/surface_reconstructor.cpp:409-479) on the uploaded particles */
 int pisurf_run_frame(pisurf_ctx* ctx, pisurf_tree_info* tree);
+/* runOurs' frame loop (src/main.cpp:330-405: one SurfReconstructor per frame in DATA_PATHES)
+ * as one persistent call: device buffers are kept across frames and grow to the largest, and
+ * frame k+1's particles are copied to the device (pinned staging, a second stream) while frame k
+ * is meshed. Frame k has n[k] particles at xyz[k] (host, or device with on_device != 0) and radius
+ * radius[k]; the other parameters are the context's. After each frame `on_mesh` receives the mesh
+ * (vertices x,y,z as double, 1-based triangles, valid only during the call); `per_frame`
+ * (nullable, nframes entries) receives each frame's statistics. Nothing carries over between
+ * frames. Afterwards pisurf_get_stats reports the last frame. */
+typedef void (*pisurf_mesh_cb)(void* user, int32_t frame, const pisurf_tree_info* tree, const double* verts,
+                               int64_t n_verts, const int32_t* tris, int64_t n_tris);
+int pisurf_run_frames(pisurf_ctx* ctx, int32_t nframes, const float* const* xyz, const int64_t* n,
+                      const float* radius, int32_t on_device, pisurf_mesh_cb on_mesh, void* user,
+                      pisurf_stats* per_frame);
 int pisurf_get_stats(pisurf_ctx* ctx, pisurf_stats* out);
 int pisurf_reset_stats(pisurf_ctx* ctx);
 /* device stream the library launches on (hipStream_t) */

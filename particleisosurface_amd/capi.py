@@ -19,9 +19,9 @@ EXPORTS = [
     "pisurf_create", "pisurf_destroy", "pisurf_last_error", "pisurf_default_params", "pisurf_set_params",
     "pisurf_upload_particles", "pisurf_hashgrid_build", "pisurf_hashgrid_download", "pisurf_neighbors",
     "pisurf_box_particles", "pisurf_anisotropy", "pisurf_download_aniso", "pisurf_iso_constants",
-    "pisurf_particle_normals", "pisurf_eval_points", "pisurf_sweep_level", "pisurf_build_tree",
+    "pisurf_particle_normals", "pisurf_eval_points", "pisurf_uniform_grid", "pisurf_sweep_level", "pisurf_build_tree",
     "pisurf_tree_download", "pisurf_qef_solve", "pisurf_bisect_edges", "pisurf_extract", "pisurf_mesh_download",
-    "pisurf_run_frame", "pisurf_get_stats", "pisurf_reset_stats", "pisurf_stream",
+    "pisurf_run_frame", "pisurf_run_frames", "pisurf_get_stats", "pisurf_reset_stats", "pisurf_stream",
 ]
 
 
@@ -58,6 +58,10 @@ class Stats(C.Structure):
 
 _lib = None
 
+# void (*)(void* user, int32 frame, const pisurf_tree_info*, const double* v, int64 nv, const int32* t, int64 nt)
+MESH_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_double), C.c_int64,
+                      C.POINTER(C.c_int32), C.c_int64)
+
 
 def load():
     """Load libpisurf.so (raises when it has not been built)."""
@@ -70,6 +74,8 @@ def load():
         L.pisurf_last_error.argtypes = [C.c_void_p]
         L.pisurf_stream.restype = C.c_void_p
         L.pisurf_stream.argtypes = [C.c_void_p]
+        L.pisurf_run_frames.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_float), C.c_int32, MESH_CB, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -259,6 +265,52 @@ class Context:
         ti = TreeInfo()
         self._c(self.L.pisurf_run_frame(self.h, C.byref(ti)), "run_frame")
         return dict(depth_max=ti.depth_max, depth_min=ti.depth_min, n_nodes=ti.n_nodes, n_levels=ti.n_levels)
+
+    def uniform_grid(self, values=True):
+        """UniformGrid sampling: (scalars (dims2, dims1, dims0) float32 or None, box[6], spacing)."""
+        dims = (C.c_int32 * 3)()
+        box = (C.c_float * 6)()
+        sp = C.c_float()
+        self._c(self.L.pisurf_uniform_grid(self.h, dims, box, C.byref(sp), None), "uniform_grid")
+        d = [dims[0], dims[1], dims[2]]
+        out = None
+        if values:
+            out = np.empty((d[2], d[1], d[0]), np.float32)
+            self._c(self.L.pisurf_uniform_grid(self.h, None, None, None, _p(out)), "uniform_grid")
+        return out, np.array(list(box), np.float32), sp.value
+
+    def run_frames(self, frames, radii, on_mesh=None):
+        """Mesh a batch of frames in one persistent call. frames: list of (n, 3) float32 arrays;
+        on_mesh(k, verts (nv, 3) float64, tris (nt, 3) int32, tree dict) is called after each frame.
+        Returns the per-frame statistics."""
+        nf = len(frames)
+        frames = [np.ascontiguousarray(f, dtype=np.float32).reshape(-1, 3) for f in frames]
+        ptrs = (C.c_void_p * max(nf, 1))(*[f.ctypes.data for f in frames])
+        ns = (C.c_int64 * max(nf, 1))(*[len(f) for f in frames])
+        rs = (C.c_float * max(nf, 1))(*[float(r) for r in radii])
+        per = (Stats * max(nf, 1))()
+        err = []
+
+        def cb(user, k, ti, v, nv, t, nt):
+            try:
+                if on_mesh is not None:
+                    tip = C.cast(ti, C.POINTER(TreeInfo)).contents
+                    tree = dict(depth_max=tip.depth_max, depth_min=tip.depth_min, n_nodes=tip.n_nodes,
+                                n_levels=tip.n_levels)
+                    va = np.ctypeslib.as_array(v, (nv, 3)).copy() if nv else np.zeros((0, 3))
+                    ta = np.ctypeslib.as_array(t, (nt, 3)).copy() if nt else np.zeros((0, 3), np.int32)
+                    on_mesh(k, va, ta, tree)
+            except Exception as e:  # an exception must not unwind through the C frames
+                err.append(e)
+
+        fn = MESH_CB(cb)
+        self._c(self.L.pisurf_run_frames(self.h, nf, ptrs, ns, rs, 0, fn, None, C.cast(per, C.c_void_p)),
+                "run_frames")
+        if err:
+            raise err[0]
+        if frames:
+            self.params.radius = float(radii[-1])
+        return [{k: getattr(per[i], k) for k, _ in Stats._fields_} for i in range(nf)]
 
     def stats(self):
         s = Stats()

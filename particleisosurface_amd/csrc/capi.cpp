@@ -11,6 +11,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pisurf.h"
@@ -38,6 +39,24 @@ struct DBuf {
         return e;
     }
     template <class T> T* as() const { return (T*)p; }
+};
+
+// page-locked host staging (hipHostMalloc; never registration of existing memory)
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
 };
 
 struct LevelBuf {
@@ -176,6 +195,10 @@ struct pisurf_ctx {
     DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals;
     // scratch for API calls
     DBuf q, out1, out2, out3;
+    // multi-frame path: next frame's particles, their staging and copy stream
+    DBuf xyz_next;
+    PinnedBuf pin;
+    hipStream_t cstream = nullptr;
     // device extraction workspace
     DBuf eb[24];
     // mesh
@@ -191,6 +214,7 @@ struct pisurf_ctx {
     ~pisurf_ctx() {
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
+        if (cstream) (void)hipStreamDestroy(cstream);
         if (stream) (void)hipStreamDestroy(stream);
     }
     hipError_t event_pair(hipEvent_t& a, hipEvent_t& b) {
@@ -1370,6 +1394,51 @@ int pisurf_eval_points(pisurf_ctx* c, const float* q, int64_t n, double* f, doub
     API_END(c)
 }
 
+int pisurf_uniform_grid(pisurf_ctx* c, int32_t dims[3], float box[6], float* spacing, float* scalars) {
+    if (!c) return PISURF_E_INVALID;
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
+    int rc = ensure_bbox(c);
+    if (rc) return rc;
+    // resizeRootBoxConstR of UniformGrid (src/marching.cpp:41-56) on the tight particle box
+    const float r = c->prm.radius;
+    UniformBox ub;
+    int64_t total = 1;
+    for (int a = 0; a < 3; a++) {
+        const float center = (c->bbox[2 * a] + c->bbox[2 * a + 1]) / 2;
+        const float length = c->bbox[2 * a + 1] - c->bbox[2 * a];
+        const float len2 = std::ceil(length / r) * r + r * 4;
+        ub.lo[a] = center - len2 / 2;
+        ub.hi[a] = center + len2 / 2;
+        ub.steps[a] = int(len2 / r);
+        ub.dims[a] = ub.steps[a] + 1;
+        if (ub.steps[a] < 1) return c->fail(PISURF_E_INVALID, "degenerate sampling box");
+        total *= ub.dims[a];
+        if (total >= ((int64_t)1 << 31)) return c->fail(PISURF_E_UNSUPPORTED, "uniform grid exceeds 2^31 points");
+    }
+    if (dims)
+        for (int a = 0; a < 3; a++) dims[a] = ub.dims[a];
+    if (box)
+        for (int a = 0; a < 3; a++) {
+            box[2 * a] = ub.lo[a];
+            box[2 * a + 1] = ub.hi[a];
+        }
+    if (spacing) *spacing = r;
+    if (!scalars) return PISURF_OK;
+    rc = need_aniso(c);
+    if (rc) return rc;
+    if (!c->consts_done) compute_constants(c);
+    HIPCHK(c, c->out1.ensure(total * sizeof(float)));
+    launch_uniform_grid(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), ub, total, c->out1.as<float>(),
+                        c->ctr(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(scalars, c->out1.p, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PISURF_OK;
+    API_END(c)
+}
+
 int pisurf_sweep_level(pisurf_ctx* c, const float* centers, const float* halfs, const int32_t* depths, int64_t n,
                        uint8_t* type, double* f, float* curv, int32_t* signchange, double* samples, double* grads) {
     if (!c) return PISURF_E_INVALID;
@@ -1571,6 +1640,90 @@ int pisurf_run_frame(pisurf_ctx* c, pisurf_tree_info* tree) {
     if (!rc) rc = do_extract(c);
     if (rc) return rc;
     c->st.ms_total += now_ms() - t0;
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_run_frames(pisurf_ctx* c, int32_t nframes, const float* const* xyz, const int64_t* n,
+                      const float* radius, int32_t on_device, pisurf_mesh_cb on_mesh, void* user,
+                      pisurf_stats* per_frame) {
+    if (!c) return PISURF_E_INVALID;
+    if (nframes < 0 || (nframes > 0 && (!xyz || !n || !radius))) return c->fail(PISURF_E_INVALID, "bad frame arrays");
+    int64_t nmax = 1;
+    for (int32_t k = 0; k < nframes; k++) {
+        const std::string fk = " (frame " + std::to_string(k) + ")";
+        if (n[k] < 0 || (n[k] > 0 && !xyz[k])) return c->fail(PISURF_E_INVALID, "bad particle buffer" + fk);
+        if (n[k] >= (int64_t)INT32_MAX / 2) return c->fail(PISURF_E_UNSUPPORTED, "particle count exceeds 2^30" + fk);
+        if (!(radius[k] > 0.0f) || !std::isfinite(radius[k])) return c->fail(PISURF_E_INVALID, "radius must be > 0" + fk);
+        nmax = std::max(nmax, n[k]);
+    }
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    if (!c->cstream) HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+    const size_t bytes_max = (size_t)nmax * 3 * sizeof(float);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, c->xyz.ensure(bytes_max));
+    HIPCHK(c, c->xyz_next.ensure(bytes_max));
+    if (!on_device) HIPCHK(c, c->pin.ensure(bytes_max));
+    // frame k -> xyz_next on the copy stream, waited for (runs on a helper thread during the previous frame)
+    auto stage = [c, xyz, n, on_device](int32_t k) -> hipError_t {
+        const size_t b = (size_t)n[k] * 3 * sizeof(float);
+        if (b == 0) return hipSuccess;
+        hipError_t e = hipSetDevice(c->device);
+        if (e != hipSuccess) return e;
+        if (on_device) {
+            e = hipMemcpyAsync(c->xyz_next.p, xyz[k], b, hipMemcpyDeviceToDevice, c->cstream);
+        } else {
+            std::memcpy(c->pin.p, xyz[k], b);
+            e = hipMemcpyAsync(c->xyz_next.p, c->pin.p, b, hipMemcpyHostToDevice, c->cstream);
+        }
+        return e != hipSuccess ? e : hipStreamSynchronize(c->cstream);
+    };
+    struct Joiner {  // the helper is joined on every exit path, exceptions included
+        std::thread t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    };
+    const pisurf_params base = c->prm;
+    if (nframes > 0) {
+        const hipError_t e = stage(0);
+        if (e != hipSuccess) return c->hipfail(e, "frame upload (frame 0)");
+    }
+    for (int32_t k = 0; k < nframes; k++) {
+        std::swap(c->xyz.p, c->xyz_next.p);
+        std::swap(c->xyz.cap, c->xyz_next.cap);
+        hipError_t staged = hipSuccess;
+        int rc = PISURF_OK;
+        pisurf_tree_info ti;
+        std::memset(&ti, 0, sizeof(ti));
+        {
+            Joiner helper;
+            if (k + 1 < nframes) helper.t = std::thread([&stage, &staged, k] { staged = stage(k + 1); });
+            pisurf_params p = base;
+            p.radius = radius[k];
+            rc = pisurf_set_params(c, &p);
+            if (!rc) rc = pisurf_reset_stats(c);
+            c->n = n[k];
+            c->bbox_done = c->grid_done = c->aniso_done = c->normals_done = c->tree_done = c->consts_done = false;
+            if (!rc && n[k] > 0) {
+                rc = pisurf_run_frame(c, &ti);
+            } else if (!rc) {
+                c->mesh_v.clear();
+                c->mesh_t.clear();
+            }
+            if (!rc && per_frame) rc = sync_counters(c);
+        }
+        if (rc) {
+            c->err += " (frame " + std::to_string(k) + ")";
+            return rc;
+        }
+        if (staged != hipSuccess) return c->hipfail(staged, "frame upload");
+        if (per_frame) per_frame[k] = c->st;
+        if (on_mesh)
+            on_mesh(user, k, &ti, c->mesh_v.data(), (int64_t)c->mesh_v.size() / 3, c->mesh_t.data(),
+                    (int64_t)c->mesh_t.size() / 3);
+    }
     return PISURF_OK;
     API_END(c)
 }

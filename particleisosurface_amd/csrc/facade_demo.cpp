@@ -1,6 +1,7 @@
 // Drives one frame through the C++ mirror of the reference classes (host_facade.h), the way the
 // reference's runOurs (src/main.cpp:330-405) drives SurfReconstructor, and writes an OBJ.
-//   pisurf_facade_demo <particles.f32> <radius> <out.obj> [precision 0|1|2] [mesh_order 0|1]
+//   pisurf_facade_demo <particles.f32> <radius> <out.obj> [precision 0|1|2] [mesh_order 0|1] [vti_dir]
+// With vti_dir, UniformGrid (the reference's runUniform path) also writes <vti_dir>/grid.vti.
 // particles.f32: raw little-endian float32 x,y,z triples.
 #include <cstdio>
 #include <cstdlib>
@@ -46,7 +47,14 @@ int main(int argc, char** argv) {
         grid->GetPIdxList(particles[0], nb);
         float f0 = 0;
         ev.SingleEval(particles[0], f0);
-        std::printf("{\"verts\": %u, \"tris\": %u, \"depth_max\": %d, \"depth_min\": %d, \"cells\": %llu, "
+        int gdims[3] = {0, 0, 0};
+        if (argc > 6) {
+            pisurf::UniformGrid ug(particles, r);
+            ug.Run(0.0f, "grid", argv[6], precision);
+            for (int a = 0; a < 3; a++) gdims[a] = ug.dims[a];
+        }
+        std::printf("{\"grid_dims\": [%d, %d, %d], ", gdims[0], gdims[1], gdims[2]);
+        std::printf("\"verts\": %u, \"tris\": %u, \"depth_max\": %d, \"depth_min\": %d, \"cells\": %llu, "
                     "\"neighbors0\": %zu, \"f0\": %.9g, \"iso\": %.9g}\n",
                     mesh.verticesNum, mesh.trianglesNum, sr.getDepthMax(), sr.getDepthMin(), grid->CellNum,
                     nb.size(), (double)f0, (double)ev.getIsoValue());

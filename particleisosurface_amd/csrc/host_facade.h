@@ -10,6 +10,8 @@
 // pisurf::Error here.
 #pragma once
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -324,6 +326,69 @@ class SurfReconstructor {
     std::shared_ptr<Device> dev_;
     int _DEPTH_MAX = 0, _DEPTH_MIN = 0;
     float _RootHalfLength = 0, _RootCenter[3] = {0, 0, 0};
+};
+
+// Image data in VTK's XML format (what vtkXMLImageDataWriter produces for UniformGrid::Run,
+// src/marching.cpp:155-185), written with raw appended data: one Float32 point array "scalars",
+// x fastest, UInt64 block header.
+inline void writeVtiFile(const std::string& fn, const int dims[3], const float origin[3], float spacing,
+                         const std::vector<float>& scalars) {
+    FILE* f = std::fopen(fn.c_str(), "wb");
+    if (!f) throw Error(PISURF_E_INVALID, "cannot open " + fn);
+    std::fprintf(f,
+                 "<?xml version=\"1.0\"?>\n<VTKFile type=\"ImageData\" version=\"1.0\" byte_order=\"LittleEndian\" "
+                 "header_type=\"UInt64\">\n  <ImageData WholeExtent=\"0 %d 0 %d 0 %d\" Origin=\"%.9g %.9g %.9g\" "
+                 "Spacing=\"%.9g %.9g %.9g\" Direction=\"1 0 0 0 1 0 0 0 1\">\n    <Piece Extent=\"0 %d 0 %d 0 %d\">\n"
+                 "      <PointData Scalars=\"scalars\">\n        <DataArray type=\"Float32\" Name=\"scalars\" "
+                 "format=\"appended\" offset=\"0\"/>\n      </PointData>\n      <CellData>\n      </CellData>\n"
+                 "    </Piece>\n  </ImageData>\n  <AppendedData encoding=\"raw\">\n   _",
+                 dims[0] - 1, dims[1] - 1, dims[2] - 1, origin[0], origin[1], origin[2], spacing, spacing, spacing,
+                 dims[0] - 1, dims[1] - 1, dims[2] - 1);
+    const uint64_t nb = (uint64_t)scalars.size() * sizeof(float);
+    std::fwrite(&nb, sizeof(nb), 1, f);
+    if (nb) std::fwrite(scalars.data(), 1, nb, f);
+    std::fprintf(f, "\n  </AppendedData>\n</VTKFile>\n");
+    std::fclose(f);
+}
+
+// include/marching.h:18-72 (constant-radius constructor; the variable-radius one is disabled in
+// the reference's field path)
+class UniformGrid {
+  public:
+    float _RADIUS = 0;
+    int _GlobalParticlesNum = 0;
+    float _BoundingBox[6] = {0, 0, 0, 0, 0, 0};
+    int dims[3] = {0, 0, 0};
+    int steps[3] = {0, 0, 0};
+    std::vector<float> _Scalars;
+    std::vector<Vector3f> _GlobalParticles;
+
+    UniformGrid(const std::vector<Vector3f>& particles, float radius, int device = 0)
+        : _RADIUS(radius), _GlobalParticles(particles), dev_(std::make_shared<Device>(device)) {
+        _GlobalParticlesNum = (int)_GlobalParticles.size();
+    }
+    // Run (src/marching.cpp:93-186): samples the field on the resized box and writes
+    // <filepath>/<filename>.vti; iso_value is unused, as in the reference
+    void Run(float iso_value, const std::string& filename, const std::string& filepath,
+             int precision = PISURF_FP32) {
+        (void)iso_value;
+        pisurf_params& p = dev_->params();
+        p.radius = _RADIUS;
+        p.precision = precision;
+        dev_->apply();
+        dev_->check(pisurf_upload_particles(dev_->get(), &_GlobalParticles[0][0], _GlobalParticlesNum, 0), "upload");
+        float sp = 0;
+        dev_->check(pisurf_uniform_grid(dev_->get(), dims, _BoundingBox, &sp, nullptr), "uniform_grid");
+        for (int a = 0; a < 3; a++) steps[a] = dims[a] - 1;
+        _Scalars.resize((size_t)dims[0] * dims[1] * dims[2]);
+        dev_->check(pisurf_uniform_grid(dev_->get(), nullptr, nullptr, nullptr, _Scalars.data()), "uniform_grid");
+        const float origin[3] = {_BoundingBox[0], _BoundingBox[2], _BoundingBox[4]};
+        writeVtiFile(filepath + "/" + filename + ".vti", dims, origin, sp, _Scalars);
+    }
+    std::shared_ptr<Device> device() const { return dev_; }
+
+  private:
+    std::shared_ptr<Device> dev_;
 };
 
 }  // namespace pisurf

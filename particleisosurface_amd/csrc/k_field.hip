@@ -83,4 +83,49 @@ void launch_normals(int prec, GridDev g, const void* ra, const void* rg, FieldCo
     }
 }
 
+// UniformGrid::gridSampling (src/marching.cpp:72-91): point i = x + dims0 * (y + dims1 * z) at
+// float(steps - x) / float(steps) * min + float(x) / float(steps) * max per axis, each operation
+// rounded on its own (the reference is built without FMA contraction)
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_uniform_grid(GridDev g, Recs<R> rec, FieldConst fc, UniformBox ub,
+                                                      int64_t total, float* __restrict__ out, Counters* ctr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vis = 0;
+    if (i < total) {
+        const int64_t plane = (int64_t)ub.dims[0] * ub.dims[1];
+        const int z = (int)(i / plane);
+        const int64_t rem = i - (int64_t)z * plane;
+        const int y = (int)(rem / ub.dims[0]);
+        const int x = (int)(rem - (int64_t)y * ub.dims[0]);
+        const int ix[3] = {x, y, z};
+        float p[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float st = (float)ub.steps[a];
+            const float u = __fdiv_rn((float)(ub.steps[a] - ix[a]), st);
+            const float v = __fdiv_rn((float)ix[a], st);
+            p[a] = __fadd_rn(__fmul_rn(u, ub.lo[a]), __fmul_rn(v, ub.hi[a]));
+        }
+        R s, gr[3];
+        vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), s, gr);
+        out[i] = float(R(fc.iso) - R(fc.sigma) * s);
+    }
+    wave_count(&ctr->field_evals, i < total ? 1ull : 0ull);
+    wave_count(&ctr->candidates, (unsigned long long)vis);
+}
+
+void launch_uniform_grid(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, UniformBox ub,
+                         int64_t total, float* out, Counters* ctr, hipStream_t s) {
+    if (total <= 0) return;
+    const unsigned nb = (unsigned)((total + 255) / 256);
+    if (prec == 2) {
+        Recs<double> r{(const double4*)ra, (const double4*)rg};
+        k_uniform_grid<double, false><<<nb, 256, 0, s>>>(g, r, fc, ub, total, out, ctr);
+    } else {
+        Recs<float> r{(const float4*)ra, (const float4*)rg};
+        if (prec == 1) k_uniform_grid<float, true><<<nb, 256, 0, s>>>(g, r, fc, ub, total, out, ctr);
+        else k_uniform_grid<float, false><<<nb, 256, 0, s>>>(g, r, fc, ub, total, out, ctr);
+    }
+}
+
 }  // namespace pisurf

@@ -55,6 +55,13 @@ void launch_eval_points(int prec, GridDev g, const void* rec_a, const void* rec_
                         int64_t n, double* f, double* grad, Counters* ctr, hipStream_t s);
 void launch_normals(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float4* spos,
                     int64_t n, float4* nrm, Counters* ctr, hipStream_t s);
+// dense sampling box: lo / hi corners, steps (intervals) and dims = steps + 1 points per axis
+struct UniformBox {
+    float lo[3], hi[3];
+    int steps[3], dims[3];
+};
+void launch_uniform_grid(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, UniformBox ub,
+                         int64_t total, float* out, Counters* ctr, hipStream_t s);
 
 // ---- octree levels ----
 struct NodeLevel {

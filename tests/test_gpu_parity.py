@@ -254,8 +254,8 @@ def test_facade_driver(tmp_path):
     xyz.astype(np.float32).tofile(pf)
     exe = os.path.join(os.path.dirname(capi.LIB_PATH), "pisurf_facade_demo")
     out = tmp_path / "m.obj"
-    res = subprocess.run([exe, str(pf), repr(float(np.float32(r))), str(out), "0", "1"], capture_output=True,
-                         text=True, timeout=120)
+    res = subprocess.run([exe, str(pf), repr(float(np.float32(r))), str(out), "0", "1", str(tmp_path)],
+                         capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     info = json.loads(res.stdout)
     o = Oracle(xyz, r)
@@ -273,6 +273,13 @@ def test_facade_driver(tmp_path):
     vb = np.array([[float(x) for x in l.split()[1:]] for l in ref_lines if l.startswith("v")])
     assert np.mean(np.linalg.norm(va - vb, axis=1) <= 1e-3 * r + 2e-6) > 0.995
     assert info["neighbors0"] > 0 and abs(info["iso"] - o.iso_constants()[1]) <= 1e-6 * info["iso"]
+    # the UniformGrid mirror wrote the same dense field as the C ABI
+    from particleisosurface_amd.vti import read_vti
+    g, origin, spacing = read_vti(str(tmp_path / "grid.vti"))
+    vals, box, sp = gpu_context(xyz, r).uniform_grid()
+    assert list(info["grid_dims"]) == [vals.shape[2], vals.shape[1], vals.shape[0]]
+    assert np.array_equal(g, vals)
+    assert np.allclose(origin, box[0::2], rtol=0, atol=1e-7) and np.allclose(spacing, sp)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
@@ -484,3 +491,80 @@ def test_sample_dedupe_is_bit_identical(case, prec):
     assert sa["sample_evals"] == sb["sample_evals"] and sa["field_evals"] == sb["field_evals"]
     assert sb["sample_unique"] == 0
     assert 0 < sa["sample_unique"] < sa["sample_evals"]
+
+
+def test_run_frames_matches_fresh_contexts():
+    """The persistent multi-frame call meshes each frame exactly as a fresh context does, with
+    frames of different sizes and radii in one batch (nothing carries over), an empty frame
+    included."""
+    frames = [synth.dam_break(), synth.lattice_sphere(), (np.zeros((0, 3), np.float32), 0.004),
+              synth.random_cloud(20000, r=0.004, extent=0.2), synth.dam_break()]
+    got = {}
+
+    def on_mesh(k, v, t, tree):
+        got[k] = (v, t, tree)
+
+    ctx = capi.Context(0)
+    ctx.set_params(0.01, precision=capi.FP32_KAHAN)
+    per = ctx.run_frames([f for f, _ in frames], [r for _, r in frames], on_mesh)
+    assert sorted(got) == list(range(len(frames)))
+    for k, (xyz, r) in enumerate(frames):
+        v, t, tree = got[k]
+        if len(xyz) == 0:
+            assert len(v) == 0 and len(t) == 0 and per[k]["field_evals"] == 0
+            continue
+        ref = gpu_context(xyz, r, precision=capi.FP32_KAHAN)
+        info = ref.run_frame()
+        v2, t2 = ref.mesh()
+        assert tree == info, k
+        assert np.array_equal(v, v2) and np.array_equal(t, t2), k
+        assert per[k]["field_evals"] == ref.stats()["field_evals"], k
+
+
+def _uniform_points(box, dims, idx):
+    """UniformGrid::gridSampling's point of linear index idx (x fastest), each float op rounded."""
+    lo, hi = box[0::2], box[1::2]
+    d0, d1 = dims[0], dims[1]
+    ix = np.stack([idx % d0, (idx // d0) % d1, idx // (d0 * d1)], 1)
+    st = (np.asarray(dims) - 1).astype(np.float32)
+    u = (st - ix.astype(np.float32)) / st
+    v = ix.astype(np.float32) / st
+    return (u * lo + v * hi).astype(np.float32)
+
+
+@pytest.mark.parametrize("prec,ref", [("fp32", "fp32"), ("fp32k", "fp64"), ("fp64", "fp64")])
+def test_uniform_grid_matches_oracle(prec, ref):
+    """Dense sampling (UniformGrid): box and dims from the tight box, values = SingleEval."""
+    rel, eps = FIELD_TOL[prec]
+    xyz, r = synth.dam_break()
+    ctx = gpu_context(xyz, r, precision=PREC[prec])
+    vals, box, sp = ctx.uniform_grid()
+    dims = (vals.shape[2], vals.shape[1], vals.shape[0])
+    lo, hi = xyz.min(0), xyz.max(0)
+    r32 = np.float32(r)
+    for a in range(3):
+        steps = dims[a] - 1
+        # ceil(len / r) * r + 4 r about the centre: at least 2 r of margin, at most 3 r
+        assert box[2 * a] <= lo[a] - 1.99 * r and box[2 * a + 1] >= hi[a] + 1.99 * r
+        assert (box[2 * a + 1] - box[2 * a]) <= (hi[a] - lo[a]) + 6.01 * r
+        assert abs(steps * r32 - (box[2 * a + 1] - box[2 * a])) <= 1.01 * r
+    assert sp == pytest.approx(r, rel=1e-7)
+    total = vals.size
+    corners = _uniform_points(box, dims, np.array([0, total - 1]))
+    assert np.array_equal(corners[0], box[0::2]) and np.array_equal(corners[1], box[1::2])
+    o = Oracle(xyz, r, fp64=(ref == "fp64"))
+    o.anisotropy()
+    ms, iso = o.iso_constants()
+    rng = np.random.default_rng(5)
+    idx = rng.choice(total, 60000, replace=False)
+    f2 = o.eval_points(_uniform_points(box, dims, idx))
+    f = vals.ravel()[idx].astype(np.float64)
+    if prec == "fp64":
+        f32 = f2.astype(np.float32)
+        assert np.all(np.abs(f - f32) <= np.spacing(np.abs(f32)) + 1e-10 * iso)
+    else:
+        S, S2 = iso - f, iso - f2
+        assert np.max(np.abs(S - S2) / np.maximum(np.abs(S2), iso)) <= rel + 6e-8
+    tie = np.abs(f2) <= eps * iso
+    assert np.array_equal((f > 0)[~tie], (f2 > 0)[~tie])
+    assert np.any(f < 0) and np.any(f > 0)
