@@ -52,6 +52,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("facade demo build failed:\n" + r.stdout + r.stderr)
     os.replace(demo + ".tmp", demo)
+    # the frame driver over a data directory (controlData.json, CSV / HDF5 / raw frames)
+    run = os.path.join(HERE, "pisurf_run")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "pisurf_run.cpp"), "-o", run + ".tmp",
+           "-L" + HERE, "-lpisurf", "-ldl", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("frame driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(run + ".tmp", run)
     return LIB
 
 

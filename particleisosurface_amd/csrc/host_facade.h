@@ -267,6 +267,25 @@ inline void writeObjFile(const Mesh& m, const std::string& fn) {
     std::fclose(f);
 }
 
+// writePlyFile (src/main.cpp:69-114): binary little-endian PLY as the reference's rply writer lays
+// it out: float32 x, y, z per vertex, faces as a uchar count and 0-based int indices
+inline void writePlyFile(const Mesh& m, const std::string& fn) {
+    FILE* f = std::fopen(fn.c_str(), "wb");
+    if (!f) throw Error(PISURF_E_INVALID, "cannot open " + fn);
+    std::fprintf(f,
+                 "ply\nformat binary_little_endian 1.0\nelement vertex %ld\nproperty float32 x\nproperty float32 y\n"
+                 "property float32 z\nelement face %ld\nproperty list uchar int vertex_indices\nend_header\n",
+                 (long)m.vertices.size(), (long)m.tris.size());
+    for (const auto& p : m.vertices) std::fwrite(p.v, sizeof(float), 3, f);
+    for (const auto& t : m.tris) {
+        const unsigned char three = 3;
+        const int32_t idx[3] = {t.v[0] - 1, t.v[1] - 1, t.v[2] - 1};
+        std::fwrite(&three, 1, 1, f);
+        std::fwrite(idx, sizeof(int32_t), 3, f);
+    }
+    std::fclose(f);
+}
+
 // include/surface_reconstructor.h:21-93
 class SurfReconstructor {
   public:

@@ -568,3 +568,32 @@ def test_uniform_grid_matches_oracle(prec, ref):
     tie = np.abs(f2) <= eps * iso
     assert np.array_equal((f > 0)[~tie], (f2 > 0)[~tie])
     assert np.any(f < 0) and np.any(f > 0)
+
+
+def test_frame_driver_directory(tmp_path):
+    """pisurf_run meshes the frames a controlData.json lists and writes one OBJ per frame, equal to
+    the C ABI's mesh of the same particles."""
+    import json
+    import os
+    import subprocess
+    xyz, r = synth.dam_break()
+    with open(tmp_path / "frame_1.csv", "w") as f:
+        f.write('"Points:0","Points:1","Points:2"\n')
+        for p in xyz:
+            f.write("%r,%r,%r\n" % (float(p[0]), float(p[1]), float(p[2])))
+    sphere, rs = synth.lattice_sphere()
+    sphere.astype(np.float32).tofile(tmp_path / "frame_2.f32")
+    (tmp_path / "controlData.json").write_text(json.dumps(
+        {"DATA_FILE": "frame_1.csv,frame_2.f32", "RADIUS": float(np.float32(r)), "OUTPUT_TYPE": "obj"}))
+    exe = os.path.join(os.path.dirname(capi.LIB_PATH), "pisurf_run")
+    res = subprocess.run([exe, str(tmp_path), str(tmp_path / "out")], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    ctx = gpu_context(xyz, float(np.float32(r)))
+    ctx.run_frame()
+    v, t = ctx.mesh()
+    lines = (tmp_path / "out" / "frame_1.obj").read_text().splitlines()
+    faces = np.array([[int(x) for x in l.split()[1:]] for l in lines if l.startswith("f")])
+    verts = np.array([[float(x) for x in l.split()[1:]] for l in lines if l.startswith("v")])
+    assert np.array_equal(faces, t) and verts.shape == v.shape
+    assert np.max(np.abs(verts - v)) <= 5e-7 + 1e-6 * np.abs(v).max()
+    assert (tmp_path / "out" / "frame_2.obj").stat().st_size > 0
