@@ -246,8 +246,8 @@ def main():
         rel = np.abs(S32 - S64)[mask] / S64[mask]
         nn = np.linalg.norm(n32, axis=1) * np.linalg.norm(n64, axis=1)
         ok = nn > 0
-        cosang = np.clip(np.sum(n32 * n64, axis=1)[ok] / nn[ok], -1, 1)
-        ang = np.degrees(np.arccos(cosang))
+        ang = np.degrees(np.arctan2(np.linalg.norm(np.cross(n32[ok], n64[ok]), axis=1),
+                                    np.sum(n32[ok] * n64[ok], axis=1)))
         out["fp64_frame_ms"] = round(ms64, 3)
         out["speedup_vs_fp64"] = round(ms64 / ms, 3)
         out["error_vs_fp64"] = {

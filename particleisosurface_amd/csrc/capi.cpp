@@ -200,7 +200,7 @@ struct pisurf_ctx {
     PinnedBuf pin;
     hipStream_t cstream = nullptr;
     // device extraction workspace
-    DBuf eb[24];
+    DBuf eb[25];
     // mesh
     std::vector<double> mesh_v;
     std::vector<int32_t> mesh_t;
@@ -208,6 +208,7 @@ struct pisurf_ctx {
     DBuf counters;
     pisurf_stats st;
     unsigned long long n_fp64_eig = 0;
+    unsigned long long host_sample_entries = 0;  // deduplicated sampling: 27 per sampled node
     std::vector<hipEvent_t> ev;  // start/stop pairs around node-sampling launches
     size_t ev_used = 0;
 
@@ -592,6 +593,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
                                       c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), nent, c->sd_head.as<int>(),
                                       c->sd_hpos.as<int>(), c->sd_rep.as<int>(), c->sd_hlist.as<int>(), &nheads,
                                       c->sd_vals.p, c->samples.p, c->tmp.as<int>(), c->ctr(), e0, e1, c->stream);
+            c->host_sample_entries += (unsigned long long)nent;
         }
     } else {
         hipEvent_t e0, e1;
@@ -892,14 +894,14 @@ int do_extract_device(pisurf_ctx* c) {
     // points -> dual cells
     HIPCHK(c, E[8].ensure((nent + 1) * 4));
     HIPCHK(c, E[9].ensure((nent + 1) * 4));
-    launch_points(tv, lidx, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(), nent, E[8].as<int>(), nullptr, nullptr, s);
+    HIPCHK(c, E[24].ensure(std::max<int64_t>(nent, 1) * 9 * 4));
+    launch_points(tv, lidx, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(), nent, E[8].as<int>(), E[24].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[8].as<int>() + nent, 0, 4, s));
     exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nent + 1, c->tmp.as<int>(), s);
     int64_t nd;
     if ((rc = read_int(c, E[9].as<int>() + nent, nd))) return rc;
     HIPCHK(c, E[10].ensure(std::max<int64_t>(nd, 1) * 9 * 4));
-    launch_points(tv, lidx, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(), nent, E[8].as<int>(), E[9].as<int>(),
-                  E[10].as<int>(), s);
+    launch_points_compact(E[8].as<int>(), E[9].as<int>(), E[24].as<int>(), nent, E[10].as<int>(), s);
     // triangles
     HIPCHK(c, E[11].ensure((nd + 1) * 4));
     HIPCHK(c, E[12].ensure((nd + 1) * 4));
@@ -1072,11 +1074,11 @@ int sync_counters(pisurf_ctx* c) {
     Counters h;
     HIPCHK(c, hipMemcpyAsync(&h, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->st.field_evals = (int64_t)h.field_evals;
+    c->st.field_evals = (int64_t)(h.field_evals + c->host_sample_entries);
     c->st.grad_evals = (int64_t)h.grad_evals;
     c->st.candidates = (int64_t)h.candidates;
     c->st.qef_solves = (int64_t)h.qef_solves;
-    c->st.sample_evals = (int64_t)h.sample_evals;
+    c->st.sample_evals = (int64_t)(h.sample_evals + c->host_sample_entries);
     c->st.sample_candidates = (int64_t)h.sample_candidates;
     c->st.sample_unique = (int64_t)h.sample_unique;
     c->st.fp64_eigensolves = (int64_t)c->n_fp64_eig;
@@ -1739,6 +1741,8 @@ int pisurf_get_stats(pisurf_ctx* c, pisurf_stats* out) {
 int pisurf_reset_stats(pisurf_ctx* c) {
     if (!c) return PISURF_E_INVALID;
     std::memset(&c->st, 0, sizeof(c->st));
+    c->host_sample_entries = 0;
+    c->n_fp64_eig = 0;
     HIPCHK(c, hipMemsetAsync(c->counters.p, 0, sizeof(Counters), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PISURF_OK;

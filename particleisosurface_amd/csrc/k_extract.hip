@@ -98,30 +98,38 @@ __device__ __forceinline__ int sign_index(const double4* leaf_node, const int n[
     return s;
 }
 
-// pass 1: flag points whose dual cell is non-trivial; pass 2: write the dual cells
+// flag points whose dual cell is non-trivial and stage the cell (8 leaves in case order, sign
+// configuration) at the point's sorted position; k_points_compact then packs the staged cells
 __global__ void k_points(TreeView tv, const int* __restrict__ lidx, const double4* __restrict__ leaf_node,
                          const uint64_t* __restrict__ skey, const int* __restrict__ sval, int64_t nent,
-                         int* __restrict__ dflag, const int* __restrict__ dpos, int* __restrict__ duals) {
+                         int* __restrict__ dflag, int* __restrict__ stage) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nent) return;
     int n[8];
     if (!gather_point(tv, lidx, skey, sval, nent, j, n)) {
-        if (!duals) dflag[j] = 0;
+        dflag[j] = 0;
         return;
     }
     const int s = sign_index(leaf_node, n);
     const uint8_t code = d_kDmcCase[s];
     const bool keep = code != 0 && code != 255;
-    if (!duals) {
-        dflag[j] = keep ? 1 : 0;
-        return;
-    }
+    dflag[j] = keep ? 1 : 0;
     if (!keep) return;
-    int* d = duals + 9 * (int64_t)dpos[j];
+    int* d = stage + 9 * j;
     const uint32_t perm = d_kDmcPerm[s];
 #pragma unroll
     for (int k = 0; k < 8; k++) d[k] = n[(perm >> (4 * k)) & 15];
     d[8] = s;
+}
+
+__global__ void k_points_compact(const int* __restrict__ dflag, const int* __restrict__ dpos,
+                                 const int* __restrict__ stage, int64_t nent, int* __restrict__ duals) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nent || !dflag[j]) return;
+    const int* src = stage + 9 * j;
+    int* d = duals + 9 * (int64_t)dpos[j];
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = src[k];
 }
 
 __device__ __forceinline__ int canon_slot(uint8_t code) {
@@ -245,8 +253,12 @@ void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double
     if (lv.n > 0) k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, leaf_node, ckey, cval, n_valid);
 }
 void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, const uint64_t* skey,
-                   const int* sval, int64_t nent, int* dflag, const int* dpos, int* duals, hipStream_t s) {
-    if (nent > 0) k_points<<<nblk(nent), 256, 0, s>>>(tv, lidx, leaf_node, skey, sval, nent, dflag, dpos, duals);
+                   const int* sval, int64_t nent, int* dflag, int* stage, hipStream_t s) {
+    if (nent > 0) k_points<<<nblk(nent), 256, 0, s>>>(tv, lidx, leaf_node, skey, sval, nent, dflag, stage);
+}
+void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
+                           hipStream_t s) {
+    if (nent > 0) k_points_compact<<<nblk(nent), 256, 0, s>>>(dflag, dpos, stage, nent, duals);
 }
 void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s) {
     if (nd > 0) k_tri_count<<<nblk(nd), 256, 0, s>>>(duals, nd, ntri);

@@ -101,10 +101,13 @@ __global__ void __launch_bounds__(256) k_uniform_grid(GridDev g, Recs<R> rec, Fi
         float p[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
+            // plain operators with contraction off in this scope: the *_rn helpers are inlined as
+            // contractable operations, and a fused product would round differently
+#pragma clang fp contract(off)
             const float st = (float)ub.steps[a];
-            const float u = __fdiv_rn((float)(ub.steps[a] - ix[a]), st);
-            const float v = __fdiv_rn((float)ix[a], st);
-            p[a] = __fadd_rn(__fmul_rn(u, ub.lo[a]), __fmul_rn(v, ub.hi[a]));
+            const float u = (float)(ub.steps[a] - ix[a]) / st;
+            const float v = (float)ix[a] / st;
+            p[a] = u * ub.lo[a] + v * ub.hi[a];
         }
         R s, gr[3];
         vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), s, gr);

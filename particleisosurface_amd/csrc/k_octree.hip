@@ -46,20 +46,23 @@ __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __r
         bc1[a] = fmaxf(b1[a], tp.b_lo[a]);
         bc2[a] = fminf(b2[a], tp.b_hi[a]);
     }
-    int lo[3], hi[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const int l = cell_coord(bc1[a], g.b0[a], g.cell, g.n[a]);
-        const int h = cell_coord(bc2[a], g.b0[a], g.cell, g.n[a]);
-        lo[a] = max(l - 1, 0);
-        hi[a] = min(h + 1, g.n[a] - 1);
-    }
     const bool need_curv = tp.calc_normals && depth >= tp.depth_min;
     float lx[3], hx[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         lx[a] = __fsub_rn(b1[a], tp.margin);
         hx[a] = __fadd_rn(b2[a], tp.margin);
+    }
+    // cells [lo-1, hi+1] of the clamped box, narrowed to the cells that can hold a particle of the
+    // grown box: the cell coordinate is monotone in the position, so a particle with lx < p < hx
+    // lies in a cell between those of lx and hx (the particles visited are the same)
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int l = cell_coord(bc1[a], g.b0[a], g.cell, g.n[a]);
+        const int h = cell_coord(bc2[a], g.b0[a], g.cell, g.n[a]);
+        lo[a] = max(max(l - 1, 0), cell_coord(lx[a], g.b0[a], g.cell, g.n[a]));
+        hi[a] = min(min(h + 1, g.n[a] - 1), cell_coord(hx[a], g.b0[a], g.cell, g.n[a]));
     }
     bool found = false;
     float sx = 0.f, sy = 0.f, sz = 0.f, area = 0.f;
@@ -221,21 +224,18 @@ __global__ void __launch_bounds__(256) k_sample_eval(GridDev g, Recs<R> rec, Fie
     wave_add(&ctr->sample_candidates, (unsigned long long)vis);
 }
 
-// every entry receives its point's value; entries are what the reference evaluates (27 per
-// sampled node), so they are what field_evals / sample_evals count
+// every entry receives its point's value (the host counts the entries: 27 per sampled node, the
+// reference's evaluations)
 __global__ void k_sample_scatter(const int* __restrict__ list, const int* __restrict__ val,
                                  const int* __restrict__ rep, const int* __restrict__ hpos, int64_t nent,
-                                 const void* __restrict__ vals, int rsz, void* __restrict__ samples, Counters* ctr) {
+                                 const void* __restrict__ vals, int rsz, void* __restrict__ samples) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nent) {
-        const int e = val[j];
-        const int64_t dst = (int64_t)list[e / 27] * 27 + e % 27;
-        const int64_t src = hpos[rep[j]];
-        if (rsz == 8) ((double*)samples)[dst] = ((const double*)vals)[src];
-        else ((float*)samples)[dst] = ((const float*)vals)[src];
-    }
-    wave_add(&ctr->field_evals, j < nent ? 1ull : 0ull);
-    wave_add(&ctr->sample_evals, j < nent ? 1ull : 0ull);
+    if (j >= nent) return;
+    const int e = val[j];
+    const int64_t dst = (int64_t)list[e / 27] * 27 + e % 27;
+    const int64_t src = hpos[rep[j]];
+    if (rsz == 8) ((double*)samples)[dst] = ((const double*)vals)[src];
+    else ((float*)samples)[dst] = ((const float*)vals)[src];
 }
 
 // centre values of leaf-depth nodes (sample slot 13), one thread per listed node
@@ -472,7 +472,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     }
     (void)hipEventRecord(e1, s);
     k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, val, rep, hpos, nent, vals,
-                                                                    prec == 2 ? 8 : 4, samples, ctr);
+                                                                    prec == 2 ? 8 : 4, samples);
 }
 
 void launch_centers(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv, const int* list,

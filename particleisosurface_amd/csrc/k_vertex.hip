@@ -22,6 +22,8 @@ __global__ void __launch_bounds__(256) k_bisect(GridDev g, Recs<R> rec, FieldCon
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int vis = 0, nev = 0;
     if (i < n) {
+        // the edge length and the interpolation round every operation as the reference does
+#pragma clang fp contract(off)
         R v1[4], v2[4];
         for (int a = 0; a < 4; a++) {
             v1[a] = R(ends[8 * i + a]);

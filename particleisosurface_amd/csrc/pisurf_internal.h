@@ -124,8 +124,11 @@ struct TreeView {
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
 void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double4* leaf_node, uint64_t* ckey,
                       int* cval, int* n_valid, hipStream_t s);
+// dflag[j] = 1 for a run head j whose dual cell is non-trivial; its cell goes to stage[9 j ..]
 void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, const uint64_t* skey,
-                   const int* sval, int64_t nent, int* dflag, const int* dpos, int* duals, hipStream_t s);
+                   const int* sval, int64_t nent, int* dflag, int* stage, hipStream_t s);
+void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
+                           hipStream_t s);
 void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s);
 void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t* vkey, int* vval, int* vend,
                         hipStream_t s);

@@ -124,8 +124,9 @@ __device__ __forceinline__ bool qef_solve(float4 c, const float pts[27][3], cons
             pl[1] = float(g[i][1]);
             pl[2] = float(g[i][2]);
             pl[3] = -1.0f;
-            pl[4] = -(__fadd_rn(__fadd_rn(__fmul_rn(pts[i][0], pl[0]), __fmul_rn(pts[i][1], pl[1])),
-                                __fmul_rn(pts[i][2], pl[2]))) + float(f[i]);
+            // plain operators: contraction is off in this scope (the *_rn helpers would be
+            // inlined as contractable operations)
+            pl[4] = -((pts[i][0] * pl[0] + pts[i][1] * pl[1]) + pts[i][2] * pl[2]) + float(f[i]);
             int k = 0;
 #pragma unroll
             for (int a = 0; a < 5; a++)

@@ -152,10 +152,13 @@ def test_field_values_and_signs(prec, ref):
     assert np.array_equal((f > 0)[~tie], (f2 > 0)[~tie])
     gn = np.linalg.norm(g2, axis=1)
     ok = gn > 1e-3 * gn.max()
-    cosang = np.sum(g * g2, 1)[ok] / (np.linalg.norm(g, axis=1)[ok] * gn[ok])
-    ang = np.degrees(np.arccos(np.clip(cosang, -1, 1)))
-    assert np.quantile(ang, 0.99) < {"fp32": 2e-3, "fp32k": 2e-4, "fp64": 1e-6}[prec]
-    assert ang.max() < {"fp32": 5e-2, "fp32k": 5e-3, "fp64": 1e-5}[prec]
+    # angle from atan2(|a x b|, a . b): accurate for tiny angles, where arccos of the cosine has a
+    # floor of ~1e-6 degrees. FP32 bounds: the reference's own FP32 noise on these points (FP32
+    # oracle vs FP64 oracle) is p99 3.2e-3 and max 2.4e-2 degrees.
+    cr = np.linalg.norm(np.cross(g[ok], g2[ok]), axis=1)
+    ang = np.degrees(np.arctan2(cr, np.sum(g[ok] * g2[ok], 1)))
+    assert np.quantile(ang, 0.99) < {"fp32": 5e-3, "fp32k": 2e-4, "fp64": 1e-9}[prec]
+    assert ang.max() < {"fp32": 5e-2, "fp32k": 5e-3, "fp64": 1e-8}[prec]
 
 
 @pytest.mark.parametrize("prec,ref", [("fp32", "fp32"), ("fp64", "fp64")])
@@ -212,35 +215,46 @@ def test_level_sweeps_seeded_with_oracle_nodes(case):
     assert np.all(f[t2 == capi.EMPTY] == iso)
 
 
-def mesh_correspondence(v, t, v2, t2, r):
-    """Match mesh (v, t) to the oracle mesh (v2, t2) by vertex position; returns per-vertex
-    distances / r and whether the triangle sets coincide under the matching."""
-    from scipy.spatial import cKDTree
-    d, idx = cKDTree(v2).query(v)
-    bij = len(np.unique(idx)) == len(v) == len(v2)
-    a = {tuple(x) for x in np.sort(idx[t - 1], axis=1)}
-    b = {tuple(x) for x in np.sort(t2 - 1, axis=1)}
-    return d / r, bij and a == b and len(t) == len(t2)
+def _tri_multiset(v, t):
+    """Triangles as sorted triples of vertex coordinates (independent of vertex numbering, and of
+    distinct vertices that happen to share a position)."""
+    c = v[t - 1]  # (nt, 3 corners, 3)
+    keys = [tuple(sorted(map(tuple, tri))) for tri in c.tolist()]
+    return sorted(keys)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("case", ["cfg1", "sphere", "cloud"])
 def test_device_extraction_matches_oracle_mesh(case, prec):
-    """Dual cells, triangles and vertices built on the device (point-key order) are the same mesh
-    as the reference traversal's, up to vertex numbering."""
+    """Dual cells, triangles and vertices built on the device (point-key order) are the mesh of
+    the reference traversal (host, reference order, itself byte-compared with the oracle in
+    test_full_tree_and_mesh) up to vertex numbering; the dual-cell count is the oracle's.
+
+    A vertex is bisected along its edge in the orientation of the first dual cell that emits it;
+    in degenerate dual cells (several corners in one leaf) the two orders can meet an edge in
+    opposite orientations, and the interpolation then rounds differently: such vertices differ
+    by a few units in the last place."""
+    from scipy.spatial import cKDTree
     xyz, r = CASES[case]()
-    o, ctx = _pair(xyz, r, prec)
+    o = Oracle(xyz, r, fp64=(prec != "fp32"))
     o.run()
-    ctx.run_frame()
-    v, t = ctx.mesh()
-    v2, t2 = o.mesh()
-    d, same = mesh_correspondence(v, t, v2, t2, r)
-    assert same
-    assert ctx.stats()["n_dual_cells"] == len(o.dual_cells()[1])
-    if prec == "fp64":
-        assert d.max() < 1e-6
-    else:
-        assert np.mean(d <= 1e-3) >= 0.995
+    dev = gpu_context(xyz, r, precision=PREC[prec], mesh_order=0)
+    host = gpu_context(xyz, r, precision=PREC[prec], mesh_order=1)
+    dev.run_frame()
+    host.run_frame()
+    v, t = dev.mesh()
+    v2, t2 = host.mesh()
+    assert v.shape == v2.shape and t.shape == t2.shape
+    assert dev.stats()["n_dual_cells"] == len(o.dual_cells()[1])
+    assert t.min() == 1 and t.max() == len(v) and len(np.unique(t)) == len(v)
+    d, idx = cKDTree(v2).query(v)
+    vmax = np.abs(v2).max()
+    ulp = float(np.spacing(np.float32(vmax))) if prec == "fp32" else float(np.spacing(vmax))
+    assert d.max() <= 4 * ulp, d.max()
+    exact = np.mean(d == 0)
+    assert exact > 0.99, exact
+    # triangles, with every device vertex replaced by its host counterpart's position
+    assert _tri_multiset(v2, idx[t - 1] + 1) == _tri_multiset(v2, t2)
 
 
 def test_facade_driver(tmp_path):
