@@ -59,6 +59,45 @@ struct PinnedBuf {
     }
 };
 
+// growable array in page-locked host memory: the mesh is copied off the device at full link speed
+template <class T> struct PinnedVec {
+    T* p = nullptr;
+    size_t n = 0, cap = 0;
+    PinnedVec() = default;
+    PinnedVec(const PinnedVec&) = delete;
+    PinnedVec& operator=(const PinnedVec&) = delete;
+    ~PinnedVec() {
+        if (p) (void)hipHostFree(p);
+    }
+    void reserve(size_t m) {
+        if (m <= cap) return;
+        size_t c2 = std::max(m, cap + cap / 2);
+        void* q = nullptr;
+        if (hipHostMalloc(&q, c2 * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        if (n) std::memcpy(q, p, n * sizeof(T));
+        if (p) (void)hipHostFree(p);
+        p = (T*)q;
+        cap = c2;
+    }
+    void resize(size_t m) {  // new elements are left uninitialised
+        reserve(m);
+        n = m;
+    }
+    void assign(size_t m, T v) {
+        resize(m);
+        for (size_t i = 0; i < m; i++) p[i] = v;
+    }
+    void push_back(T v) {
+        if (n == cap) reserve(n + 1);
+        p[n++] = v;
+    }
+    void clear() { n = 0; }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+    T* data() { return p; }
+    const T* data() const { return p; }
+};
+
 struct LevelBuf {
     int64_t n = 0;
     DBuf c, depth, id, type, empty, curv, node, flag, child, scanned, ip;
@@ -200,10 +239,10 @@ struct pisurf_ctx {
     PinnedBuf pin;
     hipStream_t cstream = nullptr;
     // device extraction workspace
-    DBuf eb[25];
+    DBuf eb[28];
     // mesh
-    std::vector<double> mesh_v;
-    std::vector<int32_t> mesh_t;
+    PinnedVec<double> mesh_v;
+    PinnedVec<int32_t> mesh_t;
     // stats
     DBuf counters;
     pisurf_stats st;
@@ -550,9 +589,14 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, c->tmp.ensure(need_tmp * 4));
     }
-    // deep levels: node boxes span a few cells, so 16 lanes per node
-    launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v,
-                      level_depth >= 0 && level_depth >= c->dmin - 1, c->stream);
+    // lanes per node ~ (y, z) cell rows of the grown node box: (width / cell + 1)^2 on average
+    int group = 64;
+    if (level_depth >= 0) {
+        const double half = std::ldexp((double)c->root_half, -level_depth);
+        const double w = (2 * half + 2 * (double)tp.margin) / (double)c->g.cell + 1;
+        group = (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
+    }
+    launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group, c->stream);
     // the level's flag / scanned / child arrays serve as the sample list until classification
     HIPCHK(c, c->cl_flag.ensure((L->n + 1) * 4));
     HIPCHK(c, c->cl_pos.ensure((L->n + 1) * 4));
@@ -818,6 +862,12 @@ struct Extractor {
     }
 };
 
+int64_t pow2_at_least(int64_t n) {
+    int64_t p = 1024;
+    while (p < n) p <<= 1;
+    return p;
+}
+
 int bits_for(int64_t v) {
     int b = 1;
     while (b < 63 && ((int64_t)1 << b) <= v) b++;
@@ -892,16 +942,27 @@ int do_extract_device(pisurf_ctx* c) {
     int64_t nent;
     if ((rc = read_int(c, E[7].as<int>(), nent))) return rc;
     // points -> dual cells
+    // distinct points: run heads of the sorted corner keys, one thread each
     HIPCHK(c, E[8].ensure((nent + 1) * 4));
     HIPCHK(c, E[9].ensure((nent + 1) * 4));
-    HIPCHK(c, E[24].ensure(std::max<int64_t>(nent, 1) * 9 * 4));
-    launch_points(tv, lidx, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(), nent, E[8].as<int>(), E[24].as<int>(), s);
+    launch_run_heads(E[3].as<uint64_t>(), nent, E[8].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[8].as<int>() + nent, 0, 4, s));
     exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nent + 1, c->tmp.as<int>(), s);
+    int64_t npts;
+    if ((rc = read_int(c, E[9].as<int>() + nent, npts))) return rc;
+    HIPCHK(c, E[25].ensure(std::max<int64_t>(npts, 1) * 4));
+    launch_head_positions(E[8].as<int>(), E[9].as<int>(), nent, E[25].as<int>(), s);
+    HIPCHK(c, E[26].ensure((npts + 1) * 4));
+    HIPCHK(c, E[27].ensure((npts + 1) * 4));
+    HIPCHK(c, E[24].ensure(std::max<int64_t>(npts, 1) * 9 * 4));
+    launch_points(tv, lidx, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(), nent, E[25].as<int>(), npts,
+                  E[26].as<int>(), E[24].as<int>(), s);
+    HIPCHK(c, hipMemsetAsync(E[26].as<int>() + npts, 0, 4, s));
+    exclusive_scan_i32(E[26].as<int>(), E[27].as<int>(), npts + 1, c->tmp.as<int>(), s);
     int64_t nd;
-    if ((rc = read_int(c, E[9].as<int>() + nent, nd))) return rc;
+    if ((rc = read_int(c, E[27].as<int>() + npts, nd))) return rc;
     HIPCHK(c, E[10].ensure(std::max<int64_t>(nd, 1) * 9 * 4));
-    launch_points_compact(E[8].as<int>(), E[9].as<int>(), E[24].as<int>(), nent, E[10].as<int>(), s);
+    launch_points_compact(E[26].as<int>(), E[27].as<int>(), E[24].as<int>(), npts, E[10].as<int>(), s);
     // triangles
     HIPCHK(c, E[11].ensure((nd + 1) * 4));
     HIPCHK(c, E[12].ensure((nd + 1) * 4));
@@ -917,13 +978,16 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[16].ensure(std::max<int64_t>(nc, 1) * 4));
     HIPCHK(c, E[17].ensure(std::max<int64_t>(nc, 1) * 8));
     launch_tri_corners(E[10].as<int>(), nd, E[12].as<int>(), E[13].as<uint64_t>(), E[14].as<int>(), E[17].as<int>(), s);
-    if ((rc = ensure_tmp(c, nc + 1))) return rc;
-    radix_sort_pairs64(E[13].as<uint64_t>(), E[14].as<int>(), E[15].as<uint64_t>(), E[16].as<int>(), nc,
-                       32 + bits_for(nleaf), c->tmp.as<int>(), s);
-    // vertices
+    // vertices: corners keyed by their unordered leaf pair, numbered by first occurrence
+    const int64_t vtab = pow2_at_least(2 * nc);
+    HIPCHK(c, E[15].ensure(vtab * 8));
+    HIPCHK(c, E[16].ensure(vtab * 4));
     HIPCHK(c, E[18].ensure((nc + 1) * 4));
     HIPCHK(c, E[19].ensure((nc + 1) * 4));
-    launch_run_heads(E[13].as<uint64_t>(), nc, E[18].as<int>(), s);
+    if ((rc = ensure_tmp(c, nc + 1))) return rc;
+    int* vslot = E[14].as<int>();  // the corner indices written there are implicit
+    launch_hash_dedupe(E[13].as<uint64_t>(), nc, E[15].as<unsigned long long>(), E[16].as<unsigned>(), vtab, vslot,
+                       E[18].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[18].as<int>() + nc, 0, 4, s));
     exclusive_scan_i32(E[18].as<int>(), E[19].as<int>(), nc + 1, c->tmp.as<int>(), s);
     int64_t nv;
@@ -931,8 +995,8 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[20].ensure(std::max<int64_t>(nc, 1) * 4));
     HIPCHK(c, E[21].ensure(std::max<int64_t>(nv, 1) * 8 * 8));
     int* cvid = E[20].as<int>();
-    launch_vertex_ids(E[13].as<uint64_t>(), E[14].as<int>(), nc, E[19].as<int>(), E[17].as<int>(), leaf_node, cvid,
-                      E[21].as<double>(), s);
+    launch_vertex_ids_hashed(vslot, E[16].as<unsigned>(), E[19].as<int>(), nc, E[17].as<int>(), leaf_node, cvid,
+                             E[21].as<double>(), s);
     HIPCHK(c, hipGetLastError());
     c->st.ms_extract_host += now_ms() - t0;  // device extraction time (topology)
     const double t1 = now_ms();
@@ -944,24 +1008,37 @@ int do_extract_device(pisurf_ctx* c) {
     c->st.ms_bisect += now_ms() - t1;
     const double t2 = now_ms();
     // triangle dedupe by sorted vertex triple; the first triangle of a repeated triple stays
-    uint32_t* tk = (uint32_t*)E[3].p;  // reuse the corner-key buffers (>= 8 * nleaf entries)
     int64_t ntk = nt;
-    HIPCHK(c, E[3].ensure(std::max<int64_t>(ntk, 1) * 8));
-    HIPCHK(c, E[4].ensure(std::max<int64_t>(ntk, 1) * 4));
-    HIPCHK(c, E[5].ensure(std::max<int64_t>(ntk, 1) * 8));
-    HIPCHK(c, E[6].ensure(std::max<int64_t>(ntk, 1) * 4));
     HIPCHK(c, E[8].ensure((ntk + 1) * 4));
     HIPCHK(c, E[9].ensure((ntk + 1) * 4));
-    tk = (uint32_t*)E[3].p;
-    launch_tri_key_hi(cvid, nt, tk, E[4].as<int>(), s);
-    radix_sort_pairs(tk, E[4].as<int>(), (uint32_t*)E[5].p, E[6].as<int>(), nt, bits_for(nv), c->tmp.as<int>(), s);
-    // second pass keys go to E[5]/E[6] (E[4] holds the first permutation)
-    HIPCHK(c, E[16].ensure(std::max<int64_t>(ntk, 1) * 8));
-    HIPCHK(c, E[15].ensure(std::max<int64_t>(ntk, 1) * 8));
-    launch_tri_key_lo(cvid, E[4].as<int>(), nt, E[5].as<uint64_t>(), E[6].as<int>(), s);
-    radix_sort_pairs64(E[5].as<uint64_t>(), E[6].as<int>(), E[15].as<uint64_t>(), E[16].as<int>(), nt,
-                       32 + bits_for(nv), c->tmp.as<int>(), s);
-    launch_tri_keep(cvid, E[6].as<int>(), nt, E[8].as<int>(), s);
+    if ((rc = ensure_tmp(c, ntk + 1))) return rc;
+    if (nv < ((int64_t)1 << 21)) {
+        // one 63-bit key per triangle, hashed
+        const int64_t ttab = pow2_at_least(2 * ntk);
+        HIPCHK(c, E[3].ensure(std::max<int64_t>(ntk, 1) * 8));
+        HIPCHK(c, E[4].ensure(std::max<int64_t>(ntk, 1) * 4));
+        HIPCHK(c, E[5].ensure(ttab * 8));
+        HIPCHK(c, E[6].ensure(ttab * 4));
+        launch_tri_key63(cvid, nt, E[3].as<uint64_t>(), s);
+        launch_hash_dedupe(E[3].as<uint64_t>(), nt, E[5].as<unsigned long long>(), E[6].as<unsigned>(), ttab,
+                           E[4].as<int>(), E[8].as<int>(), s);
+    } else {
+        // larger meshes: two stable sorts (largest id, then the lower pair) group equal triples
+        HIPCHK(c, E[3].ensure(std::max<int64_t>(ntk, 1) * 8));
+        HIPCHK(c, E[4].ensure(std::max<int64_t>(ntk, 1) * 4));
+        HIPCHK(c, E[5].ensure(std::max<int64_t>(ntk, 1) * 8));
+        HIPCHK(c, E[6].ensure(std::max<int64_t>(ntk, 1) * 4));
+        HIPCHK(c, E[15].ensure(std::max<int64_t>(ntk, 1) * 8));
+        HIPCHK(c, E[16].ensure(std::max<int64_t>(ntk, 1) * 8));
+        uint32_t* tk = (uint32_t*)E[3].p;
+        launch_tri_key_hi(cvid, nt, tk, E[4].as<int>(), s);
+        radix_sort_pairs(tk, E[4].as<int>(), (uint32_t*)E[5].p, E[6].as<int>(), nt, bits_for(nv), c->tmp.as<int>(),
+                         s);
+        launch_tri_key_lo(cvid, E[4].as<int>(), nt, E[5].as<uint64_t>(), E[6].as<int>(), s);
+        radix_sort_pairs64(E[5].as<uint64_t>(), E[6].as<int>(), E[15].as<uint64_t>(), E[16].as<int>(), nt,
+                           32 + bits_for(nv), c->tmp.as<int>(), s);
+        launch_tri_keep(cvid, E[6].as<int>(), nt, E[8].as<int>(), s);
+    }
     HIPCHK(c, hipMemsetAsync(E[8].as<int>() + nt, 0, 4, s));
     exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nt + 1, c->tmp.as<int>(), s);
     int64_t ntf;
@@ -1003,7 +1080,7 @@ int do_extract_host(pisurf_ctx* c) {
     // edge, the first dual cell to use it fixes the edge orientation; triangles are keyed by their
     // sorted vertex triple.
     std::vector<int32_t> vend;  // ordered node pair per vertex
-    std::vector<int32_t>& tris = c->mesh_t;
+    PinnedVec<int32_t>& tris = c->mesh_t;
     tris.clear();
     KeyMap vmap, tmap;
     vmap.reset(ex.duals.size() * 2 + 16);

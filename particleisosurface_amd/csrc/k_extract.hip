@@ -67,11 +67,10 @@ __device__ __forceinline__ int locate(const TreeView& tv, const int* lidx, int q
     return -1;
 }
 
-// the 8 leaves around the point of run head j; returns false if j is not a run head
-__device__ __forceinline__ bool gather_point(const TreeView& tv, const int* lidx, const uint64_t* skey,
+// the 8 leaves around the point whose corner entries start at sorted position j
+__device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx, const uint64_t* skey,
                                              const int* sval, int64_t nent, int64_t j, int n[8]) {
     const uint64_t key = skey[j];
-    if (j > 0 && skey[j - 1] == key) return false;
 #pragma unroll
     for (int o = 0; o < 8; o++) n[o] = -1;
     for (int64_t k = j; k < nent && k < j + 8 && skey[k] == key; k++) {
@@ -87,7 +86,13 @@ __device__ __forceinline__ bool gather_point(const TreeView& tv, const int* lidx
         if (n[o] >= 0) any = n[o];
     for (int o = 0; o < 8; o++)
         if (n[o] < 0) n[o] = any;  // unreachable for a consistent tree
-    return true;
+}
+
+// positions of the run heads (first entry of each distinct point), in sorted order
+__global__ void k_head_positions(const int* __restrict__ head, const int* __restrict__ hpos, int64_t n,
+                                 int* __restrict__ list) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n && head[j]) list[hpos[j]] = (int)j;
 }
 
 __device__ __forceinline__ int sign_index(const double4* leaf_node, const int n[8]) {
@@ -98,24 +103,23 @@ __device__ __forceinline__ int sign_index(const double4* leaf_node, const int n[
     return s;
 }
 
-// flag points whose dual cell is non-trivial and stage the cell (8 leaves in case order, sign
-// configuration) at the point's sorted position; k_points_compact then packs the staged cells
+// one thread per distinct point (heads[t] = its first sorted entry): flag points whose dual cell
+// is non-trivial and stage the cell (8 leaves in case order, sign configuration) at slot t;
+// k_points_compact then packs the staged cells
 __global__ void k_points(TreeView tv, const int* __restrict__ lidx, const double4* __restrict__ leaf_node,
                          const uint64_t* __restrict__ skey, const int* __restrict__ sval, int64_t nent,
-                         int* __restrict__ dflag, int* __restrict__ stage) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nent) return;
+                         const int* __restrict__ heads, int64_t npts, int* __restrict__ dflag,
+                         int* __restrict__ stage) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
     int n[8];
-    if (!gather_point(tv, lidx, skey, sval, nent, j, n)) {
-        dflag[j] = 0;
-        return;
-    }
+    gather_point(tv, lidx, skey, sval, nent, heads[t], n);
     const int s = sign_index(leaf_node, n);
     const uint8_t code = d_kDmcCase[s];
     const bool keep = code != 0 && code != 255;
-    dflag[j] = keep ? 1 : 0;
+    dflag[t] = keep ? 1 : 0;
     if (!keep) return;
-    int* d = stage + 9 * j;
+    int* d = stage + 9 * t;
     const uint32_t perm = d_kDmcPerm[s];
 #pragma unroll
     for (int k = 0; k < 8; k++) d[k] = n[(perm >> (4 * k)) & 15];
@@ -176,23 +180,65 @@ __global__ void k_run_heads(const uint64_t* __restrict__ key, int64_t n, int* __
     head[j] = (j == 0 || key[j] != key[j - 1]) ? 1 : 0;
 }
 
-// vertex ids of triangle corners and the bisection edge of each vertex (first occurrence)
-__global__ void k_vertex_ids(const uint64_t* __restrict__ key, const int* __restrict__ sval, int64_t n,
-                             const int* __restrict__ head_scan, const int* __restrict__ vend,
-                             const double4* __restrict__ leaf_node, int* __restrict__ corner_vid,
-                             double* __restrict__ ends) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const bool head = (j == 0 || key[j] != key[j - 1]);
-    const int vid = head_scan[j] + (head ? 1 : 0) - 1;
-    const int c = sval[j];
+// ---- hash-table dedupe (first occurrence wins, as Mesh::insert_vert / insert_tri do) ----
+// Open addressing with linear probing over a power-of-two table of at least twice the entries, so
+// every probe sequence ends. Keys never equal the empty marker (their top bit is clear).
+constexpr unsigned long long kHashEmpty = ~0ull;
+__device__ __forceinline__ uint64_t mix64(uint64_t k) {  // SplitMix64 finaliser
+    k ^= k >> 30;
+    k *= 0xbf58476d1ce4e5b9ull;
+    k ^= k >> 27;
+    k *= 0x94d049bb133111ebull;
+    k ^= k >> 31;
+    return k;
+}
+// slot[i] = table slot of key[i]; tmin[slot] = smallest i with that key
+__global__ void k_hash_insert(const uint64_t* __restrict__ key, int64_t n, unsigned long long* __restrict__ tkey,
+                              unsigned* __restrict__ tmin, uint64_t mask, int* __restrict__ slot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long k = key[i];
+    uint64_t h = mix64(k) & mask;
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        const unsigned long long prev = atomicCAS(&tkey[h], kHashEmpty, k);
+        if (prev == kHashEmpty || prev == k) {
+            atomicMin(&tmin[h], (unsigned)i);
+            slot[i] = (int)h;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+__global__ void k_hash_first(const int* __restrict__ slot, const unsigned* __restrict__ tmin, int64_t n,
+                             int* __restrict__ first) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) first[i] = (tmin[slot[i]] == (unsigned)i) ? 1 : 0;
+}
+// vertex id of every corner (rank of its key's first corner) and the bisection edge of each vertex,
+// oriented as its first corner's dual cell has it
+__global__ void k_vertex_ids_hashed(const int* __restrict__ slot, const unsigned* __restrict__ tmin,
+                                    const int* __restrict__ rank, int64_t n, const int* __restrict__ vend,
+                                    const double4* __restrict__ leaf_node, int* __restrict__ corner_vid,
+                                    double* __restrict__ ends) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const unsigned f = tmin[slot[c]];
+    const int vid = rank[f];
     corner_vid[c] = vid;
-    if (head) {
-        const double4 a = leaf_node[vend[2 * (int64_t)c]], b = leaf_node[vend[2 * (int64_t)c + 1]];
+    if (f == (unsigned)c) {
+        const double4 a = leaf_node[vend[2 * c]], b = leaf_node[vend[2 * c + 1]];
         double* e = ends + 8 * (int64_t)vid;
         e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
         e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
     }
+}
+// triangle key: sorted vertex triple, 21 bits each (vertex count below 2^21)
+__global__ void k_tri_key63(const int* __restrict__ cv, int64_t nt, uint64_t* __restrict__ k) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int a = cv[3 * t], b = cv[3 * t + 1], c = cv[3 * t + 2];
+    const int lo = min(a, min(b, c)), hi = max(a, max(b, c)), mid = a + b + c - lo - hi;
+    k[t] = ((uint64_t)lo << 42) | ((uint64_t)mid << 21) | (uint64_t)hi;
 }
 
 // triangle dedupe keys: sorted vertex triple (first pass by the largest id, second by the pair)
@@ -252,9 +298,29 @@ void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double
                       int* cval, int* n_valid, hipStream_t s) {
     if (lv.n > 0) k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, leaf_node, ckey, cval, n_valid);
 }
+void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned long long* tkey, unsigned* tmin, int64_t tsize,
+                        int* slot, int* first, hipStream_t s) {
+    if (n <= 0) return;
+    (void)hipMemsetAsync(tkey, 0xff, tsize * sizeof(unsigned long long), s);
+    (void)hipMemsetAsync(tmin, 0xff, tsize * sizeof(unsigned), s);
+    k_hash_insert<<<nblk(n), 256, 0, s>>>(key, n, tkey, tmin, (uint64_t)(tsize - 1), slot);
+    k_hash_first<<<nblk(n), 256, 0, s>>>(slot, tmin, n, first);
+}
+void launch_vertex_ids_hashed(const int* slot, const unsigned* tmin, const int* rank, int64_t n, const int* vend,
+                              const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s) {
+    if (n > 0) k_vertex_ids_hashed<<<nblk(n), 256, 0, s>>>(slot, tmin, rank, n, vend, leaf_node, corner_vid, ends);
+}
+void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s) {
+    if (nt > 0) k_tri_key63<<<nblk(nt), 256, 0, s>>>(cv, nt, k);
+}
+void launch_head_positions(const int* head, const int* hpos, int64_t n, int* list, hipStream_t s) {
+    if (n > 0) k_head_positions<<<nblk(n), 256, 0, s>>>(head, hpos, n, list);
+}
 void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, const uint64_t* skey,
-                   const int* sval, int64_t nent, int* dflag, int* stage, hipStream_t s) {
-    if (nent > 0) k_points<<<nblk(nent), 256, 0, s>>>(tv, lidx, leaf_node, skey, sval, nent, dflag, stage);
+                   const int* sval, int64_t nent, const int* heads, int64_t npts, int* dflag, int* stage,
+                   hipStream_t s) {
+    if (npts > 0)
+        k_points<<<nblk(npts), 256, 0, s>>>(tv, lidx, leaf_node, skey, sval, nent, heads, npts, dflag, stage);
 }
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s) {
@@ -269,10 +335,6 @@ void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t*
 }
 void launch_run_heads(const uint64_t* key, int64_t n, int* head, hipStream_t s) {
     if (n > 0) k_run_heads<<<nblk(n), 256, 0, s>>>(key, n, head);
-}
-void launch_vertex_ids(const uint64_t* key, const int* sval, int64_t n, const int* head_scan, const int* vend,
-                       const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s) {
-    if (n > 0) k_vertex_ids<<<nblk(n), 256, 0, s>>>(key, sval, n, head_scan, vend, leaf_node, corner_vid, ends);
 }
 void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s) {
     if (nt > 0) k_tri_key_hi<<<nblk(nt), 256, 0, s>>>(cv, nt, k, v);

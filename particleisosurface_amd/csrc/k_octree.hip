@@ -23,7 +23,7 @@ __device__ __forceinline__ bool leaf_depth(float half, float r) {
 }
 
 // checkEmptyAndCalcCurv (src/surface_reconstructor.cpp:152-222): a group of G lanes per node
-// (G = 64 for large boxes, 16 for the small boxes of the deep levels). The box query enumerates
+// (G = 64 for large boxes, 16 or 8 for the small boxes of the deep levels). The box query enumerates
 // cells [lo-1, hi+1] of the clamped box (GetInBoxParticles); each lane walks whole (y,z) rows of
 // that range (one contiguous window of sorted positions per row). A particle counts when it lies
 // strictly inside the unclamped box grown by r * smooth. Nodes above D_min only need emptiness and
@@ -413,16 +413,14 @@ __global__ void k_sample_grads(TreeParams tp, NodeLevel lv, const R* __restrict_
     }
 }
 
-void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, bool small,
+void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, int group,
                        hipStream_t s) {
     if (lv.n <= 0) return;
-    if (small) {
-        const int64_t threads = lv.n * 16;
-        k_empty_curv<16><<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(g, spos, nrm, tp, lv);
-    } else {
-        const int64_t threads = lv.n * kWave;
-        k_empty_curv<64><<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(g, spos, nrm, tp, lv);
-    }
+    const unsigned nb8 = (unsigned)((lv.n * 8 + 255) / 256), nb16 = (unsigned)((lv.n * 16 + 255) / 256);
+    const unsigned nb64 = (unsigned)((lv.n * kWave + 255) / 256);
+    if (group <= 8) k_empty_curv<8><<<nb8, 256, 0, s>>>(g, spos, nrm, tp, lv);
+    else if (group <= 16) k_empty_curv<16><<<nb16, 256, 0, s>>>(g, spos, nrm, tp, lv);
+    else k_empty_curv<64><<<nb64, 256, 0, s>>>(g, spos, nrm, tp, lv);
 }
 
 void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,

@@ -86,8 +86,8 @@ struct TreeParams {
     int qef_mode;
     float qef_svd_tol;
 };
-// small: every node box spans few cells (16 lanes per node instead of a wave)
-void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, bool small,
+// group: lanes per node (8, 16 or 64), about the number of (y, z) cell rows a node box spans
+void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, int group,
                        hipStream_t s);
 // flag[n+1], pos[n+1] (pos[n] = number of listed nodes), list[n]: nodes to sample; cflag / cpos /
 // clist: leaf-depth nodes that need only their centre value
@@ -124,17 +124,26 @@ struct TreeView {
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
 void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double4* leaf_node, uint64_t* ckey,
                       int* cval, int* n_valid, hipStream_t s);
-// dflag[j] = 1 for a run head j whose dual cell is non-trivial; its cell goes to stage[9 j ..]
+// first[i] = 1 when i is the smallest index with key[i] (open-addressing table of tsize slots, a
+// power of two >= 2 n; slot[i] = slot of key[i], tmin[slot] = smallest index of the key)
+void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned long long* tkey, unsigned* tmin, int64_t tsize,
+                        int* slot, int* first, hipStream_t s);
+void launch_vertex_ids_hashed(const int* slot, const unsigned* tmin, const int* rank, int64_t n, const int* vend,
+                              const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s);
+void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s);
+// list[hpos[j]] = j for every j with head[j] != 0
+void launch_head_positions(const int* head, const int* hpos, int64_t n, int* list, hipStream_t s);
+// point t (first sorted entry heads[t]): dflag[t] = 1 when its dual cell is non-trivial, the cell
+// goes to stage[9 t ..]
 void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, const uint64_t* skey,
-                   const int* sval, int64_t nent, int* dflag, int* stage, hipStream_t s);
+                   const int* sval, int64_t nent, const int* heads, int64_t npts, int* dflag, int* stage,
+                   hipStream_t s);
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s);
 void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s);
 void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t* vkey, int* vval, int* vend,
                         hipStream_t s);
 void launch_run_heads(const uint64_t* key, int64_t n, int* head, hipStream_t s);
-void launch_vertex_ids(const uint64_t* key, const int* sval, int64_t n, const int* head_scan, const int* vend,
-                       const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s);
 void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s);
 void launch_tri_key_lo(const int* cv, const int* perm, int64_t nt, uint64_t* k, int* v, hipStream_t s);
 void launch_tri_keep(const int* cv, const int* order, int64_t nt, int* keep, hipStream_t s);
