@@ -929,8 +929,8 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[4].ensure(std::max<int64_t>(nent_all, 1) * 4));
     HIPCHK(c, E[5].ensure(std::max<int64_t>(nent_all, 1) * 8));
     HIPCHK(c, E[6].ensure(std::max<int64_t>(nent_all, 1) * 4));
-    HIPCHK(c, E[7].ensure(64));
-    HIPCHK(c, hipMemsetAsync(E[7].p, 0, 64, s));
+    HIPCHK(c, E[7].ensure(64 * 128));
+    HIPCHK(c, hipMemsetAsync(E[7].p, 0, 64 * 128, s));
     double4* leaf_node = E[2].as<double4>();
     for (int L = 0; L < tv.nlev; L++)
         launch_leaf_emit(c->levels[L]->view(), tv.off[L], lidx, tv.dp, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(),
@@ -939,8 +939,13 @@ int do_extract_device(pisurf_ctx* c) {
     radix_sort_pairs64(E[3].as<uint64_t>(), E[4].as<int>(), E[5].as<uint64_t>(), E[6].as<int>(), nent_all,
                        std::min(64, 3 * tv.dp + 1), c->tmp.as<int>(), s);
     HIPCHK(c, hipGetLastError());
-    int64_t nent;
-    if ((rc = read_int(c, E[7].as<int>(), nent))) return rc;
+    int64_t nent = 0;
+    {
+        int st[64 * 32];
+        HIPCHK(c, hipMemcpyAsync(st, E[7].p, sizeof(st), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int k = 0; k < 64; k++) nent += st[32 * k];
+    }
     // points -> dual cells
     // distinct points: run heads of the sorted corner keys, one thread each
     HIPCHK(c, E[8].ensure((nent + 1) * 4));
@@ -1149,8 +1154,19 @@ int do_extract_host(pisurf_ctx* c) {
 
 int sync_counters(pisurf_ctx* c) {
     Counters h;
-    HIPCHK(c, hipMemcpyAsync(&h, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    Counters hs[kCtrStripes];
+    HIPCHK(c, hipMemcpyAsync(hs, c->counters.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memset(&h, 0, sizeof(h));
+    for (const Counters& k : hs) {
+        h.field_evals += k.field_evals;
+        h.grad_evals += k.grad_evals;
+        h.candidates += k.candidates;
+        h.qef_solves += k.qef_solves;
+        h.sample_evals += k.sample_evals;
+        h.sample_candidates += k.sample_candidates;
+        h.sample_unique += k.sample_unique;
+    }
     c->st.field_evals = (int64_t)(h.field_evals + c->host_sample_entries);
     c->st.grad_evals = (int64_t)h.grad_evals;
     c->st.candidates = (int64_t)h.candidates;
@@ -1201,8 +1217,8 @@ int pisurf_create(pisurf_ctx** out, int device) {
     }
     pisurf_default_params(&c->prm);
     std::memset(&c->st, 0, sizeof(c->st));
-    if (c->counters.ensure(sizeof(Counters)) != hipSuccess ||
-        hipMemset(c->counters.p, 0, sizeof(Counters)) != hipSuccess) {
+    if (c->counters.ensure(kCtrStripes * sizeof(Counters)) != hipSuccess ||
+        hipMemset(c->counters.p, 0, kCtrStripes * sizeof(Counters)) != hipSuccess) {
         delete c;
         return PISURF_E_HIP;
     }
@@ -1820,7 +1836,7 @@ int pisurf_reset_stats(pisurf_ctx* c) {
     std::memset(&c->st, 0, sizeof(c->st));
     c->host_sample_entries = 0;
     c->n_fp64_eig = 0;
-    HIPCHK(c, hipMemsetAsync(c->counters.p, 0, sizeof(Counters), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->counters.p, 0, kCtrStripes * sizeof(Counters), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PISURF_OK;
 }

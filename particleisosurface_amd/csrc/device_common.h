@@ -118,66 +118,83 @@ struct FieldConst {
     const float4* lo;
 };
 
-// Field sum S(q) = sum_j det_j K(|G_j (q - xMean_j)|) over the 27-cell candidates of q (without
-// sigma), optionally with the poly6-form gradient sum (src/evaluator.cpp:408-415,437-463).
-// Returns the number of candidates visited.
+// squared distance of a candidate offset (one expression shared by every field kernel, so the
+// compiler forms the same operations everywhere)
+template <class R> __device__ __forceinline__ R dist2(R dx, R dy, R dz) { return dx * dx + dy * dy + dz * dz; }
+
+// contribution of one candidate that passed the cheap reject test: offset (dx, dy, dz) = q - a.xyz,
+// records g0 / g1 and, in compensated FP32, the residual lo of the particle mean
 template <class R, bool KAHAN, bool GRAD>
-__device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, const FieldConst& fc, float fx,
-                                         float fy, float fz, R px, R py, R pz, R& sum, R grad[3]) {
+__device__ __forceinline__ void field_contrib(const FieldConst& fc, R inv_h2, R h2, R dx, R dy, R dz, R d2,
+                                              const typename Vec4<R>::T& g0, const typename Vec4<R>::T& g1,
+                                              const float4& lo, Acc<R, KAHAN>& acc, Acc<R, KAHAN>* ga) {
+    if (KAHAN && sizeof(R) == 4) {
+        dx -= R(lo.x);
+        dy -= R(lo.y);
+        dz -= R(lo.z);
+    }
+    if (fc.use_ani) {
+        const R gx = g0.x * dx + g0.y * dy + g0.z * dz;
+        const R gy = g0.y * dx + g0.w * dy + g1.x * dz;
+        const R gz = g0.z * dx + g1.x * dy + g1.y * dz;
+        const R n2 = gx * gx + gy * gy + gz * gz;
+        if (fc.use_poly6) {
+            const R t = h2 - n2;
+            if (n2 <= h2) acc.add(g1.z * (t * t * t));
+        } else {
+            acc.add(g1.z * bspline_q2<R>(n2 * inv_h2));
+        }
+        if (GRAD && n2 <= h2) {
+            const R t = h2 - n2;
+            const R w = R(-6) * g1.z * t * t;
+            ga[0].add(w * gx);
+            ga[1].add(w * gy);
+            ga[2].add(w * gz);
+        }
+    } else {
+        if (d2 <= h2) {
+            const R t = h2 - d2;
+            acc.add(t * t * t);
+            if (GRAD) {
+                const R w = R(-6) * t * t;
+                ga[0].add(w * dx);
+                ga[1].add(w * dy);
+                ga[2].add(w * dz);
+            }
+        }
+    }
+}
+
+// the nine (y, z) row windows of the 27-cell neighbourhood of q's cell; returns the candidate count
+__device__ __forceinline__ int field_rows(const GridDev& g, float fx, float fy, float fz, int rlo[9], int rhi[9]) {
     int ix, iy, iz;
     cell_of(g, fx, fy, fz, ix, iy, iz);
-    Acc<R, KAHAN> acc, ga[3];
-    const R inv_h2 = R(fc.inv_h2);
-    const R h2 = R(fc.h2);
-    // all 9 row windows first (independent loads in flight together)
-    int rlo[9], rhi[9];
     int visited = 0;
 #pragma unroll
     for (int r = 0; r < 9; r++) {
         if (!row_range(g, ix, iy, iz, r, rlo[r], rhi[r])) rlo[r] = rhi[r] = 0;
         visited += rhi[r] - rlo[r];
     }
+    return visited;
+}
+
+// Field sum S(q) = sum_j det_j K(|G_j (q - xMean_j)|) over the 27-cell candidates of q (without
+// sigma), optionally with the poly6-form gradient sum (src/evaluator.cpp:408-415,437-463).
+// Candidates are taken in the reference's order (GetPIdxList: rows z-outer, then y, each row's
+// cells x-ascending, sorted particles within a cell). Returns the number of candidates visited.
+template <class R, bool KAHAN, bool GRAD>
+__device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, const FieldConst& fc, float fx,
+                                         float fy, float fz, R px, R py, R pz, R& sum, R grad[3]) {
+    Acc<R, KAHAN> acc, ga[3];
+    const R inv_h2 = R(fc.inv_h2);
+    const R h2 = R(fc.h2);
+    // all 9 row windows first (independent loads in flight together)
+    int rlo[9], rhi[9];
+    const int visited = field_rows(g, fx, fy, fz, rlo, rhi);
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     auto contrib = [&](int j, R dx, R dy, R dz, R d2) {
-        {
-            if (KAHAN && sizeof(R) == 4) {
-                const float4 l = fc.lo[j];
-                dx -= R(l.x);
-                dy -= R(l.y);
-                dz -= R(l.z);
-            }
-            if (fc.use_ani) {
-                const auto g0 = rec.g[2 * j];
-                const auto g1 = rec.g[2 * j + 1];
-                const R gx = g0.x * dx + g0.y * dy + g0.z * dz;
-                const R gy = g0.y * dx + g0.w * dy + g1.x * dz;
-                const R gz = g0.z * dx + g1.x * dy + g1.y * dz;
-                const R n2 = gx * gx + gy * gy + gz * gz;
-                if (fc.use_poly6) {
-                    const R t = h2 - n2;
-                    if (n2 <= h2) acc.add(g1.z * (t * t * t));
-                } else {
-                    acc.add(g1.z * bspline_q2<R>(n2 * inv_h2));
-                }
-                if (GRAD && n2 <= h2) {
-                    const R t = h2 - n2;
-                    const R w = R(-6) * g1.z * t * t;
-                    ga[0].add(w * gx);
-                    ga[1].add(w * gy);
-                    ga[2].add(w * gz);
-                }
-            } else {
-                if (d2 <= h2) {
-                    const R t = h2 - d2;
-                    acc.add(t * t * t);
-                    if (GRAD) {
-                        const R w = R(-6) * t * t;
-                        ga[0].add(w * dx);
-                        ga[1].add(w * dy);
-                        ga[2].add(w * dz);
-                    }
-                }
-            }
-        }
+        const float4 lo = (KAHAN && sizeof(R) == 4) ? fc.lo[j] : zero4;
+        field_contrib<R, KAHAN, GRAD>(fc, inv_h2, h2, dx, dy, dz, d2, rec.g[2 * j], rec.g[2 * j + 1], lo, acc, ga);
     };
     for (int r = 0; r < 9; r++) {
         int j = rlo[r];
@@ -189,8 +206,8 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
             const R x1 = px - a1.x, y1 = py - a1.y, z1 = pz - a1.z;
             const R x2 = px - a2.x, y2 = py - a2.y, z2 = pz - a2.z;
             const R x3 = px - a3.x, y3 = py - a3.y, z3 = pz - a3.z;
-            const R q0 = x0 * x0 + y0 * y0 + z0 * z0, q1 = x1 * x1 + y1 * y1 + z1 * z1;
-            const R q2 = x2 * x2 + y2 * y2 + z2 * z2, q3 = x3 * x3 + y3 * y3 + z3 * z3;
+            const R q0 = dist2(x0, y0, z0), q1 = dist2(x1, y1, z1);
+            const R q2 = dist2(x2, y2, z2), q3 = dist2(x3, y3, z3);
             if (q0 < a0.w) contrib(j, x0, y0, z0, q0);
             if (q1 < a1.w) contrib(j + 1, x1, y1, z1, q1);
             if (q2 < a2.w) contrib(j + 2, x2, y2, z2, q2);
@@ -199,7 +216,7 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
         for (; j < e; j++) {
             const auto a = rec.a[j];
             const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-            const R d2 = dx * dx + dy * dy + dz * dz;
+            const R d2 = dist2(dx, dy, dz);
             if (d2 < a.w) contrib(j, dx, dy, dz, d2);
         }
     }

@@ -49,7 +49,8 @@ __global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ l
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) valid += __shfl_xor(valid, o, kWave);
-    if ((threadIdx.x & 63) == 0 && valid) atomicAdd(n_valid, valid);
+    // striped like the event counters: 64 copies, 128 bytes apart
+    if ((threadIdx.x & 63) == 0 && valid) atomicAdd(n_valid + 32 * (blockIdx.x & 63), valid);
 }
 
 // leaf containing the finest cell q (descent from the root)

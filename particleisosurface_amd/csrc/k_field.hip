@@ -4,13 +4,6 @@
 
 namespace pisurf {
 
-// counter reduction: one 64-bit atomic per wave
-__device__ __forceinline__ void wave_count(unsigned long long* ctr, unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
-}
-
 template <class R, bool K, bool GRAD>
 __global__ void __launch_bounds__(256) k_eval_points(GridDev g, Recs<R> rec, FieldConst fc, const float* __restrict__ q,
                                                      int64_t n, double* __restrict__ f, double* __restrict__ grad,
@@ -29,8 +22,8 @@ __global__ void __launch_bounds__(256) k_eval_points(GridDev g, Recs<R> rec, Fie
             grad[3 * i + 2] = double(sigma * gr[2]);
         }
     }
-    wave_count(GRAD ? &ctr->grad_evals : &ctr->field_evals, i < n ? 1ull : 0ull);
-    wave_count(&ctr->candidates, (unsigned long long)vis);
+    wave_count(GRAD ? &ctr_stripe(ctr)->grad_evals : &ctr_stripe(ctr)->field_evals, i < n ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
 template <class R, bool K>
@@ -45,8 +38,8 @@ __global__ void __launch_bounds__(256) k_normals(GridDev g, Recs<R> rec, FieldCo
         const R sigma = R(fc.sigma);
         nrm[p] = make_float4(float(sigma * gr[0]), float(sigma * gr[1]), float(sigma * gr[2]), 0.0f);
     }
-    wave_count(&ctr->grad_evals, p < n ? 1ull : 0ull);
-    wave_count(&ctr->candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->grad_evals, p < n ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
 void launch_eval_points(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, const float* q, int64_t n,
@@ -113,8 +106,8 @@ __global__ void __launch_bounds__(256) k_uniform_grid(GridDev g, Recs<R> rec, Fi
         vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), s, gr);
         out[i] = float(R(fc.iso) - R(fc.sigma) * s);
     }
-    wave_count(&ctr->field_evals, i < total ? 1ull : 0ull);
-    wave_count(&ctr->candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->field_evals, i < total ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
 void launch_uniform_grid(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, UniformBox ub,

@@ -10,12 +10,6 @@ enum : uint8_t { kEmpty = 0, kInternal = 1, kLeaf = 2 };  // include/iso_method_
 
 template <class R> struct R4t { typedef typename Vec4<R>::T T; };
 
-__device__ __forceinline__ void wave_add(unsigned long long* ctr, unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
-}
-
 // leaf test of afterSampleEval: (cellsize - min_radius) < 1e-8 in double
 __device__ __forceinline__ bool leaf_depth(float half, float r) {
     const float cellsize = __fmul_rn(2.0f, half);
@@ -219,9 +213,9 @@ __global__ void __launch_bounds__(256) k_sample_eval(GridDev g, Recs<R> rec, Fie
         vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), sum, gr);
         out[h] = R(fc.iso) - R(fc.sigma) * sum;
     }
-    wave_add(&ctr->candidates, (unsigned long long)vis);
-    wave_add(&ctr->sample_unique, h < nheads ? 1ull : 0ull);
-    wave_add(&ctr->sample_candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->sample_unique, h < nheads ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->sample_candidates, (unsigned long long)vis);
 }
 
 // every entry receives its point's value (the host counts the entries: 27 per sampled node, the
@@ -254,8 +248,8 @@ __global__ void __launch_bounds__(256) k_centers(GridDev g, Recs<R> rec, FieldCo
         samples[node * 27 + 13] = R(fc.iso) - R(fc.sigma) * sum;
         did = true;
     }
-    wave_add(&ctr->field_evals, did ? 1ull : 0ull);
-    wave_add(&ctr->candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->field_evals, did ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 __global__ void k_sample_list(NodeLevel lv, const int* __restrict__ flag, const int* __restrict__ pos,
                               int* __restrict__ list) {
@@ -283,10 +277,10 @@ __global__ void __launch_bounds__(256) k_samples(GridDev g, Recs<R> rec, FieldCo
         samples[node * 27 + s] = R(fc.iso) - R(fc.sigma) * sum;
         did = true;
     }
-    wave_add(&ctr->field_evals, did ? 1ull : 0ull);
-    wave_add(&ctr->candidates, (unsigned long long)vis);
-    wave_add(&ctr->sample_evals, did ? 1ull : 0ull);
-    wave_add(&ctr->sample_candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->field_evals, did ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->sample_evals, did ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->sample_candidates, (unsigned long long)vis);
 }
 
 // afterSampleEval classification (src/surface_reconstructor.cpp:237-273) + leaf values
@@ -361,9 +355,9 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
         lv.child[node] = -1;
         if (sc_out) sc_out[node] = sc ? 1 : 0;
     }
-    wave_add(&ctr->field_evals, evals);
-    wave_add(&ctr->qef_solves, qefs);
-    wave_add(&ctr->candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->field_evals, evals);
+    wave_count(&ctr_stripe(ctr)->qef_solves, qefs);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
 // Children of INTERNAL nodes (TNode ctor, src/iso_method_ours.cpp:25-38): child t has

@@ -5,12 +5,6 @@
 
 namespace pisurf {
 
-__device__ __forceinline__ void wave_add_v(unsigned long long* ctr, unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
-}
-
 // One thread per edge. Endpoints are (x, y, z, f) with f of opposite signs (f > 0 is +1).
 // The midpoint replaces the endpoint of equal sign and the length is halved (not recomputed)
 // until it is at most r/2; then the zero crossing is interpolated, snapping to an endpoint
@@ -59,8 +53,8 @@ __global__ void __launch_bounds__(256) k_bisect(GridDev g, Recs<R> rec, FieldCon
         for (int a = 0; a < 3; a++) out[3 * i + a] = double(o[a]);
         if (nev_out) nev_out[i] = nev;
     }
-    wave_add_v(&ctr->field_evals, (unsigned long long)nev);
-    wave_add_v(&ctr->candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->field_evals, (unsigned long long)nev);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
 void launch_bisect(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, float half_r, const double* ends,
@@ -109,9 +103,9 @@ __global__ void __launch_bounds__(64) k_qef(GridDev g, Recs<R> rec, FieldConst f
         if (err / R(__fmul_rn(2.0f, c.w)) < R(0.0001)) fv = R(0);
         out[i] = make_double4(double(vx), double(vy), double(vz), double(fv));
     }
-    wave_add_v(&ctr->qef_solves, i < n ? 1ull : 0ull);
-    wave_add_v(&ctr->field_evals, i < n ? 1ull : 0ull);
-    wave_add_v(&ctr->candidates, (unsigned long long)vis);
+    wave_count(&ctr_stripe(ctr)->qef_solves, i < n ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->field_evals, i < n ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
 void launch_qef(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, TreeParams tp, const float4* cen,

@@ -47,10 +47,21 @@ void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams
                  void* rec_g, void* rec_lo, unsigned long long* n_fp64_eig, hipStream_t s);
 
 // ---- field ----
-struct Counters {
+// Event counters, spread over kCtrStripes copies one cache line apart: a block adds to the copy of
+// its index, so the per-wave atomics of a launch do not all serialise on one address. The host
+// sums the copies.
+constexpr int kCtrStripes = 64;
+struct alignas(128) Counters {
     unsigned long long field_evals, grad_evals, candidates, qef_solves, sample_evals, sample_candidates,
         sample_unique;
 };
+__device__ __forceinline__ Counters* ctr_stripe(Counters* c) { return c + (blockIdx.x & (kCtrStripes - 1)); }
+// wave-reduced add: one atomic per wave (every lane of the wave must call it)
+__device__ __forceinline__ void wave_count(unsigned long long* ctr, unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
+}
 void launch_eval_points(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float* q,
                         int64_t n, double* f, double* grad, Counters* ctr, hipStream_t s);
 void launch_normals(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float4* spos,
@@ -123,7 +134,7 @@ struct TreeView {
 };
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
 void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double4* leaf_node, uint64_t* ckey,
-                      int* cval, int* n_valid, hipStream_t s);
+                      int* cval, int* n_valid /* 64 stripes, 32 ints apart */, hipStream_t s);
 // first[i] = 1 when i is the smallest index with key[i] (open-addressing table of tsize slots, a
 // power of two >= 2 n; slot[i] = slot of key[i], tmin[slot] = smallest index of the key)
 void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned long long* tkey, unsigned* tmin, int64_t tsize,
