@@ -28,6 +28,19 @@ BYTES_PER_CAND = {capi.FP32: 52, capi.FP32_KAHAN: 52, capi.FP64: 104}
 BYTES_FIXED = 16 + 27 * 8
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/*_traffic.md, lines `traffic_bytes_per_launch <kernel> = <bytes>`), or None."""
+    import glob
+    import re
+    for f in sorted(glob.glob(os.path.join(HERE, "profiles", "*_traffic.md"))):
+        for line in open(f):
+            m = re.match(r"traffic_bytes_per_launch (\w+) = (\d+)", line.strip())
+            if m and m.group(1) == kernel:
+                return int(m.group(2)), os.path.relpath(f, HERE)
+    return None, None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -193,20 +206,31 @@ def main():
                                             st["ms_normals"] / args.steps, st["ms_tree"] / args.steps,
                                             st["ms_extract_host"] / args.steps, st["ms_bisect"] / args.steps))
     value = evals_per_frame / (dt / args.steps) / args.steps
-    # roofline of the node-sampling kernel (the dominant field-evaluation kernel): algorithmic bytes
-    # of the evaluations it performs (one per distinct sample point when deduplicated)
+    # roofline of the dominant field-evaluation kernel: the cell-grouped sampling kernel when a level
+    # uses it (the deepest sampled level holds most points), else the per-point sampling kernel;
+    # algorithmic bytes of the evaluations it performs (one per distinct sample point)
+    def kernel_roof(name, evals, cands, launches, ms):
+        k_ms = ms / max(launches, 1)
+        alg = evals * BYTES_FIXED + cands * BYTES_PER_CAND[prec]
+        achieved = alg / max(launches, 1) / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+        traffic, source = measured_traffic(name)
+        return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=source, kernel=name,
+                    launches_per_frame=launches / args.steps, avg_launch_ms=round(k_ms, 4),
+                    evals_per_launch=evals / max(launches, 1), candidates_per_eval=cands / max(evals, 1),
+                    algorithmic_bytes_per_launch=alg / max(launches, 1),
+                    evals_per_s=evals / (ms * 1e-3) if ms > 0 else None)
     performed = st["sample_unique"] if st["sample_unique"] > 0 else st["sample_evals"]
-    k_ms = st["ms_sample_kernel"] / max(st["sample_launches"], 1)
-    alg_bytes = performed * BYTES_FIXED + st["sample_candidates"] * BYTES_PER_CAND[prec]
-    achieved = alg_bytes / max(st["sample_launches"], 1) / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
-    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_sample_eval" if st["sample_unique"] > 0 else "k_samples",
-                launches_per_frame=st["sample_launches"] / args.steps,
-                avg_launch_ms=round(k_ms, 4), evals_per_launch=performed / max(st["sample_launches"], 1),
-                candidates_per_eval=st["sample_candidates"] / max(performed, 1),
-                sample_kernel_evals_per_s=performed / (st["ms_sample_kernel"] * 1e-3)
-                if st["ms_sample_kernel"] > 0 else None)
+    pt_evals = performed - st["cell_evals"]
+    pt_cands = st["sample_candidates"] - st["cell_candidates"]
+    pt_name = "k_sample_eval" if st["sample_unique"] > 0 else "k_samples"
+    per_point = kernel_roof(pt_name, pt_evals, pt_cands, st["sample_launches"], st["ms_sample_kernel"])
+    if st["cell_launches"] > 0:
+        roof = kernel_roof("k_cell_eval", st["cell_evals"], st["cell_candidates"], st["cell_launches"],
+                           st["ms_cell_kernel"])
+        roof["other_sampling_kernel"] = per_point
+    else:
+        roof = per_point
     v32, t32 = ctx.mesh()
     out = {
         "metric": "field evals/s + frame mesh time @4M particles, FP32-comp vs FP64 at stated err",

@@ -88,12 +88,16 @@ typedef struct pisurf_stats {
     int64_t candidates;        /* candidate particles visited by all field evaluations */
     int64_t sample_evals;      /* evaluations of the node-sampling kernel (27 per sampled node) */
     int64_t sample_candidates; /* candidates visited by the node-sampling kernel */
-    int64_t sample_launches;   /* launches of the node-sampling kernel */
+    int64_t sample_launches;   /* launches of the per-point node-sampling kernel */
     int64_t fp64_eigensolves;  /* clamped spectra re-solved in FP64 (PISURF_FP32_KAHAN) */
     double ms_grid, ms_aniso, ms_normals, ms_tree, ms_extract_host, ms_bisect, ms_total;
-    double ms_sample_kernel;   /* summed device time of the node-sampling kernel (HIP events) */
+    double ms_sample_kernel;   /* summed device time of the per-point node-sampling kernel (HIP events) */
     int64_t n_dual_cells, n_verts, n_tris, n_nodes;
     int64_t sample_unique;     /* sample evaluations performed after deduplication (<= sample_evals) */
+    /* the cell-grouped sampling kernel (levels with many sample points per hash cell); its
+     * evaluations and candidates are included in sample_unique / sample_candidates */
+    int64_t cell_evals, cell_candidates, cell_launches;
+    double ms_cell_kernel;
 } pisurf_stats;
 
 /* ---- context ---- */

@@ -231,7 +231,7 @@ struct pisurf_ctx {
     DBuf samples, cl_flag, cl_pos, cl_list;
     // sample deduplication: keys / entries (ping-pong), heads, scanned heads, representatives,
     // head list, per-point values
-    DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals;
+    DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals, sd_cwork;
     // scratch for API calls
     DBuf q, out1, out2, out3;
     // multi-frame path: next frame's particles, their staging and copy stream
@@ -249,6 +249,7 @@ struct pisurf_ctx {
     unsigned long long n_fp64_eig = 0;
     unsigned long long host_sample_entries = 0;  // deduplicated sampling: 27 per sampled node
     std::vector<hipEvent_t> ev;  // start/stop pairs around node-sampling launches
+    std::vector<int> ev_kind;
     size_t ev_used = 0;
 
     ~pisurf_ctx() {
@@ -257,7 +258,8 @@ struct pisurf_ctx {
         if (cstream) (void)hipStreamDestroy(cstream);
         if (stream) (void)hipStreamDestroy(stream);
     }
-    hipError_t event_pair(hipEvent_t& a, hipEvent_t& b) {
+    // start/stop events around one evaluation launch; kind 0: per-point sampling kernel, 1: cell kernel
+    hipError_t event_pair(hipEvent_t& a, hipEvent_t& b, size_t* slot = nullptr) {
         while (ev.size() < ev_used + 2) {
             hipEvent_t e;
             hipError_t r = hipEventCreate(&e);
@@ -266,15 +268,24 @@ struct pisurf_ctx {
         }
         a = ev[ev_used];
         b = ev[ev_used + 1];
+        if (ev_kind.size() < ev_used / 2 + 1) ev_kind.resize(ev_used / 2 + 1);
+        ev_kind[ev_used / 2] = 0;
+        if (slot) *slot = ev_used / 2;
         ev_used += 2;
         return hipSuccess;
     }
-    // sum the recorded sampling-kernel intervals (stream already synchronized)
+    // sum the recorded intervals per kind (stream already synchronized)
     void harvest_events() {
         for (size_t i = 0; i + 1 < ev_used; i += 2) {
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) st.ms_sample_kernel += ms;
-            st.sample_launches++;
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) ms = 0.f;
+            if (ev_kind[i / 2]) {
+                st.ms_cell_kernel += ms;
+                st.cell_launches++;
+            } else {
+                st.ms_sample_kernel += ms;
+                st.sample_launches++;
+            }
         }
         ev_used = 0;
     }
@@ -304,6 +315,18 @@ struct pisurf_ctx {
 namespace {
 
 int prec_of(const pisurf_ctx* c) { return c->prm.precision; }
+
+int64_t pow2_at_least(int64_t n) {
+    int64_t p = 1024;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+int bits_for(int64_t v) {
+    int b = 1;
+    while (b < 63 && ((int64_t)1 << b) <= v) b++;
+    return b;
+}
 
 FieldConst field_const(const pisurf_ctx* c) {
     FieldConst fc;
@@ -622,22 +645,33 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             HIPCHK(c, c->sd_rep.ensure(nent * 4));
             HIPCHK(c, c->sd_hlist.ensure(nent * 4));
             HIPCHK(c, c->sd_vals.ensure(nent * rsz));
+            HIPCHK(c, c->sd_cwork.ensure(3 * (nent + 2) * 4));
             const int64_t need = std::max(radix_tmp_ints(nent), scan_tmp_ints(nent + 1)) + 64;
             HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, need * 4)));
             const float origin[3] = {c->root_c[0] - c->root_half, c->root_c[1] - c->root_half,
                                      c->root_c[2] - c->root_half};
-            launch_sample_dedupe_keys(v, L->child.as<int>(), nent, origin, c->sd_key.as<uint64_t>(),
+            // many sample points per hash cell (>= 32 for a full cell): group them by cell and
+            // evaluate each cell's points against one staged candidate list
+            const double half = std::ldexp((double)c->root_half, -level_depth);
+            const double per_cell = std::pow((double)c->g.cell / half, 3.0);
+            const int cellmode = per_cell >= 32.0 ? 1 : 0;
+            const uint64_t groups = (uint64_t)(c->g.n[0] + 2) * (uint64_t)(c->g.n[1] + 2) * (uint64_t)(c->g.n[2] + 2);
+            const int kbits = cellmode ? std::min(64, bits_for((int64_t)groups + 1) + 9) : 3 * (level_depth + 2);
+            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, cellmode, c->sd_key.as<uint64_t>(),
                                       c->sd_val.as<int>(), c->stream);
             radix_sort_pairs64(c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
-                               c->sd_val2.as<int>(), nent, 3 * (level_depth + 2), c->tmp.as<int>(), c->stream);
+                               c->sd_val2.as<int>(), nent, kbits, c->tmp.as<int>(), c->stream);
             hipEvent_t e0, e1;
-            HIPCHK(c, c->event_pair(e0, e1));
-            int64_t nheads = 0;
+            size_t eslot = 0;
+            HIPCHK(c, c->event_pair(e0, e1, &eslot));
+            int64_t nheads = 0, ngroups = 0;
             launch_sample_dedupe_eval(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, L->child.as<int>(),
-                                      c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), nent, c->sd_head.as<int>(),
-                                      c->sd_hpos.as<int>(), c->sd_rep.as<int>(), c->sd_hlist.as<int>(), &nheads,
-                                      c->sd_vals.p, c->samples.p, c->tmp.as<int>(), c->ctr(), e0, e1, c->stream);
+                                      c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), nent, cellmode,
+                                      c->sd_head.as<int>(), c->sd_hpos.as<int>(), c->sd_rep.as<int>(),
+                                      c->sd_hlist.as<int>(), c->sd_cwork.as<int>(), &nheads, &ngroups, c->sd_vals.p,
+                                      c->samples.p, c->tmp.as<int>(), c->ctr(), e0, e1, c->stream);
             c->host_sample_entries += (unsigned long long)nent;
+            c->ev_kind[eslot] = ngroups > 0 ? 1 : 0;
         }
     } else {
         hipEvent_t e0, e1;
@@ -862,17 +896,6 @@ struct Extractor {
     }
 };
 
-int64_t pow2_at_least(int64_t n) {
-    int64_t p = 1024;
-    while (p < n) p <<= 1;
-    return p;
-}
-
-int bits_for(int64_t v) {
-    int b = 1;
-    while (b < 63 && ((int64_t)1 << b) <= v) b++;
-    return b;
-}
 
 int read_int(pisurf_ctx* c, const int* dptr, int64_t& out) {
     int v = 0;
@@ -1166,6 +1189,8 @@ int sync_counters(pisurf_ctx* c) {
         h.sample_evals += k.sample_evals;
         h.sample_candidates += k.sample_candidates;
         h.sample_unique += k.sample_unique;
+        h.cell_evals += k.cell_evals;
+        h.cell_candidates += k.cell_candidates;
     }
     c->st.field_evals = (int64_t)(h.field_evals + c->host_sample_entries);
     c->st.grad_evals = (int64_t)h.grad_evals;
@@ -1174,6 +1199,8 @@ int sync_counters(pisurf_ctx* c) {
     c->st.sample_evals = (int64_t)(h.sample_evals + c->host_sample_entries);
     c->st.sample_candidates = (int64_t)h.sample_candidates;
     c->st.sample_unique = (int64_t)h.sample_unique;
+    c->st.cell_evals = (int64_t)h.cell_evals;
+    c->st.cell_candidates = (int64_t)h.cell_candidates;
     c->st.fp64_eigensolves = (int64_t)c->n_fp64_eig;
     return PISURF_OK;
 }

@@ -18,6 +18,27 @@ namespace pisurf {
 
 __device__ __forceinline__ bool is_leaf_type(uint8_t t) { return t == 2 || t == 0; }
 
+// point keys: Morton interleave of the 21-bit finest-level coordinates (x in bit 0), so dual cells,
+// and the vertices numbered from them, come out in space-filling order
+__device__ __forceinline__ uint64_t morton_spread(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+__device__ __forceinline__ uint32_t morton_compact(uint64_t v) {
+    v &= 0x1249249249249249ull;
+    v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ull;
+    v = (v ^ (v >> 4)) & 0x100f00f00f00f00full;
+    v = (v ^ (v >> 8)) & 0x1f0000ff0000ffull;
+    v = (v ^ (v >> 16)) & 0x1f00000000ffffull;
+    v = (v ^ (v >> 32)) & 0x1fffffull;
+    return (uint32_t)v;
+}
+
 __global__ void k_leaf_flag(NodeLevel lv, int64_t off, int* __restrict__ flag) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= lv.n) return;
@@ -25,7 +46,7 @@ __global__ void k_leaf_flag(NodeLevel lv, int64_t off, int* __restrict__ flag) {
 }
 
 // leaf records and the 8 corner keys of every leaf (corners on the root boundary get the
-// sentinel key ~0 and sort last)
+// sentinel key ~0 and sort last; valid keys use 3 * dp bits)
 __global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp,
                             double4* __restrict__ leaf_node, uint64_t* __restrict__ ckey, int* __restrict__ cval,
                             int* __restrict__ n_valid) {
@@ -42,7 +63,8 @@ __global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ l
             const int64_t X = (int64_t)(p.x + (c & 1)) << sh, Y = (int64_t)(p.y + ((c >> 1) & 1)) << sh,
                           Z = (int64_t)(p.z + ((c >> 2) & 1)) << sh;
             const bool in = X >= 1 && X <= hi && Y >= 1 && Y <= hi && Z >= 1 && Z <= hi;
-            ckey[8 * (int64_t)L + c] = in ? ((uint64_t)Z << (2 * dp)) | ((uint64_t)Y << dp) | (uint64_t)X : ~0ull;
+            ckey[8 * (int64_t)L + c] =
+                in ? morton_spread(X) | (morton_spread(Y) << 1) | (morton_spread(Z) << 2) : ~0ull;
             cval[8 * (int64_t)L + c] = 8 * L + c;
             valid += in;
         }
@@ -78,8 +100,7 @@ __device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx
         const int v = sval[k];
         n[(v & 7) ^ 7] = v >> 3;  // a leaf with P as corner c lies in octant c ^ 7 of P
     }
-    const uint64_t m = ((uint64_t)1 << tv.dp) - 1;
-    const int px = (int)(key & m), py = (int)((key >> tv.dp) & m), pz = (int)(key >> (2 * tv.dp));
+    const int px = (int)morton_compact(key), py = (int)morton_compact(key >> 1), pz = (int)morton_compact(key >> 2);
     int any = -1;
     for (int o = 0; o < 8; o++)
         if (n[o] < 0) n[o] = locate(tv, lidx, px - 1 + (o & 1), py - 1 + ((o >> 1) & 1), pz - 1 + ((o >> 2) & 1));

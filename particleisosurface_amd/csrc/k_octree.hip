@@ -140,8 +140,21 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v) {
     return x;
 }
 
-__global__ void k_sample_keys(NodeLevel lv, const int* __restrict__ list, int64_t nent, float ox, float oy, float oz,
-                              uint64_t* __restrict__ key, int* __restrict__ val) {
+// Hash cell group of a point for the cell-grouped evaluation: its cell (CalcXYZIdx) in the grid
+// extended by one cell on every side; points farther out have an empty 27-cell window and share
+// the last group. Points of one group have the same candidate list in the same order.
+__device__ __forceinline__ uint64_t cell_group(const GridDev& g, float x, float y, float z) {
+    int ix, iy, iz;
+    cell_of(g, x, y, z, ix, iy, iz);
+    const uint64_t ex = g.n[0] + 2, ey = g.n[1] + 2, ez = g.n[2] + 2;
+    if (ix < -1 || ix > g.n[0] || iy < -1 || iy > g.n[1] || iz < -1 || iz > g.n[2]) return ex * ey * ez;
+    return ((uint64_t)(iz + 1) * ey + (uint64_t)(iy + 1)) * ex + (uint64_t)(ix + 1);
+}
+
+// cellmode 0: Morton code of the lattice position; cellmode 1: cell group, then the lattice
+// position modulo 8 per axis (distinct inside a cell spanning fewer than 8 lattice steps)
+__global__ void k_sample_keys(GridDev g, NodeLevel lv, const int* __restrict__ list, int64_t nent, float ox,
+                              float oy, float oz, int cellmode, uint64_t* __restrict__ key, int* __restrict__ val) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nent) return;
     const int node = list[t / 27];
@@ -155,7 +168,10 @@ __global__ void k_sample_keys(NodeLevel lv, const int* __restrict__ list, int64_
         const float u = rintf((p[a] - o[a]) / c.w);
         q[a] = (uint32_t)fminf(fmaxf(u, 0.0f), 2097151.0f);
     }
-    key[t] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    if (cellmode)
+        key[t] = (cell_group(g, p[0], p[1], p[2]) << 9) | (q[0] & 7) | ((q[1] & 7) << 3) | ((q[2] & 7) << 6);
+    else
+        key[t] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
     val[t] = (int)t;
 }
 
@@ -216,6 +232,105 @@ __global__ void __launch_bounds__(256) k_sample_eval(GridDev g, Recs<R> rec, Fie
     wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
     wave_count(&ctr_stripe(ctr)->sample_unique, h < nheads ? 1ull : 0ull);
     wave_count(&ctr_stripe(ctr)->sample_candidates, (unsigned long long)vis);
+}
+
+// ---- cell-grouped evaluation (levels with many sample points per hash cell) ----
+// cflag[h] = 1 where the cell group changes along the distinct points (cell-major order)
+__global__ void k_cell_flags(const uint64_t* __restrict__ key, const int* __restrict__ hlist, int64_t nheads,
+                             int* __restrict__ cflag) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h > nheads) return;
+    if (h == nheads) {
+        cflag[h] = 0;
+        return;
+    }
+    cflag[h] = (h == 0 || (key[hlist[h]] >> 9) != (key[hlist[h - 1]] >> 9)) ? 1 : 0;
+}
+// cstart[k] = first distinct point of cell group k; cstart[ngroups] = nheads
+__global__ void k_cell_starts(const int* __restrict__ cflag, const int* __restrict__ cpos, int64_t nheads,
+                              int* __restrict__ cstart) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h > nheads) return;
+    if (h == nheads) cstart[cpos[h]] = (int)nheads;
+    else if (cflag[h]) cstart[cpos[h]] = (int)h;
+}
+
+// One workgroup per cell group: the group's 27-cell candidates are staged through LDS in the
+// reference's order (rows z-outer, then y; x-ascending cells; sorted particles), kCellChunk at a
+// time, and every thread sums them for one point. Each point sees exactly the candidate sequence
+// of field_sum, with the same per-candidate arithmetic.
+constexpr int kCellChunk = 256;
+constexpr int kCellThreads = 128;
+template <class R, bool K>
+__global__ void __launch_bounds__(kCellThreads) k_cell_eval(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
+                                                            const int* __restrict__ list, const int* __restrict__ val,
+                                                            const int* __restrict__ hlist,
+                                                            const int* __restrict__ cstart, R* __restrict__ out,
+                                                            Counters* ctr) {
+    typedef typename Vec4<R>::T V4;
+    constexpr bool LO = K && sizeof(R) == 4;
+    __shared__ V4 sa[kCellChunk], sg0[kCellChunk], sg1[kCellChunk];
+    __shared__ float4 slo[LO ? kCellChunk : 1];
+    const int64_t grp = blockIdx.x;
+    const int h0 = cstart[grp], h1 = cstart[grp + 1];
+    float p0[3];
+    entry_point(lv, list, val[hlist[h0]], p0);
+    int rlo[9], rhi[9], roff[10];
+    const int total = field_rows(g, p0[0], p0[1], p0[2], rlo, rhi);
+    roff[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 9; r++) roff[r + 1] = roff[r] + (rhi[r] - rlo[r]);
+    const R inv_h2 = R(fc.inv_h2), h2 = R(fc.h2);
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int hb = h0; hb < h1; hb += kCellThreads) {
+        const int h = hb + threadIdx.x;
+        const bool act = h < h1;
+        R px = R(0), py = R(0), pz = R(0);
+        if (act) {
+            float p[3];
+            entry_point(lv, list, val[hlist[h]], p);
+            px = R(p[0]);
+            py = R(p[1]);
+            pz = R(p[2]);
+        }
+        Acc<R, K> acc, ga[3];
+        for (int base = 0; base < total; base += kCellChunk) {
+            const int n = min(kCellChunk, total - base);
+            __syncthreads();  // the previous chunk has been consumed
+            for (int t = threadIdx.x; t < n; t += kCellThreads) {
+                const int li = base + t;
+                int r = 0;
+                while (li >= roff[r + 1]) r++;
+                const int j = rlo[r] + (li - roff[r]);
+                sa[t] = rec.a[j];
+                sg0[t] = rec.g[2 * j];
+                sg1[t] = rec.g[2 * j + 1];
+                if (LO) slo[t] = fc.lo[j];
+            }
+            __syncthreads();
+            if (act) {
+                for (int t = 0; t < n; t++) {
+                    const V4 a = sa[t];
+                    const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+                    const R d2 = dist2(dx, dy, dz);
+                    if (d2 < a.w)
+                        field_contrib<R, K, false>(fc, inv_h2, h2, dx, dy, dz, d2, sg0[t], sg1[t],
+                                                   LO ? slo[LO ? t : 0] : zero4, acc, ga);
+                }
+            }
+        }
+        if (act) out[h] = R(fc.iso) - R(fc.sigma) * acc.get();
+    }
+    // one add per group: unique points, and candidates as the per-point kernel counts them
+    if (threadIdx.x == 0) {
+        Counters* c = ctr_stripe(ctr);
+        const unsigned long long np = (unsigned long long)(h1 - h0), cand = np * (unsigned long long)total;
+        atomicAdd(&c->sample_unique, np);
+        atomicAdd(&c->sample_candidates, cand);
+        atomicAdd(&c->candidates, cand);
+        atomicAdd(&c->cell_evals, np);
+        atomicAdd(&c->cell_candidates, cand);
+    }
 }
 
 // every entry receives its point's value (the host counts the entries: 27 per sampled node, the
@@ -427,18 +542,20 @@ void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* l
     k_sample_list<<<(unsigned)((lv.n + 255) / 256), 256, 0, s>>>(lv, cflag, cpos, clist);
 }
 
-void launch_sample_dedupe_keys(NodeLevel lv, const int* list, int64_t nent, const float origin[3], uint64_t* key,
-                               int* val, hipStream_t s) {
+void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
+                               int cellmode, uint64_t* key, int* val, hipStream_t s) {
     if (nent <= 0) return;
-    k_sample_keys<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(lv, list, nent, origin[0], origin[1], origin[2], key,
-                                                                 val);
+    k_sample_keys<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(g, lv, list, nent, origin[0], origin[1], origin[2],
+                                                                 cellmode, key, val);
 }
 
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
-                               const int* list, const uint64_t* key, const int* val, int64_t nent, int* head, int* hpos,
-                               int* rep, int* hlist, int64_t* nheads_out, void* vals, void* samples, int* tmp,
-                               Counters* ctr, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+                               const int* list, const uint64_t* key, const int* val, int64_t nent, int cellmode,
+                               int* head, int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out,
+                               int64_t* ngroups_out, void* vals, void* samples, int* tmp, Counters* ctr,
+                               hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
     *nheads_out = 0;
+    *ngroups_out = 0;
     if (nent <= 0) return;
     const unsigned nb = (unsigned)((nent + 1 + 255) / 256);
     k_sample_heads<<<nb, 256, 0, s>>>(lv, list, key, val, nent, head, rep);
@@ -448,18 +565,46 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     (void)hipMemcpyAsync(&nh, hpos + nent, 4, hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);
     *nheads_out = nh;
+    int ng = 0;
+    int* cflag = cwork;
+    int* cpos = cwork + (nent + 2);
+    int* cstart = cwork + 2 * (nent + 2);
+    if (cellmode && nh > 0) {
+        const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
+        k_cell_flags<<<nbh, 256, 0, s>>>(key, hlist, nh, cflag);
+        exclusive_scan_i32(cflag, cpos, nh + 1, tmp, s);
+        k_cell_starts<<<nbh, 256, 0, s>>>(cflag, cpos, nh, cstart);
+        (void)hipMemcpyAsync(&ng, cpos + nh, 4, hipMemcpyDeviceToHost, s);
+        (void)hipStreamSynchronize(s);
+        *ngroups_out = ng;
+    }
     const unsigned ne = (unsigned)((nh + 255) / 256);
     (void)hipEventRecord(e0, s);
     if (nh > 0) {
         if (prec == 2) {
             Recs<double> r{(const double4*)ra, (const double4*)rg};
-            k_sample_eval<double, false><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (double*)vals, ctr);
+            if (ng > 0)
+                k_cell_eval<double, false><<<ng, kCellThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, cstart,
+                                                                        (double*)vals, ctr);
+            else
+                k_sample_eval<double, false><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (double*)vals, ctr);
         } else {
             Recs<float> r{(const float4*)ra, (const float4*)rg};
-            if (prec == 1)
-                k_sample_eval<float, true><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (float*)vals, ctr);
-            else
-                k_sample_eval<float, false><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (float*)vals, ctr);
+            if (prec == 1) {
+                if (ng > 0)
+                    k_cell_eval<float, true><<<ng, kCellThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, cstart,
+                                                                         (float*)vals, ctr);
+                else
+                    k_sample_eval<float, true><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (float*)vals,
+                                                                  ctr);
+            } else {
+                if (ng > 0)
+                    k_cell_eval<float, false><<<ng, kCellThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, cstart,
+                                                                          (float*)vals, ctr);
+                else
+                    k_sample_eval<float, false><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (float*)vals,
+                                                                   ctr);
+            }
         }
     }
     (void)hipEventRecord(e1, s);
