@@ -238,6 +238,9 @@ struct pisurf_ctx {
     DBuf xyz_next;
     PinnedBuf pin;
     hipStream_t cstream = nullptr;
+    // mesh download stream and the events around the bisection
+    hipStream_t dstream = nullptr;
+    hipEvent_t eb0 = nullptr, eb1 = nullptr;
     // device extraction workspace
     DBuf eb[28];
     // mesh
@@ -256,6 +259,9 @@ struct pisurf_ctx {
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
         if (cstream) (void)hipStreamDestroy(cstream);
+        if (dstream) (void)hipStreamDestroy(dstream);
+        if (eb0) (void)hipEventDestroy(eb0);
+        if (eb1) (void)hipEventDestroy(eb1);
         if (stream) (void)hipStreamDestroy(stream);
     }
     // start/stop events around one evaluation launch; kind 0: per-point sampling kernel, 1: cell kernel
@@ -628,7 +634,8 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
                        c->cl_pos.as<int>(), c->cl_list.as<int>(), c->tmp.as<int>(), c->stream);
     // Deduplicated sampling needs one node size per level (lattice keys) and 21-bit lattice
     // coordinates: 2^(depth + 1) half-widths span the root box.
-    const bool dedupe = c->prm.sample_dedupe && level_depth >= 0 && level_depth <= 19;
+    // levels above D_min sample nothing (wants_samples), so they skip the count read-back
+    const bool dedupe = c->prm.sample_dedupe && level_depth >= 0 && level_depth <= 19 && level_depth >= c->dmin;
     if (dedupe) {
         int k = 0;
         HIPCHK(c, hipMemcpyAsync(&k, L->scanned.as<int>() + L->n, 4, hipMemcpyDeviceToHost, c->stream));
@@ -1027,13 +1034,19 @@ int do_extract_device(pisurf_ctx* c) {
                              E[21].as<double>(), s);
     HIPCHK(c, hipGetLastError());
     c->st.ms_extract_host += now_ms() - t0;  // device extraction time (topology)
-    const double t1 = now_ms();
     HIPCHK(c, E[22].ensure(std::max<int64_t>(nv, 1) * 3 * 8));
+    if (!c->eb0) HIPCHK(c, hipEventCreate(&c->eb0));
+    if (!c->eb1) HIPCHK(c, hipEventCreate(&c->eb1));
+    if (!c->dstream) HIPCHK(c, hipStreamCreateWithFlags(&c->dstream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventRecord(c->eb0, s));
     launch_bisect(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->prm.radius / 2, E[21].as<double>(), nv,
                   E[22].as<double>(), nullptr, c->ctr(), s);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(s));
-    c->st.ms_bisect += now_ms() - t1;
+    HIPCHK(c, hipEventRecord(c->eb1, s));
+    // the vertices go to the host on a second stream while the triangles are deduplicated
+    c->mesh_v.resize(nv * 3);
+    HIPCHK(c, hipStreamWaitEvent(c->dstream, c->eb1, 0));
+    if (nv) HIPCHK(c, hipMemcpyAsync(c->mesh_v.data(), E[22].p, nv * 3 * 8, hipMemcpyDeviceToHost, c->dstream));
     const double t2 = now_ms();
     // triangle dedupe by sorted vertex triple; the first triangle of a repeated triple stays
     int64_t ntk = nt;
@@ -1074,11 +1087,14 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[23].ensure(std::max<int64_t>(ntf, 1) * 3 * 4));
     launch_tri_out(cvid, E[8].as<int>(), E[9].as<int>(), nt, E[23].as<int>(), s);
     HIPCHK(c, hipGetLastError());
-    c->mesh_v.resize(nv * 3);
     c->mesh_t.resize(ntf * 3);
-    if (nv) HIPCHK(c, hipMemcpyAsync(c->mesh_v.data(), E[22].p, nv * 3 * 8, hipMemcpyDeviceToHost, s));
     if (ntf) HIPCHK(c, hipMemcpyAsync(c->mesh_t.data(), E[23].p, ntf * 3 * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
+    HIPCHK(c, hipStreamSynchronize(c->dstream));
+    {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->eb0, c->eb1) == hipSuccess) c->st.ms_bisect += ms;
+    }
     c->st.ms_extract_host += now_ms() - t2;
     c->st.n_dual_cells = nd;
     c->st.n_verts = nv;
