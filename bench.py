@@ -41,6 +41,19 @@ def measured_traffic(kernel):
     return None, None
 
 
+def device_memory_in_use():
+    """Bytes of device memory in use (hipMemGetInfo), or None."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+            return None
+        return total.value - free.value
+    except OSError:
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -126,7 +139,7 @@ def copy_calibration(mib):
                       "launches": 5}), flush=True)
 
 
-def batch_leg(prec, dedupe):
+def batch_leg(prec, dedupe, qef):
     """24 frames of uneven size: one persistent run_frames call vs. a loop of the plain path (a
     fresh context per frame, as the reference builds a new SurfReconstructor per frame)."""
     t = time.perf_counter()
@@ -140,7 +153,7 @@ def batch_leg(prec, dedupe):
         tris[0] += len(t_)
 
     ctx = capi.Context(0)
-    ctx.set_params(rs[0], precision=prec, sample_dedupe=dedupe)
+    ctx.set_params(rs[0], precision=prec, sample_dedupe=dedupe, qef_mode=qef)
     ctx.run_frames(xs[:2], rs[:2])  # warm-up: buffers, code objects
     tris[0] = 0
     t = time.perf_counter()
@@ -152,7 +165,7 @@ def batch_leg(prec, dedupe):
     tris_l = 0
     for x, r in zip(xs, rs):
         c = capi.Context(0)
-        c.set_params(r, precision=prec, sample_dedupe=dedupe)
+        c.set_params(r, precision=prec, sample_dedupe=dedupe, qef_mode=qef)
         c.upload(x)
         c.run_frame()
         tris_l += len(c.mesh()[1])
@@ -175,6 +188,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--precision", default="fp32k", choices=["fp32", "fp32k", "fp64"])
     ap.add_argument("--no-dedupe", action="store_true", help="evaluate all 27 samples of every node")
+    ap.add_argument("--qef", type=int, default=0, choices=[0, 1, 2],
+                    help="leaf vertex: 0 centre (the reference as shipped), 1 float QEF, 2 FP64 truncated-SVD QEF")
+    ap.add_argument("--workload", default="crown", choices=["crown", "pour"],
+                    help="crown: crown splash (~4.0 M, r=0.001); pour: multi-body pour (~12.1 M, r=0.0005, depth 12)")
+    ap.add_argument("--depth-max", type=int, default=-1, help="octree depth override (-1: 12 for pour, else derived)")
     ap.add_argument("--batch", action="store_true",
                     help="also mesh the 24-frame batch (50 k - 3 M particles per frame, radii by frame) "
                          "through the persistent multi-frame call and through a loop of fresh contexts")
@@ -190,12 +208,24 @@ def main():
         sys.exit(2)
 
     t = time.perf_counter()
-    xyz, r = synth.crown_splash(pool_layers=args.pool_layers)
-    log("workload: crown splash, %d particles, r=%g (%.1f s to generate)" % (len(xyz), r, time.perf_counter() - t))
+    if args.workload == "pour":
+        xyz, r = synth.multibody_pour()
+        wname = "multi-body pour (config 5), %d particles, r=0.0005, pool 46 layers" % len(xyz)
+    else:
+        xyz, r = synth.crown_splash(pool_layers=args.pool_layers)
+        wname = "crown splash (config 3), %d particles, r=0.001, pool %d layers" % (len(xyz), args.pool_layers)
+    depth = args.depth_max if args.depth_max >= 0 else (12 if args.workload == "pour" else 0)
+    log("workload: %s (%.1f s to generate)" % (wname, time.perf_counter() - t))
 
     prec = {"fp32": capi.FP32, "fp32k": capi.FP32_KAHAN, "fp64": capi.FP64}[args.precision]
+    mem0 = device_memory_in_use()
     ctx = capi.Context(0)
-    ctx.set_params(r, precision=prec, sample_dedupe=not args.no_dedupe)
+
+    def set_params(precision):
+        ctx.set_params(r, precision=precision, sample_dedupe=not args.no_dedupe, qef_mode=args.qef,
+                       depth_max_override=depth)
+
+    set_params(prec)
     ctx.upload(xyz)  # resident in HBM before the timed region
     info, st, dt = frame(ctx, args.steps, args.warmup)
     ms = 1e3 * dt / args.steps
@@ -206,9 +236,8 @@ def main():
                                             st["ms_normals"] / args.steps, st["ms_tree"] / args.steps,
                                             st["ms_extract_host"] / args.steps, st["ms_bisect"] / args.steps))
     value = evals_per_frame / (dt / args.steps) / args.steps
-    # roofline of the dominant field-evaluation kernel: the cell-grouped sampling kernel when a level
-    # uses it (the deepest sampled level holds most points), else the per-point sampling kernel;
-    # algorithmic bytes of the evaluations it performs (one per distinct sample point)
+    # roofline of the dominant field-evaluation kernel, the node-sampling kernel: algorithmic bytes of
+    # the evaluations it performs (one per distinct sample point when deduplicated)
     def kernel_roof(name, evals, cands, launches, ms):
         k_ms = ms / max(launches, 1)
         alg = evals * BYTES_FIXED + cands * BYTES_PER_CAND[prec]
@@ -220,17 +249,13 @@ def main():
                     evals_per_launch=evals / max(launches, 1), candidates_per_eval=cands / max(evals, 1),
                     algorithmic_bytes_per_launch=alg / max(launches, 1),
                     evals_per_s=evals / (ms * 1e-3) if ms > 0 else None)
-    performed = st["sample_unique"] if st["sample_unique"] > 0 else st["sample_evals"]
-    pt_evals = performed - st["cell_evals"]
-    pt_cands = st["sample_candidates"] - st["cell_candidates"]
-    pt_name = "k_sample_eval" if st["sample_unique"] > 0 else "k_samples"
-    per_point = kernel_roof(pt_name, pt_evals, pt_cands, st["sample_launches"], st["ms_sample_kernel"])
-    if st["cell_launches"] > 0:
-        roof = kernel_roof("k_cell_eval", st["cell_evals"], st["cell_candidates"], st["cell_launches"],
-                           st["ms_cell_kernel"])
-        roof["other_sampling_kernel"] = per_point
+    if st["run_launches"] > 0:
+        roof = kernel_roof("k_run_samples", st["run_evals"], st["run_candidates"], st["run_launches"],
+                           st["ms_run_kernel"])
     else:
-        roof = per_point
+        roof = kernel_roof("k_samples", st["sample_evals"], st["sample_candidates"], st["sample_launches"],
+                           st["ms_sample_kernel"])
+    performed = st["sample_unique"] if st["sample_unique"] > 0 else st["sample_evals"]
     v32, t32 = ctx.mesh()
     out = {
         "metric": "field evals/s + frame mesh time @4M particles, FP32-comp vs FP64 at stated err",
@@ -238,9 +263,10 @@ def main():
         "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"fp32": "f32", "fp32k": "f32 (compensated; f64 eigensolve when clamped)", "fp64": "f64"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": "crown splash (config 3), %d particles, r=0.001, pool %d layers"
-                               % (len(xyz), args.pool_layers), "precision": args.precision,
+        "config": {"workload": wname, "precision": args.precision, "qef_mode": args.qef,
                    "depth_max": info["depth_max"], "depth_min": info["depth_min"]},
+        "qef_solves_per_frame": st["qef_solves"] // args.steps,
+        "device_memory_bytes": device_memory_in_use() - mem0 if mem0 is not None else None,
         "frame_ms": round(ms, 3),
         "field_evals_per_frame": evals_per_frame // args.steps,
         "field_evals_performed_per_frame": (evals_per_frame - st["sample_evals"] + performed) // args.steps,
@@ -259,7 +285,7 @@ def main():
         f32 = ctx.eval_points(q)
         n32 = ctx.particle_normals()
         iso = ctx.iso_constants()[1]
-        ctx.set_params(r, precision=capi.FP64, sample_dedupe=not args.no_dedupe)
+        set_params(capi.FP64)
         info64, st64, dt64 = frame(ctx, 1, 1)
         ms64 = 1e3 * dt64
         v64, t64 = ctx.mesh()
@@ -283,11 +309,11 @@ def main():
             "same_tree": bool(info64["n_nodes"] == info["n_nodes"]),
         }
         log("fp64 frame %.2f ms, speedup %.2fx" % (ms64, ms64 / ms))
-        ctx.set_params(r, precision=prec, sample_dedupe=not args.no_dedupe)
+        set_params(prec)
         info, st, dt = frame(ctx, 1, 0)
 
     if args.batch:
-        out["batch"] = batch_leg(prec, not args.no_dedupe)
+        out["batch"] = batch_leg(prec, not args.no_dedupe, args.qef)
         log("batch: %s" % out["batch"])
 
     if not args.no_cpu:

@@ -94,10 +94,11 @@ typedef struct pisurf_stats {
     double ms_sample_kernel;   /* summed device time of the per-point node-sampling kernel (HIP events) */
     int64_t n_dual_cells, n_verts, n_tris, n_nodes;
     int64_t sample_unique;     /* sample evaluations performed after deduplication (<= sample_evals) */
-    /* the cell-grouped sampling kernel (levels with many sample points per hash cell); its
-     * evaluations and candidates are included in sample_unique / sample_candidates */
-    int64_t cell_evals, cell_candidates, cell_launches;
-    double ms_cell_kernel;
+    /* the run kernel of deduplicated sampling (distinct points grouped by hash cell, candidates
+     * staged in LDS); its evaluations and candidates are included in sample_unique /
+     * sample_candidates */
+    int64_t run_evals, run_candidates, run_launches;
+    double ms_run_kernel;
 } pisurf_stats;
 
 /* ---- context ---- */

@@ -53,8 +53,8 @@ class Stats(C.Structure):
                 ("ms_normals", C.c_double), ("ms_tree", C.c_double), ("ms_extract_host", C.c_double),
                 ("ms_bisect", C.c_double), ("ms_total", C.c_double), ("ms_sample_kernel", C.c_double),
                 ("n_dual_cells", C.c_int64), ("n_verts", C.c_int64), ("n_tris", C.c_int64), ("n_nodes", C.c_int64),
-                ("sample_unique", C.c_int64), ("cell_evals", C.c_int64), ("cell_candidates", C.c_int64),
-                ("cell_launches", C.c_int64), ("ms_cell_kernel", C.c_double)]
+                ("sample_unique", C.c_int64), ("run_evals", C.c_int64), ("run_candidates", C.c_int64),
+                ("run_launches", C.c_int64), ("ms_run_kernel", C.c_double)]
 
 
 _lib = None

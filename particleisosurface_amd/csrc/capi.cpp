@@ -264,7 +264,7 @@ struct pisurf_ctx {
         if (eb1) (void)hipEventDestroy(eb1);
         if (stream) (void)hipStreamDestroy(stream);
     }
-    // start/stop events around one evaluation launch; kind 0: per-point sampling kernel, 1: cell kernel
+    // start/stop events around one evaluation launch; kind 0: per-point sampling kernel, 1: run kernel
     hipError_t event_pair(hipEvent_t& a, hipEvent_t& b, size_t* slot = nullptr) {
         while (ev.size() < ev_used + 2) {
             hipEvent_t e;
@@ -286,8 +286,8 @@ struct pisurf_ctx {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) ms = 0.f;
             if (ev_kind[i / 2]) {
-                st.ms_cell_kernel += ms;
-                st.cell_launches++;
+                st.ms_run_kernel += ms;
+                st.run_launches++;
             } else {
                 st.ms_sample_kernel += ms;
                 st.sample_launches++;
@@ -652,33 +652,30 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             HIPCHK(c, c->sd_rep.ensure(nent * 4));
             HIPCHK(c, c->sd_hlist.ensure(nent * 4));
             HIPCHK(c, c->sd_vals.ensure(nent * rsz));
-            HIPCHK(c, c->sd_cwork.ensure(3 * (nent + 2) * 4));
+            HIPCHK(c, c->sd_cwork.ensure(4 * (nent + 2) * 4));
             const int64_t need = std::max(radix_tmp_ints(nent), scan_tmp_ints(nent + 1)) + 64;
             HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, need * 4)));
             const float origin[3] = {c->root_c[0] - c->root_half, c->root_c[1] - c->root_half,
                                      c->root_c[2] - c->root_half};
-            // many sample points per hash cell (>= 32 for a full cell): group them by cell and
-            // evaluate each cell's points against one staged candidate list
-            const double half = std::ldexp((double)c->root_half, -level_depth);
-            const double per_cell = std::pow((double)c->g.cell / half, 3.0);
-            const int cellmode = per_cell >= 32.0 ? 1 : 0;
+            // distinct points grouped by hash cell (cell-major keys), evaluated in runs of cells
             const uint64_t groups = (uint64_t)(c->g.n[0] + 2) * (uint64_t)(c->g.n[1] + 2) * (uint64_t)(c->g.n[2] + 2);
-            const int kbits = cellmode ? std::min(64, bits_for((int64_t)groups + 1) + 9) : 3 * (level_depth + 2);
-            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, cellmode, c->sd_key.as<uint64_t>(),
+            const int kbits = std::min(64, bits_for((int64_t)groups + 1) + 9);
+            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, 1, c->sd_key.as<uint64_t>(),
                                       c->sd_val.as<int>(), c->stream);
             radix_sort_pairs64(c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
                                c->sd_val2.as<int>(), nent, kbits, c->tmp.as<int>(), c->stream);
             hipEvent_t e0, e1;
             size_t eslot = 0;
             HIPCHK(c, c->event_pair(e0, e1, &eslot));
-            int64_t nheads = 0, ngroups = 0;
+            int64_t nheads = 0, nruns = 0;
+            int kx = 0;
             launch_sample_dedupe_eval(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, L->child.as<int>(),
-                                      c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), nent, cellmode,
-                                      c->sd_head.as<int>(), c->sd_hpos.as<int>(), c->sd_rep.as<int>(),
-                                      c->sd_hlist.as<int>(), c->sd_cwork.as<int>(), &nheads, &ngroups, c->sd_vals.p,
-                                      c->samples.p, c->tmp.as<int>(), c->ctr(), e0, e1, c->stream);
+                                      c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), nent, c->sd_head.as<int>(),
+                                      c->sd_hpos.as<int>(), c->sd_rep.as<int>(), c->sd_hlist.as<int>(),
+                                      c->sd_cwork.as<int>(), &nheads, &nruns, &kx, c->sd_vals.p, c->samples.p,
+                                      c->tmp.as<int>(), c->ctr(), e0, e1, c->stream);
             c->host_sample_entries += (unsigned long long)nent;
-            c->ev_kind[eslot] = ngroups > 0 ? 1 : 0;
+            c->ev_kind[eslot] = nruns > 0 ? 1 : 0;
         }
     } else {
         hipEvent_t e0, e1;
@@ -1205,8 +1202,8 @@ int sync_counters(pisurf_ctx* c) {
         h.sample_evals += k.sample_evals;
         h.sample_candidates += k.sample_candidates;
         h.sample_unique += k.sample_unique;
-        h.cell_evals += k.cell_evals;
-        h.cell_candidates += k.cell_candidates;
+        h.run_evals += k.run_evals;
+        h.run_candidates += k.run_candidates;
     }
     c->st.field_evals = (int64_t)(h.field_evals + c->host_sample_entries);
     c->st.grad_evals = (int64_t)h.grad_evals;
@@ -1215,8 +1212,8 @@ int sync_counters(pisurf_ctx* c) {
     c->st.sample_evals = (int64_t)(h.sample_evals + c->host_sample_entries);
     c->st.sample_candidates = (int64_t)h.sample_candidates;
     c->st.sample_unique = (int64_t)h.sample_unique;
-    c->st.cell_evals = (int64_t)h.cell_evals;
-    c->st.cell_candidates = (int64_t)h.cell_candidates;
+    c->st.run_evals = (int64_t)h.run_evals;
+    c->st.run_candidates = (int64_t)h.run_candidates;
     c->st.fp64_eigensolves = (int64_t)c->n_fp64_eig;
     return PISURF_OK;
 }

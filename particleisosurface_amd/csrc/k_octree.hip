@@ -3,6 +3,7 @@
 // TNode, src/iso_method_ours.cpp:16-99). One launch of each kernel covers all nodes of a level.
 #include "pisurf_internal.h"
 #include "qef_device.h"
+#include "run_eval.h"
 
 namespace pisurf {
 
@@ -214,26 +215,6 @@ __global__ void k_head_list(const int* __restrict__ head, const int* __restrict_
     if (j < nent && head[j]) hlist[hpos[j]] = (int)j;
 }
 
-// one evaluation per distinct point, in Morton order
-template <class R, bool K>
-__global__ void __launch_bounds__(256) k_sample_eval(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
-                                                     const int* __restrict__ list, const int* __restrict__ val,
-                                                     const int* __restrict__ hlist, int64_t nheads, R* __restrict__ out,
-                                                     Counters* ctr) {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int vis = 0;
-    if (h < nheads) {
-        float p[3];
-        entry_point(lv, list, val[hlist[h]], p);
-        R sum, gr[3];
-        vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), sum, gr);
-        out[h] = R(fc.iso) - R(fc.sigma) * sum;
-    }
-    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
-    wave_count(&ctr_stripe(ctr)->sample_unique, h < nheads ? 1ull : 0ull);
-    wave_count(&ctr_stripe(ctr)->sample_candidates, (unsigned long long)vis);
-}
-
 // ---- cell-grouped evaluation (levels with many sample points per hash cell) ----
 // cflag[h] = 1 where the cell group changes along the distinct points (cell-major order)
 __global__ void k_cell_flags(const uint64_t* __restrict__ key, const int* __restrict__ hlist, int64_t nheads,
@@ -255,81 +236,61 @@ __global__ void k_cell_starts(const int* __restrict__ cflag, const int* __restri
     else if (cflag[h]) cstart[cpos[h]] = (int)h;
 }
 
-// One workgroup per cell group: the group's 27-cell candidates are staged through LDS in the
-// reference's order (rows z-outer, then y; x-ascending cells; sorted particles), kCellChunk at a
-// time, and every thread sums them for one point. Each point sees exactly the candidate sequence
-// of field_sum, with the same per-candidate arithmetic.
-constexpr int kCellChunk = 256;
-constexpr int kCellThreads = 128;
-template <class R, bool K>
-__global__ void __launch_bounds__(kCellThreads) k_cell_eval(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
-                                                            const int* __restrict__ list, const int* __restrict__ val,
-                                                            const int* __restrict__ hlist,
-                                                            const int* __restrict__ cstart, R* __restrict__ out,
-                                                            Counters* ctr) {
-    typedef typename Vec4<R>::T V4;
-    constexpr bool LO = K && sizeof(R) == 4;
-    __shared__ V4 sa[kCellChunk], sg0[kCellChunk], sg1[kCellChunk];
-    __shared__ float4 slo[LO ? kCellChunk : 1];
-    const int64_t grp = blockIdx.x;
-    const int h0 = cstart[grp], h1 = cstart[grp + 1];
-    float p0[3];
-    entry_point(lv, list, val[hlist[h0]], p0);
-    int rlo[9], rhi[9], roff[10];
-    const int total = field_rows(g, p0[0], p0[1], p0[2], rlo, rhi);
-    roff[0] = 0;
-#pragma unroll
-    for (int r = 0; r < 9; r++) roff[r + 1] = roff[r] + (rhi[r] - rlo[r]);
-    const R inv_h2 = R(fc.inv_h2), h2 = R(fc.h2);
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int hb = h0; hb < h1; hb += kCellThreads) {
-        const int h = hb + threadIdx.x;
-        const bool act = h < h1;
-        R px = R(0), py = R(0), pz = R(0);
-        if (act) {
-            float p[3];
-            entry_point(lv, list, val[hlist[h]], p);
-            px = R(p[0]);
-            py = R(p[1]);
-            pz = R(p[2]);
-        }
-        Acc<R, K> acc, ga[3];
-        for (int base = 0; base < total; base += kCellChunk) {
-            const int n = min(kCellChunk, total - base);
-            __syncthreads();  // the previous chunk has been consumed
-            for (int t = threadIdx.x; t < n; t += kCellThreads) {
-                const int li = base + t;
-                int r = 0;
-                while (li >= roff[r + 1]) r++;
-                const int j = rlo[r] + (li - roff[r]);
-                sa[t] = rec.a[j];
-                sg0[t] = rec.g[2 * j];
-                sg1[t] = rec.g[2 * j + 1];
-                if (LO) slo[t] = fc.lo[j];
-            }
-            __syncthreads();
-            if (act) {
-                for (int t = 0; t < n; t++) {
-                    const V4 a = sa[t];
-                    const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-                    const R d2 = dist2(dx, dy, dz);
-                    if (d2 < a.w)
-                        field_contrib<R, K, false>(fc, inv_h2, h2, dx, dy, dz, d2, sg0[t], sg1[t],
-                                                   LO ? slo[LO ? t : 0] : zero4, acc, ga);
-                }
-            }
-        }
-        if (act) out[h] = R(fc.iso) - R(fc.sigma) * acc.get();
+// runs of cell groups: rflag[k] = 1 where the run id (cell row, x / kx) changes along the groups
+__global__ void k_run_flags(GridDev g, const uint64_t* __restrict__ key, const int* __restrict__ hlist,
+                            const int* __restrict__ cstart, int64_t ngroups, int kx, int* __restrict__ rflag) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > ngroups) return;
+    if (k == ngroups) {
+        rflag[k] = 0;
+        return;
     }
-    // one add per group: unique points, and candidates as the per-point kernel counts them
+    auto run_id = [&](int64_t grp) {
+        const uint64_t gid = key[hlist[cstart[grp]]] >> 9;
+        const uint64_t ex = g.n[0] + 2, ey = g.n[1] + 2;
+        const int ix = (int)(gid % ex) - 1, iy = (int)((gid / ex) % ey) - 1, iz = (int)(gid / (ex * ey)) - 1;
+        return run_of_cell(g, ix, iy, iz, kx);  // the empty-window group maps past every row
+    };
+    rflag[k] = (k == 0 || run_id(k) != run_id(k - 1)) ? 1 : 0;
+}
+// rstart[j] = first distinct point of run j; rstart[nruns] = nheads (= cstart[ngroups])
+__global__ void k_run_starts(const int* __restrict__ rflag, const int* __restrict__ rpos,
+                             const int* __restrict__ cstart, int64_t ngroups, int* __restrict__ rstart) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > ngroups) return;
+    if (k == ngroups || rflag[k]) rstart[rpos[k]] = cstart[k];
+}
+
+// distinct sample points of a level (cell-major), evaluated run by run
+template <class R> struct SampleSrc {
+    NodeLevel lv;
+    const int* list;
+    const int* val;
+    const int* hlist;
+    R* out;
+    R iso, sigma;
+    __device__ __forceinline__ void point(int h, float p[3]) const { entry_point(lv, list, val[hlist[h]], p); }
+    __device__ __forceinline__ void store(int h, R sum, const R*) const { out[h] = iso - sigma * sum; }
+};
+
+template <class R, bool K>
+__global__ void __launch_bounds__(kRunThreads) k_run_samples(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
+                                                             const int* __restrict__ list, const int* __restrict__ val,
+                                                             const int* __restrict__ hlist,
+                                                             const int* __restrict__ rstart, R* __restrict__ out,
+                                                             Counters* ctr) {
+    __shared__ RunLds<R, K && sizeof(R) == 4> lds;
+    const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
+    SampleSrc<R> src{lv, list, val, hlist, out, R(fc.iso), R(fc.sigma)};
+    unsigned long long vis = 0;
+    run_eval<R, K, false>(g, rec, fc, src, h0, h1, lds, &vis);
+    Counters* c = ctr_stripe(ctr);
+    wave_count(&c->sample_candidates, vis);
+    wave_count(&c->candidates, vis);
+    wave_count(&c->run_candidates, vis);
     if (threadIdx.x == 0) {
-        Counters* c = ctr_stripe(ctr);
-        const unsigned long long np = (unsigned long long)(h1 - h0), cand = np * (unsigned long long)total;
-        atomicAdd(&c->sample_unique, np);
-        atomicAdd(&c->sample_candidates, cand);
-        atomicAdd(&c->candidates, cand);
-        atomicAdd(&c->cell_evals, np);
-        atomicAdd(&c->cell_candidates, cand);
+        atomicAdd(&c->sample_unique, (unsigned long long)(h1 - h0));
+        atomicAdd(&c->run_evals, (unsigned long long)(h1 - h0));
     }
 }
 
@@ -550,12 +511,13 @@ void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t
 }
 
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
-                               const int* list, const uint64_t* key, const int* val, int64_t nent, int cellmode,
-                               int* head, int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out,
-                               int64_t* ngroups_out, void* vals, void* samples, int* tmp, Counters* ctr,
-                               hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+                               const int* list, const uint64_t* key, const int* val, int64_t nent, int* head,
+                               int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
+                               int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
+                               hipEvent_t e1, hipStream_t s) {
     *nheads_out = 0;
-    *ngroups_out = 0;
+    *nruns_out = 0;
+    *kx_out = 0;
     if (nent <= 0) return;
     const unsigned nb = (unsigned)((nent + 1 + 255) / 256);
     k_sample_heads<<<nb, 256, 0, s>>>(lv, list, key, val, nent, head, rep);
@@ -565,46 +527,50 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     (void)hipMemcpyAsync(&nh, hpos + nent, 4, hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);
     *nheads_out = nh;
-    int ng = 0;
-    int* cflag = cwork;
-    int* cpos = cwork + (nent + 2);
+    // workspace: [flags | positions | group starts, then run starts]: (nent + 2) + (nent + 2) +
+    // 2 (nent + 2) ints
+    int* flag = cwork;
+    int* pos = cwork + (nent + 2);
     int* cstart = cwork + 2 * (nent + 2);
-    if (cellmode && nh > 0) {
-        const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
-        k_cell_flags<<<nbh, 256, 0, s>>>(key, hlist, nh, cflag);
-        exclusive_scan_i32(cflag, cpos, nh + 1, tmp, s);
-        k_cell_starts<<<nbh, 256, 0, s>>>(cflag, cpos, nh, cstart);
-        (void)hipMemcpyAsync(&ng, cpos + nh, 4, hipMemcpyDeviceToHost, s);
-        (void)hipStreamSynchronize(s);
-        *ngroups_out = ng;
-    }
-    const unsigned ne = (unsigned)((nh + 255) / 256);
-    (void)hipEventRecord(e0, s);
+    int* rstart = nullptr;
+    int nr = 0;
     if (nh > 0) {
+        // cell groups (points of one hash cell)
+        const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
+        k_cell_flags<<<nbh, 256, 0, s>>>(key, hlist, nh, flag);
+        exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
+        k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
+        int ng = 0;
+        (void)hipMemcpyAsync(&ng, pos + nh, 4, hipMemcpyDeviceToHost, s);
+        (void)hipStreamSynchronize(s);
+        // runs of about kRunThreads points: kx cells of a row per run (a power of two <= 64)
+        const double per_cell = (double)nh / (double)std::max(ng, 1);
+        int kx = 1;
+        while (kx < 64 && (kx * 2) * per_cell <= (double)kRunThreads) kx *= 2;
+        *kx_out = kx;
+        const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
+        k_run_flags<<<nbg, 256, 0, s>>>(g, key, hlist, cstart, ng, kx, flag);
+        exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
+        rstart = cstart + (ng + 1);  // (ng + 1) + (nruns + 1) <= 2 (nent + 1)
+        k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
+        (void)hipMemcpyAsync(&nr, pos + ng, 4, hipMemcpyDeviceToHost, s);
+        (void)hipStreamSynchronize(s);
+        *nruns_out = nr;
+    }
+    (void)hipEventRecord(e0, s);
+    if (nr > 0) {
         if (prec == 2) {
             Recs<double> r{(const double4*)ra, (const double4*)rg};
-            if (ng > 0)
-                k_cell_eval<double, false><<<ng, kCellThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, cstart,
-                                                                        (double*)vals, ctr);
-            else
-                k_sample_eval<double, false><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (double*)vals, ctr);
+            k_run_samples<double, false><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                                                                    (double*)vals, ctr);
         } else {
             Recs<float> r{(const float4*)ra, (const float4*)rg};
-            if (prec == 1) {
-                if (ng > 0)
-                    k_cell_eval<float, true><<<ng, kCellThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, cstart,
-                                                                         (float*)vals, ctr);
-                else
-                    k_sample_eval<float, true><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (float*)vals,
-                                                                  ctr);
-            } else {
-                if (ng > 0)
-                    k_cell_eval<float, false><<<ng, kCellThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, cstart,
-                                                                          (float*)vals, ctr);
-                else
-                    k_sample_eval<float, false><<<ne, 256, 0, s>>>(g, r, fc, lv, list, val, hlist, nh, (float*)vals,
-                                                                   ctr);
-            }
+            if (prec == 1)
+                k_run_samples<float, true><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                                                                      (float*)vals, ctr);
+            else
+                k_run_samples<float, false><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                                                                       (float*)vals, ctr);
         }
     }
     (void)hipEventRecord(e1, s);

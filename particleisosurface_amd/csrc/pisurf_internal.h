@@ -53,7 +53,7 @@ void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams
 constexpr int kCtrStripes = 64;
 struct alignas(128) Counters {
     unsigned long long field_evals, grad_evals, candidates, qef_solves, sample_evals, sample_candidates,
-        sample_unique, cell_evals, cell_candidates;
+        sample_unique, run_evals, run_candidates;
 };
 __device__ __forceinline__ Counters* ctr_stripe(Counters* c) { return c + (blockIdx.x & (kCtrStripes - 1)); }
 // wave-reduced add: one atomic per wave (every lane of the wave must call it)
@@ -106,17 +106,18 @@ void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* l
                         int* clist, int* tmp, hipStream_t s);
 void launch_centers(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                     const int* list, const int* count, void* samples, Counters* ctr, hipStream_t s);
-// sample deduplication: keys for the nent = 27 * listed entries (cellmode 0: Morton order of the
-// lattice position; 1: hash cell group first), then (after a 64-bit key sort) heads, one
-// evaluation per distinct point between events e0 / e1 (per point, or one workgroup per cell
-// group in cellmode), and the scatter to samples. cwork holds 3 (nent + 2) ints.
+// sample deduplication: keys for the nent = 27 * listed entries (cellmode 1: hash cell group,
+// then the lattice position modulo 8; 0: Morton order of the lattice position), then (after a
+// 64-bit key sort) heads, the distinct points grouped into runs of cells, one workgroup per run
+// between events e0 / e1, and the scatter to samples. cwork holds 4 (nent + 2) ints; kx_out is the
+// number of cells of a row per run.
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
                                int cellmode, uint64_t* key, int* val, hipStream_t s);
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
-                               const int* list, const uint64_t* key, const int* val, int64_t nent, int cellmode,
-                               int* head, int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out,
-                               int64_t* ngroups_out, void* vals, void* samples, int* tmp, Counters* ctr,
-                               hipEvent_t e0, hipEvent_t e1, hipStream_t s);
+                               const int* list, const uint64_t* key, const int* val, int64_t nent, int* head,
+                               int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
+                               int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
+                               hipEvent_t e1, hipStream_t s);
 void launch_samples(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
                     NodeLevel lv, const int* list, const int* count, void* samples /*n*27 R*/, Counters* ctr,
                     hipStream_t s);
