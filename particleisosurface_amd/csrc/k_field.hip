@@ -1,6 +1,7 @@
 // Field evaluation at arbitrary points (Evaluator::SingleEval / SingleEvalWithGrad,
 // src/evaluator.cpp:60-125) and particle normals (CalcParticlesNormal, :363-375).
 #include "pisurf_internal.h"
+#include "run_eval.h"
 
 namespace pisurf {
 
@@ -26,20 +27,39 @@ __global__ void __launch_bounds__(256) k_eval_points(GridDev g, Recs<R> rec, Fie
     wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
-template <class R, bool K>
-__global__ void __launch_bounds__(256) k_normals(GridDev g, Recs<R> rec, FieldConst fc, const float4* __restrict__ spos,
-                                                 int64_t n, float4* __restrict__ nrm, Counters* ctr) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int vis = 0;
-    if (p < n) {
-        const float4 pp = spos[p];
-        R s, gr[3];
-        vis = field_sum<R, K, true>(g, rec, fc, pp.x, pp.y, pp.z, R(pp.x), R(pp.y), R(pp.z), s, gr);
-        const R sigma = R(fc.sigma);
-        nrm[p] = make_float4(float(sigma * gr[0]), float(sigma * gr[1]), float(sigma * gr[2]), 0.0f);
+// CalcParticlesNormal over runs: the particles of kx consecutive cells of one (y, z) row of the
+// grid (their sorted range is contiguous), candidates staged through LDS
+template <class R> struct ParticleSrc {
+    const float4* spos;
+    float4* nrm;
+    R sigma;
+    __device__ __forceinline__ void point(int h, float p[3]) const {
+        const float4 q = spos[h];
+        p[0] = q.x;
+        p[1] = q.y;
+        p[2] = q.z;
     }
-    wave_count(&ctr_stripe(ctr)->grad_evals, p < n ? 1ull : 0ull);
-    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
+    __device__ __forceinline__ void store(int h, R, const R* gr) const {
+        nrm[h] = make_float4(float(sigma * gr[0]), float(sigma * gr[1]), float(sigma * gr[2]), 0.0f);
+    }
+};
+
+template <class R, bool K>
+__global__ void __launch_bounds__(kRunThreads) k_run_normals(GridDev g, Recs<R> rec, FieldConst fc,
+                                                             const float4* __restrict__ spos, int kx,
+                                                             float4* __restrict__ nrm, Counters* ctr) {
+    __shared__ RunLds<R, K && sizeof(R) == 4> lds;
+    const int per_row = (g.n[0] + kx - 1) / kx;
+    const int row = blockIdx.x / per_row, x0 = (blockIdx.x % per_row) * kx;
+    const int base = row * g.n[0];
+    const int h0 = g.off[base + x0], h1 = g.off[base + min(x0 + kx, g.n[0])];
+    if (h0 >= h1) return;  // uniform over the block
+    ParticleSrc<R> src{spos, nrm, R(fc.sigma)};
+    unsigned long long vis = 0;
+    run_eval<R, K, true>(g, rec, fc, src, h0, h1, lds, &vis);
+    Counters* c = ctr_stripe(ctr);
+    wave_count(&c->candidates, vis);
+    if (threadIdx.x == 0) atomicAdd(&c->grad_evals, (unsigned long long)(h1 - h0));
 }
 
 void launch_eval_points(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, const float* q, int64_t n,
@@ -65,14 +85,20 @@ void launch_eval_points(int prec, GridDev g, const void* ra, const void* rg, Fie
 void launch_normals(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, const float4* spos, int64_t n,
                     float4* nrm, Counters* ctr, hipStream_t s) {
     if (n <= 0) return;
-    const unsigned nb = (unsigned)((n + 255) / 256);
+    // kx cells per run: about kRunThreads particles in bulk fluid, where particles sit on a lattice of
+    // spacing 2 r (8 per cell of 4 r)
+    const int kx = 16;
+    const int per_row = (g.n[0] + kx - 1) / kx;
+    const int64_t nb64 = (int64_t)per_row * g.n[1] * g.n[2];
+    if (nb64 > INT32_MAX) return;  // grids this large are refused when the grid is built
+    const unsigned nb = (unsigned)nb64;
     if (prec == 2) {
         Recs<double> r{(const double4*)ra, (const double4*)rg};
-        k_normals<double, false><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
+        k_run_normals<double, false><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
     } else {
         Recs<float> r{(const float4*)ra, (const float4*)rg};
-        if (prec == 1) k_normals<float, true><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
-        else k_normals<float, false><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
+        if (prec == 1) k_run_normals<float, true><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
+        else k_run_normals<float, false><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
     }
 }
 
