@@ -685,8 +685,33 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
                        L->scanned.as<int>() + L->n, c->samples.p, c->ctr(), c->stream);
         HIPCHK(c, hipEventRecord(e1, c->stream));
     }
-    launch_centers(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), c->cl_pos.as<int>() + L->n,
-                   c->samples.p, c->ctr(), c->stream);
+    // centre values of leaf-depth nodes (QEF off): only the leaf-depth level has them
+    bool leaf_level = false;
+    if (level_depth >= 0) {
+        const float half = std::ldexp(c->root_half, -level_depth);
+        leaf_level = double((2.0f * half) - c->prm.radius) < 1e-8;
+    }
+    if (level_depth < 0) {
+        launch_centers(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), c->cl_pos.as<int>() + L->n,
+                       c->samples.p, c->ctr(), c->stream);
+    } else if (leaf_level && c->prm.qef_mode == 0 && level_depth >= c->dmin) {
+        int k = 0;
+        HIPCHK(c, hipMemcpyAsync(&k, c->cl_pos.as<int>() + L->n, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (k > 0) {
+            HIPCHK(c, c->sd_key.ensure((size_t)k * 8));
+            HIPCHK(c, c->sd_key2.ensure((size_t)k * 8));
+            HIPCHK(c, c->sd_val.ensure((size_t)k * 4));
+            HIPCHK(c, c->sd_val2.ensure((size_t)k * 4));
+            HIPCHK(c, c->sd_cwork.ensure(4 * ((size_t)k + 2) * 4));
+            const int64_t need = std::max(radix_tmp_ints(k), scan_tmp_ints((int64_t)k + 1)) + 64;
+            HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, need * 4)));
+            launch_centers_runs(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), k,
+                                c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
+                                c->sd_val2.as<int>(), c->sd_cwork.as<int>(), c->tmp.as<int>(), c->samples.p, c->ctr(),
+                                c->stream);
+        }
+    }
     launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, sc_dev, c->ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
     return PISURF_OK;

@@ -225,7 +225,9 @@ __global__ void k_cell_flags(const uint64_t* __restrict__ key, const int* __rest
         cflag[h] = 0;
         return;
     }
-    cflag[h] = (h == 0 || (key[hlist[h]] >> 9) != (key[hlist[h - 1]] >> 9)) ? 1 : 0;
+    // hlist NULL: the points are the sorted entries themselves
+    const int64_t a = hlist ? hlist[h] : h, b = h > 0 ? (hlist ? hlist[h - 1] : h - 1) : 0;
+    cflag[h] = (h == 0 || (key[a] >> 9) != (key[b] >> 9)) ? 1 : 0;
 }
 // cstart[k] = first distinct point of cell group k; cstart[ngroups] = nheads
 __global__ void k_cell_starts(const int* __restrict__ cflag, const int* __restrict__ cpos, int64_t nheads,
@@ -246,7 +248,7 @@ __global__ void k_run_flags(GridDev g, const uint64_t* __restrict__ key, const i
         return;
     }
     auto run_id = [&](int64_t grp) {
-        const uint64_t gid = key[hlist[cstart[grp]]] >> 9;
+        const uint64_t gid = key[hlist ? hlist[cstart[grp]] : cstart[grp]] >> 9;
         const uint64_t ex = g.n[0] + 2, ey = g.n[1] + 2;
         const int ix = (int)(gid % ex) - 1, iy = (int)((gid / ex) % ey) - 1, iz = (int)(gid / (ex * ey)) - 1;
         return run_of_cell(g, ix, iy, iz, kx);  // the empty-window group maps past every row
@@ -259,6 +261,84 @@ __global__ void k_run_starts(const int* __restrict__ rflag, const int* __restric
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k > ngroups) return;
     if (k == ngroups || rflag[k]) rstart[rpos[k]] = cstart[k];
+}
+
+// leaf-depth centres keyed by cell group (the centres are distinct, one per node)
+__global__ void k_center_keys(GridDev g, NodeLevel lv, const int* __restrict__ list, int64_t k,
+                              uint64_t* __restrict__ key, int* __restrict__ val) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= k) return;
+    const float4 c = lv.c[list[e]];
+    key[e] = cell_group(g, c.x, c.y, c.z) << 9;
+    val[e] = (int)e;
+}
+
+template <class R> struct CenterSrc {
+    NodeLevel lv;
+    const int* list;
+    const int* val;
+    R* samples;
+    R iso, sigma;
+    __device__ __forceinline__ void point(int h, float p[3]) const {
+        const float4 c = lv.c[list[val[h]]];
+        p[0] = c.x;
+        p[1] = c.y;
+        p[2] = c.z;
+    }
+    __device__ __forceinline__ void store(int h, R sum, const R*) const {
+        samples[(int64_t)list[val[h]] * 27 + 13] = iso - sigma * sum;
+    }
+};
+
+template <class R, bool K>
+__global__ void __launch_bounds__(kRunThreads) k_run_centers(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
+                                                             const int* __restrict__ list, const int* __restrict__ val,
+                                                             const int* __restrict__ rstart, R* __restrict__ samples,
+                                                             Counters* ctr) {
+    __shared__ RunLds<R, K && sizeof(R) == 4> lds;
+    const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
+    CenterSrc<R> src{lv, list, val, samples, R(fc.iso), R(fc.sigma)};
+    unsigned long long vis = 0;
+    run_eval<R, K, false>(g, rec, fc, src, h0, h1, lds, &vis);
+    Counters* c = ctr_stripe(ctr);
+    wave_count(&c->candidates, vis);
+    if (threadIdx.x == 0) atomicAdd(&c->field_evals, (unsigned long long)(h1 - h0));
+}
+
+// Cell groups and runs of nh points in cell-major order (key[hlist[h]] >> 9 is the cell group of
+// point h; hlist NULL: point h is entry h). cwork: flags | positions | group starts, run starts
+// ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of runs; *rstart_out points into
+// cwork. Synchronises the stream twice (group and run counts).
+static int build_runs(GridDev g, const uint64_t* key, const int* hlist, int nh, int* cwork, int* tmp,
+                      int** rstart_out, int* kx_out, hipStream_t s) {
+    int* flag = cwork;
+    int* pos = cwork + (nh + 2);
+    int* cstart = cwork + 2 * (int64_t)(nh + 2);
+    *rstart_out = cstart;
+    *kx_out = 0;
+    if (nh <= 0) return 0;
+    const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
+    k_cell_flags<<<nbh, 256, 0, s>>>(key, hlist, nh, flag);
+    exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
+    k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
+    int ng = 0;
+    (void)hipMemcpyAsync(&ng, pos + nh, 4, hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    // runs of about kRunThreads points: kx cells of a row per run (a power of two <= 64)
+    const double per_cell = (double)nh / (double)std::max(ng, 1);
+    int kx = 1;
+    while (kx < 64 && (kx * 2) * per_cell <= (double)kRunThreads) kx *= 2;
+    *kx_out = kx;
+    const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
+    k_run_flags<<<nbg, 256, 0, s>>>(g, key, hlist, cstart, ng, kx, flag);
+    exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
+    int* rstart = cstart + (ng + 1);  // (ng + 1) + (nruns + 1) <= 2 (nh + 1)
+    k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
+    int nr = 0;
+    (void)hipMemcpyAsync(&nr, pos + ng, 4, hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    *rstart_out = rstart;
+    return nr;
 }
 
 // distinct sample points of a level (cell-major), evaluated run by run
@@ -527,36 +607,11 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     (void)hipMemcpyAsync(&nh, hpos + nent, 4, hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);
     *nheads_out = nh;
-    // workspace: [flags | positions | group starts, then run starts]: (nent + 2) + (nent + 2) +
-    // 2 (nent + 2) ints
-    int* flag = cwork;
-    int* pos = cwork + (nent + 2);
-    int* cstart = cwork + 2 * (nent + 2);
     int* rstart = nullptr;
-    int nr = 0;
-    if (nh > 0) {
-        // cell groups (points of one hash cell)
-        const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
-        k_cell_flags<<<nbh, 256, 0, s>>>(key, hlist, nh, flag);
-        exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
-        k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
-        int ng = 0;
-        (void)hipMemcpyAsync(&ng, pos + nh, 4, hipMemcpyDeviceToHost, s);
-        (void)hipStreamSynchronize(s);
-        // runs of about kRunThreads points: kx cells of a row per run (a power of two <= 64)
-        const double per_cell = (double)nh / (double)std::max(ng, 1);
-        int kx = 1;
-        while (kx < 64 && (kx * 2) * per_cell <= (double)kRunThreads) kx *= 2;
-        *kx_out = kx;
-        const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
-        k_run_flags<<<nbg, 256, 0, s>>>(g, key, hlist, cstart, ng, kx, flag);
-        exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
-        rstart = cstart + (ng + 1);  // (ng + 1) + (nruns + 1) <= 2 (nent + 1)
-        k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
-        (void)hipMemcpyAsync(&nr, pos + ng, 4, hipMemcpyDeviceToHost, s);
-        (void)hipStreamSynchronize(s);
-        *nruns_out = nr;
-    }
+    int kx = 0;
+    const int nr = build_runs(g, key, hlist, nh, cwork, tmp, &rstart, &kx, s);
+    *nruns_out = nr;
+    *kx_out = kx;
     (void)hipEventRecord(e0, s);
     if (nr > 0) {
         if (prec == 2) {
@@ -589,6 +644,34 @@ void launch_centers(int prec, GridDev g, const void* ra, const void* rg, FieldCo
         Recs<float> r{(const float4*)ra, (const float4*)rg};
         if (prec == 1) k_centers<float, true><<<nb, 256, 0, s>>>(g, r, fc, lv, list, count, (float*)samples, ctr);
         else k_centers<float, false><<<nb, 256, 0, s>>>(g, r, fc, lv, list, count, (float*)samples, ctr);
+    }
+}
+
+void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
+                         const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
+                         int* tmp, void* samples, Counters* ctr, hipStream_t s) {
+    if (k <= 0) return;
+    const uint64_t groups = (uint64_t)(g.n[0] + 2) * (uint64_t)(g.n[1] + 2) * (uint64_t)(g.n[2] + 2);
+    int bits = 1;
+    while (bits < 55 && ((uint64_t)1 << bits) <= groups) bits++;
+    k_center_keys<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(g, lv, list, k, key, val);
+    radix_sort_pairs64(key, val, key2, val2, k, bits + 9, tmp, s);
+    int* rstart = nullptr;
+    int kx = 0;
+    const int nr = build_runs(g, key, nullptr, (int)k, cwork, tmp, &rstart, &kx, s);
+    if (nr <= 0) return;
+    if (prec == 2) {
+        Recs<double> r{(const double4*)ra, (const double4*)rg};
+        k_run_centers<double, false><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, rstart, (double*)samples,
+                                                                ctr);
+    } else {
+        Recs<float> r{(const float4*)ra, (const float4*)rg};
+        if (prec == 1)
+            k_run_centers<float, true><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, rstart, (float*)samples,
+                                                                  ctr);
+        else
+            k_run_centers<float, false><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, rstart, (float*)samples,
+                                                                   ctr);
     }
 }
 

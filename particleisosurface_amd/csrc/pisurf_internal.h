@@ -106,6 +106,11 @@ void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* l
                         int* clist, int* tmp, hipStream_t s);
 void launch_centers(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                     const int* list, const int* count, void* samples, Counters* ctr, hipStream_t s);
+// the same for k listed centres (count known on the host), grouped by hash cell and evaluated in
+// runs of cells; key / val / key2 / val2 hold k entries, cwork 4 (k + 2) ints
+void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
+                         const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
+                         int* tmp, void* samples, Counters* ctr, hipStream_t s);
 // sample deduplication: keys for the nent = 27 * listed entries (cellmode 1: hash cell group,
 // then the lattice position modulo 8; 0: Morton order of the lattice position), then (after a
 // 64-bit key sort) heads, the distinct points grouped into runs of cells, one workgroup per run
