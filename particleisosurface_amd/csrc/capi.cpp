@@ -519,10 +519,10 @@ int do_aniso(pisurf_ctx* c) {
     ap.xmean_delta = 0.9f;
     ap.use_poly6 = c->prm.use_poly6;
     ap.use_ani = c->prm.use_ani;
-    launch_xmean(prec, c->g, c->spos.as<float4>(), c->n, ap, c->xm.p, c->stream);
+    launch_xmean(prec, c->g, c->spos.as<float4>(), c->n, c->ginfo.n_invalid, ap, c->xm.p, c->stream);
     unsigned long long* nfp = (unsigned long long*)(c->small.as<char>() + 128);
-    launch_gmat(prec, c->g, c->spos.as<float4>(), c->n, ap, c->xm.p, c->rec_a.p, c->rec_g.p, c->rec_lo.p, nfp,
-                c->stream);
+    launch_gmat(prec, c->g, c->spos.as<float4>(), c->n, c->ginfo.n_invalid, ap, c->xm.p, c->rec_a.p, c->rec_g.p,
+                c->rec_lo.p, nfp, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&c->n_fp64_eig, nfp, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -545,7 +545,7 @@ int do_normals(pisurf_ctx* c) {
     const double t0 = now_ms();
     HIPCHK(c, c->nrm.ensure(c->n * sizeof(float4)));
     launch_normals(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->spos.as<float4>(), c->n,
-                   c->nrm.as<float4>(), c->ctr(), c->stream);
+                   c->ginfo.n_invalid, c->nrm.as<float4>(), c->ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->normals_done = true;

@@ -140,11 +140,11 @@ __device__ __forceinline__ void field_contrib(const FieldConst& fc, R inv_h2, R 
         const R n2 = gx * gx + gy * gy + gz * gz;
         if (fc.use_poly6) {
             const R t = h2 - n2;
-            if (n2 <= h2) acc.add(g1.z * (t * t * t));
+            if (!(n2 > h2)) acc.add(g1.z * (t * t * t));
         } else {
             acc.add(g1.z * bspline_q2<R>(n2 * inv_h2));
         }
-        if (GRAD && n2 <= h2) {
+        if (GRAD && !(n2 > h2)) {
             const R t = h2 - n2;
             const R w = R(-6) * g1.z * t * t;
             ga[0].add(w * gx);
@@ -152,7 +152,7 @@ __device__ __forceinline__ void field_contrib(const FieldConst& fc, R inv_h2, R 
             ga[2].add(w * gz);
         }
     } else {
-        if (d2 <= h2) {
+        if (!(d2 > h2)) {
             const R t = h2 - d2;
             acc.add(t * t * t);
             if (GRAD) {
@@ -199,7 +199,9 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
     for (int r = 0; r < 9; r++) {
         int j = rlo[r];
         const int e = rhi[r];
-        // four candidates per step: their records are requested before any is used
+        // four candidates per step: their records are requested before any is used. The culls
+        // are written !(d2 >= R^2) so that a NaN record is visited and poisons the sum, as the
+        // reference's kernels do for it.
         for (; j + 4 <= e; j += 4) {
             const auto a0 = rec.a[j], a1 = rec.a[j + 1], a2 = rec.a[j + 2], a3 = rec.a[j + 3];
             const R x0 = px - a0.x, y0 = py - a0.y, z0 = pz - a0.z;
@@ -208,16 +210,16 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
             const R x3 = px - a3.x, y3 = py - a3.y, z3 = pz - a3.z;
             const R q0 = dist2(x0, y0, z0), q1 = dist2(x1, y1, z1);
             const R q2 = dist2(x2, y2, z2), q3 = dist2(x3, y3, z3);
-            if (q0 < a0.w) contrib(j, x0, y0, z0, q0);
-            if (q1 < a1.w) contrib(j + 1, x1, y1, z1, q1);
-            if (q2 < a2.w) contrib(j + 2, x2, y2, z2, q2);
-            if (q3 < a3.w) contrib(j + 3, x3, y3, z3, q3);
+            if (!(q0 >= a0.w)) contrib(j, x0, y0, z0, q0);
+            if (!(q1 >= a1.w)) contrib(j + 1, x1, y1, z1, q1);
+            if (!(q2 >= a2.w)) contrib(j + 2, x2, y2, z2, q2);
+            if (!(q3 >= a3.w)) contrib(j + 3, x3, y3, z3, q3);
         }
         for (; j < e; j++) {
             const auto a = rec.a[j];
             const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
             const R d2 = dist2(dx, dy, dz);
-            if (d2 < a.w) contrib(j, dx, dy, dz, d2);
+            if (!(d2 >= a.w)) contrib(j, dx, dy, dz, d2);
         }
     }
     sum = acc.get();

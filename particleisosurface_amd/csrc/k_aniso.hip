@@ -161,14 +161,16 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
     }
     // squared world radius outside which |G d|^2 >= (support)^2: 4 h^2 (B-spline) or h^2 (poly6)
     const R supp = ap.use_poly6 ? h2 : R(4) * h2;
-    const R R2 = supp / (gmin * gmin) * R(1.001);
+    R R2 = supp / (gmin * gmin) * R(1.001);
+    if (R2 != R2) R2 = R(INFINITY);  // 0 / 0 covariance: never culled, its NaN reaches every sum
     rec_a[p] = mk4(xi.x, xi.y, xi.z, R2);
     rec_g[2 * p] = mk4(G[0], G[1], G[2], G[3]);
     rec_g[2 * p + 1] = mk4(G[4], G[5], det, R(0));
     if (REL) rec_lo[p] = li;
 }
 
-void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, void* xm, hipStream_t s) {
+void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, int64_t, AnisoParams ap, void* xm,
+                  hipStream_t s) {
     if (n <= 0) return;
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (prec == 2) k_xmean<double, false><<<nb, 256, 0, s>>>(g, spos, n, ap, (double4*)xm);
@@ -176,8 +178,8 @@ void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, AnisoParam
     else k_xmean<float, false><<<nb, 256, 0, s>>>(g, spos, n, ap, (float4*)xm);
 }
 
-void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, const void* xm, void* rec_a,
-                 void* rec_g, void* rec_lo, unsigned long long* n_fp64, hipStream_t s) {
+void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, int64_t, AnisoParams ap, const void* xm,
+                 void* rec_a, void* rec_g, void* rec_lo, unsigned long long* n_fp64, hipStream_t s) {
     if (n <= 0) return;
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (prec == 2)

@@ -27,6 +27,23 @@ __global__ void __launch_bounds__(256) k_eval_points(GridDev g, Recs<R> rec, Fie
     wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
+// per-particle form for grids that leave particles outside (shifted cell windows, no runs)
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_normals_pt(GridDev g, Recs<R> rec, FieldConst fc, const float4* __restrict__ spos,
+                                                 int64_t n, float4* __restrict__ nrm, Counters* ctr) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vis = 0;
+    if (p < n) {
+        const float4 pp = spos[p];
+        R s, gr[3];
+        vis = field_sum<R, K, true>(g, rec, fc, pp.x, pp.y, pp.z, R(pp.x), R(pp.y), R(pp.z), s, gr);
+        const R sigma = R(fc.sigma);
+        nrm[p] = make_float4(float(sigma * gr[0]), float(sigma * gr[1]), float(sigma * gr[2]), 0.0f);
+    }
+    wave_count(&ctr_stripe(ctr)->grad_evals, p < n ? 1ull : 0ull);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
+}
+
 // CalcParticlesNormal over runs: the particles of kx consecutive cells of one (y, z) row of the
 // grid (their sorted range is contiguous), candidates staged through LDS
 template <class R> struct ParticleSrc {
@@ -49,14 +66,11 @@ __global__ void __launch_bounds__(kRunThreads) k_run_normals(GridDev g, Recs<R> 
                                                              const float4* __restrict__ spos, int kx,
                                                              float4* __restrict__ nrm, Counters* ctr) {
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
-    const int per_row = (g.n[0] + kx - 1) / kx;
-    const int row = blockIdx.x / per_row, x0 = (blockIdx.x % per_row) * kx;
-    const int base = row * g.n[0];
-    const int h0 = g.off[base + x0], h1 = g.off[base + min(x0 + kx, g.n[0])];
+    int h0, h1;
+    particle_run(g, kx, h0, h1);
     if (h0 >= h1) return;  // uniform over the block
     ParticleSrc<R> src{spos, nrm, R(fc.sigma)};
-    unsigned long long vis = 0;
-    run_eval<R, K, true>(g, rec, fc, src, h0, h1, lds, &vis);
+    const unsigned long long vis = run_eval<R, K, true>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
     if (threadIdx.x == 0) atomicAdd(&c->grad_evals, (unsigned long long)(h1 - h0));
@@ -83,8 +97,20 @@ void launch_eval_points(int prec, GridDev g, const void* ra, const void* rg, Fie
 }
 
 void launch_normals(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, const float4* spos, int64_t n,
-                    float4* nrm, Counters* ctr, hipStream_t s) {
+                    int64_t n_invalid, float4* nrm, Counters* ctr, hipStream_t s) {
     if (n <= 0) return;
+    if (n_invalid > 0) {
+        const unsigned nb = (unsigned)((n + 255) / 256);
+        if (prec == 2) {
+            Recs<double> r{(const double4*)ra, (const double4*)rg};
+            k_normals_pt<double, false><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
+        } else {
+            Recs<float> r{(const float4*)ra, (const float4*)rg};
+            if (prec == 1) k_normals_pt<float, true><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
+            else k_normals_pt<float, false><<<nb, 256, 0, s>>>(g, r, fc, spos, n, nrm, ctr);
+        }
+        return;
+    }
     // kx cells per run: about kRunThreads particles in bulk fluid, where particles sit on a lattice of
     // spacing 2 r (8 per cell of 4 r)
     const int kx = 16;

@@ -298,8 +298,7 @@ __global__ void __launch_bounds__(kRunThreads) k_run_centers(GridDev g, Recs<R> 
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
     CenterSrc<R> src{lv, list, val, samples, R(fc.iso), R(fc.sigma)};
-    unsigned long long vis = 0;
-    run_eval<R, K, false>(g, rec, fc, src, h0, h1, lds, &vis);
+    const unsigned long long vis = run_eval<R, K, false>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
     if (threadIdx.x == 0) atomicAdd(&c->field_evals, (unsigned long long)(h1 - h0));
@@ -362,8 +361,7 @@ __global__ void __launch_bounds__(kRunThreads) k_run_samples(GridDev g, Recs<R> 
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
     SampleSrc<R> src{lv, list, val, hlist, out, R(fc.iso), R(fc.sigma)};
-    unsigned long long vis = 0;
-    run_eval<R, K, false>(g, rec, fc, src, h0, h1, lds, &vis);
+    const unsigned long long vis = run_eval<R, K, false>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->sample_candidates, vis);
     wave_count(&c->candidates, vis);

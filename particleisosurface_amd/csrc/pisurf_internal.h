@@ -41,10 +41,11 @@ struct AnisoParams {
     int use_poly6;
     int use_ani;
 };
-void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, void* xm /*R x4*/,
-                  hipStream_t s);
-void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, AnisoParams ap, const void* xm, void* rec_a,
-                 void* rec_g, void* rec_lo, unsigned long long* n_fp64_eig, hipStream_t s);
+// one thread per sorted particle (n_invalid: particles outside the grid; these kernels handle any)
+void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, int64_t n_invalid, AnisoParams ap,
+                  void* xm /*R x4*/, hipStream_t s);
+void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, int64_t n_invalid, AnisoParams ap, const void* xm,
+                 void* rec_a, void* rec_g, void* rec_lo, unsigned long long* n_fp64_eig, hipStream_t s);
 
 // ---- field ----
 // Event counters, spread over kCtrStripes copies one cache line apart: a block adds to the copy of
@@ -65,7 +66,7 @@ __device__ __forceinline__ void wave_count(unsigned long long* ctr, unsigned lon
 void launch_eval_points(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float* q,
                         int64_t n, double* f, double* grad, Counters* ctr, hipStream_t s);
 void launch_normals(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float4* spos,
-                    int64_t n, float4* nrm, Counters* ctr, hipStream_t s);
+                    int64_t n, int64_t n_invalid, float4* nrm, Counters* ctr, hipStream_t s);
 // dense sampling box: lo / hi corners, steps (intervals) and dims = steps + 1 points per axis
 struct UniformBox {
     float lo[3], hi[3];

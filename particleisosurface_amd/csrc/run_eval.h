@@ -1,13 +1,13 @@
-// Field sums for runs of query points that share a hash-cell row, with the candidates staged
+// Neighbour sums for runs of query points that share a hash-cell row, with the candidates staged
 // through LDS.
 //
 // A run is a set of consecutive points (cell-major order) whose cells lie in one (y, z) row of
 // the hash grid, with x in [xa, xb]. For each of the nine neighbour rows (z-outer, then y, as in
 // GetPIdxList) the union of the run's windows, cells [xa - 1, xb + 1], is one contiguous range of
-// sorted particles. It is staged through LDS in chunks. Each thread then walks its own point's
-// sub-range: cells [x - 1, x + 1], particles in sorted order. Every point therefore sees exactly
-// the candidate sequence of field_sum, with the same per-candidate arithmetic (dist2,
-// field_contrib), and its sum is bit-identical to the per-point kernels.
+// sorted particles; the nine ranges are concatenated and staged through LDS in chunks. Each thread
+// then walks its own point's sub-ranges: cells [x - 1, x + 1] of every row, particles in sorted
+// order. Every point therefore sees exactly the candidate sequence of the per-point kernels, and a
+// policy that does the per-point arithmetic gives bit-identical sums.
 #pragma once
 #include "pisurf_internal.h"
 
@@ -16,28 +16,28 @@ namespace pisurf {
 constexpr int kRunThreads = 128;
 constexpr int kRunChunk = 256;
 
-template <class R, bool LO> struct RunLds {
-    typename Vec4<R>::T a[kRunChunk], g0[kRunChunk], g1[kRunChunk];
-    float4 lo[LO ? kRunChunk : 1];
-};
-
-// Src: point(h, float p[3]) gives point h; store(h, sum, grad) receives its sums (without sigma).
-// *visited receives the candidates visited by this thread's points.
-template <class R, bool K, bool GRAD, class Src>
-__device__ __forceinline__ void run_eval(const GridDev& g, const Recs<R>& rec, const FieldConst& fc, const Src& src,
-                                         int h0, int h1, RunLds<R, K && sizeof(R) == 4>& lds,
-                                         unsigned long long* visited) {
-    typedef typename Vec4<R>::T V4;
-    constexpr bool LO = K && sizeof(R) == 4;
+// Policy interface of run_visit:
+//   typename Pol::Lds     staging buffers for kRunChunk candidates
+//   typename Pol::State   per-point accumulators
+//   void point(int h, float p[3]) const                 query point h
+//   void stage(Lds&, int t, int j) const                candidate j -> slot t
+//   void begin(State&, int h, const float p[3]) const   (inactive threads too, with h = -1)
+//   void visit(State&, const Lds&, int t, int j) const  candidate in slot t (global index j)
+//   void end(State&, int h, int ncand) const            active points: ncand = candidates in its
+//                                                       27-cell window
+// Returns the candidates visited by this thread's points.
+template <class Pol>
+__device__ __forceinline__ unsigned long long run_visit(const GridDev& g, const Pol& pol, int h0, int h1,
+                                                        typename Pol::Lds& lds) {
     float pa[3], pb[3];
-    src.point(h0, pa);
-    src.point(h1 - 1, pb);
+    pol.point(h0, pa);
+    pol.point(h1 - 1, pb);
     int xa, ya, za, xb, yb, zb;
     cell_of(g, pa[0], pa[1], pa[2], xa, ya, za);
     cell_of(g, pb[0], pb[1], pb[2], xb, yb, zb);
     if (xb < xa) xb = xa;  // only in the group of points with empty windows
-    // the run's nine union row windows, concatenated in row order: row r is [w0[r], w0[r] + len)
-    // at concatenated offset roff[r]
+    // the run's nine union row windows, concatenated in row order: row r starts at sorted position
+    // w0[r] and at concatenated offset roff[r]
     int w0[9], roff[10];
     roff[0] = 0;
 #pragma unroll
@@ -57,21 +57,24 @@ __device__ __forceinline__ void run_eval(const GridDev& g, const Recs<R>& rec, c
     // a run of one cell (not the empty-window group): every point's windows are the run's rows
     const bool one_cell = xa == xb && xa >= -1 && xa <= g.n[0] && ya >= -1 && ya <= g.n[1] && za >= -1 &&
                           za <= g.n[2];
-    const R inv_h2 = R(fc.inv_h2), h2 = R(fc.h2);
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     unsigned long long vis = 0;
     for (int hb = h0; hb < h1; hb += kRunThreads) {
         const int h = hb + (int)threadIdx.x;
         const bool act = h < h1;
         float p[3] = {0.f, 0.f, 0.f};
         // this point's windows as concatenated ranges (sub-ranges of the run's rows)
-        int clo[9], chi[9];
+        int clo[9], chi[9], ncand = 0;
 #pragma unroll
         for (int r = 0; r < 9; r++) clo[r] = chi[r] = 0;
         if (act) {
-            src.point(h, p);
+            pol.point(h, p);
             if (one_cell) {
-                vis += (unsigned long long)total;
+#pragma unroll
+                for (int r = 0; r < 9; r++) {
+                    clo[r] = roff[r];
+                    chi[r] = roff[r + 1];
+                }
+                ncand = total;
             } else {
                 int ix, iy, iz;
                 cell_of(g, p[0], p[1], p[2], ix, iy, iz);
@@ -81,13 +84,14 @@ __device__ __forceinline__ void run_eval(const GridDev& g, const Recs<R>& rec, c
                     if (row_range(g, ix, iy, iz, r, lo, hi)) {
                         clo[r] = roff[r] + (lo - w0[r]);
                         chi[r] = roff[r] + (hi - w0[r]);
-                        vis += (unsigned long long)(hi - lo);
+                        ncand += hi - lo;
                     }
                 }
             }
         }
-        const R px = R(p[0]), py = R(p[1]), pz = R(p[2]);
-        Acc<R, K> acc, ga[3];
+        vis += (unsigned long long)ncand;
+        typename Pol::State st;
+        pol.begin(st, act ? h : -1, p);
         for (int cb = 0; cb < total; cb += kRunChunk) {
             const int n = min(kRunChunk, total - cb);
             if (hb > h0 || cb > 0) __syncthreads();  // the previous chunk has been consumed
@@ -98,36 +102,73 @@ __device__ __forceinline__ void run_eval(const GridDev& g, const Recs<R>& rec, c
 #pragma unroll
                     for (int r = 0; r < 9; r++)  // the row holding concatenated position c
                         if (c >= roff[r] && c < roff[r + 1]) j = w0[r] + (c - roff[r]);
-                    lds.a[t] = rec.a[j];
-                    lds.g0[t] = rec.g[2 * j];
-                    lds.g1[t] = rec.g[2 * j + 1];
-                    if (LO) lds.lo[t] = fc.lo[j];
+                    pol.stage(lds, t, j);
                 }
             }
             __syncthreads();
-            auto visit = [&](int t0, int t1) {
-                for (int t = t0; t < t1; t++) {
-                    const V4 a = lds.a[t];
-                    const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-                    const R d2 = dist2(dx, dy, dz);
-                    if (d2 < a.w)
-                        field_contrib<R, K, GRAD>(fc, inv_h2, h2, dx, dy, dz, d2, lds.g0[t], lds.g1[t],
-                                                  LO ? lds.lo[LO ? t : 0] : zero4, acc, ga);
-                }
-            };
-            if (one_cell) {
-                if (act) visit(0, n);  // the windows are the whole concatenated sequence
-            } else {
 #pragma unroll
-                for (int r = 0; r < 9; r++) visit(max(clo[r], cb) - cb, min(chi[r], cb + n) - cb);
+            for (int r = 0; r < 9; r++) {
+                const int t0 = max(clo[r], cb) - cb, t1 = min(chi[r], cb + n) - cb;
+                const int jr = w0[r] - roff[r] + cb;  // global index of slot t in row r: jr + t
+                for (int t = t0; t < t1; t++) pol.visit(st, lds, t, jr + t);
             }
         }
-        if (act) {
-            R gr[3] = {ga[0].get(), ga[1].get(), ga[2].get()};
-            src.store(h, acc.get(), gr);
+        if (act) pol.end(st, h, ncand);
+    }
+    return vis;
+}
+
+// ---- field sums (SingleEval / SingleEvalWithGrad): records of device_common.h ----
+template <class R, bool LO> struct RunLds {
+    typename Vec4<R>::T a[kRunChunk], g0[kRunChunk], g1[kRunChunk];
+    float4 lo[LO ? kRunChunk : 1];
+};
+
+// Src: point(h, float p[3]) gives point h; store(h, sum, grad) receives its sums (without sigma)
+template <class R, bool K, bool GRAD, class Src> struct FieldPol {
+    static constexpr bool LO = K && sizeof(R) == 4;
+    typedef RunLds<R, LO> Lds;
+    struct State {
+        R px, py, pz;
+        Acc<R, K> acc, ga[3];
+    };
+    Recs<R> rec;
+    FieldConst fc;
+    R inv_h2, h2;
+    Src src;
+    __device__ __forceinline__ void point(int h, float p[3]) const { src.point(h, p); }
+    __device__ __forceinline__ void stage(Lds& l, int t, int j) const {
+        l.a[t] = rec.a[j];
+        l.g0[t] = rec.g[2 * j];
+        l.g1[t] = rec.g[2 * j + 1];
+        if (LO) l.lo[t] = fc.lo[j];
+    }
+    __device__ __forceinline__ void begin(State& s, int, const float p[3]) const {
+        s.px = R(p[0]);
+        s.py = R(p[1]);
+        s.pz = R(p[2]);
+    }
+    __device__ __forceinline__ void visit(State& s, const Lds& l, int t, int) const {
+        const typename Vec4<R>::T a = l.a[t];
+        const R dx = s.px - a.x, dy = s.py - a.y, dz = s.pz - a.z;
+        const R d2 = dist2(dx, dy, dz);
+        if (!(d2 >= a.w)) {
+            const float4 lo = LO ? l.lo[LO ? t : 0] : make_float4(0.f, 0.f, 0.f, 0.f);
+            field_contrib<R, K, GRAD>(fc, inv_h2, h2, dx, dy, dz, d2, l.g0[t], l.g1[t], lo, s.acc, s.ga);
         }
     }
-    *visited = vis;
+    __device__ __forceinline__ void end(State& s, int h, int) const {
+        R gr[3] = {s.ga[0].get(), s.ga[1].get(), s.ga[2].get()};
+        src.store(h, s.acc.get(), gr);
+    }
+};
+
+template <class R, bool K, bool GRAD, class Src>
+__device__ __forceinline__ unsigned long long run_eval(const GridDev& g, const Recs<R>& rec, const FieldConst& fc,
+                                                       const Src& src, int h0, int h1,
+                                                       RunLds<R, K && sizeof(R) == 4>& lds) {
+    const FieldPol<R, K, GRAD, Src> pol{rec, fc, R(fc.inv_h2), R(fc.h2), src};
+    return run_visit(g, pol, h0, h1, lds);
 }
 
 // run id of a hash cell (extended by one cell on every side): its (y, z) row and x / kx
@@ -136,6 +177,16 @@ __device__ __forceinline__ uint64_t run_of_cell(const GridDev& g, int ix, int iy
     const uint64_t per_row = (ex + kx - 1) / kx;
     if (ix < -1 || ix > g.n[0] || iy < -1 || iy > g.n[1] || iz < -1 || iz > g.n[2]) return ey * ez * per_row;
     return ((uint64_t)(iz + 1) * ey + (uint64_t)(iy + 1)) * per_row + (uint64_t)(ix + 1) / kx;
+}
+
+// particle runs: block b covers the particles of cells [x0, x0 + kx) of one grid row (contiguous
+// in sorted order)
+__device__ __forceinline__ void particle_run(const GridDev& g, int kx, int& h0, int& h1) {
+    const int per_row = (g.n[0] + kx - 1) / kx;
+    const int row = blockIdx.x / per_row, x0 = (blockIdx.x % per_row) * kx;
+    const int base = row * g.n[0];
+    h0 = g.off[base + x0];
+    h1 = g.off[base + min(x0 + kx, g.n[0])];
 }
 
 }  // namespace pisurf

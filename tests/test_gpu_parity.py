@@ -611,3 +611,41 @@ def test_frame_driver_directory(tmp_path):
     assert np.array_equal(faces, t) and verts.shape == v.shape
     assert np.max(np.abs(verts - v)) <= 5e-7 + 1e-6 * np.abs(v).max()
     assert (tmp_path / "out" / "frame_2.obj").stat().st_size > 0
+
+
+@pytest.mark.parametrize("prec", ["fp32k", "fp64"])
+@pytest.mark.parametrize("cut,n_out", [(2, 1), (4, 30)])
+def test_anisotropy_and_normals_with_clipping_box(prec, cut, n_out):
+    """A grid built on a box that leaves particles outside (their hash is -1): the shifted cell
+    windows of the reference apply, and anisotropy / normals take the per-particle kernels.
+    With one particle outside every window shifts by one; with 30 most covariances become 0 / 0."""
+    xyz, r = synth.dam_break()
+    lo, hi = xyz.min(0), xyz.max(0)
+    # clips the top of the droplet by cut radii
+    bbox = np.array([lo[0], hi[0], lo[1], hi[1] - cut * r, lo[2], hi[2]], np.float32)
+    o, ctx = _pair(xyz, r, prec)
+    go = o.hashgrid_build(bbox)
+    g = ctx.hashgrid_build(bbox)
+    assert g["n_invalid"] == go["n_invalid"] == n_out
+    o.anisotropy()
+    ctx.anisotropy()
+    xm, G, d = ctx.aniso()
+    xm2, G2, d2 = o.aniso()
+    tol = {"fp32k": (1e-5 * r, 2e-5, 2e-5), "fp64": (1e-12, 1e-9, 1e-9)}[prec]
+
+    def close(a, b, t):
+        # the shifted windows leave some particles without neighbours: their covariance is 0 / 0,
+        # NaN in the reference as well
+        nan = np.isnan(b)
+        assert np.array_equal(np.isnan(a), nan)
+        assert np.max(np.abs(a[~nan] - b[~nan]), initial=0.0) <= t
+
+    close(xm, xm2, tol[0])
+    close(G, G2, tol[1])
+    close(d, d2, tol[2])
+    assert np.isfinite(G2).any()
+    o.iso_constants()
+    n = ctx.particle_normals()
+    n2 = o.particle_normals()
+    # normals are stored as float (Vector3f in the reference): FP64 sums are held to that rounding
+    close(n, n2, {"fp32k": 1e-6, "fp64": 1e-7}[prec] * np.max(np.abs(n2[np.isfinite(n2)]), initial=1.0))
