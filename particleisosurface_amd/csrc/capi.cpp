@@ -241,6 +241,7 @@ struct pisurf_ctx {
     // mesh download stream and the events around the bisection
     hipStream_t dstream = nullptr;
     hipEvent_t eb0 = nullptr, eb1 = nullptr;
+    std::vector<hipEvent_t> ev_chunk;  // ends of the bisection chunks
     // device extraction workspace
     DBuf eb[28];
     // mesh
@@ -262,6 +263,8 @@ struct pisurf_ctx {
         if (dstream) (void)hipStreamDestroy(dstream);
         if (eb0) (void)hipEventDestroy(eb0);
         if (eb1) (void)hipEventDestroy(eb1);
+        for (auto e : ev_chunk)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
     // start/stop events around one evaluation launch; kind 0: per-point sampling kernel, 1: run kernel
@@ -1055,21 +1058,10 @@ int do_extract_device(pisurf_ctx* c) {
     launch_vertex_ids_hashed(vslot, E[16].as<unsigned>(), E[19].as<int>(), nc, E[17].as<int>(), leaf_node, cvid,
                              E[21].as<double>(), s);
     HIPCHK(c, hipGetLastError());
-    c->st.ms_extract_host += now_ms() - t0;  // device extraction time (topology)
     HIPCHK(c, E[22].ensure(std::max<int64_t>(nv, 1) * 3 * 8));
     if (!c->eb0) HIPCHK(c, hipEventCreate(&c->eb0));
     if (!c->eb1) HIPCHK(c, hipEventCreate(&c->eb1));
     if (!c->dstream) HIPCHK(c, hipStreamCreateWithFlags(&c->dstream, hipStreamNonBlocking));
-    HIPCHK(c, hipEventRecord(c->eb0, s));
-    launch_bisect(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->prm.radius / 2, E[21].as<double>(), nv,
-                  E[22].as<double>(), nullptr, c->ctr(), s);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->eb1, s));
-    // the vertices go to the host on a second stream while the triangles are deduplicated
-    c->mesh_v.resize(nv * 3);
-    HIPCHK(c, hipStreamWaitEvent(c->dstream, c->eb1, 0));
-    if (nv) HIPCHK(c, hipMemcpyAsync(c->mesh_v.data(), E[22].p, nv * 3 * 8, hipMemcpyDeviceToHost, c->dstream));
-    const double t2 = now_ms();
     // triangle dedupe by sorted vertex triple; the first triangle of a repeated triple stays
     int64_t ntk = nt;
     HIPCHK(c, E[8].ensure((ntk + 1) * 4));
@@ -1109,8 +1101,35 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[23].ensure(std::max<int64_t>(ntf, 1) * 3 * 4));
     launch_tri_out(cvid, E[8].as<int>(), E[9].as<int>(), nt, E[23].as<int>(), s);
     HIPCHK(c, hipGetLastError());
+    c->st.ms_extract_host += now_ms() - t0;  // device extraction time (topology)
+    const double t2 = now_ms();
+    // The triangles go to the host on a second stream while the vertices are placed; the
+    // bisection runs in chunks, and each chunk's vertices follow the triangles down as soon as
+    // the chunk is done.
     c->mesh_t.resize(ntf * 3);
-    if (ntf) HIPCHK(c, hipMemcpyAsync(c->mesh_t.data(), E[23].p, ntf * 3 * 4, hipMemcpyDeviceToHost, s));
+    c->mesh_v.resize(nv * 3);
+    HIPCHK(c, hipEventRecord(c->eb0, s));
+    HIPCHK(c, hipStreamWaitEvent(c->dstream, c->eb0, 0));
+    if (ntf) HIPCHK(c, hipMemcpyAsync(c->mesh_t.data(), E[23].p, ntf * 3 * 4, hipMemcpyDeviceToHost, c->dstream));
+    if (c->ev_chunk.empty()) {
+        c->ev_chunk.resize(2, nullptr);
+        for (auto& e : c->ev_chunk) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    // two halves from 2^20 vertices on: smaller launches lose more to the tail of the bisection
+    // (threads take different numbers of steps) than the overlap gains
+    const int64_t chunk = nv >= ((int64_t)1 << 20) ? (nv + 1) / 2 : std::max<int64_t>(nv, 1);
+    int ci = 0;
+    for (int64_t v0 = 0; v0 < nv; v0 += chunk, ci++) {
+        const int64_t m = std::min(chunk, nv - v0);
+        launch_bisect(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->prm.radius / 2,
+                      E[21].as<double>() + 8 * v0, m, E[22].as<double>() + 3 * v0, nullptr, c->ctr(), s);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev_chunk[ci], s));
+        HIPCHK(c, hipStreamWaitEvent(c->dstream, c->ev_chunk[ci], 0));
+        HIPCHK(c, hipMemcpyAsync(c->mesh_v.data() + 3 * v0, E[22].as<double>() + 3 * v0, m * 3 * 8,
+                                 hipMemcpyDeviceToHost, c->dstream));
+    }
+    HIPCHK(c, hipEventRecord(c->eb1, s));
     HIPCHK(c, hipStreamSynchronize(s));
     HIPCHK(c, hipStreamSynchronize(c->dstream));
     {
