@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kRunThreads) k_run_normals(GridDev g, Recs<R> 
     particle_run(g, kx, h0, h1);
     if (h0 >= h1) return;  // uniform over the block
     ParticleSrc<R> src{spos, nrm, R(fc.sigma)};
-    const unsigned long long vis = run_eval<R, K, true>(g, rec, fc, src, h0, h1, lds);
+    const unsigned long long vis = run_eval<kRunThreads, R, K, true>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
     if (threadIdx.x == 0) atomicAdd(&c->grad_evals, (unsigned long long)(h1 - h0));

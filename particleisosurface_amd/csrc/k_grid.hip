@@ -1,6 +1,7 @@
 // Hash grid on the device: bounding box, cell keys, stable LSD radix sort, CSR cell offsets,
 // and the neighbour / box queries of HashGrid (src/hash_grid.cpp).
 #include <cfloat>
+#include <cstring>
 
 #include "pisurf_internal.h"
 
@@ -187,6 +188,41 @@ void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, i
 
 void radix_sort_pairs64(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
     radix_sort_impl<uint64_t>(k0, v0, k1, v1, n, bits, tmp, s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Small read-backs (counts, bounding box) in the frame's level loop. The wait polls an event
+// instead of sleeping in the driver, so a count that is ready a few microseconds after the copy
+// is seen then, not after an interrupt wake-up; the copy lands in pinned memory so it is a plain
+// asynchronous transfer. Each host thread keeps its own staging buffer and event until the
+// process ends.
+// ---------------------------------------------------------------------------------------------
+namespace {
+struct Readback {
+    void* pin = nullptr;
+    hipEvent_t ev = nullptr;
+};
+thread_local Readback tl_readback;
+}  // namespace
+
+hipError_t stream_wait(hipStream_t s) {
+    Readback& r = tl_readback;
+    hipError_t e;
+    if (!r.ev && (e = hipEventCreateWithFlags(&r.ev, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventRecord(r.ev, s)) != hipSuccess) return e;
+    while ((e = hipEventQuery(r.ev)) == hipErrorNotReady) __builtin_ia32_pause();
+    return e;
+}
+
+hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s) {
+    if (bytes > kReadbackBytes) return hipErrorInvalidValue;
+    Readback& r = tl_readback;
+    hipError_t e;
+    if (!r.pin && (e = hipHostMalloc(&r.pin, kReadbackBytes, hipHostMallocDefault)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(r.pin, dptr, bytes, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = stream_wait(s)) != hipSuccess) return e;
+    std::memcpy(host, r.pin, bytes);
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -5,6 +5,9 @@
 #include "qef_device.h"
 #include "run_eval.h"
 
+#include <cmath>
+#include <type_traits>
+
 namespace pisurf {
 
 enum : uint8_t { kEmpty = 0, kInternal = 1, kLeaf = 2 };  // include/iso_method_ours.h:25-27
@@ -290,43 +293,59 @@ template <class R> struct CenterSrc {
     }
 };
 
-template <class R, bool K>
-__global__ void __launch_bounds__(kRunThreads) k_run_centers(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
+template <class R, bool K, int T>
+__global__ void __launch_bounds__(T) k_run_centers(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
                                                              const int* __restrict__ list, const int* __restrict__ val,
                                                              const int* __restrict__ rstart, R* __restrict__ samples,
                                                              Counters* ctr) {
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
     CenterSrc<R> src{lv, list, val, samples, R(fc.iso), R(fc.sigma)};
-    const unsigned long long vis = run_eval<R, K, false>(g, rec, fc, src, h0, h1, lds);
+    const unsigned long long vis = run_eval<T, R, K, false>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
     if (threadIdx.x == 0) atomicAdd(&c->field_evals, (unsigned long long)(h1 - h0));
 }
 
+// Block size T (128 or 256) and cells per run kx for an average of p points per cell: kx p is
+// about 90 % of T (a run may hold more points than the average), and the shape that keeps more of
+// the block's threads busy wins; a cell of more than T points is taken in several passes.
+static void run_shape(double p, int* kx_out, int* t_out) {
+    double best = -1.0;
+    for (int t : {128, 256}) {
+        const int kx = std::max(1, std::min(128, (int)(0.9 * t / std::max(p, 1e-9))));
+        const double pts = kx * p;
+        const double util = pts / (t * std::ceil(pts / t));
+        if (util > best + 0.02) {
+            best = util;
+            *kx_out = kx;
+            *t_out = t;
+        }
+    }
+}
+
 // Cell groups and runs of nh points in cell-major order (key[hlist[h]] >> 9 is the cell group of
 // point h; hlist NULL: point h is entry h). cwork: flags | positions | group starts, run starts
-// ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of runs; *rstart_out points into
-// cwork. Synchronises the stream twice (group and run counts).
+// ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of runs, to be evaluated by blocks
+// of *t_out threads; *rstart_out points into cwork. Synchronises the stream twice (group and run
+// counts).
 static int build_runs(GridDev g, const uint64_t* key, const int* hlist, int nh, int* cwork, int* tmp,
-                      int** rstart_out, int* kx_out, hipStream_t s) {
+                      int** rstart_out, int* kx_out, int* t_out, hipStream_t s) {
     int* flag = cwork;
     int* pos = cwork + (nh + 2);
     int* cstart = cwork + 2 * (int64_t)(nh + 2);
     *rstart_out = cstart;
     *kx_out = 0;
+    *t_out = kRunThreads;
     if (nh <= 0) return 0;
     const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
     k_cell_flags<<<nbh, 256, 0, s>>>(key, hlist, nh, flag);
     exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
     k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
     int ng = 0;
-    (void)hipMemcpyAsync(&ng, pos + nh, 4, hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    // runs of about kRunThreads points: kx cells of a row per run (a power of two <= 64)
-    const double per_cell = (double)nh / (double)std::max(ng, 1);
+    (void)read_device(&ng, pos + nh, 4, s);
     int kx = 1;
-    while (kx < 64 && (kx * 2) * per_cell <= (double)kRunThreads) kx *= 2;
+    run_shape((double)nh / (double)std::max(ng, 1), &kx, t_out);
     *kx_out = kx;
     const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
     k_run_flags<<<nbg, 256, 0, s>>>(g, key, hlist, cstart, ng, kx, flag);
@@ -334,8 +353,7 @@ static int build_runs(GridDev g, const uint64_t* key, const int* hlist, int nh, 
     int* rstart = cstart + (ng + 1);  // (ng + 1) + (nruns + 1) <= 2 (nh + 1)
     k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
     int nr = 0;
-    (void)hipMemcpyAsync(&nr, pos + ng, 4, hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
+    (void)read_device(&nr, pos + ng, 4, s);
     *rstart_out = rstart;
     return nr;
 }
@@ -352,8 +370,8 @@ template <class R> struct SampleSrc {
     __device__ __forceinline__ void store(int h, R sum, const R*) const { out[h] = iso - sigma * sum; }
 };
 
-template <class R, bool K>
-__global__ void __launch_bounds__(kRunThreads) k_run_samples(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
+template <class R, bool K, int T>
+__global__ void __launch_bounds__(T) k_run_samples(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
                                                              const int* __restrict__ list, const int* __restrict__ val,
                                                              const int* __restrict__ hlist,
                                                              const int* __restrict__ rstart, R* __restrict__ out,
@@ -361,7 +379,7 @@ __global__ void __launch_bounds__(kRunThreads) k_run_samples(GridDev g, Recs<R> 
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
     SampleSrc<R> src{lv, list, val, hlist, out, R(fc.iso), R(fc.sigma)};
-    const unsigned long long vis = run_eval<R, K, false>(g, rec, fc, src, h0, h1, lds);
+    const unsigned long long vis = run_eval<T, R, K, false>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->sample_candidates, vis);
     wave_count(&c->candidates, vis);
@@ -602,29 +620,33 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     exclusive_scan_i32(head, hpos, nent + 1, tmp, s);
     k_head_list<<<nb, 256, 0, s>>>(head, hpos, nent, hlist);
     int nh = 0;
-    (void)hipMemcpyAsync(&nh, hpos + nent, 4, hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
+    (void)read_device(&nh, hpos + nent, 4, s);
     *nheads_out = nh;
     int* rstart = nullptr;
-    int kx = 0;
-    const int nr = build_runs(g, key, hlist, nh, cwork, tmp, &rstart, &kx, s);
+    int kx = 0, T = kRunThreads;
+    const int nr = build_runs(g, key, hlist, nh, cwork, tmp, &rstart, &kx, &T, s);
     *nruns_out = nr;
     *kx_out = kx;
     (void)hipEventRecord(e0, s);
     if (nr > 0) {
-        if (prec == 2) {
-            Recs<double> r{(const double4*)ra, (const double4*)rg};
-            k_run_samples<double, false><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
-                                                                    (double*)vals, ctr);
-        } else {
-            Recs<float> r{(const float4*)ra, (const float4*)rg};
-            if (prec == 1)
-                k_run_samples<float, true><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+        auto go = [&](auto tag) {
+            constexpr int TT = decltype(tag)::value;
+            if (prec == 2) {
+                Recs<double> r{(const double4*)ra, (const double4*)rg};
+                k_run_samples<double, false, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                                                                   (double*)vals, ctr);
+            } else {
+                Recs<float> r{(const float4*)ra, (const float4*)rg};
+                if (prec == 1)
+                    k_run_samples<float, true, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                                                                     (float*)vals, ctr);
+                else
+                    k_run_samples<float, false, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
                                                                       (float*)vals, ctr);
-            else
-                k_run_samples<float, false><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
-                                                                       (float*)vals, ctr);
-        }
+            }
+        };
+        if (T == 256) go(std::integral_constant<int, 256>());
+        else go(std::integral_constant<int, 128>());
     }
     (void)hipEventRecord(e1, s);
     k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, val, rep, hpos, nent, vals,
@@ -655,22 +677,27 @@ void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, Fi
     k_center_keys<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(g, lv, list, k, key, val);
     radix_sort_pairs64(key, val, key2, val2, k, bits + 9, tmp, s);
     int* rstart = nullptr;
-    int kx = 0;
-    const int nr = build_runs(g, key, nullptr, (int)k, cwork, tmp, &rstart, &kx, s);
+    int kx = 0, T = kRunThreads;
+    const int nr = build_runs(g, key, nullptr, (int)k, cwork, tmp, &rstart, &kx, &T, s);
     if (nr <= 0) return;
-    if (prec == 2) {
-        Recs<double> r{(const double4*)ra, (const double4*)rg};
-        k_run_centers<double, false><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, rstart, (double*)samples,
-                                                                ctr);
-    } else {
-        Recs<float> r{(const float4*)ra, (const float4*)rg};
-        if (prec == 1)
-            k_run_centers<float, true><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, rstart, (float*)samples,
-                                                                  ctr);
-        else
-            k_run_centers<float, false><<<nr, kRunThreads, 0, s>>>(g, r, fc, lv, list, val, rstart, (float*)samples,
-                                                                   ctr);
-    }
+    auto go = [&](auto tag) {
+        constexpr int TT = decltype(tag)::value;
+        if (prec == 2) {
+            Recs<double> r{(const double4*)ra, (const double4*)rg};
+            k_run_centers<double, false, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart, (double*)samples,
+                                                               ctr);
+        } else {
+            Recs<float> r{(const float4*)ra, (const float4*)rg};
+            if (prec == 1)
+                k_run_centers<float, true, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart, (float*)samples,
+                                                                 ctr);
+            else
+                k_run_centers<float, false, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart,
+                                                                  (float*)samples, ctr);
+        }
+    };
+    if (T == 256) go(std::integral_constant<int, 256>());
+    else go(std::integral_constant<int, 128>());
 }
 
 void launch_samples(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, TreeParams tp, NodeLevel lv,

@@ -17,6 +17,13 @@ int64_t radix_tmp_ints(int64_t n);
 void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s);
 void radix_sort_pairs64(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s);
 
+// ---- read-backs ----
+// wait for the stream's work so far by polling an event (no driver sleep)
+hipError_t stream_wait(hipStream_t s);
+// copy bytes (<= kReadbackBytes) of device memory to host after the stream's earlier work
+constexpr size_t kReadbackBytes = 16384;
+hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s);
+
 // ---- grid ----
 void launch_bbox(const float* xyz, int64_t n, float* partial /*>= 6*1024*/, float* out6, int* nan_count,
                  hipStream_t s);

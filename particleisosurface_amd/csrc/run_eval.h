@@ -13,7 +13,7 @@
 
 namespace pisurf {
 
-constexpr int kRunThreads = 128;
+constexpr int kRunThreads = 128;  // block size of the particle runs (normals)
 constexpr int kRunChunk = 256;
 
 // Policy interface of run_visit:
@@ -25,8 +25,8 @@ constexpr int kRunChunk = 256;
 //   void visit(State&, const Lds&, int t, int j) const  candidate in slot t (global index j)
 //   void end(State&, int h, int ncand) const            active points: ncand = candidates in its
 //                                                       27-cell window
-// Returns the candidates visited by this thread's points.
-template <class Pol>
+// T is the block size. Returns the candidates visited by this thread's points.
+template <int T, class Pol>
 __device__ __forceinline__ unsigned long long run_visit(const GridDev& g, const Pol& pol, int h0, int h1,
                                                         typename Pol::Lds& lds) {
     float pa[3], pb[3];
@@ -58,7 +58,7 @@ __device__ __forceinline__ unsigned long long run_visit(const GridDev& g, const 
     const bool one_cell = xa == xb && xa >= -1 && xa <= g.n[0] && ya >= -1 && ya <= g.n[1] && za >= -1 &&
                           za <= g.n[2];
     unsigned long long vis = 0;
-    for (int hb = h0; hb < h1; hb += kRunThreads) {
+    for (int hb = h0; hb < h1; hb += T) {
         const int h = hb + (int)threadIdx.x;
         const bool act = h < h1;
         float p[3] = {0.f, 0.f, 0.f};
@@ -96,7 +96,7 @@ __device__ __forceinline__ unsigned long long run_visit(const GridDev& g, const 
             const int n = min(kRunChunk, total - cb);
             if (hb > h0 || cb > 0) __syncthreads();  // the previous chunk has been consumed
             if (!(hb > h0 && total <= kRunChunk)) {    // a single chunk stays staged for every point
-                for (int t = threadIdx.x; t < n; t += kRunThreads) {
+                for (int t = threadIdx.x; t < n; t += T) {
                     const int c = cb + t;
                     int j = 0;
 #pragma unroll
@@ -163,12 +163,12 @@ template <class R, bool K, bool GRAD, class Src> struct FieldPol {
     }
 };
 
-template <class R, bool K, bool GRAD, class Src>
+template <int T, class R, bool K, bool GRAD, class Src>
 __device__ __forceinline__ unsigned long long run_eval(const GridDev& g, const Recs<R>& rec, const FieldConst& fc,
                                                        const Src& src, int h0, int h1,
                                                        RunLds<R, K && sizeof(R) == 4>& lds) {
     const FieldPol<R, K, GRAD, Src> pol{rec, fc, R(fc.inv_h2), R(fc.h2), src};
-    return run_visit(g, pol, h0, h1, lds);
+    return run_visit<T>(g, pol, h0, h1, lds);
 }
 
 // run id of a hash cell (extended by one cell on every side): its (y, z) row and x / kx
