@@ -585,6 +585,7 @@ TreeParams tree_params(const pisurf_ctx* c) {
     tp.depth_min = c->dmin;
     tp.depth_max = c->dmax;
     tp.calc_normals = c->prm.calc_normals;
+    tp.curv = 1;
     tp.qef_mode = c->prm.qef_mode;
     tp.qef_svd_tol = c->prm.qef_svd_tol;
     return tp;
@@ -610,7 +611,15 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     const int prec = prec_of(c);
     const size_t rsz = (prec == PISURF_FP64) ? 8 : 4;
     HIPCHK(c, c->samples.ensure(L->n * 27 * rsz));
-    const TreeParams tp = tree_params(c);
+    // leaf-depth level (cellsize - r < 1e-8, afterSampleEval): its nodes are leaves whatever their
+    // curvature, which the reference computes there and discards
+    bool leaf_level = false;
+    if (level_depth >= 0) {
+        const float half = std::ldexp(c->root_half, -level_depth);
+        leaf_level = double((2.0f * half) - c->prm.radius) < 1e-8;
+    }
+    TreeParams tp = tree_params(c);
+    if (leaf_level) tp.curv = 0;
     const FieldConst fc = field_const(c);
     const NodeLevel v = L->view();
     const int64_t need_tmp = scan_tmp_ints(L->n + 1) + 64;
@@ -685,11 +694,6 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
         HIPCHK(c, hipEventRecord(e1, c->stream));
     }
     // centre values of leaf-depth nodes (QEF off): only the leaf-depth level has them
-    bool leaf_level = false;
-    if (level_depth >= 0) {
-        const float half = std::ldexp(c->root_half, -level_depth);
-        leaf_level = double((2.0f * half) - c->prm.radius) < 1e-8;
-    }
     if (level_depth < 0) {
         launch_centers(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), c->cl_pos.as<int>() + L->n,
                        c->samples.p, c->ctr(), c->stream);
@@ -1031,15 +1035,15 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[17].ensure(std::max<int64_t>(nc, 1) * 8));
     launch_tri_corners(E[10].as<int>(), nd, E[12].as<int>(), E[13].as<uint64_t>(), E[14].as<int>(), E[17].as<int>(), s);
     // vertices: corners keyed by their unordered leaf pair, numbered by first occurrence
-    const int64_t vtab = pow2_at_least(2 * nc);
-    HIPCHK(c, E[15].ensure(vtab * 8));
-    HIPCHK(c, E[16].ensure(vtab * 4));
+    // about six corners per distinct vertex on a surface: a table of 1.25 nc slots or more stays
+    // sparse, and holds every key even if all were distinct
+    const int64_t vtab = pow2_at_least(nc + nc / 4 + 1);
+    HIPCHK(c, E[15].ensure(vtab * sizeof(HashSlot)));
     HIPCHK(c, E[18].ensure((nc + 1) * 4));
     HIPCHK(c, E[19].ensure((nc + 1) * 4));
     if ((rc = ensure_tmp(c, nc + 1))) return rc;
     int* vslot = E[14].as<int>();  // the corner indices written there are implicit
-    launch_hash_dedupe(E[13].as<uint64_t>(), nc, E[15].as<unsigned long long>(), E[16].as<unsigned>(), vtab, vslot,
-                       E[18].as<int>(), s);
+    launch_hash_dedupe(E[13].as<uint64_t>(), nc, E[15].as<HashSlot>(), vtab, vslot, E[18].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[18].as<int>() + nc, 0, 4, s));
     exclusive_scan_i32(E[18].as<int>(), E[19].as<int>(), nc + 1, c->tmp.as<int>(), s);
     int64_t nv;
@@ -1047,7 +1051,7 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[20].ensure(std::max<int64_t>(nc, 1) * 4));
     HIPCHK(c, E[21].ensure(std::max<int64_t>(nv, 1) * 8 * 8));
     int* cvid = E[20].as<int>();
-    launch_vertex_ids_hashed(vslot, E[16].as<unsigned>(), E[19].as<int>(), nc, E[17].as<int>(), leaf_node, cvid,
+    launch_vertex_ids_hashed(vslot, E[15].as<HashSlot>(), E[19].as<int>(), nc, E[17].as<int>(), leaf_node, cvid,
                              E[21].as<double>(), s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, E[22].ensure(std::max<int64_t>(nv, 1) * 3 * 8));
@@ -1064,11 +1068,9 @@ int do_extract_device(pisurf_ctx* c) {
         const int64_t ttab = pow2_at_least(2 * ntk);
         HIPCHK(c, E[3].ensure(std::max<int64_t>(ntk, 1) * 8));
         HIPCHK(c, E[4].ensure(std::max<int64_t>(ntk, 1) * 4));
-        HIPCHK(c, E[5].ensure(ttab * 8));
-        HIPCHK(c, E[6].ensure(ttab * 4));
+        HIPCHK(c, E[5].ensure(ttab * sizeof(HashSlot)));
         launch_tri_key63(cvid, nt, E[3].as<uint64_t>(), s);
-        launch_hash_dedupe(E[3].as<uint64_t>(), nt, E[5].as<unsigned long long>(), E[6].as<unsigned>(), ttab,
-                           E[4].as<int>(), E[8].as<int>(), s);
+        launch_hash_dedupe(E[3].as<uint64_t>(), nt, E[5].as<HashSlot>(), ttab, E[4].as<int>(), E[8].as<int>(), s);
     } else {
         // larger meshes: two stable sorts (largest id, then the lower pair) group equal triples
         HIPCHK(c, E[3].ensure(std::max<int64_t>(ntk, 1) * 8));

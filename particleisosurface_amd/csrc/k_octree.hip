@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __r
         bc1[a] = fmaxf(b1[a], tp.b_lo[a]);
         bc2[a] = fminf(b2[a], tp.b_hi[a]);
     }
-    const bool need_curv = tp.calc_normals && depth >= tp.depth_min;
+    const bool need_curv = tp.calc_normals && tp.curv && depth >= tp.depth_min;
     float lx[3], hx[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
