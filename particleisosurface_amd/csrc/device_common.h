@@ -59,11 +59,12 @@ template <class R> __device__ __forceinline__ R wij(R d, R h, R inv_h3) {
 // Cubic B-spline shape without sigma (src/evaluator.cpp:383-396), argument q^2.
 template <class R> __device__ __forceinline__ R bspline_q2(R q2);
 template <> __device__ __forceinline__ float bspline_q2<float>(float q2) {
-    if (!(q2 < 4.0f)) return 0.0f;
+    // both pieces, then a select: the same values as branching on q, without divergence
     const float q = __builtin_amdgcn_sqrtf(q2);
-    if (q <= 1.0f) return 1.0f + q2 * (-1.5f + 0.75f * q);
+    const float inner = 1.0f + q2 * (-1.5f + 0.75f * q);
     const float t = 2.0f - q;
-    return 0.25f * t * t * t;
+    const float outer = 0.25f * t * t * t;
+    return (q2 < 4.0f) ? ((q <= 1.0f) ? inner : outer) : 0.0f;
 }
 template <> __device__ __forceinline__ double bspline_q2<double>(double q2) {
     if (!(q2 < 4.0)) return 0.0;
@@ -81,6 +82,19 @@ template <> __device__ __forceinline__ double dsqrt<double>(double x) { return s
 template <class R, bool K> struct Acc {
     R s = R(0), c = R(0);
     __device__ __forceinline__ void add(R v) {
+        if (K) {
+            const R y = v - c;
+            const R t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += v;
+        }
+    }
+    // the same, never fused with the product that made v: the field sums of every kernel then
+    // round each contribution the same way, whatever code surrounds them
+    __device__ __forceinline__ void add_unfused(R v) {
+#pragma clang fp contract(off)
         if (K) {
             const R y = v - c;
             const R t = s + y;
@@ -122,44 +136,53 @@ struct FieldConst {
 // compiler forms the same operations everywhere)
 template <class R> __device__ __forceinline__ R dist2(R dx, R dy, R dz) { return dx * dx + dy * dy + dz * dz; }
 
+// kernel choice of the field sums: read from FieldConst at run time, or fixed at compile time to
+// the default anisotropic B-spline (use_ani = 1, use_poly6 = 0)
+constexpr int kModeAny = 0, kModeAniBspline = 1;
+__host__ __device__ inline int field_mode(const FieldConst& fc) {
+    return (fc.use_ani && !fc.use_poly6) ? kModeAniBspline : kModeAny;
+}
+
 // contribution of one candidate that passed the cheap reject test: offset (dx, dy, dz) = q - a.xyz,
 // records g0 / g1 and, in compensated FP32, the residual lo of the particle mean
-template <class R, bool KAHAN, bool GRAD>
+template <class R, bool KAHAN, bool GRAD, int MODE = kModeAny>
 __device__ __forceinline__ void field_contrib(const FieldConst& fc, R inv_h2, R h2, R dx, R dy, R dz, R d2,
                                               const typename Vec4<R>::T& g0, const typename Vec4<R>::T& g1,
                                               const float4& lo, Acc<R, KAHAN>& acc, Acc<R, KAHAN>* ga) {
+    const bool ani = MODE == kModeAniBspline || fc.use_ani;
+    const bool poly6 = MODE != kModeAniBspline && fc.use_poly6;
     if (KAHAN && sizeof(R) == 4) {
         dx -= R(lo.x);
         dy -= R(lo.y);
         dz -= R(lo.z);
     }
-    if (fc.use_ani) {
+    if (ani) {
         const R gx = g0.x * dx + g0.y * dy + g0.z * dz;
         const R gy = g0.y * dx + g0.w * dy + g1.x * dz;
         const R gz = g0.z * dx + g1.x * dy + g1.y * dz;
         const R n2 = gx * gx + gy * gy + gz * gz;
-        if (fc.use_poly6) {
+        if (poly6) {
             const R t = h2 - n2;
-            if (!(n2 > h2)) acc.add(g1.z * (t * t * t));
+            if (!(n2 > h2)) acc.add_unfused(g1.z * (t * t * t));
         } else {
-            acc.add(g1.z * bspline_q2<R>(n2 * inv_h2));
+            acc.add_unfused(g1.z * bspline_q2<R>(n2 * inv_h2));
         }
         if (GRAD && !(n2 > h2)) {
             const R t = h2 - n2;
             const R w = R(-6) * g1.z * t * t;
-            ga[0].add(w * gx);
-            ga[1].add(w * gy);
-            ga[2].add(w * gz);
+            ga[0].add_unfused(w * gx);
+            ga[1].add_unfused(w * gy);
+            ga[2].add_unfused(w * gz);
         }
     } else {
         if (!(d2 > h2)) {
             const R t = h2 - d2;
-            acc.add(t * t * t);
+            acc.add_unfused(t * t * t);
             if (GRAD) {
                 const R w = R(-6) * t * t;
-                ga[0].add(w * dx);
-                ga[1].add(w * dy);
-                ga[2].add(w * dz);
+                ga[0].add_unfused(w * dx);
+                ga[1].add_unfused(w * dy);
+                ga[2].add_unfused(w * dz);
             }
         }
     }
@@ -182,7 +205,7 @@ __device__ __forceinline__ int field_rows(const GridDev& g, float fx, float fy, 
 // sigma), optionally with the poly6-form gradient sum (src/evaluator.cpp:408-415,437-463).
 // Candidates are taken in the reference's order (GetPIdxList: rows z-outer, then y, each row's
 // cells x-ascending, sorted particles within a cell). Returns the number of candidates visited.
-template <class R, bool KAHAN, bool GRAD>
+template <class R, bool KAHAN, bool GRAD, int MODE = kModeAny>
 __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, const FieldConst& fc, float fx,
                                          float fy, float fz, R px, R py, R pz, R& sum, R grad[3]) {
     Acc<R, KAHAN> acc, ga[3];
@@ -194,7 +217,8 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     auto contrib = [&](int j, R dx, R dy, R dz, R d2) {
         const float4 lo = (KAHAN && sizeof(R) == 4) ? fc.lo[j] : zero4;
-        field_contrib<R, KAHAN, GRAD>(fc, inv_h2, h2, dx, dy, dz, d2, rec.g[2 * j], rec.g[2 * j + 1], lo, acc, ga);
+        field_contrib<R, KAHAN, GRAD, MODE>(fc, inv_h2, h2, dx, dy, dz, d2, rec.g[2 * j], rec.g[2 * j + 1], lo, acc,
+                                            ga);
     };
     for (int r = 0; r < 9; r++) {
         int j = rlo[r];

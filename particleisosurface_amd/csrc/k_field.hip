@@ -3,6 +3,8 @@
 #include "pisurf_internal.h"
 #include "run_eval.h"
 
+#include <type_traits>
+
 namespace pisurf {
 
 template <class R, bool K, bool GRAD>
@@ -61,7 +63,7 @@ template <class R> struct ParticleSrc {
     }
 };
 
-template <class R, bool K>
+template <class R, bool K, int MODE>
 __global__ void __launch_bounds__(kRunThreads) k_run_normals(GridDev g, Recs<R> rec, FieldConst fc,
                                                              const float4* __restrict__ spos, int kx,
                                                              float4* __restrict__ nrm, Counters* ctr) {
@@ -70,7 +72,7 @@ __global__ void __launch_bounds__(kRunThreads) k_run_normals(GridDev g, Recs<R> 
     particle_run(g, kx, h0, h1);
     if (h0 >= h1) return;  // uniform over the block
     ParticleSrc<R> src{spos, nrm, R(fc.sigma)};
-    const unsigned long long vis = run_eval<kRunThreads, R, K, true>(g, rec, fc, src, h0, h1, lds);
+    const unsigned long long vis = run_eval<kRunThreads, R, K, true, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
     if (threadIdx.x == 0) atomicAdd(&c->grad_evals, (unsigned long long)(h1 - h0));
@@ -118,14 +120,19 @@ void launch_normals(int prec, GridDev g, const void* ra, const void* rg, FieldCo
     const int64_t nb64 = (int64_t)per_row * g.n[1] * g.n[2];
     if (nb64 > INT32_MAX) return;  // grids this large are refused when the grid is built
     const unsigned nb = (unsigned)nb64;
-    if (prec == 2) {
-        Recs<double> r{(const double4*)ra, (const double4*)rg};
-        k_run_normals<double, false><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
-    } else {
-        Recs<float> r{(const float4*)ra, (const float4*)rg};
-        if (prec == 1) k_run_normals<float, true><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
-        else k_run_normals<float, false><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
-    }
+    auto go = [&](auto mode) {
+        constexpr int M = decltype(mode)::value;
+        if (prec == 2) {
+            Recs<double> r{(const double4*)ra, (const double4*)rg};
+            k_run_normals<double, false, M><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
+        } else {
+            Recs<float> r{(const float4*)ra, (const float4*)rg};
+            if (prec == 1) k_run_normals<float, true, M><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
+            else k_run_normals<float, false, M><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
+        }
+    };
+    if (field_mode(fc) == kModeAniBspline) go(std::integral_constant<int, kModeAniBspline>());
+    else go(std::integral_constant<int, kModeAny>());
 }
 
 // UniformGrid::gridSampling (src/marching.cpp:72-91): point i = x + dims0 * (y + dims1 * z) at

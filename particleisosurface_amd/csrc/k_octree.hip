@@ -293,7 +293,7 @@ template <class R> struct CenterSrc {
     }
 };
 
-template <class R, bool K, int T>
+template <class R, bool K, int T, int MODE>
 __global__ void __launch_bounds__(T) k_run_centers(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
                                                              const int* __restrict__ list, const int* __restrict__ val,
                                                              const int* __restrict__ rstart, R* __restrict__ samples,
@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(T) k_run_centers(GridDev g, Recs<R> rec, Field
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
     CenterSrc<R> src{lv, list, val, samples, R(fc.iso), R(fc.sigma)};
-    const unsigned long long vis = run_eval<T, R, K, false>(g, rec, fc, src, h0, h1, lds);
+    const unsigned long long vis = run_eval<T, R, K, false, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
     if (threadIdx.x == 0) atomicAdd(&c->field_evals, (unsigned long long)(h1 - h0));
@@ -370,7 +370,7 @@ template <class R> struct SampleSrc {
     __device__ __forceinline__ void store(int h, R sum, const R*) const { out[h] = iso - sigma * sum; }
 };
 
-template <class R, bool K, int T>
+template <class R, bool K, int T, int MODE>
 __global__ void __launch_bounds__(T) k_run_samples(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
                                                              const int* __restrict__ list, const int* __restrict__ val,
                                                              const int* __restrict__ hlist,
@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(T) k_run_samples(GridDev g, Recs<R> rec, Field
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
     SampleSrc<R> src{lv, list, val, hlist, out, R(fc.iso), R(fc.sigma)};
-    const unsigned long long vis = run_eval<T, R, K, false>(g, rec, fc, src, h0, h1, lds);
+    const unsigned long long vis = run_eval<T, R, K, false, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->sample_candidates, vis);
     wave_count(&c->candidates, vis);
@@ -629,24 +629,27 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     *kx_out = kx;
     (void)hipEventRecord(e0, s);
     if (nr > 0) {
-        auto go = [&](auto tag) {
-            constexpr int TT = decltype(tag)::value;
+        auto go = [&](auto tag, auto mode) {
+            constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
             if (prec == 2) {
                 Recs<double> r{(const double4*)ra, (const double4*)rg};
-                k_run_samples<double, false, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
-                                                                   (double*)vals, ctr);
+                k_run_samples<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                                                                      (double*)vals, ctr);
             } else {
                 Recs<float> r{(const float4*)ra, (const float4*)rg};
                 if (prec == 1)
-                    k_run_samples<float, true, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
-                                                                     (float*)vals, ctr);
+                    k_run_samples<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                                                                        (float*)vals, ctr);
                 else
-                    k_run_samples<float, false, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
-                                                                      (float*)vals, ctr);
+                    k_run_samples<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                                                                         (float*)vals, ctr);
             }
         };
-        if (T == 256) go(std::integral_constant<int, 256>());
-        else go(std::integral_constant<int, 128>());
+        using M0 = std::integral_constant<int, kModeAny>;
+        using M1 = std::integral_constant<int, kModeAniBspline>;
+        const bool m1 = field_mode(fc) == kModeAniBspline;
+        if (T == 256) m1 ? go(std::integral_constant<int, 256>(), M1()) : go(std::integral_constant<int, 256>(), M0());
+        else m1 ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
     }
     (void)hipEventRecord(e1, s);
     k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, val, rep, hpos, nent, vals,
@@ -680,24 +683,27 @@ void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, Fi
     int kx = 0, T = kRunThreads;
     const int nr = build_runs(g, key, nullptr, (int)k, cwork, tmp, &rstart, &kx, &T, s);
     if (nr <= 0) return;
-    auto go = [&](auto tag) {
-        constexpr int TT = decltype(tag)::value;
+    auto go = [&](auto tag, auto mode) {
+        constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
         if (prec == 2) {
             Recs<double> r{(const double4*)ra, (const double4*)rg};
-            k_run_centers<double, false, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart, (double*)samples,
-                                                               ctr);
+            k_run_centers<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart,
+                                                                  (double*)samples, ctr);
         } else {
             Recs<float> r{(const float4*)ra, (const float4*)rg};
             if (prec == 1)
-                k_run_centers<float, true, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart, (float*)samples,
-                                                                 ctr);
+                k_run_centers<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart,
+                                                                    (float*)samples, ctr);
             else
-                k_run_centers<float, false, TT><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart,
-                                                                  (float*)samples, ctr);
+                k_run_centers<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart,
+                                                                     (float*)samples, ctr);
         }
     };
-    if (T == 256) go(std::integral_constant<int, 256>());
-    else go(std::integral_constant<int, 128>());
+    using M0 = std::integral_constant<int, kModeAny>;
+    using M1 = std::integral_constant<int, kModeAniBspline>;
+    const bool m1 = field_mode(fc) == kModeAniBspline;
+    if (T == 256) m1 ? go(std::integral_constant<int, 256>(), M1()) : go(std::integral_constant<int, 256>(), M0());
+    else m1 ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
 }
 
 void launch_samples(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, TreeParams tp, NodeLevel lv,

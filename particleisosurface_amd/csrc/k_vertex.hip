@@ -3,13 +3,15 @@
 #include "pisurf_internal.h"
 #include "qef_device.h"
 
+#include <type_traits>
+
 namespace pisurf {
 
 // One thread per edge. Endpoints are (x, y, z, f) with f of opposite signs (f > 0 is +1).
 // The midpoint replaces the endpoint of equal sign and the length is halved (not recomputed)
 // until it is at most r/2; then the zero crossing is interpolated, snapping to an endpoint
 // when the ratio is below 0.1 or above 0.9.
-template <class R, bool K>
+template <class R, bool K, int MODE>
 __global__ void __launch_bounds__(256) k_bisect(GridDev g, Recs<R> rec, FieldConst fc, float half_r,
                                                 const double* __restrict__ ends, int64_t n, double* __restrict__ out,
                                                 int* __restrict__ nev_out, Counters* ctr) {
@@ -32,7 +34,8 @@ __global__ void __launch_bounds__(256) k_bisect(GridDev g, Recs<R> rec, FieldCon
             R m[4];
             for (int a = 0; a < 3; a++) m[a] = (v1[a] + v2[a]) / R(2);
             R sum, g3[3];
-            vis += field_sum<R, K, false>(g, rec, fc, float(m[0]), float(m[1]), float(m[2]), m[0], m[1], m[2], sum, g3);
+            vis += field_sum<R, K, false, MODE>(g, rec, fc, float(m[0]), float(m[1]), float(m[2]), m[0], m[1], m[2],
+                                                sum, g3);
             nev++;
             m[3] = R(fc.iso) - R(fc.sigma) * sum;
             const int sm = m[3] > R(0) ? 1 : -1, s1 = v1[3] > R(0) ? 1 : -1;
@@ -61,14 +64,19 @@ void launch_bisect(int prec, GridDev g, const void* ra, const void* rg, FieldCon
                    int64_t n, double* out, int* nev, Counters* ctr, hipStream_t s) {
     if (n <= 0) return;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    if (prec == 2) {
-        Recs<double> r{(const double4*)ra, (const double4*)rg};
-        k_bisect<double, false><<<nb, 256, 0, s>>>(g, r, fc, half_r, ends, n, out, nev, ctr);
-    } else {
-        Recs<float> r{(const float4*)ra, (const float4*)rg};
-        if (prec == 1) k_bisect<float, true><<<nb, 256, 0, s>>>(g, r, fc, half_r, ends, n, out, nev, ctr);
-        else k_bisect<float, false><<<nb, 256, 0, s>>>(g, r, fc, half_r, ends, n, out, nev, ctr);
-    }
+    auto go = [&](auto mode) {
+        constexpr int M = decltype(mode)::value;
+        if (prec == 2) {
+            Recs<double> r{(const double4*)ra, (const double4*)rg};
+            k_bisect<double, false, M><<<nb, 256, 0, s>>>(g, r, fc, half_r, ends, n, out, nev, ctr);
+        } else {
+            Recs<float> r{(const float4*)ra, (const float4*)rg};
+            if (prec == 1) k_bisect<float, true, M><<<nb, 256, 0, s>>>(g, r, fc, half_r, ends, n, out, nev, ctr);
+            else k_bisect<float, false, M><<<nb, 256, 0, s>>>(g, r, fc, half_r, ends, n, out, nev, ctr);
+        }
+    };
+    if (field_mode(fc) == kModeAniBspline) go(std::integral_constant<int, kModeAniBspline>());
+    else go(std::integral_constant<int, kModeAny>());
 }
 
 // Standalone QEF over given samples (n x 27 x 4) and gradients (n x 27 x 3).

@@ -124,8 +124,9 @@ template <class R, bool LO> struct RunLds {
     float4 lo[LO ? kRunChunk : 1];
 };
 
-// Src: point(h, float p[3]) gives point h; store(h, sum, grad) receives its sums (without sigma)
-template <class R, bool K, bool GRAD, class Src> struct FieldPol {
+// Src: point(h, float p[3]) gives point h; store(h, sum, grad) receives its sums (without sigma).
+// MODE: kernel choice of field_contrib.
+template <class R, bool K, bool GRAD, int MODE, class Src> struct FieldPol {
     static constexpr bool LO = K && sizeof(R) == 4;
     typedef RunLds<R, LO> Lds;
     struct State {
@@ -154,7 +155,7 @@ template <class R, bool K, bool GRAD, class Src> struct FieldPol {
         const R d2 = dist2(dx, dy, dz);
         if (!(d2 >= a.w)) {
             const float4 lo = LO ? l.lo[LO ? t : 0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            field_contrib<R, K, GRAD>(fc, inv_h2, h2, dx, dy, dz, d2, l.g0[t], l.g1[t], lo, s.acc, s.ga);
+            field_contrib<R, K, GRAD, MODE>(fc, inv_h2, h2, dx, dy, dz, d2, l.g0[t], l.g1[t], lo, s.acc, s.ga);
         }
     }
     __device__ __forceinline__ void end(State& s, int h, int) const {
@@ -163,11 +164,11 @@ template <class R, bool K, bool GRAD, class Src> struct FieldPol {
     }
 };
 
-template <int T, class R, bool K, bool GRAD, class Src>
+template <int T, class R, bool K, bool GRAD, int MODE, class Src>
 __device__ __forceinline__ unsigned long long run_eval(const GridDev& g, const Recs<R>& rec, const FieldConst& fc,
                                                        const Src& src, int h0, int h1,
                                                        RunLds<R, K && sizeof(R) == 4>& lds) {
-    const FieldPol<R, K, GRAD, Src> pol{rec, fc, R(fc.inv_h2), R(fc.h2), src};
+    const FieldPol<R, K, GRAD, MODE, Src> pol{rec, fc, R(fc.inv_h2), R(fc.h2), src};
     return run_visit<T>(g, pol, h0, h1, lds);
 }
 
