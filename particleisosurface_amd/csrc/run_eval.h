@@ -23,7 +23,7 @@ constexpr int kRunChunk = 256;
 //   void stage(Lds&, int t, int j) const                candidate j -> slot t
 //   void begin(State&, int h, const float p[3]) const   (inactive threads too, with h = -1)
 //   void visit(State&, const Lds&, int t, int j) const  candidate in slot t (global index j)
-//   void visit2(State&, const Lds&, int t, int j) const candidates t and t + 1, in that order
+//   void visitn<N>(State&, const Lds&, int t) const    candidates t .. t + N - 1, in that order
 //   void end(State&, int h, int ncand) const            active points: ncand = candidates in its
 //                                                       27-cell window
 // T is the block size. Returns the candidates visited by this thread's points.
@@ -112,8 +112,8 @@ __device__ __forceinline__ unsigned long long run_visit(const GridDev& g, const 
                 const int t0 = max(clo[r], cb) - cb, t1 = min(chi[r], cb + n) - cb;
                 const int jr = w0[r] - roff[r] + cb;  // global index of slot t in row r: jr + t
                 int t = t0;
-                for (; t + 2 <= t1; t += 2) pol.visit2(st, lds, t, jr + t);
-                if (t < t1) pol.visit(st, lds, t, jr + t);
+                for (; t + 4 <= t1; t += 4) pol.template visitn<4>(st, lds, t);
+                for (; t < t1; t++) pol.visit(st, lds, t, jr + t);
             }
         }
         if (act) pol.end(st, h, ncand);
@@ -161,20 +161,26 @@ template <class R, bool K, bool GRAD, int MODE, class Src> struct FieldPol {
             field_contrib<R, K, GRAD, MODE>(fc, inv_h2, h2, dx, dy, dz, d2, l.g0[t], l.g1[t], lo, s.acc, s.ga);
         }
     }
-    // both positions are read before either is used, so the two LDS reads overlap
-    __device__ __forceinline__ void visit2(State& s, const Lds& l, int t, int) const {
-        const typename Vec4<R>::T a0 = l.a[t], a1 = l.a[t + 1];
-        const R dx0 = s.px - a0.x, dy0 = s.py - a0.y, dz0 = s.pz - a0.z;
-        const R dx1 = s.px - a1.x, dy1 = s.py - a1.y, dz1 = s.pz - a1.z;
-        const R q0 = dist2(dx0, dy0, dz0), q1 = dist2(dx1, dy1, dz1);
-        if (!(q0 >= a0.w)) {
-            const float4 lo = LO ? l.lo[LO ? t : 0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            field_contrib<R, K, GRAD, MODE>(fc, inv_h2, h2, dx0, dy0, dz0, q0, l.g0[t], l.g1[t], lo, s.acc, s.ga);
+    // the N positions are read before any is used, so their LDS reads overlap
+    template <int N> __device__ __forceinline__ void visitn(State& s, const Lds& l, int t) const {
+        typename Vec4<R>::T a[N];
+        R dx[N], dy[N], dz[N], q[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) a[k] = l.a[t + k];
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            dx[k] = s.px - a[k].x;
+            dy[k] = s.py - a[k].y;
+            dz[k] = s.pz - a[k].z;
+            q[k] = dist2(dx[k], dy[k], dz[k]);
         }
-        if (!(q1 >= a1.w)) {
-            const float4 lo = LO ? l.lo[LO ? t + 1 : 0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            field_contrib<R, K, GRAD, MODE>(fc, inv_h2, h2, dx1, dy1, dz1, q1, l.g0[t + 1], l.g1[t + 1], lo, s.acc,
-                                            s.ga);
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            if (!(q[k] >= a[k].w)) {
+                const float4 lo = LO ? l.lo[LO ? t + k : 0] : make_float4(0.f, 0.f, 0.f, 0.f);
+                field_contrib<R, K, GRAD, MODE>(fc, inv_h2, h2, dx[k], dy[k], dz[k], q[k], l.g0[t + k], l.g1[t + k],
+                                                lo, s.acc, s.ga);
+            }
         }
     }
     __device__ __forceinline__ void end(State& s, int h, int) const {
