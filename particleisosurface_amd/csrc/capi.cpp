@@ -668,17 +668,19 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             // distinct points grouped by hash cell (cell-major keys), evaluated in runs of cells
             const uint64_t groups = (uint64_t)(c->g.n[0] + 2) * (uint64_t)(c->g.n[1] + 2) * (uint64_t)(c->g.n[2] + 2);
             const int kbits = std::min(64, bits_for((int64_t)groups + 1) + 9);
-            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, 1, c->sd_key.as<uint64_t>(),
+            const int vbits = pack_vbits(kbits, nent);
+            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, 1, vbits, c->sd_key.as<uint64_t>(),
                                       c->sd_val.as<int>(), c->stream);
-            radix_sort_pairs64(c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
-                               c->sd_val2.as<int>(), nent, kbits, c->tmp.as<int>(), c->stream);
+            const SortedKV skv = radix_sort_entries(c->sd_key.as<uint64_t>(), c->sd_val.as<int>(),
+                                                    c->sd_key2.as<uint64_t>(), c->sd_val2.as<int>(), nent, kbits,
+                                                    vbits, c->tmp.as<int>(), c->stream);
             hipEvent_t e0, e1;
             size_t eslot = 0;
             HIPCHK(c, c->event_pair(e0, e1, &eslot));
             int64_t nheads = 0, nruns = 0;
             int kx = 0;
-            launch_sample_dedupe_eval(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, L->child.as<int>(),
-                                      c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), nent, c->sd_head.as<int>(),
+            launch_sample_dedupe_eval(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, L->child.as<int>(), skv, nent,
+                                      c->sd_head.as<int>(),
                                       c->sd_hpos.as<int>(), c->sd_rep.as<int>(), c->sd_hlist.as<int>(),
                                       c->sd_cwork.as<int>(), &nheads, &nruns, &kx, c->sd_vals.p, c->samples.p,
                                       c->tmp.as<int>(), c->ctr(), e0, e1, c->stream);
@@ -984,12 +986,15 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[7].ensure(64 * 128));
     HIPCHK(c, hipMemsetAsync(E[7].p, 0, 64 * 128, s));
     double4* leaf_node = E[2].as<double4>();
+    // corners outside the finest lattice carry all-ones keys and sort after the nent valid ones
+    const int ckbits = std::min(64, 3 * tv.dp + 1);
+    const int cvbits = pack_vbits(ckbits, nent_all);
     for (int L = 0; L < tv.nlev; L++)
-        launch_leaf_emit(c->levels[L]->view(), tv.off[L], lidx, tv.dp, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(),
-                         E[7].as<int>(), s);
+        launch_leaf_emit(c->levels[L]->view(), tv.off[L], lidx, tv.dp, ckbits, cvbits, leaf_node, E[3].as<uint64_t>(),
+                         E[4].as<int>(), E[7].as<int>(), s);
     if ((rc = ensure_tmp(c, nent_all + 1))) return rc;
-    radix_sort_pairs64(E[3].as<uint64_t>(), E[4].as<int>(), E[5].as<uint64_t>(), E[6].as<int>(), nent_all,
-                       std::min(64, 3 * tv.dp + 1), c->tmp.as<int>(), s);
+    const SortedKV ckv = radix_sort_entries(E[3].as<uint64_t>(), E[4].as<int>(), E[5].as<uint64_t>(), E[6].as<int>(),
+                                            nent_all, ckbits, cvbits, c->tmp.as<int>(), s);
     HIPCHK(c, hipGetLastError());
     int64_t nent = 0;
     {
@@ -1001,7 +1006,7 @@ int do_extract_device(pisurf_ctx* c) {
     // distinct points: run heads of the sorted corner keys, one thread each
     HIPCHK(c, E[8].ensure((nent + 1) * 4));
     HIPCHK(c, E[9].ensure((nent + 1) * 4));
-    launch_run_heads(E[3].as<uint64_t>(), nent, E[8].as<int>(), s);
+    launch_run_heads(ckv, nent, E[8].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[8].as<int>() + nent, 0, 4, s));
     exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nent + 1, c->tmp.as<int>(), s);
     int64_t npts;
@@ -1011,8 +1016,7 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[26].ensure((npts + 1) * 4));
     HIPCHK(c, E[27].ensure((npts + 1) * 4));
     HIPCHK(c, E[24].ensure(std::max<int64_t>(npts, 1) * 9 * 4));
-    launch_points(tv, lidx, leaf_node, E[3].as<uint64_t>(), E[4].as<int>(), nent, E[25].as<int>(), npts,
-                  E[26].as<int>(), E[24].as<int>(), s);
+    launch_points(tv, lidx, leaf_node, ckv, nent, E[25].as<int>(), npts, E[26].as<int>(), E[24].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[26].as<int>() + npts, 0, 4, s));
     exclusive_scan_i32(E[26].as<int>(), E[27].as<int>(), npts + 1, c->tmp.as<int>(), s);
     int64_t nd;

@@ -47,7 +47,7 @@ __global__ void k_leaf_flag(NodeLevel lv, int64_t off, int* __restrict__ flag) {
 
 // leaf records and the 8 corner keys of every leaf (corners on the root boundary get the
 // sentinel key ~0 and sort last; valid keys use 3 * dp bits)
-__global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp,
+__global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp, int kbits, int vbits,
                             double4* __restrict__ leaf_node, uint64_t* __restrict__ ckey, int* __restrict__ cval,
                             int* __restrict__ n_valid) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -63,9 +63,15 @@ __global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ l
             const int64_t X = (int64_t)(p.x + (c & 1)) << sh, Y = (int64_t)(p.y + ((c >> 1) & 1)) << sh,
                           Z = (int64_t)(p.z + ((c >> 2) & 1)) << sh;
             const bool in = X >= 1 && X <= hi && Y >= 1 && Y <= hi && Z >= 1 && Z <= hi;
-            ckey[8 * (int64_t)L + c] =
-                in ? morton_spread(X) | (morton_spread(Y) << 1) | (morton_spread(Z) << 2) : ~0ull;
-            cval[8 * (int64_t)L + c] = 8 * L + c;
+            const uint64_t kmax = kbits >= 64 ? ~0ull : ((1ull << kbits) - 1);
+            const uint64_t k = in ? morton_spread(X) | (morton_spread(Y) << 1) | (morton_spread(Z) << 2) : kmax;
+            const int64_t e = 8 * (int64_t)L + c;
+            if (vbits) {
+                ckey[e] = (k << vbits) | (uint64_t)e;
+            } else {
+                ckey[e] = k;
+                cval[e] = (int)e;
+            }
             valid += in;
         }
     }
@@ -91,13 +97,13 @@ __device__ __forceinline__ int locate(const TreeView& tv, const int* lidx, int q
 }
 
 // the 8 leaves around the point whose corner entries start at sorted position j
-__device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx, const uint64_t* skey,
-                                             const int* sval, int64_t nent, int64_t j, int n[8]) {
-    const uint64_t key = skey[j];
+__device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx, const SortedKV& kv, int64_t nent,
+                                             int64_t j, int n[8]) {
+    const uint64_t key = kv.key(j);
 #pragma unroll
     for (int o = 0; o < 8; o++) n[o] = -1;
-    for (int64_t k = j; k < nent && k < j + 8 && skey[k] == key; k++) {
-        const int v = sval[k];
+    for (int64_t k = j; k < nent && k < j + 8 && kv.key(k) == key; k++) {
+        const int v = kv.val(k);
         n[(v & 7) ^ 7] = v >> 3;  // a leaf with P as corner c lies in octant c ^ 7 of P
     }
     const int px = (int)morton_compact(key), py = (int)morton_compact(key >> 1), pz = (int)morton_compact(key >> 2);
@@ -129,13 +135,12 @@ __device__ __forceinline__ int sign_index(const double4* leaf_node, const int n[
 // is non-trivial and stage the cell (8 leaves in case order, sign configuration) at slot t;
 // k_points_compact then packs the staged cells
 __global__ void k_points(TreeView tv, const int* __restrict__ lidx, const double4* __restrict__ leaf_node,
-                         const uint64_t* __restrict__ skey, const int* __restrict__ sval, int64_t nent,
-                         const int* __restrict__ heads, int64_t npts, int* __restrict__ dflag,
-                         int* __restrict__ stage) {
+                         SortedKV kv, int64_t nent, const int* __restrict__ heads, int64_t npts,
+                         int* __restrict__ dflag, int* __restrict__ stage) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npts) return;
     int n[8];
-    gather_point(tv, lidx, skey, sval, nent, heads[t], n);
+    gather_point(tv, lidx, kv, nent, heads[t], n);
     const int s = sign_index(leaf_node, n);
     const uint8_t code = d_kDmcCase[s];
     const bool keep = code != 0 && code != 255;
@@ -196,10 +201,10 @@ __global__ void k_tri_corners(const int* __restrict__ duals, int64_t nd, const i
         }
 }
 
-__global__ void k_run_heads(const uint64_t* __restrict__ key, int64_t n, int* __restrict__ head) {
+__global__ void k_run_heads(SortedKV kv, int64_t n, int* __restrict__ head) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    head[j] = (j == 0 || key[j] != key[j - 1]) ? 1 : 0;
+    head[j] = (j == 0 || kv.key(j) != kv.key(j - 1)) ? 1 : 0;
 }
 
 // ---- hash-table dedupe (first occurrence wins, as Mesh::insert_vert / insert_tri do) ----
@@ -318,9 +323,10 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s) {
     if (lv.n > 0) k_leaf_flag<<<nblk(lv.n), 256, 0, s>>>(lv, off, flag);
 }
-void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double4* leaf_node, uint64_t* ckey,
-                      int* cval, int* n_valid, hipStream_t s) {
-    if (lv.n > 0) k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, leaf_node, ckey, cval, n_valid);
+void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, int kbits, int vbits, double4* leaf_node,
+                      uint64_t* ckey, int* cval, int* n_valid, hipStream_t s) {
+    if (lv.n > 0)
+        k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, kbits, vbits, leaf_node, ckey, cval, n_valid);
 }
 void launch_hash_dedupe(const uint64_t* key, int64_t n, HashSlot* table, int64_t tsize, int* slot, int* first,
                         hipStream_t s) {
@@ -339,11 +345,9 @@ void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s) {
 void launch_head_positions(const int* head, const int* hpos, int64_t n, int* list, hipStream_t s) {
     if (n > 0) k_head_positions<<<nblk(n), 256, 0, s>>>(head, hpos, n, list);
 }
-void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, const uint64_t* skey,
-                   const int* sval, int64_t nent, const int* heads, int64_t npts, int* dflag, int* stage,
-                   hipStream_t s) {
-    if (npts > 0)
-        k_points<<<nblk(npts), 256, 0, s>>>(tv, lidx, leaf_node, skey, sval, nent, heads, npts, dflag, stage);
+void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, SortedKV kv, int64_t nent,
+                   const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s) {
+    if (npts > 0) k_points<<<nblk(npts), 256, 0, s>>>(tv, lidx, leaf_node, kv, nent, heads, npts, dflag, stage);
 }
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s) {
@@ -356,8 +360,8 @@ void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t*
                         hipStream_t s) {
     if (nd > 0) k_tri_corners<<<nblk(nd), 256, 0, s>>>(duals, nd, toff, vkey, vval, vend);
 }
-void launch_run_heads(const uint64_t* key, int64_t n, int* head, hipStream_t s) {
-    if (n > 0) k_run_heads<<<nblk(n), 256, 0, s>>>(key, n, head);
+void launch_run_heads(SortedKV kv, int64_t n, int* head, hipStream_t s) {
+    if (n > 0) k_run_heads<<<nblk(n), 256, 0, s>>>(kv, n, head);
 }
 void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s) {
     if (nt > 0) k_tri_key_hi<<<nblk(nt), 256, 0, s>>>(cv, nt, k, v);

@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(kRsT) k_rs_hist(const KT* __restrict__ keys, i
     bh[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-template <class KT>
+template <class KT, bool VAL>
 __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin, const int* __restrict__ vin,
                                                      KT* __restrict__ kout, int* __restrict__ vout, int64_t n,
                                                      int shift, const int* __restrict__ bscan, int nblocks) {
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin,
         const int64_t i = (int64_t)blockIdx.x * kRsTile + r * kRsT + t;
         const bool valid = i < n;
         const KT k = valid ? kin[i] : KT(0);
-        const int v = valid ? vin[i] : 0;
+        const int v = (VAL && valid) ? vin[i] : 0;
         const int d = (int)((k >> shift) & 255u);
         unsigned long long m = __ballot(valid);
 #pragma unroll
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin,
         if (valid) {
             const int pos = base[d] + wc[w][d] + rank;
             kout[pos] = k;
-            vout[pos] = v;
+            if (VAL) vout[pos] = v;
         }
         __syncthreads();
         base[t] += tot[t];
@@ -158,9 +158,10 @@ int64_t radix_tmp_ints(int64_t n) {
     return 2 * 256 * nb + scan_tmp_ints(256 * nb) + 16;
 }
 
+// Sorts by bits [lo_bit, lo_bit + bits) of the keys; v0 / v1 NULL: keys only. Returns the buffer
+// holding the result (k0 or k1; with values, v0 or v1 alike).
 template <class KT>
-static void radix_sort_impl(KT* k0, int* v0, KT* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
-    if (n <= 1) return;
+static KT* radix_passes(KT* k0, int* v0, KT* k1, int* v1, int64_t n, int lo_bit, int bits, int* tmp, hipStream_t s) {
     const int nb = (int)((n + kRsTile - 1) / kRsTile);
     int* bh = tmp;
     int* bscan = tmp + 256 * (int64_t)nb;
@@ -169,18 +170,26 @@ static void radix_sort_impl(KT* k0, int* v0, KT* k1, int* v1, int64_t n, int bit
     KT *ks = k0, *kd = k1;
     int *vs = v0, *vd = v1;
     for (int p = 0; p < passes; p++) {
-        const int shift = 8 * p;
+        const int shift = lo_bit + 8 * p;
         k_rs_hist<KT><<<nb, kRsT, 0, s>>>(ks, n, shift, bh, nb);
         exclusive_scan_i32(bh, bscan, 256 * (int64_t)nb, stmp, s);
-        k_rs_scatter<KT><<<nb, kRsT, 0, s>>>(ks, vs, kd, vd, n, shift, bscan, nb);
+        if (v0) k_rs_scatter<KT, true><<<nb, kRsT, 0, s>>>(ks, vs, kd, vd, n, shift, bscan, nb);
+        else k_rs_scatter<KT, false><<<nb, kRsT, 0, s>>>(ks, nullptr, kd, nullptr, n, shift, bscan, nb);
         KT* tk = ks; ks = kd; kd = tk;
         int* tv = vs; vs = vd; vd = tv;
     }
-    if (ks != k0) {
-        (void)hipMemcpyAsync(k0, ks, n * sizeof(KT), hipMemcpyDeviceToDevice, s);
-        (void)hipMemcpyAsync(v0, vs, n * sizeof(int), hipMemcpyDeviceToDevice, s);
+    return ks;
+}
+
+template <class KT>
+static void radix_sort_impl(KT* k0, int* v0, KT* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
+    if (n <= 1) return;
+    if (radix_passes<KT>(k0, v0, k1, v1, n, 0, bits, tmp, s) != k0) {
+        (void)hipMemcpyAsync(k0, k1, n * sizeof(KT), hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(v0, v1, n * sizeof(int), hipMemcpyDeviceToDevice, s);
     }
 }
+
 
 void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
     radix_sort_impl<uint32_t>(k0, v0, k1, v1, n, bits, tmp, s);
@@ -188,6 +197,22 @@ void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, i
 
 void radix_sort_pairs64(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s) {
     radix_sort_impl<uint64_t>(k0, v0, k1, v1, n, bits, tmp, s);
+}
+
+int pack_vbits(int bits, int64_t n) {
+    int vb = 1;
+    while (vb < 31 && ((int64_t)1 << vb) < n) vb++;
+    return (((int64_t)1 << vb) >= n && bits + vb <= 64) ? vb : 0;
+}
+
+SortedKV radix_sort_entries(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_t n, int bits, int vbits, int* tmp,
+                            hipStream_t s) {
+    if (vbits <= 0) {
+        radix_sort_impl<uint64_t>(k0, v0, k1, v1, n, bits, tmp, s);
+        return SortedKV{k0, v0, 0};
+    }
+    if (n <= 1) return SortedKV{k0, nullptr, vbits};
+    return SortedKV{radix_passes<uint64_t>(k0, nullptr, k1, nullptr, n, vbits, bits, tmp, s), nullptr, vbits};
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -158,7 +158,8 @@ __device__ __forceinline__ uint64_t cell_group(const GridDev& g, float x, float 
 // cellmode 0: Morton code of the lattice position; cellmode 1: cell group, then the lattice
 // position modulo 8 per axis (distinct inside a cell spanning fewer than 8 lattice steps)
 __global__ void k_sample_keys(GridDev g, NodeLevel lv, const int* __restrict__ list, int64_t nent, float ox,
-                              float oy, float oz, int cellmode, uint64_t* __restrict__ key, int* __restrict__ val) {
+                              float oy, float oz, int cellmode, int vbits, uint64_t* __restrict__ key,
+                              int* __restrict__ val) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nent) return;
     const int node = list[t / 27];
@@ -172,11 +173,15 @@ __global__ void k_sample_keys(GridDev g, NodeLevel lv, const int* __restrict__ l
         const float u = rintf((p[a] - o[a]) / c.w);
         q[a] = (uint32_t)fminf(fmaxf(u, 0.0f), 2097151.0f);
     }
-    if (cellmode)
-        key[t] = (cell_group(g, p[0], p[1], p[2]) << 9) | (q[0] & 7) | ((q[1] & 7) << 3) | ((q[2] & 7) << 6);
-    else
-        key[t] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
-    val[t] = (int)t;
+    const uint64_t k = cellmode ? (cell_group(g, p[0], p[1], p[2]) << 9) | (q[0] & 7) | ((q[1] & 7) << 3) |
+                                      ((q[2] & 7) << 6)
+                                : spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    if (vbits) {
+        key[t] = (k << vbits) | (uint64_t)t;
+    } else {
+        key[t] = k;
+        val[t] = (int)t;
+    }
 }
 
 __device__ __forceinline__ void entry_point(const NodeLevel& lv, const int* list, int e, float p[3]) {
@@ -185,9 +190,8 @@ __device__ __forceinline__ void entry_point(const NodeLevel& lv, const int* list
 
 // head[j] = 1 when no earlier entry of the same key run has the same coordinates; rep[j] = sorted
 // position of the first entry with j's coordinates
-__global__ void k_sample_heads(NodeLevel lv, const int* __restrict__ list, const uint64_t* __restrict__ key,
-                               const int* __restrict__ val, int64_t nent, int* __restrict__ head,
-                               int* __restrict__ rep) {
+__global__ void k_sample_heads(NodeLevel lv, const int* __restrict__ list, SortedKV kv, int64_t nent,
+                               int* __restrict__ head, int* __restrict__ rep) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j > nent) return;
     if (j == nent) {
@@ -195,13 +199,14 @@ __global__ void k_sample_heads(NodeLevel lv, const int* __restrict__ list, const
         return;
     }
     int64_t s0 = j;
-    while (s0 > 0 && j - s0 < 64 && key[s0 - 1] == key[j]) s0--;
+    const uint64_t kj = kv.key(j);
+    while (s0 > 0 && j - s0 < 64 && kv.key(s0 - 1) == kj) s0--;
     float p[3];
-    entry_point(lv, list, val[j], p);
+    entry_point(lv, list, kv.val(j), p);
     int64_t r = j;
     for (int64_t k = s0; k < j; k++) {
         float q[3];
-        entry_point(lv, list, val[k], q);
+        entry_point(lv, list, kv.val(k), q);
         if (q[0] == p[0] && q[1] == p[1] && q[2] == p[2] && __float_as_uint(q[0]) == __float_as_uint(p[0]) &&
             __float_as_uint(q[1]) == __float_as_uint(p[1]) && __float_as_uint(q[2]) == __float_as_uint(p[2])) {
             r = k;
@@ -220,8 +225,7 @@ __global__ void k_head_list(const int* __restrict__ head, const int* __restrict_
 
 // ---- cell-grouped evaluation (levels with many sample points per hash cell) ----
 // cflag[h] = 1 where the cell group changes along the distinct points (cell-major order)
-__global__ void k_cell_flags(const uint64_t* __restrict__ key, const int* __restrict__ hlist, int64_t nheads,
-                             int* __restrict__ cflag) {
+__global__ void k_cell_flags(SortedKV kv, const int* __restrict__ hlist, int64_t nheads, int* __restrict__ cflag) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h > nheads) return;
     if (h == nheads) {
@@ -230,7 +234,7 @@ __global__ void k_cell_flags(const uint64_t* __restrict__ key, const int* __rest
     }
     // hlist NULL: the points are the sorted entries themselves
     const int64_t a = hlist ? hlist[h] : h, b = h > 0 ? (hlist ? hlist[h - 1] : h - 1) : 0;
-    cflag[h] = (h == 0 || (key[a] >> 9) != (key[b] >> 9)) ? 1 : 0;
+    cflag[h] = (h == 0 || (kv.key(a) >> 9) != (kv.key(b) >> 9)) ? 1 : 0;
 }
 // cstart[k] = first distinct point of cell group k; cstart[ngroups] = nheads
 __global__ void k_cell_starts(const int* __restrict__ cflag, const int* __restrict__ cpos, int64_t nheads,
@@ -242,8 +246,8 @@ __global__ void k_cell_starts(const int* __restrict__ cflag, const int* __restri
 }
 
 // runs of cell groups: rflag[k] = 1 where the run id (cell row, x / kx) changes along the groups
-__global__ void k_run_flags(GridDev g, const uint64_t* __restrict__ key, const int* __restrict__ hlist,
-                            const int* __restrict__ cstart, int64_t ngroups, int kx, int* __restrict__ rflag) {
+__global__ void k_run_flags(GridDev g, SortedKV kv, const int* __restrict__ hlist, const int* __restrict__ cstart,
+                            int64_t ngroups, int kx, int* __restrict__ rflag) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k > ngroups) return;
     if (k == ngroups) {
@@ -251,7 +255,7 @@ __global__ void k_run_flags(GridDev g, const uint64_t* __restrict__ key, const i
         return;
     }
     auto run_id = [&](int64_t grp) {
-        const uint64_t gid = key[hlist ? hlist[cstart[grp]] : cstart[grp]] >> 9;
+        const uint64_t gid = kv.key(hlist ? hlist[cstart[grp]] : cstart[grp]) >> 9;
         const uint64_t ex = g.n[0] + 2, ey = g.n[1] + 2;
         const int ix = (int)(gid % ex) - 1, iy = (int)((gid / ex) % ey) - 1, iz = (int)(gid / (ex * ey)) - 1;
         return run_of_cell(g, ix, iy, iz, kx);  // the empty-window group maps past every row
@@ -267,40 +271,45 @@ __global__ void k_run_starts(const int* __restrict__ rflag, const int* __restric
 }
 
 // leaf-depth centres keyed by cell group (the centres are distinct, one per node)
-__global__ void k_center_keys(GridDev g, NodeLevel lv, const int* __restrict__ list, int64_t k,
+__global__ void k_center_keys(GridDev g, NodeLevel lv, const int* __restrict__ list, int64_t k, int vbits,
                               uint64_t* __restrict__ key, int* __restrict__ val) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= k) return;
     const float4 c = lv.c[list[e]];
-    key[e] = cell_group(g, c.x, c.y, c.z) << 9;
-    val[e] = (int)e;
+    const uint64_t kk = cell_group(g, c.x, c.y, c.z) << 9;
+    if (vbits) {
+        key[e] = (kk << vbits) | (uint64_t)e;
+    } else {
+        key[e] = kk;
+        val[e] = (int)e;
+    }
 }
 
 template <class R> struct CenterSrc {
     NodeLevel lv;
     const int* list;
-    const int* val;
+    SortedKV kv;
     R* samples;
     R iso, sigma;
     __device__ __forceinline__ void point(int h, float p[3]) const {
-        const float4 c = lv.c[list[val[h]]];
+        const float4 c = lv.c[list[kv.val(h)]];
         p[0] = c.x;
         p[1] = c.y;
         p[2] = c.z;
     }
     __device__ __forceinline__ void store(int h, R sum, const R*) const {
-        samples[(int64_t)list[val[h]] * 27 + 13] = iso - sigma * sum;
+        samples[(int64_t)list[kv.val(h)] * 27 + 13] = iso - sigma * sum;
     }
 };
 
 template <class R, bool K, int T, int MODE>
 __global__ void __launch_bounds__(T) k_run_centers(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
-                                                             const int* __restrict__ list, const int* __restrict__ val,
+                                                             const int* __restrict__ list, SortedKV kv,
                                                              const int* __restrict__ rstart, R* __restrict__ samples,
                                                              Counters* ctr) {
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
-    CenterSrc<R> src{lv, list, val, samples, R(fc.iso), R(fc.sigma)};
+    CenterSrc<R> src{lv, list, kv, samples, R(fc.iso), R(fc.sigma)};
     const unsigned long long vis = run_eval<T, R, K, false, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
@@ -329,7 +338,7 @@ static void run_shape(double p, int* kx_out, int* t_out) {
 // ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of runs, to be evaluated by blocks
 // of *t_out threads; *rstart_out points into cwork. Synchronises the stream twice (group and run
 // counts).
-static int build_runs(GridDev g, const uint64_t* key, const int* hlist, int nh, int* cwork, int* tmp,
+static int build_runs(GridDev g, SortedKV kv, const int* hlist, int nh, int* cwork, int* tmp,
                       int** rstart_out, int* kx_out, int* t_out, hipStream_t s) {
     int* flag = cwork;
     int* pos = cwork + (nh + 2);
@@ -339,7 +348,7 @@ static int build_runs(GridDev g, const uint64_t* key, const int* hlist, int nh, 
     *t_out = kRunThreads;
     if (nh <= 0) return 0;
     const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
-    k_cell_flags<<<nbh, 256, 0, s>>>(key, hlist, nh, flag);
+    k_cell_flags<<<nbh, 256, 0, s>>>(kv, hlist, nh, flag);
     exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
     k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
     int ng = 0;
@@ -348,7 +357,7 @@ static int build_runs(GridDev g, const uint64_t* key, const int* hlist, int nh, 
     run_shape((double)nh / (double)std::max(ng, 1), &kx, t_out);
     *kx_out = kx;
     const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
-    k_run_flags<<<nbg, 256, 0, s>>>(g, key, hlist, cstart, ng, kx, flag);
+    k_run_flags<<<nbg, 256, 0, s>>>(g, kv, hlist, cstart, ng, kx, flag);
     exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
     int* rstart = cstart + (ng + 1);  // (ng + 1) + (nruns + 1) <= 2 (nh + 1)
     k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
@@ -362,23 +371,23 @@ static int build_runs(GridDev g, const uint64_t* key, const int* hlist, int nh, 
 template <class R> struct SampleSrc {
     NodeLevel lv;
     const int* list;
-    const int* val;
+    SortedKV kv;
     const int* hlist;
     R* out;
     R iso, sigma;
-    __device__ __forceinline__ void point(int h, float p[3]) const { entry_point(lv, list, val[hlist[h]], p); }
+    __device__ __forceinline__ void point(int h, float p[3]) const { entry_point(lv, list, kv.val(hlist[h]), p); }
     __device__ __forceinline__ void store(int h, R sum, const R*) const { out[h] = iso - sigma * sum; }
 };
 
 template <class R, bool K, int T, int MODE>
 __global__ void __launch_bounds__(T) k_run_samples(GridDev g, Recs<R> rec, FieldConst fc, NodeLevel lv,
-                                                             const int* __restrict__ list, const int* __restrict__ val,
+                                                             const int* __restrict__ list, SortedKV kv,
                                                              const int* __restrict__ hlist,
                                                              const int* __restrict__ rstart, R* __restrict__ out,
                                                              Counters* ctr) {
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
-    SampleSrc<R> src{lv, list, val, hlist, out, R(fc.iso), R(fc.sigma)};
+    SampleSrc<R> src{lv, list, kv, hlist, out, R(fc.iso), R(fc.sigma)};
     const unsigned long long vis = run_eval<T, R, K, false, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->sample_candidates, vis);
@@ -392,12 +401,12 @@ __global__ void __launch_bounds__(T) k_run_samples(GridDev g, Recs<R> rec, Field
 
 // every entry receives its point's value (the host counts the entries: 27 per sampled node, the
 // reference's evaluations)
-__global__ void k_sample_scatter(const int* __restrict__ list, const int* __restrict__ val,
-                                 const int* __restrict__ rep, const int* __restrict__ hpos, int64_t nent,
+__global__ void k_sample_scatter(const int* __restrict__ list, SortedKV kv, const int* __restrict__ rep,
+                                 const int* __restrict__ hpos, int64_t nent,
                                  const void* __restrict__ vals, int rsz, void* __restrict__ samples) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nent) return;
-    const int e = val[j];
+    const int e = kv.val(j);
     const int64_t dst = (int64_t)list[e / 27] * 27 + e % 27;
     const int64_t src = hpos[rep[j]];
     if (rsz == 8) ((double*)samples)[dst] = ((const double*)vals)[src];
@@ -600,14 +609,14 @@ void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* l
 }
 
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
-                               int cellmode, uint64_t* key, int* val, hipStream_t s) {
+                               int cellmode, int vbits, uint64_t* key, int* val, hipStream_t s) {
     if (nent <= 0) return;
     k_sample_keys<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(g, lv, list, nent, origin[0], origin[1], origin[2],
-                                                                 cellmode, key, val);
+                                                                 cellmode, vbits, key, val);
 }
 
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
-                               const int* list, const uint64_t* key, const int* val, int64_t nent, int* head,
+                               const int* list, SortedKV kv, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
                                hipEvent_t e1, hipStream_t s) {
@@ -616,7 +625,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     *kx_out = 0;
     if (nent <= 0) return;
     const unsigned nb = (unsigned)((nent + 1 + 255) / 256);
-    k_sample_heads<<<nb, 256, 0, s>>>(lv, list, key, val, nent, head, rep);
+    k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, nent, head, rep);
     exclusive_scan_i32(head, hpos, nent + 1, tmp, s);
     k_head_list<<<nb, 256, 0, s>>>(head, hpos, nent, hlist);
     int nh = 0;
@@ -624,7 +633,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     *nheads_out = nh;
     int* rstart = nullptr;
     int kx = 0, T = kRunThreads;
-    const int nr = build_runs(g, key, hlist, nh, cwork, tmp, &rstart, &kx, &T, s);
+    const int nr = build_runs(g, kv, hlist, nh, cwork, tmp, &rstart, &kx, &T, s);
     *nruns_out = nr;
     *kx_out = kx;
     (void)hipEventRecord(e0, s);
@@ -633,15 +642,15 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
             constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
             if (prec == 2) {
                 Recs<double> r{(const double4*)ra, (const double4*)rg};
-                k_run_samples<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                k_run_samples<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart,
                                                                       (double*)vals, ctr);
             } else {
                 Recs<float> r{(const float4*)ra, (const float4*)rg};
                 if (prec == 1)
-                    k_run_samples<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                    k_run_samples<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart,
                                                                         (float*)vals, ctr);
                 else
-                    k_run_samples<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, hlist, rstart,
+                    k_run_samples<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart,
                                                                          (float*)vals, ctr);
             }
         };
@@ -652,7 +661,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
         else m1 ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
     }
     (void)hipEventRecord(e1, s);
-    k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, val, rep, hpos, nent, vals,
+    k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, kv, rep, hpos, nent, vals,
                                                                     prec == 2 ? 8 : 4, samples);
 }
 
@@ -677,25 +686,26 @@ void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, Fi
     const uint64_t groups = (uint64_t)(g.n[0] + 2) * (uint64_t)(g.n[1] + 2) * (uint64_t)(g.n[2] + 2);
     int bits = 1;
     while (bits < 55 && ((uint64_t)1 << bits) <= groups) bits++;
-    k_center_keys<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(g, lv, list, k, key, val);
-    radix_sort_pairs64(key, val, key2, val2, k, bits + 9, tmp, s);
+    const int vbits = pack_vbits(bits + 9, k);
+    k_center_keys<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(g, lv, list, k, vbits, key, val);
+    const SortedKV kv = radix_sort_entries(key, val, key2, val2, k, bits + 9, vbits, tmp, s);
     int* rstart = nullptr;
     int kx = 0, T = kRunThreads;
-    const int nr = build_runs(g, key, nullptr, (int)k, cwork, tmp, &rstart, &kx, &T, s);
+    const int nr = build_runs(g, kv, nullptr, (int)k, cwork, tmp, &rstart, &kx, &T, s);
     if (nr <= 0) return;
     auto go = [&](auto tag, auto mode) {
         constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
         if (prec == 2) {
             Recs<double> r{(const double4*)ra, (const double4*)rg};
-            k_run_centers<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart,
+            k_run_centers<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
                                                                   (double*)samples, ctr);
         } else {
             Recs<float> r{(const float4*)ra, (const float4*)rg};
             if (prec == 1)
-                k_run_centers<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart,
+                k_run_centers<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
                                                                     (float*)samples, ctr);
             else
-                k_run_centers<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, val, rstart,
+                k_run_centers<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
                                                                      (float*)samples, ctr);
         }
     };

@@ -16,6 +16,26 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_
 int64_t radix_tmp_ints(int64_t n);
 void radix_sort_pairs(uint32_t* k0, int* v0, uint32_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s);
 void radix_sort_pairs64(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_t n, int bits, int* tmp, hipStream_t s);
+// Sorted entries: packed words (vbits > 0: the key above vbits bits of value, v unused) or
+// separate key / value arrays (vbits = 0). Packed, a sort pass makes one scattered store per
+// entry instead of two.
+struct SortedKV {
+    const uint64_t* k;
+    const int* v;
+    int vbits;
+    __device__ __forceinline__ uint64_t key(int64_t i) const { return vbits ? (k[i] >> vbits) : k[i]; }
+    __device__ __forceinline__ int val(int64_t i) const {
+        return vbits ? (int)(k[i] & ((1ull << vbits) - 1)) : v[i];
+    }
+};
+// value bits to pack beside a key of `bits` bits for values 0 .. n - 1, or 0 when they do not fit
+// one word
+int pack_vbits(int bits, int64_t n);
+// stable sort of n entries by the low `bits` bits of their key. vbits > 0: k0 holds packed words
+// with keys below 2^bits, k1 is the ping-pong buffer, and the result stays where the last pass put
+// it (k0 or k1); vbits = 0: key / value arrays, result in k0 / v0.
+SortedKV radix_sort_entries(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_t n, int bits, int vbits, int* tmp,
+                            hipStream_t s);
 
 // ---- read-backs ----
 // wait for the stream's work so far by polling an event (no driver sleep)
@@ -126,9 +146,9 @@ void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec
 // between events e0 / e1, and the scatter to samples. cwork holds 4 (nent + 2) ints; kx_out is the
 // number of cells of a row per run.
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
-                               int cellmode, uint64_t* key, int* val, hipStream_t s);
+                               int cellmode, int vbits, uint64_t* key, int* val, hipStream_t s);
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
-                               const int* list, const uint64_t* key, const int* val, int64_t nent, int* head,
+                               const int* list, SortedKV kv, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
                                hipEvent_t e1, hipStream_t s);
@@ -151,8 +171,10 @@ struct TreeView {
     const int* child[kMaxLevels];
 };
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
-void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, double4* leaf_node, uint64_t* ckey,
-                      int* cval, int* n_valid /* 64 stripes, 32 ints apart */, hipStream_t s);
+// corner keys: Morton code of the corner (kbits bits; all ones outside the finest lattice), packed
+// with the entry index when vbits > 0
+void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, int kbits, int vbits, double4* leaf_node,
+                      uint64_t* ckey, int* cval, int* n_valid /* 64 stripes, 32 ints apart */, hipStream_t s);
 // hash-table slot: a key and the smallest index that inserted it
 struct alignas(16) HashSlot {
     unsigned long long key;
@@ -171,15 +193,14 @@ void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s);
 void launch_head_positions(const int* head, const int* hpos, int64_t n, int* list, hipStream_t s);
 // point t (first sorted entry heads[t]): dflag[t] = 1 when its dual cell is non-trivial, the cell
 // goes to stage[9 t ..]
-void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, const uint64_t* skey,
-                   const int* sval, int64_t nent, const int* heads, int64_t npts, int* dflag, int* stage,
-                   hipStream_t s);
+void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, SortedKV kv, int64_t nent,
+                   const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s);
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s);
 void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s);
 void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t* vkey, int* vval, int* vend,
                         hipStream_t s);
-void launch_run_heads(const uint64_t* key, int64_t n, int* head, hipStream_t s);
+void launch_run_heads(SortedKV kv, int64_t n, int* head, hipStream_t s);
 void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s);
 void launch_tri_key_lo(const int* cv, const int* perm, int64_t nt, uint64_t* k, int* v, hipStream_t s);
 void launch_tri_keep(const int* cv, const int* order, int64_t nt, int* keep, hipStream_t s);
