@@ -634,7 +634,13 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
         const double w = (2 * half + 2 * (double)tp.margin) / (double)c->g.cell + 1;
         group = (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
     }
-    launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group, c->stream);
+    // At D_min, where most nodes lie inside the fluid and their samples do not change sign, the
+    // frame tests emptiness first and takes the curvature after sampling, for the nodes whose
+    // samples change sign; deeper levels hold the children of such nodes, and mixed API sweeps,
+    // do both at once. (Above D_min classification reads no curvature.)
+    const bool split_curv = level_depth >= 0 && level_depth == c->dmin;
+    launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group,
+                      split_curv ? kEmptyOnly : kEmptyAndCurv, nullptr, 0, c->stream);
     // the level's flag / scanned / child arrays serve as the sample list until classification
     HIPCHK(c, c->cl_flag.ensure((L->n + 1) * 4));
     HIPCHK(c, c->cl_pos.ensure((L->n + 1) * 4));
@@ -716,6 +722,9 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
                                 c->stream);
         }
     }
+    if (split_curv)
+        launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group, kCurvOnly, c->samples.p,
+                          (int)rsz, c->stream);
     launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, sc_dev, c->ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
     return PISURF_OK;

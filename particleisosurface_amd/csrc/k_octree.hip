@@ -26,13 +26,49 @@ __device__ __forceinline__ bool leaf_depth(float half, float r) {
 // that range (one contiguous window of sorted positions per row). A particle counts when it lies
 // strictly inside the unclamped box grown by r * smooth. Nodes above D_min only need emptiness and
 // stop at the first hit.
-template <int G>
+// nodes whose 27 samples are needed: non-empty, depth >= D_min, and not at leaf depth unless the
+// QEF consumes them (the shipped build samples leaf-depth nodes but never uses the result)
+__device__ __forceinline__ bool wants_samples(const NodeLevel& lv, const TreeParams& tp, int64_t node) {
+    if (lv.empty[node]) return false;
+    if (lv.depth[node] < tp.depth_min) return false;
+    if (tp.qef_mode == 0 && leaf_depth(lv.c[node].w, tp.radius)) return false;
+    return true;
+}
+
+// leaf-depth nodes that need their centre value and have no samples (QEF off)
+__device__ __forceinline__ bool wants_center(const NodeLevel& lv, const TreeParams& tp, int64_t node) {
+    return !lv.empty[node] && tp.qef_mode == 0 && leaf_depth(lv.c[node].w, tp.radius) &&
+           lv.depth[node] >= tp.depth_min;
+}
+
+template <int G, int PASS>
 __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __restrict__ spos,
-                                                    const float4* __restrict__ nrm, TreeParams tp, NodeLevel lv) {
+                                                    const float4* __restrict__ nrm, TreeParams tp, NodeLevel lv,
+                                                    const void* __restrict__ samples, int rsz) {
     const int64_t node = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const int lane = threadIdx.x % G;
     const int gbase = (threadIdx.x & 63) - lane;  // first lane of this group within the wave
     if (node >= lv.n) return;
+    if (PASS == kCurvOnly) {
+        // classification reads curv only for sampled nodes whose samples change sign (the exits
+        // below are uniform over the node's lanes)
+        if (!tp.curv || !wants_samples(lv, tp, node)) return;
+        bool sc = false;
+        if (rsz == 8) {
+            const double* f = (const double*)samples + node * 27;
+            const bool s0 = f[0] >= 0.0;
+            for (int k = 1; k < 27; k++) sc = sc || ((f[k] >= 0.0) != s0);
+        } else {
+            const float* f = (const float*)samples + node * 27;
+            const bool s0 = f[0] >= 0.0f;
+            for (int k = 1; k < 27; k++) sc = sc || ((f[k] >= 0.0f) != s0);
+        }
+        if (!sc) return;
+        if (!tp.calc_normals) {  // no normals: curv = 1 (no normal length)
+            if (lane == 0) lv.curv[node] = 1.0f;
+            return;
+        }
+    }
     const float4 c = lv.c[node];
     const int depth = lv.depth[node];
     const float cc[3] = {c.x, c.y, c.z};
@@ -44,7 +80,8 @@ __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __r
         bc1[a] = fmaxf(b1[a], tp.b_lo[a]);
         bc2[a] = fminf(b2[a], tp.b_hi[a]);
     }
-    const bool need_curv = tp.calc_normals && tp.curv && depth >= tp.depth_min;
+    const bool need_curv = PASS == kCurvOnly || (PASS == kEmptyAndCurv && tp.calc_normals && tp.curv &&
+                                                 depth >= tp.depth_min);
     float lx[3], hx[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -97,25 +134,11 @@ __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __r
         area += __shfl_xor(area, o, kWave);
     }
     if (lane == 0) {
-        lv.empty[node] = any ? 0 : 1;
-        lv.curv[node] = (area == 0.f) ? 1.0f : sqrtf(sx * sx + sy * sy + sz * sz) / area;
+        if (PASS != kCurvOnly) lv.empty[node] = any ? 0 : 1;
+        if (PASS != kEmptyOnly) lv.curv[node] = (area == 0.f) ? 1.0f : sqrtf(sx * sx + sy * sy + sz * sz) / area;
     }
 }
 
-// nodes whose 27 samples are needed: non-empty, depth >= D_min, and not at leaf depth unless the
-// QEF consumes them (the shipped build samples leaf-depth nodes but never uses the result)
-__device__ __forceinline__ bool wants_samples(const NodeLevel& lv, const TreeParams& tp, int64_t node) {
-    if (lv.empty[node]) return false;
-    if (lv.depth[node] < tp.depth_min) return false;
-    if (tp.qef_mode == 0 && leaf_depth(lv.c[node].w, tp.radius)) return false;
-    return true;
-}
-
-// leaf-depth nodes that need their centre value and have no samples (QEF off)
-__device__ __forceinline__ bool wants_center(const NodeLevel& lv, const TreeParams& tp, int64_t node) {
-    return !lv.empty[node] && tp.qef_mode == 0 && leaf_depth(lv.c[node].w, tp.radius) &&
-           lv.depth[node] >= tp.depth_min;
-}
 
 // compaction of the nodes to evaluate: flag, then (after a scan) the list. A listed node gets its
 // 27 samples, or only its centre value (stored in sample slot 13) when it sits at leaf depth.
@@ -589,13 +612,19 @@ __global__ void k_sample_grads(TreeParams tp, NodeLevel lv, const R* __restrict_
 }
 
 void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, int group,
-                       hipStream_t s) {
+                       int pass, const void* samples, int rsz, hipStream_t s) {
     if (lv.n <= 0) return;
     const unsigned nb8 = (unsigned)((lv.n * 8 + 255) / 256), nb16 = (unsigned)((lv.n * 16 + 255) / 256);
     const unsigned nb64 = (unsigned)((lv.n * kWave + 255) / 256);
-    if (group <= 8) k_empty_curv<8><<<nb8, 256, 0, s>>>(g, spos, nrm, tp, lv);
-    else if (group <= 16) k_empty_curv<16><<<nb16, 256, 0, s>>>(g, spos, nrm, tp, lv);
-    else k_empty_curv<64><<<nb64, 256, 0, s>>>(g, spos, nrm, tp, lv);
+    auto go = [&](auto tag) {
+        constexpr int P = decltype(tag)::value;
+        if (group <= 8) k_empty_curv<8, P><<<nb8, 256, 0, s>>>(g, spos, nrm, tp, lv, samples, rsz);
+        else if (group <= 16) k_empty_curv<16, P><<<nb16, 256, 0, s>>>(g, spos, nrm, tp, lv, samples, rsz);
+        else k_empty_curv<64, P><<<nb64, 256, 0, s>>>(g, spos, nrm, tp, lv, samples, rsz);
+    };
+    if (pass == kEmptyOnly) go(std::integral_constant<int, kEmptyOnly>());
+    else if (pass == kCurvOnly) go(std::integral_constant<int, kCurvOnly>());
+    else go(std::integral_constant<int, kEmptyAndCurv>());
 }
 
 void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,

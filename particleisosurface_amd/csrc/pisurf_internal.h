@@ -126,9 +126,13 @@ struct TreeParams {
     int qef_mode;
     float qef_svd_tol;
 };
-// group: lanes per node (8, 16 or 64), about the number of (y, z) cell rows a node box spans
+// group: lanes per node (8, 16 or 64), about the number of (y, z) cell rows a node box spans.
+// pass: kEmptyAndCurv both; kEmptyOnly emptiness alone (stops at the first particle found);
+// kCurvOnly the curvature of the sampled nodes whose 27 samples (R = rsz bytes) change sign, the
+// only nodes whose classification reads it.
+constexpr int kEmptyAndCurv = 0, kEmptyOnly = 1, kCurvOnly = 2;
 void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, int group,
-                       hipStream_t s);
+                       int pass, const void* samples, int rsz, hipStream_t s);
 // flag[n+1], pos[n+1] (pos[n] = number of listed nodes), list[n]: nodes to sample; cflag / cpos /
 // clist: leaf-depth nodes that need only their centre value
 void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,
