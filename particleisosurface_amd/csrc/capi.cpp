@@ -1051,12 +1051,12 @@ int do_extract_device(pisurf_ctx* c) {
     // about six corners per distinct vertex on a surface: a table of 1.25 nc slots or more stays
     // sparse, and holds every key even if all were distinct
     const int64_t vtab = pow2_at_least(nc + nc / 4 + 1);
-    HIPCHK(c, E[15].ensure(vtab * sizeof(HashSlot)));
+    HIPCHK(c, E[15].ensure(vtab * sizeof(unsigned)));
     HIPCHK(c, E[18].ensure((nc + 1) * 4));
     HIPCHK(c, E[19].ensure((nc + 1) * 4));
     if ((rc = ensure_tmp(c, nc + 1))) return rc;
     int* vslot = E[14].as<int>();  // the corner indices written there are implicit
-    launch_hash_dedupe(E[13].as<uint64_t>(), nc, E[15].as<HashSlot>(), vtab, vslot, E[18].as<int>(), s);
+    launch_hash_dedupe(E[13].as<uint64_t>(), nc, E[15].as<unsigned>(), vtab, vslot, E[18].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[18].as<int>() + nc, 0, 4, s));
     exclusive_scan_i32(E[18].as<int>(), E[19].as<int>(), nc + 1, c->tmp.as<int>(), s);
     int64_t nv;
@@ -1064,7 +1064,7 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[20].ensure(std::max<int64_t>(nc, 1) * 4));
     HIPCHK(c, E[21].ensure(std::max<int64_t>(nv, 1) * 8 * 8));
     int* cvid = E[20].as<int>();
-    launch_vertex_ids_hashed(vslot, E[15].as<HashSlot>(), E[19].as<int>(), nc, E[17].as<int>(), leaf_node, cvid,
+    launch_vertex_ids_hashed(vslot, E[15].as<unsigned>(), E[19].as<int>(), nc, E[17].as<int>(), leaf_node, cvid,
                              E[21].as<double>(), s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, E[22].ensure(std::max<int64_t>(nv, 1) * 3 * 8));
@@ -1081,9 +1081,9 @@ int do_extract_device(pisurf_ctx* c) {
         const int64_t ttab = pow2_at_least(2 * ntk);
         HIPCHK(c, E[3].ensure(std::max<int64_t>(ntk, 1) * 8));
         HIPCHK(c, E[4].ensure(std::max<int64_t>(ntk, 1) * 4));
-        HIPCHK(c, E[5].ensure(ttab * sizeof(HashSlot)));
+        HIPCHK(c, E[5].ensure(ttab * sizeof(unsigned)));
         launch_tri_key63(cvid, nt, E[3].as<uint64_t>(), s);
-        launch_hash_dedupe(E[3].as<uint64_t>(), nt, E[5].as<HashSlot>(), ttab, E[4].as<int>(), E[8].as<int>(), s);
+        launch_hash_dedupe(E[3].as<uint64_t>(), nt, E[5].as<unsigned>(), ttab, E[4].as<int>(), E[8].as<int>(), s);
     } else {
         // larger meshes: two stable sorts (largest id, then the lower pair) group equal triples
         HIPCHK(c, E[3].ensure(std::max<int64_t>(ntk, 1) * 8));

@@ -208,11 +208,11 @@ __global__ void k_run_heads(SortedKV kv, int64_t n, int* __restrict__ head) {
 }
 
 // ---- hash-table dedupe (first occurrence wins, as Mesh::insert_vert / insert_tri do) ----
-// Open addressing with linear probing over a power-of-two table of more entries than keys, so
-// every probe sequence ends. A slot holds the key and the smallest index that inserted it in one
-// 16-byte record, so an insert touches one cache line. Keys never equal the empty marker (their
-// top bit is clear).
-constexpr unsigned long long kHashEmpty = ~0ull;
+// Open addressing with linear probing over a power-of-two table of more slots than keys, so every
+// probe sequence ends. A slot holds only the smallest index that inserted its key; an insert
+// compares keys through key[table[h]], and takes one atomic (the claim of an empty slot, or the
+// minimum on a found one) instead of a key exchange followed by a minimum.
+constexpr unsigned kSlotEmpty = 0xffffffffu;
 __device__ __forceinline__ uint64_t mix64(uint64_t k) {  // SplitMix64 finaliser
     k ^= k >> 30;
     k *= 0xbf58476d1ce4e5b9ull;
@@ -221,37 +221,46 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k) {  // SplitMix64 finaliser
     k ^= k >> 31;
     return k;
 }
-// slot[i] = table slot of key[i]; tmin[slot] = smallest i with that key
-__global__ void k_hash_insert(const uint64_t* __restrict__ key, int64_t n, HashSlot* __restrict__ table,
-                              uint64_t mask, int* __restrict__ slot) {
+// slot[i] = table slot of key[i]; table[slot] = smallest i with that key
+__global__ void k_hash_insert(const uint64_t* __restrict__ key, int64_t n, unsigned* table, uint64_t mask,
+                              int* __restrict__ slot) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const unsigned long long k = key[i];
+    const uint64_t k = key[i];
     uint64_t h = mix64(k) & mask;
     for (uint64_t probe = 0; probe <= mask; probe++) {
-        const unsigned long long prev = atomicCAS(&table[h].key, kHashEmpty, k);
-        if (prev == kHashEmpty || prev == k) {
-            atomicMin(&table[h].min, (unsigned)i);
+        // a plain read may be stale: an empty slot is claimed only through the exchange, and a
+        // slot's index only decreases among indices of one key
+        unsigned cur = table[h];
+        if (cur == kSlotEmpty) {
+            cur = atomicCAS(&table[h], kSlotEmpty, (unsigned)i);
+            if (cur == kSlotEmpty) {
+                slot[i] = (int)h;
+                return;
+            }
+        }
+        if (key[cur] == k) {
+            if (cur > (unsigned)i) atomicMin(&table[h], (unsigned)i);
             slot[i] = (int)h;
             return;
         }
         h = (h + 1) & mask;
     }
 }
-__global__ void k_hash_first(const int* __restrict__ slot, const HashSlot* __restrict__ table, int64_t n,
+__global__ void k_hash_first(const int* __restrict__ slot, const unsigned* __restrict__ table, int64_t n,
                              int* __restrict__ first) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) first[i] = (table[slot[i]].min == (unsigned)i) ? 1 : 0;
+    if (i < n) first[i] = (table[slot[i]] == (unsigned)i) ? 1 : 0;
 }
 // vertex id of every corner (rank of its key's first corner) and the bisection edge of each vertex,
 // oriented as its first corner's dual cell has it
-__global__ void k_vertex_ids_hashed(const int* __restrict__ slot, const HashSlot* __restrict__ table,
+__global__ void k_vertex_ids_hashed(const int* __restrict__ slot, const unsigned* __restrict__ table,
                                     const int* __restrict__ rank, int64_t n, const int* __restrict__ vend,
                                     const double4* __restrict__ leaf_node, int* __restrict__ corner_vid,
                                     double* __restrict__ ends) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const unsigned f = table[slot[c]].min;
+    const unsigned f = table[slot[c]];
     const int vid = rank[f];
     corner_vid[c] = vid;
     if (f == (unsigned)c) {
@@ -328,14 +337,14 @@ void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, int kb
     if (lv.n > 0)
         k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, kbits, vbits, leaf_node, ckey, cval, n_valid);
 }
-void launch_hash_dedupe(const uint64_t* key, int64_t n, HashSlot* table, int64_t tsize, int* slot, int* first,
+void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned* table, int64_t tsize, int* slot, int* first,
                         hipStream_t s) {
     if (n <= 0) return;
-    (void)hipMemsetAsync(table, 0xff, tsize * sizeof(HashSlot), s);  // empty keys, no index
+    (void)hipMemsetAsync(table, 0xff, tsize * sizeof(unsigned), s);  // every slot empty
     k_hash_insert<<<nblk(n), 256, 0, s>>>(key, n, table, (uint64_t)(tsize - 1), slot);
     k_hash_first<<<nblk(n), 256, 0, s>>>(slot, table, n, first);
 }
-void launch_vertex_ids_hashed(const int* slot, const HashSlot* table, const int* rank, int64_t n, const int* vend,
+void launch_vertex_ids_hashed(const int* slot, const unsigned* table, const int* rank, int64_t n, const int* vend,
                               const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s) {
     if (n > 0) k_vertex_ids_hashed<<<nblk(n), 256, 0, s>>>(slot, table, rank, n, vend, leaf_node, corner_vid, ends);
 }

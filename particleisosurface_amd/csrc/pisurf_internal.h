@@ -179,18 +179,12 @@ void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
 // with the entry index when vbits > 0
 void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, int kbits, int vbits, double4* leaf_node,
                       uint64_t* ckey, int* cval, int* n_valid /* 64 stripes, 32 ints apart */, hipStream_t s);
-// hash-table slot: a key and the smallest index that inserted it
-struct alignas(16) HashSlot {
-    unsigned long long key;
-    unsigned min;
-    unsigned pad;
-};
 // first[i] = 1 when i is the smallest index with key[i] (open-addressing table of tsize slots, a
 // power of two above the number of distinct keys, with room to spare; slot[i] = slot of key[i],
-// table[slot].min = smallest index of the key)
-void launch_hash_dedupe(const uint64_t* key, int64_t n, HashSlot* table, int64_t tsize, int* slot, int* first,
+// table[slot] = smallest index of the key; n < 2^32 - 1)
+void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned* table, int64_t tsize, int* slot, int* first,
                         hipStream_t s);
-void launch_vertex_ids_hashed(const int* slot, const HashSlot* table, const int* rank, int64_t n, const int* vend,
+void launch_vertex_ids_hashed(const int* slot, const unsigned* table, const int* rank, int64_t n, const int* vend,
                               const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s);
 void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s);
 // list[hpos[j]] = j for every j with head[j] != 0
