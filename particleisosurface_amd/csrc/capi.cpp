@@ -243,7 +243,7 @@ struct pisurf_ctx {
     hipEvent_t eb0 = nullptr, eb1 = nullptr;
     std::vector<hipEvent_t> ev_chunk;  // ends of the bisection chunks
     // device extraction workspace
-    DBuf eb[28];
+    DBuf eb[29];
     // mesh
     PinnedVec<double> mesh_v;
     PinnedVec<int32_t> mesh_t;
@@ -995,12 +995,13 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[7].ensure(64 * 128));
     HIPCHK(c, hipMemsetAsync(E[7].p, 0, 64 * 128, s));
     double4* leaf_node = E[2].as<double4>();
+    HIPCHK(c, E[28].ensure(std::max<int64_t>(nleaf, 1)));
     // corners outside the finest lattice carry all-ones keys and sort after the nent valid ones
     const int ckbits = std::min(64, 3 * tv.dp + 1);
     const int cvbits = pack_vbits(ckbits, nent_all);
     for (int L = 0; L < tv.nlev; L++)
-        launch_leaf_emit(c->levels[L]->view(), tv.off[L], lidx, tv.dp, ckbits, cvbits, leaf_node, E[3].as<uint64_t>(),
-                         E[4].as<int>(), E[7].as<int>(), s);
+        launch_leaf_emit(c->levels[L]->view(), tv.off[L], lidx, tv.dp, ckbits, cvbits, leaf_node, E[28].as<uint8_t>(),
+                         E[3].as<uint64_t>(), E[4].as<int>(), E[7].as<int>(), s);
     if ((rc = ensure_tmp(c, nent_all + 1))) return rc;
     const SortedKV ckv = radix_sort_entries(E[3].as<uint64_t>(), E[4].as<int>(), E[5].as<uint64_t>(), E[6].as<int>(),
                                             nent_all, ckbits, cvbits, c->tmp.as<int>(), s);
@@ -1025,7 +1026,8 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[26].ensure((npts + 1) * 4));
     HIPCHK(c, E[27].ensure((npts + 1) * 4));
     HIPCHK(c, E[24].ensure(std::max<int64_t>(npts, 1) * 9 * 4));
-    launch_points(tv, lidx, leaf_node, ckv, nent, E[25].as<int>(), npts, E[26].as<int>(), E[24].as<int>(), s);
+    launch_points(tv, lidx, E[28].as<uint8_t>(), ckv, nent, E[25].as<int>(), npts, E[26].as<int>(), E[24].as<int>(),
+                  s);
     HIPCHK(c, hipMemsetAsync(E[26].as<int>() + npts, 0, 4, s));
     exclusive_scan_i32(E[26].as<int>(), E[27].as<int>(), npts + 1, c->tmp.as<int>(), s);
     int64_t nd;

@@ -48,13 +48,15 @@ __global__ void k_leaf_flag(NodeLevel lv, int64_t off, int* __restrict__ flag) {
 // leaf records and the 8 corner keys of every leaf (corners on the root boundary get the
 // sentinel key ~0 and sort last; valid keys use 3 * dp bits)
 __global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp, int kbits, int vbits,
-                            double4* __restrict__ leaf_node, uint64_t* __restrict__ ckey, int* __restrict__ cval,
-                            int* __restrict__ n_valid) {
+                            double4* __restrict__ leaf_node, uint8_t* __restrict__ leaf_pos,
+                            uint64_t* __restrict__ ckey, int* __restrict__ cval, int* __restrict__ n_valid) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int valid = 0;
     if (i < lv.n && is_leaf_type(lv.type[i])) {
         const int L = lidx[off + i];
-        leaf_node[L] = lv.node[i];
+        const double4 nd = lv.node[i];
+        leaf_node[L] = nd;
+        leaf_pos[L] = nd.w > 0.0 ? 1 : 0;  // sign of the leaf value, read by the point pass
         const int4 p = lv.ip[i];
         const int sh = dp - p.w;
         const int64_t hi = ((int64_t)1 << dp) - 1;
@@ -123,25 +125,25 @@ __global__ void k_head_positions(const int* __restrict__ head, const int* __rest
     if (j < n && head[j]) list[hpos[j]] = (int)j;
 }
 
-__device__ __forceinline__ int sign_index(const double4* leaf_node, const int n[8]) {
+__device__ __forceinline__ int sign_index(const uint8_t* leaf_pos, const int n[8]) {
     int s = 0;
 #pragma unroll
     for (int o = 0; o < 8; o++)
-        if (leaf_node[n[o]].w > 0.0) s |= 1 << o;
+        if (leaf_pos[n[o]]) s |= 1 << o;
     return s;
 }
 
 // one thread per distinct point (heads[t] = its first sorted entry): flag points whose dual cell
 // is non-trivial and stage the cell (8 leaves in case order, sign configuration) at slot t;
 // k_points_compact then packs the staged cells
-__global__ void k_points(TreeView tv, const int* __restrict__ lidx, const double4* __restrict__ leaf_node,
+__global__ void k_points(TreeView tv, const int* __restrict__ lidx, const uint8_t* __restrict__ leaf_pos,
                          SortedKV kv, int64_t nent, const int* __restrict__ heads, int64_t npts,
                          int* __restrict__ dflag, int* __restrict__ stage) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npts) return;
     int n[8];
     gather_point(tv, lidx, kv, nent, heads[t], n);
-    const int s = sign_index(leaf_node, n);
+    const int s = sign_index(leaf_pos, n);
     const uint8_t code = d_kDmcCase[s];
     const bool keep = code != 0 && code != 255;
     dflag[t] = keep ? 1 : 0;
@@ -333,9 +335,10 @@ void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s) {
     if (lv.n > 0) k_leaf_flag<<<nblk(lv.n), 256, 0, s>>>(lv, off, flag);
 }
 void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, int kbits, int vbits, double4* leaf_node,
-                      uint64_t* ckey, int* cval, int* n_valid, hipStream_t s) {
+                      uint8_t* leaf_pos, uint64_t* ckey, int* cval, int* n_valid, hipStream_t s) {
     if (lv.n > 0)
-        k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, kbits, vbits, leaf_node, ckey, cval, n_valid);
+        k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, kbits, vbits, leaf_node, leaf_pos, ckey, cval,
+                                               n_valid);
 }
 void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned* table, int64_t tsize, int* slot, int* first,
                         hipStream_t s) {
@@ -354,9 +357,9 @@ void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s) {
 void launch_head_positions(const int* head, const int* hpos, int64_t n, int* list, hipStream_t s) {
     if (n > 0) k_head_positions<<<nblk(n), 256, 0, s>>>(head, hpos, n, list);
 }
-void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, SortedKV kv, int64_t nent,
+void launch_points(const TreeView& tv, const int* lidx, const uint8_t* leaf_pos, SortedKV kv, int64_t nent,
                    const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s) {
-    if (npts > 0) k_points<<<nblk(npts), 256, 0, s>>>(tv, lidx, leaf_node, kv, nent, heads, npts, dflag, stage);
+    if (npts > 0) k_points<<<nblk(npts), 256, 0, s>>>(tv, lidx, leaf_pos, kv, nent, heads, npts, dflag, stage);
 }
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s) {

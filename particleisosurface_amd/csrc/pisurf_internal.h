@@ -177,8 +177,10 @@ struct TreeView {
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
 // corner keys: Morton code of the corner (kbits bits; all ones outside the finest lattice), packed
 // with the entry index when vbits > 0
+// leaf_node[L] = vertex and value of leaf L, leaf_pos[L] = (value > 0)
 void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, int kbits, int vbits, double4* leaf_node,
-                      uint64_t* ckey, int* cval, int* n_valid /* 64 stripes, 32 ints apart */, hipStream_t s);
+                      uint8_t* leaf_pos, uint64_t* ckey, int* cval, int* n_valid /* 64 stripes, 32 ints apart */,
+                      hipStream_t s);
 // first[i] = 1 when i is the smallest index with key[i] (open-addressing table of tsize slots, a
 // power of two above the number of distinct keys, with room to spare; slot[i] = slot of key[i],
 // table[slot] = smallest index of the key; n < 2^32 - 1)
@@ -191,7 +193,7 @@ void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s);
 void launch_head_positions(const int* head, const int* hpos, int64_t n, int* list, hipStream_t s);
 // point t (first sorted entry heads[t]): dflag[t] = 1 when its dual cell is non-trivial, the cell
 // goes to stage[9 t ..]
-void launch_points(const TreeView& tv, const int* lidx, const double4* leaf_node, SortedKV kv, int64_t nent,
+void launch_points(const TreeView& tv, const int* lidx, const uint8_t* leaf_pos, SortedKV kv, int64_t nent,
                    const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s);
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s);
