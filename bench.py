@@ -28,12 +28,14 @@ BYTES_PER_CAND = {capi.FP32: 52, capi.FP32_KAHAN: 52, capi.FP64: 104}
 BYTES_FIXED = 16 + 27 * 8
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    (profiles/*_traffic.md, lines `traffic_bytes_per_launch <kernel> = <bytes>`), or None."""
+def measured_traffic(kernel, workload_tag):
+    """HBM bytes per launch of `kernel` on the workload tagged `workload_tag` (e.g. crown_splash_4M)
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    profiles/<kernel>_<workload_tag>_traffic.md (lines `traffic_bytes_per_launch <kernel> = <bytes>`),
+    or None when no summary covers that kernel on that workload."""
     import glob
     import re
-    for f in sorted(glob.glob(os.path.join(HERE, "profiles", "*_traffic.md"))):
+    for f in sorted(glob.glob(os.path.join(HERE, "profiles", "%s_%s_traffic.md" % (kernel, workload_tag)))):
         for line in open(f):
             m = re.match(r"traffic_bytes_per_launch (\w+) = (\d+)", line.strip())
             if m and m.group(1) == kernel:
@@ -211,10 +213,16 @@ def main():
     if args.workload == "pour":
         xyz, r = synth.multibody_pour()
         wname = "multi-body pour (config 5), %d particles, r=0.0005, pool 46 layers" % len(xyz)
+        wtag = "multibody_pour_12M"
     else:
         xyz, r = synth.crown_splash(pool_layers=args.pool_layers)
         wname = "crown splash (config 3), %d particles, r=0.001, pool %d layers" % (len(xyz), args.pool_layers)
+        wtag = "crown_splash_4M" if args.pool_layers == 42 else "crown_splash_%dlayers" % args.pool_layers
     depth = args.depth_max if args.depth_max >= 0 else (12 if args.workload == "pour" else 0)
+    if args.precision != "fp32k":  # the committed traffic summaries cover compensated FP32
+        wtag += "_" + args.precision
+    if args.depth_max >= 0:
+        wtag += "_depth%d" % args.depth_max
     log("workload: %s (%.1f s to generate)" % (wname, time.perf_counter() - t))
 
     prec = {"fp32": capi.FP32, "fp32k": capi.FP32_KAHAN, "fp64": capi.FP64}[args.precision]
@@ -242,7 +250,7 @@ def main():
         k_ms = ms / max(launches, 1)
         alg = evals * BYTES_FIXED + cands * BYTES_PER_CAND[prec]
         achieved = alg / max(launches, 1) / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
-        traffic, source = measured_traffic(name)
+        traffic, source = measured_traffic(name, wtag)
         return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=source, kernel=name,
                     launches_per_frame=launches / args.steps, avg_launch_ms=round(k_ms, 4),
