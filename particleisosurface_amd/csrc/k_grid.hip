@@ -81,10 +81,8 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_
 }
 
 // ---------------------------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits. Per tile (16 rounds of 256 keys): digit histogram, a
-// global digit-major exclusive scan of the per-tile histograms, then a stable scatter where the
-// rank of a key inside its round comes from wave64 ballots (lanes with equal digit) and a
-// per-digit prefix over the 4 waves.
+// Stable LSD radix sort, 8-bit digits. Per tile of 4096 keys: digit histogram, a global
+// digit-major exclusive scan of the per-tile histograms, then a stable scatter (below).
 // ---------------------------------------------------------------------------------------------
 constexpr int kRsT = 256, kRsRounds = 16, kRsTile = kRsT * kRsRounds;
 
@@ -103,24 +101,38 @@ __global__ void __launch_bounds__(kRsT) k_rs_hist(const KT* __restrict__ keys, i
     bh[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Scatter of one tile: wave w takes the tile's keys [w * 1024, (w + 1) * 1024) in 16 rounds of 64.
+// Within a round, lanes of equal digit find each other with 8 ballots; a wave's running count per
+// digit lives in LDS (the one leader lane per digit advances it), so a key's offset among its
+// wave's keys of that digit needs no barrier. The per-wave counts then give every key its place
+// in the tile sorted by digit; the tile is staged in LDS in that order and written out by
+// consecutive threads, so each digit's run of keys goes to consecutive addresses.
 template <class KT, bool VAL>
 __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin, const int* __restrict__ vin,
                                                      KT* __restrict__ kout, int* __restrict__ vout, int64_t n,
                                                      int shift, const int* __restrict__ bscan, int nblocks) {
     __shared__ int base[256];
-    __shared__ int wc[4][256];
-    __shared__ int tot[256];
+    __shared__ int cnt[4][256];
+    __shared__ int dstart[256];
+    __shared__ int wsum[4];
+    __shared__ KT sk[kRsTile];
+    __shared__ int sv[VAL ? kRsTile : 1];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     base[t] = bscan[(int64_t)t * nblocks + blockIdx.x];
-    wc[0][t] = wc[1][t] = wc[2][t] = wc[3][t] = 0;
+    cnt[0][t] = cnt[1][t] = cnt[2][t] = cnt[3][t] = 0;
     __syncthreads();
     const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const int64_t tile0 = (int64_t)blockIdx.x * kRsTile;
+    const int64_t chunk = tile0 + (int64_t)w * (kRsRounds * kWave);
+    KT k[kRsRounds];
+    int v[kRsRounds], off[kRsRounds];
+#pragma unroll
     for (int r = 0; r < kRsRounds; r++) {
-        const int64_t i = (int64_t)blockIdx.x * kRsTile + r * kRsT + t;
+        const int64_t i = chunk + r * kWave + lane;
         const bool valid = i < n;
-        const KT k = valid ? kin[i] : KT(0);
-        const int v = (VAL && valid) ? vin[i] : 0;
-        const int d = (int)((k >> shift) & 255u);
+        k[r] = valid ? kin[i] : KT(0);
+        v[r] = (VAL && valid) ? vin[i] : 0;
+        const int d = (int)((k[r] >> shift) & 255u);
         unsigned long long m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -128,28 +140,42 @@ __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin,
             m &= ((d >> b) & 1) ? bb : ~bb;
         }
         const int rank = __popcll(m & lt);
-        if (valid && rank == 0) wc[w][d] = __popcll(m);
-        __syncthreads();
-        {
-            int sacc = 0;
+        const int before = cnt[w][d];
+        off[r] = valid ? before + rank : -1;
+        if (valid && rank == 0) cnt[w][d] = before + __popcll(m);
+    }
+    __syncthreads();
+    // thread t owns digit t: per-wave offsets, then the digit's start in the sorted tile
+    int tot = 0;
 #pragma unroll
-            for (int ww = 0; ww < 4; ww++) {
-                const int c = wc[ww][t];
-                wc[ww][t] = sacc;
-                sacc += c;
-            }
-            tot[t] = sacc;
-        }
-        __syncthreads();
-        if (valid) {
-            const int pos = base[d] + wc[w][d] + rank;
-            kout[pos] = k;
-            if (VAL) vout[pos] = v;
-        }
-        __syncthreads();
-        base[t] += tot[t];
-        wc[0][t] = wc[1][t] = wc[2][t] = wc[3][t] = 0;
-        __syncthreads();
+    for (int ww = 0; ww < 4; ww++) {
+        const int c = cnt[ww][t];
+        cnt[ww][t] = tot;
+        tot += c;
+    }
+    const int inc = wave_incl_scan(tot, lane);
+    if (lane == kWave - 1) wsum[w] = inc;
+    __syncthreads();
+    int wpre = 0;
+    for (int ww = 0; ww < w; ww++) wpre += wsum[ww];
+    dstart[t] = wpre + inc - tot;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRsRounds; r++) {
+        if (off[r] < 0) continue;
+        const int d = (int)((k[r] >> shift) & 255u);
+        const int tp = dstart[d] + cnt[w][d] + off[r];
+        sk[tp] = k[r];
+        if (VAL) sv[tp] = v[r];
+    }
+    __syncthreads();
+    const int nt = (int)min((int64_t)kRsTile, n - tile0);
+    for (int e = t; e < nt; e += kRsT) {
+        const KT key = sk[e];
+        const int d = (int)((key >> shift) & 255u);
+        const int pos = base[d] + (e - dstart[d]);
+        kout[pos] = key;
+        if (VAL) vout[pos] = sv[e];
     }
 }
 
