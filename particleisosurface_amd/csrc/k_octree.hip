@@ -212,32 +212,49 @@ __device__ __forceinline__ void entry_point(const NodeLevel& lv, const int* list
 }
 
 // head[j] = 1 when no earlier entry of the same key run has the same coordinates; rep[j] = sorted
-// position of the first entry with j's coordinates
-__global__ void k_sample_heads(NodeLevel lv, const int* __restrict__ list, SortedKV kv, int64_t nent,
-                               int* __restrict__ head, int* __restrict__ rep) {
+// position of the first entry with j's coordinates. A run (a lattice point shared by up to 8
+// nodes) is short: the 7 entries before j come from the lanes before j by shuffle, and only
+// entries of the run before those (or before the wave) are recomputed.
+__device__ __forceinline__ bool same_point(const float p[3], const float q[3]) {
+    return q[0] == p[0] && q[1] == p[1] && q[2] == p[2] && __float_as_uint(q[0]) == __float_as_uint(p[0]) &&
+           __float_as_uint(q[1]) == __float_as_uint(p[1]) && __float_as_uint(q[2]) == __float_as_uint(p[2]);
+}
+__global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* __restrict__ list, SortedKV kv,
+                                                      int64_t nent, int* __restrict__ head, int* __restrict__ rep) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j > nent) return;
-    if (j == nent) {
-        head[j] = 0;
-        return;
-    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool act = j < nent;
+    float p[3] = {0.f, 0.f, 0.f};
     int64_t s0 = j;
-    const uint64_t kj = kv.key(j);
-    while (s0 > 0 && j - s0 < 64 && kv.key(s0 - 1) == kj) s0--;
-    float p[3];
-    entry_point(lv, list, kv.val(j), p);
+    if (act) {
+        entry_point(lv, list, kv.val(j), p);
+        const uint64_t kj = kv.key(j);
+        while (s0 > 0 && j - s0 < 64 && kv.key(s0 - 1) == kj) s0--;
+    }
+    // entries [lo, j) are read from the lanes before this one
+    const int64_t lo = max(s0, max(j - 7, j - lane));
     int64_t r = j;
-    for (int64_t k = s0; k < j; k++) {
+    for (int64_t k = s0; k < lo && r == j; k++) {
         float q[3];
         entry_point(lv, list, kv.val(k), q);
-        if (q[0] == p[0] && q[1] == p[1] && q[2] == p[2] && __float_as_uint(q[0]) == __float_as_uint(p[0]) &&
-            __float_as_uint(q[1]) == __float_as_uint(p[1]) && __float_as_uint(q[2]) == __float_as_uint(p[2])) {
-            r = k;
-            break;
-        }
+        if (same_point(p, q)) r = k;
     }
-    head[j] = (r == j) ? 1 : 0;
-    rep[j] = (int)r;
+    float q[8][3];
+#pragma unroll
+    for (int o = 1; o <= 7; o++) {
+        q[o][0] = __shfl_up(p[0], o, kWave);
+        q[o][1] = __shfl_up(p[1], o, kWave);
+        q[o][2] = __shfl_up(p[2], o, kWave);
+    }
+#pragma unroll
+    for (int o = 7; o >= 1; o--)
+        if (r == j && j - o >= lo && same_point(p, q[o])) r = j - o;
+    if (act) {
+        head[j] = (r == j) ? 1 : 0;
+        rep[j] = (int)r;
+    } else if (j == nent) {
+        head[j] = 0;
+    }
 }
 
 __global__ void k_head_list(const int* __restrict__ head, const int* __restrict__ hpos, int64_t nent,

@@ -25,13 +25,15 @@ template <class R> __device__ __forceinline__ void fd_grads27(const R f[27], flo
 // Sample point s (0..26, index z*9 + y*3 + x) of a node box (TNode::GenerateSampling,
 // src/iso_method_ours.cpp:52-75) with correctly rounded, uncontracted float ops.
 __device__ __forceinline__ void sample_point(float4 c, int s, float p[3]) {
+    // every operation rounded on its own, as the reference computes it (and alike in every kernel)
+#pragma clang fp contract(off)
     const int t[3] = {s % 3, (s / 3) % 3, s / 9};
     const float cc[3] = {c.x, c.y, c.z};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const float mn = __fsub_rn(cc[a], c.w), mx = __fadd_rn(cc[a], c.w);
+        const float mn = cc[a] - c.w, mx = cc[a] + c.w;
         const float u = float(t[a]) / 2.0f;
-        p[a] = __fadd_rn(__fmul_rn(1.0f - u, mn), __fmul_rn(u, mx));
+        p[a] = (1.0f - u) * mn + u * mx;
     }
 }
 
