@@ -1014,15 +1014,11 @@ int do_extract_device(pisurf_ctx* c) {
     }
     // points -> dual cells
     // distinct points: run heads of the sorted corner keys, one thread each
-    HIPCHK(c, E[8].ensure((nent + 1) * 4));
-    HIPCHK(c, E[9].ensure((nent + 1) * 4));
-    launch_run_heads(ckv, nent, E[8].as<int>(), s);
-    HIPCHK(c, hipMemsetAsync(E[8].as<int>() + nent, 0, 4, s));
-    exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nent + 1, c->tmp.as<int>(), s);
+    HIPCHK(c, E[25].ensure(std::max<int64_t>(nent, 1) * 4));
+    HIPCHK(c, E[9].ensure(head_list_tmp_ints(nent) * 4));
+    launch_head_list(ckv, nent, E[25].as<int>(), E[9].as<int>(), c->tmp.as<int>(), s);
     int64_t npts;
-    if ((rc = read_int(c, E[9].as<int>() + nent, npts))) return rc;
-    HIPCHK(c, E[25].ensure(std::max<int64_t>(npts, 1) * 4));
-    launch_head_positions(E[8].as<int>(), E[9].as<int>(), nent, E[25].as<int>(), s);
+    if ((rc = read_int(c, head_list_count(E[9].as<int>(), nent), npts))) return rc;
     HIPCHK(c, E[26].ensure((npts + 1) * 4));
     HIPCHK(c, E[27].ensure((npts + 1) * 4));
     HIPCHK(c, E[24].ensure(std::max<int64_t>(npts, 1) * 9 * 4));

@@ -79,6 +79,16 @@ template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __bu
 template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
 
 // Compensated (Kahan) accumulator; K = false degenerates to a plain sum.
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
 template <class R, bool K> struct Acc {
     R s = R(0), c = R(0);
     __device__ __forceinline__ void add(R v) {

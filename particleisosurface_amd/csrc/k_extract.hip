@@ -118,12 +118,53 @@ __device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx
         if (n[o] < 0) n[o] = any;  // unreachable for a consistent tree
 }
 
-// positions of the run heads (first entry of each distinct point), in sorted order
-__global__ void k_head_positions(const int* __restrict__ head, const int* __restrict__ hpos, int64_t n,
-                                 int* __restrict__ list) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n && head[j]) list[hpos[j]] = (int)j;
+// List of the run heads of n sorted entries (entry j with j = 0 or key(j) != key(j - 1)), in
+// order: per tile of 2048 entries the heads are counted, the counts scanned, and each tile writes
+// its heads at its offset; the flags are recomputed from the keys, never stored.
+constexpr int kHeadT = 256, kHeadI = 8, kHeadTile = kHeadT * kHeadI;
+__device__ __forceinline__ int head_flags(const SortedKV& kv, int64_t n, int64_t base, bool f[kHeadI]) {
+    int c = 0;
+    uint64_t prev = (base > 0 && base - 1 < n) ? kv.key(base - 1) : 0;
+#pragma unroll
+    for (int q = 0; q < kHeadI; q++) {
+        const int64_t j = base + q;
+        f[q] = false;
+        if (j < n) {
+            const uint64_t k = kv.key(j);
+            f[q] = (j == 0 || k != prev);
+            prev = k;
+        }
+        c += f[q];
+    }
+    return c;
 }
+__global__ void __launch_bounds__(kHeadT) k_head_count(SortedKV kv, int64_t n, int* __restrict__ tile_count) {
+    __shared__ int wsum[kHeadT / kWave];
+    bool f[kHeadI];
+    int c = head_flags(kv, n, (int64_t)blockIdx.x * kHeadTile + (int64_t)threadIdx.x * kHeadI, f);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_count[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+__global__ void __launch_bounds__(kHeadT) k_head_write(SortedKV kv, int64_t n, const int* __restrict__ tile_off,
+                                                       int* __restrict__ list) {
+    __shared__ int wsum[kHeadT / kWave];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kHeadTile + (int64_t)t * kHeadI;
+    bool f[kHeadI];
+    const int c = head_flags(kv, n, base, f);
+    const int incl = wave_incl_scan(c, lane);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int pos = tile_off[blockIdx.x] + incl - c;
+    for (int k = 0; k < w; k++) pos += wsum[k];
+#pragma unroll
+    for (int q = 0; q < kHeadI; q++)
+        if (f[q]) list[pos++] = (int)(base + q);
+}
+
 
 __device__ __forceinline__ int sign_index(const uint8_t* leaf_pos, const int n[8]) {
     int s = 0;
@@ -203,11 +244,6 @@ __global__ void k_tri_corners(const int* __restrict__ duals, int64_t nd, const i
         }
 }
 
-__global__ void k_run_heads(SortedKV kv, int64_t n, int* __restrict__ head) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    head[j] = (j == 0 || kv.key(j) != kv.key(j - 1)) ? 1 : 0;
-}
 
 // ---- hash-table dedupe (first occurrence wins, as Mesh::insert_vert / insert_tri do) ----
 // Open addressing with linear probing over a power-of-two table of more slots than keys, so every
@@ -354,8 +390,19 @@ void launch_vertex_ids_hashed(const int* slot, const unsigned* table, const int*
 void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s) {
     if (nt > 0) k_tri_key63<<<nblk(nt), 256, 0, s>>>(cv, nt, k);
 }
-void launch_head_positions(const int* head, const int* hpos, int64_t n, int* list, hipStream_t s) {
-    if (n > 0) k_head_positions<<<nblk(n), 256, 0, s>>>(head, hpos, n, list);
+static_assert(kHeadTile == 2048, "head_list_count assumes tiles of 2048 entries");
+int64_t head_list_tmp_ints(int64_t n) { return 2 * ((n + kHeadTile - 1) / kHeadTile + 1); }
+void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_tmp, hipStream_t s) {
+    const int64_t nb = (n + kHeadTile - 1) / kHeadTile;
+    int* tile_off = tiles + (nb + 1);
+    if (n <= 0) {
+        (void)hipMemsetAsync(tile_off, 0, sizeof(int), s);
+        return;
+    }
+    k_head_count<<<(unsigned)nb, kHeadT, 0, s>>>(kv, n, tiles);
+    (void)hipMemsetAsync(tiles + nb, 0, sizeof(int), s);
+    exclusive_scan_i32(tiles, tile_off, nb + 1, scan_tmp, s);
+    k_head_write<<<(unsigned)nb, kHeadT, 0, s>>>(kv, n, tile_off, list);
 }
 void launch_points(const TreeView& tv, const int* lidx, const uint8_t* leaf_pos, SortedKV kv, int64_t nent,
                    const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s) {
@@ -371,9 +418,6 @@ void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s) {
 void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t* vkey, int* vval, int* vend,
                         hipStream_t s) {
     if (nd > 0) k_tri_corners<<<nblk(nd), 256, 0, s>>>(duals, nd, toff, vkey, vval, vend);
-}
-void launch_run_heads(SortedKV kv, int64_t n, int* head, hipStream_t s) {
-    if (n > 0) k_run_heads<<<nblk(n), 256, 0, s>>>(kv, n, head);
 }
 void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s) {
     if (nt > 0) k_tri_key_hi<<<nblk(nt), 256, 0, s>>>(cv, nt, k, v);

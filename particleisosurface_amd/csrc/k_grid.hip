@@ -12,15 +12,6 @@ namespace pisurf {
 // ---------------------------------------------------------------------------------------------
 constexpr int kScanT = 256, kScanI = 8, kScanTile = kScanT * kScanI;
 
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        int t = __shfl_up(v, o, kWave);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
 __global__ void __launch_bounds__(kScanT) k_scan_tile(const int* __restrict__ in, int* __restrict__ out, int64_t n,
                                                       int* __restrict__ sums) {
     __shared__ int wsum[kScanT / kWave];

@@ -167,7 +167,7 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v) {
     return x;
 }
 
-// Hash cell group of a point for the cell-grouped evaluation: its cell (CalcXYZIdx) in the grid
+// Hash cell group of a point for the run evaluation: its cell (CalcXYZIdx) in the grid
 // extended by one cell on every side; points farther out have an empty 27-cell window and share
 // the last group. Points of one group have the same candidate list in the same order.
 __device__ __forceinline__ uint64_t cell_group(const GridDev& g, float x, float y, float z) {
@@ -263,7 +263,7 @@ __global__ void k_head_list(const int* __restrict__ head, const int* __restrict_
     if (j < nent && head[j]) hlist[hpos[j]] = (int)j;
 }
 
-// ---- cell-grouped evaluation (levels with many sample points per hash cell) ----
+// ---- groups of points by hash cell, and runs of groups (run_eval.h) ----
 // cflag[h] = 1 where the cell group changes along the distinct points (cell-major order)
 __global__ void k_cell_flags(SortedKV kv, const int* __restrict__ hlist, int64_t nheads, int* __restrict__ cflag) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

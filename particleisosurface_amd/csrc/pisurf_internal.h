@@ -189,8 +189,6 @@ void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned* table, int64_t
 void launch_vertex_ids_hashed(const int* slot, const unsigned* table, const int* rank, int64_t n, const int* vend,
                               const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s);
 void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s);
-// list[hpos[j]] = j for every j with head[j] != 0
-void launch_head_positions(const int* head, const int* hpos, int64_t n, int* list, hipStream_t s);
 // point t (first sorted entry heads[t]): dflag[t] = 1 when its dual cell is non-trivial, the cell
 // goes to stage[9 t ..]
 void launch_points(const TreeView& tv, const int* lidx, const uint8_t* leaf_pos, SortedKV kv, int64_t nent,
@@ -200,7 +198,12 @@ void launch_points_compact(const int* dflag, const int* dpos, const int* stage, 
 void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s);
 void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t* vkey, int* vval, int* vend,
                         hipStream_t s);
-void launch_run_heads(SortedKV kv, int64_t n, int* head, hipStream_t s);
+// list[0 .. h) = the run heads of n sorted entries in order, and h in *head_list_count(tiles, n).
+// tiles holds head_list_tmp_ints(n) ints; scan_tmp is exclusive_scan_i32 workspace for
+// n / 2048 + 2 ints.
+int64_t head_list_tmp_ints(int64_t n);
+inline int* head_list_count(int* tiles, int64_t n) { return tiles + 2 * ((n + 2047) / 2048) + 1; }
+void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_tmp, hipStream_t s);
 void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s);
 void launch_tri_key_lo(const int* cv, const int* perm, int64_t nt, uint64_t* k, int* v, hipStream_t s);
 void launch_tri_keep(const int* cv, const int* order, int64_t nt, int* keep, hipStream_t s);
