@@ -661,7 +661,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             HIPCHK(c, c->sd_key2.ensure(nent * 8));
             HIPCHK(c, c->sd_val.ensure(nent * 4));
             HIPCHK(c, c->sd_val2.ensure(nent * 4));
-            HIPCHK(c, c->sd_head.ensure((nent + 1) * 4));
+            HIPCHK(c, c->sd_head.ensure((2 * ((nent + 255) / 256) + 2) * 4));
             HIPCHK(c, c->sd_hpos.ensure((nent + 1) * 4));
             HIPCHK(c, c->sd_rep.ensure(nent * 4));
             HIPCHK(c, c->sd_hlist.ensure(nent * 4));

@@ -220,7 +220,9 @@ __device__ __forceinline__ bool same_point(const float p[3], const float q[3]) {
            __float_as_uint(q[1]) == __float_as_uint(p[1]) && __float_as_uint(q[2]) == __float_as_uint(p[2]);
 }
 __global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* __restrict__ list, SortedKV kv,
-                                                      int64_t nent, int* __restrict__ head, int* __restrict__ rep) {
+                                                      int64_t nent, int* __restrict__ block_heads,
+                                                      int* __restrict__ rep) {
+    __shared__ int wsum[4];
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (kWave - 1);
     const bool act = j < nent;
@@ -249,18 +251,34 @@ __global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* _
 #pragma unroll
     for (int o = 7; o >= 1; o--)
         if (r == j && j - o >= lo && same_point(p, q[o])) r = j - o;
-    if (act) {
-        head[j] = (r == j) ? 1 : 0;
-        rep[j] = (int)r;
-    } else if (j == nent) {
-        head[j] = 0;
-    }
+    if (act) rep[j] = (int)r;
+    // heads of this block of 256 entries (a head is its own representative)
+    int c = (act && r == j) ? 1 : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+    if (lane == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) block_heads[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-__global__ void k_head_list(const int* __restrict__ head, const int* __restrict__ hpos, int64_t nent,
-                            int* __restrict__ hlist) {
+// hlist[h] = sorted position of the h-th head, hpos[j] = h for each head j (block_off: scanned
+// per-block head counts of k_sample_heads' blocks of 256)
+__global__ void __launch_bounds__(256) k_head_list(const int* __restrict__ rep, int64_t nent,
+                                                   const int* __restrict__ block_off, int* __restrict__ hlist,
+                                                   int* __restrict__ hpos) {
+    __shared__ int wsum[4];
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nent && head[j]) hlist[hpos[j]] = (int)j;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int f = (j < nent && rep[j] == (int)j) ? 1 : 0;
+    const int incl = wave_incl_scan(f, lane);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int pos = block_off[blockIdx.x] + incl - f;
+    for (int k = 0; k < w; k++) pos += wsum[k];
+    if (f) {
+        hlist[pos] = (int)j;
+        hpos[j] = pos;
+    }
 }
 
 // ---- groups of points by hash cell, and runs of groups (run_eval.h) ----
@@ -670,12 +688,15 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     *nruns_out = 0;
     *kx_out = 0;
     if (nent <= 0) return;
-    const unsigned nb = (unsigned)((nent + 1 + 255) / 256);
+    // head holds the per-block head counts (nb + 1) and their scan (nb + 1)
+    const unsigned nb = (unsigned)((nent + 255) / 256);
+    int* block_off = head + (nb + 1);
     k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, nent, head, rep);
-    exclusive_scan_i32(head, hpos, nent + 1, tmp, s);
-    k_head_list<<<nb, 256, 0, s>>>(head, hpos, nent, hlist);
+    (void)hipMemsetAsync(head + nb, 0, sizeof(int), s);
+    exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
+    k_head_list<<<nb, 256, 0, s>>>(rep, nent, block_off, hlist, hpos);
     int nh = 0;
-    (void)read_device(&nh, hpos + nent, 4, s);
+    (void)read_device(&nh, block_off + nb, 4, s);
     *nheads_out = nh;
     int* rstart = nullptr;
     int kx = 0, T = kRunThreads;

@@ -147,8 +147,8 @@ void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec
 // sample deduplication: keys for the nent = 27 * listed entries (cellmode 1: hash cell group,
 // then the lattice position modulo 8; 0: Morton order of the lattice position), then (after a
 // 64-bit key sort) heads, the distinct points grouped into runs of cells, one workgroup per run
-// between events e0 / e1, and the scatter to samples. cwork holds 4 (nent + 2) ints; kx_out is the
-// number of cells of a row per run.
+// between events e0 / e1, and the scatter to samples. head holds 2 (nent / 256 + 2) ints, hpos /
+// rep / hlist nent, cwork 4 (nent + 2); kx_out is the number of cells of a row per run.
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
                                int cellmode, int vbits, uint64_t* key, int* val, hipStream_t s);
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
