@@ -523,7 +523,7 @@ __global__ void __launch_bounds__(256) k_samples(GridDev g, Recs<R> rec, FieldCo
 }
 
 // afterSampleEval classification (src/surface_reconstructor.cpp:237-273) + leaf values
-template <class R, bool K>
+template <class R, bool K, bool QEF>
 __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldConst fc, TreeParams tp, NodeLevel lv,
                                                   const R* __restrict__ samples, int32_t* __restrict__ sc_out,
                                                   Counters* ctr) {
@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
                 type = kInternal;
             } else {
                 type = kLeaf;
-                if (tp.qef_mode != 0 && has_samples) {
+                if (QEF && tp.qef_mode != 0 && has_samples) {
                     R f[27], gr[27][3];
                     float pts[27][3];
                     for (int s = 0; s < 27; s++) {
@@ -801,14 +801,23 @@ void launch_classify(int prec, GridDev g, const void* ra, const void* rg, FieldC
                      const void* samples, int32_t* sc, Counters* ctr, hipStream_t s) {
     if (lv.n <= 0) return;
     const unsigned nb = (unsigned)((lv.n + 255) / 256);
-    if (prec == 2) {
-        Recs<double> r{(const double4*)ra, (const double4*)rg};
-        k_classify<double, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const double*)samples, sc, ctr);
-    } else {
-        Recs<float> r{(const float4*)ra, (const float4*)rg};
-        if (prec == 1) k_classify<float, true><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const float*)samples, sc, ctr);
-        else k_classify<float, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const float*)samples, sc, ctr);
-    }
+    // the QEF code is compiled in only when used: it takes ~200 registers, which would cut the
+    // occupancy of the common case
+    auto go = [&](auto qef) {
+        constexpr bool Q = decltype(qef)::value;
+        if (prec == 2) {
+            Recs<double> r{(const double4*)ra, (const double4*)rg};
+            k_classify<double, false, Q><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const double*)samples, sc, ctr);
+        } else {
+            Recs<float> r{(const float4*)ra, (const float4*)rg};
+            if (prec == 1)
+                k_classify<float, true, Q><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const float*)samples, sc, ctr);
+            else
+                k_classify<float, false, Q><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const float*)samples, sc, ctr);
+        }
+    };
+    if (tp.qef_mode != 0) go(std::true_type());
+    else go(std::false_type());
 }
 
 void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s) {
