@@ -632,7 +632,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     if (level_depth >= 0) {
         const double half = std::ldexp((double)c->root_half, -level_depth);
         const double w = (2 * half + 2 * (double)tp.margin) / (double)c->g.cell + 1;
-        group = (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
+        group = (w * w <= 6) ? 4 : (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
     }
     // At D_min, where most nodes lie inside the fluid and their samples do not change sign, the
     // frame tests emptiness first and takes the curvature after sampling, for the nodes whose

@@ -649,11 +649,13 @@ __global__ void k_sample_grads(TreeParams tp, NodeLevel lv, const R* __restrict_
 void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, int group,
                        int pass, const void* samples, int rsz, hipStream_t s) {
     if (lv.n <= 0) return;
-    const unsigned nb8 = (unsigned)((lv.n * 8 + 255) / 256), nb16 = (unsigned)((lv.n * 16 + 255) / 256);
+    const unsigned nb4 = (unsigned)((lv.n * 4 + 255) / 256), nb8 = (unsigned)((lv.n * 8 + 255) / 256);
+    const unsigned nb16 = (unsigned)((lv.n * 16 + 255) / 256);
     const unsigned nb64 = (unsigned)((lv.n * kWave + 255) / 256);
     auto go = [&](auto tag) {
         constexpr int P = decltype(tag)::value;
-        if (group <= 8) k_empty_curv<8, P><<<nb8, 256, 0, s>>>(g, spos, nrm, tp, lv, samples, rsz);
+        if (group <= 4) k_empty_curv<4, P><<<nb4, 256, 0, s>>>(g, spos, nrm, tp, lv, samples, rsz);
+        else if (group <= 8) k_empty_curv<8, P><<<nb8, 256, 0, s>>>(g, spos, nrm, tp, lv, samples, rsz);
         else if (group <= 16) k_empty_curv<16, P><<<nb16, 256, 0, s>>>(g, spos, nrm, tp, lv, samples, rsz);
         else k_empty_curv<64, P><<<nb64, 256, 0, s>>>(g, spos, nrm, tp, lv, samples, rsz);
     };

@@ -126,7 +126,7 @@ struct TreeParams {
     int qef_mode;
     float qef_svd_tol;
 };
-// group: lanes per node (8, 16 or 64), about the number of (y, z) cell rows a node box spans.
+// group: lanes per node (4, 8, 16 or 64), about the number of (y, z) cell rows a node box spans.
 // pass: kEmptyAndCurv both; kEmptyOnly emptiness alone (stops at the first particle found);
 // kCurvOnly the curvature of the sampled nodes whose 27 samples (R = rsz bytes) change sign, the
 // only nodes whose classification reads it.
