@@ -212,7 +212,7 @@ struct pisurf_ctx {
     bool grid_done = false;
     pisurf_grid_info ginfo;
     GridDev g;
-    DBuf keys, vals, keys2, vals2, cellcount, off, tmp, spos, part, small;
+    DBuf keys, vals, keys2, vals2, off, tmp, spos, part, small;
     // anisotropy (records are for prec_rec)
     bool aniso_done = false;
     int prec_rec = 0;
@@ -456,13 +456,11 @@ int do_grid(pisurf_ctx* c, const float* bbox, float inf) {
     HIPCHK(c, c->vals.ensure(n * 4));
     HIPCHK(c, c->keys2.ensure(n * 4));
     HIPCHK(c, c->vals2.ensure(n * 4));
-    HIPCHK(c, c->cellcount.ensure((ncell + 1) * 4));
     HIPCHK(c, c->off.ensure((ncell + 1) * 4));
     HIPCHK(c, c->spos.ensure(n * sizeof(float4)));
     const int64_t tmp_ints = std::max(radix_tmp_ints(n), scan_tmp_ints(ncell + 1)) + 64;
     HIPCHK(c, c->tmp.ensure(tmp_ints * 4));
     HIPCHK(c, c->small.ensure(256));
-    HIPCHK(c, hipMemsetAsync(c->cellcount.p, 0, (ncell + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->small.p, 0, 64, c->stream));
     GridDev g;
     for (int a = 0; a < 3; a++) {
@@ -473,13 +471,12 @@ int do_grid(pisurf_ctx* c, const float* bbox, float inf) {
     g.ncell = (int)ncell;
     g.off = c->off.as<int>();
     int* ninv_d = c->small.as<int>();
-    launch_keys(c->xyz.as<float>(), n, g, c->keys.as<uint32_t>(), c->vals.as<int>(), c->cellcount.as<int>(), ninv_d,
-                c->stream);
+    launch_keys(c->xyz.as<float>(), n, g, c->keys.as<uint32_t>(), c->vals.as<int>(), ninv_d, c->stream);
     int bits = 1;
     while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)ncell) bits++;
     radix_sort_pairs(c->keys.as<uint32_t>(), c->vals.as<int>(), c->keys2.as<uint32_t>(), c->vals2.as<int>(), n, bits,
                      c->tmp.as<int>(), c->stream);
-    exclusive_scan_i32(c->cellcount.as<int>(), c->off.as<int>(), ncell + 1, c->tmp.as<int>(), c->stream);
+    launch_cell_offsets(c->keys.as<uint32_t>(), n, ninv_d, ncell, c->off.as<int>(), c->stream);
     launch_gather_pos(c->xyz.as<float>(), c->vals.as<int>(), n, c->spos.as<float4>(), c->stream);
     HIPCHK(c, hipGetLastError());
     int ninv = 0;

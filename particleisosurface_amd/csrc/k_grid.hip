@@ -339,7 +339,7 @@ void launch_bbox(const float* xyz, int64_t n, float* partial, float* out6, int* 
 // out-of-range key -1 sorts first, as the signed keys do in the reference.
 // ---------------------------------------------------------------------------------------------
 __global__ void k_keys(const float* __restrict__ xyz, int64_t n, GridDev g, uint32_t* __restrict__ keys,
-                       int* __restrict__ vals, int* __restrict__ cell_count, int* __restrict__ n_invalid) {
+                       int* __restrict__ vals, int* __restrict__ n_invalid) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int ix, iy, iz;
@@ -348,14 +348,34 @@ __global__ void k_keys(const float* __restrict__ xyz, int64_t n, GridDev g, uint
     const int key = valid ? (iz * g.n[1] + iy) * g.n[0] + ix : -1;
     keys[i] = (uint32_t)(key + 1);
     vals[i] = (int)i;
-    if (valid) atomicAdd(&cell_count[key], 1);
-    else atomicAdd(n_invalid, 1);
+    if (!valid) atomicAdd(n_invalid, 1);
 }
 
-void launch_keys(const float* xyz, int64_t n, GridDev g, uint32_t* keys, int* vals, int* cell_count, int* n_invalid,
-                 hipStream_t s) {
+void launch_keys(const float* xyz, int64_t n, GridDev g, uint32_t* keys, int* vals, int* n_invalid, hipStream_t s) {
     if (n <= 0) return;
-    k_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(xyz, n, g, keys, vals, cell_count, n_invalid);
+    k_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(xyz, n, g, keys, vals, n_invalid);
+}
+
+// off[c] = number of valid particles in cells below c (FindStartEnd's counts, used by the
+// reference as positions into the full sorted list): the first sorted position whose stored key
+// (cell + 1, 0 outside the grid) exceeds c, less the particles outside. One thread per cell,
+// binary search over the sorted keys.
+__global__ void k_cell_offsets(const uint32_t* __restrict__ skeys, int64_t n, const int* __restrict__ n_invalid,
+                               int64_t ncell, int* __restrict__ off) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncell) return;
+    int64_t lo = 0, hi = n;  // first position with skeys[p] >= c + 1
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)skeys[mid] < c + 1) lo = mid + 1;
+        else hi = mid;
+    }
+    off[c] = (int)(lo - *n_invalid);
+}
+
+void launch_cell_offsets(const uint32_t* skeys, int64_t n, const int* n_invalid, int64_t ncell, int* off,
+                         hipStream_t s) {
+    k_cell_offsets<<<(unsigned)((ncell + 1 + 255) / 256), 256, 0, s>>>(skeys, n, n_invalid, ncell, off);
 }
 
 __global__ void k_gather_pos(const float* __restrict__ xyz, const int* __restrict__ sidx, int64_t n,

@@ -47,8 +47,10 @@ hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s
 // ---- grid ----
 void launch_bbox(const float* xyz, int64_t n, float* partial /*>= 6*1024*/, float* out6, int* nan_count,
                  hipStream_t s);
-void launch_keys(const float* xyz, int64_t n, GridDev g, uint32_t* keys, int* vals, int* cell_count, int* n_invalid,
-                 hipStream_t s);
+void launch_keys(const float* xyz, int64_t n, GridDev g, uint32_t* keys, int* vals, int* n_invalid, hipStream_t s);
+// off[0 .. ncell] from the sorted keys (cell + 1, 0 outside the grid)
+void launch_cell_offsets(const uint32_t* skeys, int64_t n, const int* n_invalid, int64_t ncell, int* off,
+                         hipStream_t s);
 void launch_gather_pos(const float* xyz, const int* sidx, int64_t n, float4* spos, hipStream_t s);
 void launch_neighbor_count(GridDev g, const float* q, int64_t nq, int* counts, hipStream_t s);
 void launch_neighbor_write(GridDev g, const float* q, int64_t nq, const int* offsets, const int* sidx, int* ids,
