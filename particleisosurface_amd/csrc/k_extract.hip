@@ -60,13 +60,25 @@ __global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ l
         const int4 p = lv.ip[i];
         const int sh = dp - p.w;
         const int64_t hi = ((int64_t)1 << dp) - 1;
+        const uint64_t kmax = kbits >= 64 ? ~0ull : ((1ull << kbits) - 1);
+        // the two coordinates per axis of the 8 corners, and their spread bits
+        int64_t cx[2], cy[2], cz[2];
+        uint64_t mx[2], my[2], mz[2];
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+            cx[o] = (int64_t)(p.x + o) << sh;
+            cy[o] = (int64_t)(p.y + o) << sh;
+            cz[o] = (int64_t)(p.z + o) << sh;
+            mx[o] = morton_spread(cx[o]);
+            my[o] = morton_spread(cy[o]) << 1;
+            mz[o] = morton_spread(cz[o]) << 2;
+        }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            const int64_t X = (int64_t)(p.x + (c & 1)) << sh, Y = (int64_t)(p.y + ((c >> 1) & 1)) << sh,
-                          Z = (int64_t)(p.z + ((c >> 2) & 1)) << sh;
+            const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+            const int64_t X = cx[ox], Y = cy[oy], Z = cz[oz];
             const bool in = X >= 1 && X <= hi && Y >= 1 && Y <= hi && Z >= 1 && Z <= hi;
-            const uint64_t kmax = kbits >= 64 ? ~0ull : ((1ull << kbits) - 1);
-            const uint64_t k = in ? morton_spread(X) | (morton_spread(Y) << 1) | (morton_spread(Z) << 2) : kmax;
+            const uint64_t k = in ? (mx[ox] | my[oy] | mz[oz]) : kmax;
             const int64_t e = 8 * (int64_t)L + c;
             if (vbits) {
                 ckey[e] = (k << vbits) | (uint64_t)e;
