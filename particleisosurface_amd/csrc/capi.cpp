@@ -1114,15 +1114,21 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, hipStreamWaitEvent(c->dstream, c->eb0, 0));
     if (ntf) HIPCHK(c, hipMemcpyAsync(c->mesh_t.data(), E[23].p, ntf * 3 * 4, hipMemcpyDeviceToHost, c->dstream));
     if (c->ev_chunk.empty()) {
-        c->ev_chunk.resize(2, nullptr);
+        c->ev_chunk.resize(3, nullptr);
         for (auto& e : c->ev_chunk) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    // two halves from 2^20 vertices on: smaller launches lose more to the tail of the bisection
-    // (threads take different numbers of steps) than the overlap gains
-    const int64_t chunk = nv >= ((int64_t)1 << 20) ? (nv + 1) / 2 : std::max<int64_t>(nv, 1);
-    int ci = 0;
-    for (int64_t v0 = 0; v0 < nv; v0 += chunk, ci++) {
-        const int64_t m = std::min(chunk, nv - v0);
+    // From 2^20 vertices on, three parts of 40, 35 and 25 %: the download of the last part is
+    // the only one not hidden; smaller launches would lose more to the tail of the bisection
+    // (threads take different numbers of steps) than the overlap gains.
+    int64_t part_end[3] = {nv, nv, nv};
+    if (nv >= ((int64_t)1 << 20)) {
+        part_end[0] = nv * 2 / 5;
+        part_end[1] = nv * 3 / 4;
+    }
+    int64_t v0 = 0;
+    for (int ci = 0; ci < 3 && v0 < nv; ci++) {
+        const int64_t m = part_end[ci] - v0;
+        if (m <= 0) continue;
         launch_bisect(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->prm.radius / 2,
                       E[21].as<double>() + 8 * v0, m, E[22].as<double>() + 3 * v0, nullptr, c->ctr(), s);
         HIPCHK(c, hipGetLastError());
@@ -1130,6 +1136,7 @@ int do_extract_device(pisurf_ctx* c) {
         HIPCHK(c, hipStreamWaitEvent(c->dstream, c->ev_chunk[ci], 0));
         HIPCHK(c, hipMemcpyAsync(c->mesh_v.data() + 3 * v0, E[22].as<double>() + 3 * v0, m * 3 * 8,
                                  hipMemcpyDeviceToHost, c->dstream));
+        v0 = part_end[ci];
     }
     HIPCHK(c, hipEventRecord(c->eb1, s));
     HIPCHK(c, stream_wait(s));
