@@ -3,17 +3,9 @@
 // cells, so their 9 candidate row ranges coincide and neighbour reads are shared in L1/L2.
 #include "pisurf_internal.h"
 
-#include <type_traits>
-
 namespace pisurf {
 
 template <class R> struct R4 { typedef typename Vec4<R>::T T; };
-
-// Sums of the anisotropy passes: compensated FP32 accumulates its float terms in FP64 (a
-// conversion and one add per term instead of the four operations of a compensated float sum, and
-// at least as accurate); the other modes sum in their own precision.
-template <class R, bool K>
-using AnisoAcc = typename std::conditional<K && sizeof(R) == 4, Acc<double, false>, Acc<R, K>>::type;
 
 __device__ __forceinline__ float4 mk4(float x, float y, float z, float w) { return make_float4(x, y, z, w); }
 __device__ __forceinline__ double4 mk4(double x, double y, double z, double w) { return make_double4(x, y, z, w); }
@@ -42,7 +34,7 @@ __global__ void __launch_bounds__(256) k_xmean(GridDev g, const float4* __restri
     const R nI = ap.nI, nI2 = ap.nI2, inv_h3 = R(1) / (nI * nI * nI);
     // compensated FP32 sums the offsets x_j - x_i (exact for nearby floats) instead of x_j
     constexpr bool REL = K && sizeof(R) == 4;
-    AnisoAcc<R, K> sx, sy, sz, sw;
+    Acc<R, K> sx, sy, sz, sw;
     for (int r = 0; r < 9; r++) {
         for (int j = lo[r]; j < hi[r]; j++) {
             if (j == (int)p) continue;
@@ -58,7 +50,7 @@ __global__ void __launch_bounds__(256) k_xmean(GridDev g, const float4* __restri
         }
     }
     R ox = px, oy = py, oz = pz;
-    const auto ws = sw.get();
+    const R ws = sw.get();
     if (REL) {
         float4 lo4 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (!ap.use_poly6 && ws > R(0)) {
@@ -82,9 +74,9 @@ __global__ void __launch_bounds__(256) k_xmean(GridDev g, const float4* __restri
     }
     if (!ap.use_poly6 && ws > R(0)) {
         const R a = R(1.0f - ap.xmean_delta), b = R(ap.xmean_delta);
-        ox = px * a + R(sx.get() / ws) * b;
-        oy = py * a + R(sy.get() / ws) * b;
-        oz = pz * a + R(sz.get() / ws) * b;
+        ox = px * a + (sx.get() / ws) * b;
+        oy = py * a + (sy.get() / ws) * b;
+        oz = pz * a + (sz.get() / ws) * b;
     }
     xm[p] = mk4(ox, oy, oz, R(0));
 }
@@ -114,7 +106,7 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
     const R nI = ap.nI, nI2 = ap.nI2, inv_h3 = R(1) / (nI * nI * nI);
     constexpr bool REL = K && sizeof(R) == 4;
     const float4 li = REL ? ((const float4*)xm)[n + p] : make_float4(0.f, 0.f, 0.f, 0.f);
-    AnisoAcc<R, K> c[6], sw;
+    Acc<R, K> c[6], sw;
     const R base = ap.use_poly6 ? R(1.5) : h2;
     for (int r = 0; r < 9; r++) {
         int lo, hi;
@@ -149,9 +141,9 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
             c[5].add(wz * wz * w);
         }
     }
-    const auto ws = sw.get();
-    R cov[6] = {R((c[0].get() + base) / ws), R(c[1].get() / ws), R(c[2].get() / ws),
-                R((c[3].get() + base) / ws), R(c[4].get() / ws), R((c[5].get() + base) / ws)};
+    const R ws = sw.get();
+    R cov[6] = {(c[0].get() + base) / ws, c[1].get() / ws, c[2].get() / ws,
+                (c[3].get() + base) / ws, c[4].get() / ws, (c[5].get() + base) / ws};
     const R extra = ap.use_poly6 ? R(0.5) : R(1);
     R G[6], det, gmin;
     bool clamped;
