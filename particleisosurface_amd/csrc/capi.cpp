@@ -672,7 +672,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             const uint64_t groups = (uint64_t)(c->g.n[0] + 2) * (uint64_t)(c->g.n[1] + 2) * (uint64_t)(c->g.n[2] + 2);
             const int kbits = std::min(64, bits_for((int64_t)groups + 1) + 9);
             const int vbits = pack_vbits(kbits, nent);
-            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, 1, vbits, c->sd_key.as<uint64_t>(),
+            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, vbits, c->sd_key.as<uint64_t>(),
                                       c->sd_val.as<int>(), c->stream);
             const SortedKV skv = radix_sort_entries(c->sd_key.as<uint64_t>(), c->sd_val.as<int>(),
                                                     c->sd_key2.as<uint64_t>(), c->sd_val2.as<int>(), nent, kbits,

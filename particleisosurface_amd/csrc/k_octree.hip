@@ -153,19 +153,10 @@ __global__ void k_sample_flag(TreeParams tp, NodeLevel lv, int* __restrict__ fla
 
 // ---- per-level sample deduplication --------------------------------------------------------
 // Neighbouring nodes share sample points on their faces, edges and corners. Entries (listed node,
-// sample) are keyed by the Morton code of their lattice position round((p - origin) / half), so a
-// shared point's copies sort next to each other and evaluation runs in space-filling order; only
-// copies with bitwise-identical float coordinates are merged, so every value is exactly what the
-// node's own evaluation would produce.
-__device__ __forceinline__ uint64_t spread3(uint32_t v) {
-    uint64_t x = v & 0x1fffff;
-    x = (x | (x << 32)) & 0x1f00000000ffffull;
-    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
-    x = (x | (x << 8)) & 0x100f00f00f00f00full;
-    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
-    x = (x | (x << 2)) & 0x1249249249249249ull;
-    return x;
-}
+// sample) are keyed by the point's hash cell group and lattice position, so a shared point's
+// copies sort next to each other and the distinct points come out grouped by cell for the run
+// evaluation; only copies with bitwise-identical float coordinates are merged, so every value is
+// exactly what the node's own evaluation would produce.
 
 // Hash cell group of a point for the run evaluation: its cell (CalcXYZIdx) in the grid
 // extended by one cell on every side; points farther out have an empty 27-cell window and share
@@ -178,11 +169,10 @@ __device__ __forceinline__ uint64_t cell_group(const GridDev& g, float x, float 
     return ((uint64_t)(iz + 1) * ey + (uint64_t)(iy + 1)) * ex + (uint64_t)(ix + 1);
 }
 
-// cellmode 0: Morton code of the lattice position; cellmode 1: cell group, then the lattice
-// position modulo 8 per axis (distinct inside a cell spanning fewer than 8 lattice steps)
+// key of sample entry t: the point's cell group, then its lattice position modulo 8 per axis
+// (distinct inside a cell spanning fewer than 8 lattice steps; equal points have equal keys)
 __global__ void k_sample_keys(GridDev g, NodeLevel lv, const int* __restrict__ list, int64_t nent, float ox,
-                              float oy, float oz, int cellmode, int vbits, uint64_t* __restrict__ key,
-                              int* __restrict__ val) {
+                              float oy, float oz, int vbits, uint64_t* __restrict__ key, int* __restrict__ val) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nent) return;
     const int node = list[t / 27];
@@ -196,9 +186,7 @@ __global__ void k_sample_keys(GridDev g, NodeLevel lv, const int* __restrict__ l
         const float u = rintf((p[a] - o[a]) / c.w);
         q[a] = (uint32_t)fminf(fmaxf(u, 0.0f), 2097151.0f);
     }
-    const uint64_t k = cellmode ? (cell_group(g, p[0], p[1], p[2]) << 9) | (q[0] & 7) | ((q[1] & 7) << 3) |
-                                      ((q[2] & 7) << 6)
-                                : spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    const uint64_t k = (cell_group(g, p[0], p[1], p[2]) << 9) | (q[0] & 7) | ((q[1] & 7) << 3) | ((q[2] & 7) << 6);
     if (vbits) {
         key[t] = (k << vbits) | (uint64_t)t;
     } else {
@@ -675,10 +663,10 @@ void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* l
 }
 
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
-                               int cellmode, int vbits, uint64_t* key, int* val, hipStream_t s) {
+                               int vbits, uint64_t* key, int* val, hipStream_t s) {
     if (nent <= 0) return;
     k_sample_keys<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(g, lv, list, nent, origin[0], origin[1], origin[2],
-                                                                 cellmode, vbits, key, val);
+                                                                 vbits, key, val);
 }
 
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
