@@ -670,9 +670,13 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
                                      c->root_c[2] - c->root_half};
             // distinct points grouped by hash cell (cell-major keys), evaluated in runs of cells
             const uint64_t groups = (uint64_t)(c->g.n[0] + 2) * (uint64_t)(c->g.n[1] + 2) * (uint64_t)(c->g.n[2] + 2);
-            const int kbits = std::min(64, bits_for((int64_t)groups + 1) + 9);
+            // three lattice bits per axis: fewer would save a sort pass at the coarser levels but
+            // lengthen the runs of equal keys (shared cells, and the points outside the grid, which
+            // form one group), which costs the heads pass more than the sort gains
+            const int lb = 3;
+            const int kbits = std::min(64, bits_for((int64_t)groups + 1) + 3 * lb);
             const int vbits = pack_vbits(kbits, nent);
-            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, vbits, c->sd_key.as<uint64_t>(),
+            launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, lb, vbits, c->sd_key.as<uint64_t>(),
                                       c->sd_val.as<int>(), c->stream);
             const SortedKV skv = radix_sort_entries(c->sd_key.as<uint64_t>(), c->sd_val.as<int>(),
                                                     c->sd_key2.as<uint64_t>(), c->sd_val2.as<int>(), nent, kbits,
@@ -682,7 +686,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             HIPCHK(c, c->event_pair(e0, e1, &eslot));
             int64_t nheads = 0, nruns = 0;
             int kx = 0;
-            launch_sample_dedupe_eval(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, L->child.as<int>(), skv, nent,
+            launch_sample_dedupe_eval(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, L->child.as<int>(), skv, 3 * lb, nent,
                                       c->sd_head.as<int>(),
                                       c->sd_hpos.as<int>(), c->sd_rep.as<int>(), c->sd_hlist.as<int>(),
                                       c->sd_cwork.as<int>(), &nheads, &nruns, &kx, c->sd_vals.p, c->samples.p,

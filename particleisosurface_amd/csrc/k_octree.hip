@@ -169,10 +169,12 @@ __device__ __forceinline__ uint64_t cell_group(const GridDev& g, float x, float 
     return ((uint64_t)(iz + 1) * ey + (uint64_t)(iy + 1)) * ex + (uint64_t)(ix + 1);
 }
 
-// key of sample entry t: the point's cell group, then its lattice position modulo 8 per axis
-// (distinct inside a cell spanning fewer than 8 lattice steps; equal points have equal keys)
+// key of sample entry t: the point's cell group, then its lattice position modulo 2^lb per axis
+// (lb bits: enough for the lattice steps a cell spans). Equal points have equal keys; distinct
+// points that share one are told apart by the heads pass, which compares coordinates.
 __global__ void k_sample_keys(GridDev g, NodeLevel lv, const int* __restrict__ list, int64_t nent, float ox,
-                              float oy, float oz, int vbits, uint64_t* __restrict__ key, int* __restrict__ val) {
+                              float oy, float oz, int lb, int vbits, uint64_t* __restrict__ key,
+                              int* __restrict__ val) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nent) return;
     const int node = list[t / 27];
@@ -186,7 +188,9 @@ __global__ void k_sample_keys(GridDev g, NodeLevel lv, const int* __restrict__ l
         const float u = rintf((p[a] - o[a]) / c.w);
         q[a] = (uint32_t)fminf(fmaxf(u, 0.0f), 2097151.0f);
     }
-    const uint64_t k = (cell_group(g, p[0], p[1], p[2]) << 9) | (q[0] & 7) | ((q[1] & 7) << 3) | ((q[2] & 7) << 6);
+    const uint32_t m = (1u << lb) - 1;
+    const uint64_t k = (cell_group(g, p[0], p[1], p[2]) << (3 * lb)) | (q[0] & m) | ((q[1] & m) << lb) |
+                       ((uint64_t)(q[2] & m) << (2 * lb));
     if (vbits) {
         key[t] = (k << vbits) | (uint64_t)t;
     } else {
@@ -271,7 +275,8 @@ __global__ void __launch_bounds__(256) k_head_list(const int* __restrict__ rep, 
 
 // ---- groups of points by hash cell, and runs of groups (run_eval.h) ----
 // cflag[h] = 1 where the cell group changes along the distinct points (cell-major order)
-__global__ void k_cell_flags(SortedKV kv, const int* __restrict__ hlist, int64_t nheads, int* __restrict__ cflag) {
+__global__ void k_cell_flags(SortedKV kv, int gshift, const int* __restrict__ hlist, int64_t nheads,
+                             int* __restrict__ cflag) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h > nheads) return;
     if (h == nheads) {
@@ -280,7 +285,7 @@ __global__ void k_cell_flags(SortedKV kv, const int* __restrict__ hlist, int64_t
     }
     // hlist NULL: the points are the sorted entries themselves
     const int64_t a = hlist ? hlist[h] : h, b = h > 0 ? (hlist ? hlist[h - 1] : h - 1) : 0;
-    cflag[h] = (h == 0 || (kv.key(a) >> 9) != (kv.key(b) >> 9)) ? 1 : 0;
+    cflag[h] = (h == 0 || (kv.key(a) >> gshift) != (kv.key(b) >> gshift)) ? 1 : 0;
 }
 // cstart[k] = first distinct point of cell group k; cstart[ngroups] = nheads
 __global__ void k_cell_starts(const int* __restrict__ cflag, const int* __restrict__ cpos, int64_t nheads,
@@ -292,8 +297,8 @@ __global__ void k_cell_starts(const int* __restrict__ cflag, const int* __restri
 }
 
 // runs of cell groups: rflag[k] = 1 where the run id (cell row, x / kx) changes along the groups
-__global__ void k_run_flags(GridDev g, SortedKV kv, const int* __restrict__ hlist, const int* __restrict__ cstart,
-                            int64_t ngroups, int kx, int* __restrict__ rflag) {
+__global__ void k_run_flags(GridDev g, SortedKV kv, int gshift, const int* __restrict__ hlist,
+                            const int* __restrict__ cstart, int64_t ngroups, int kx, int* __restrict__ rflag) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k > ngroups) return;
     if (k == ngroups) {
@@ -301,7 +306,7 @@ __global__ void k_run_flags(GridDev g, SortedKV kv, const int* __restrict__ hlis
         return;
     }
     auto run_id = [&](int64_t grp) {
-        const uint64_t gid = kv.key(hlist ? hlist[cstart[grp]] : cstart[grp]) >> 9;
+        const uint64_t gid = kv.key(hlist ? hlist[cstart[grp]] : cstart[grp]) >> gshift;
         const uint64_t ex = g.n[0] + 2, ey = g.n[1] + 2;
         const int ix = (int)(gid % ex) - 1, iy = (int)((gid / ex) % ey) - 1, iz = (int)(gid / (ex * ey)) - 1;
         return run_of_cell(g, ix, iy, iz, kx);  // the empty-window group maps past every row
@@ -322,7 +327,7 @@ __global__ void k_center_keys(GridDev g, NodeLevel lv, const int* __restrict__ l
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= k) return;
     const float4 c = lv.c[list[e]];
-    const uint64_t kk = cell_group(g, c.x, c.y, c.z) << 9;
+    const uint64_t kk = cell_group(g, c.x, c.y, c.z);
     if (vbits) {
         key[e] = (kk << vbits) | (uint64_t)e;
     } else {
@@ -379,12 +384,12 @@ static void run_shape(double p, int* kx_out, int* t_out) {
     }
 }
 
-// Cell groups and runs of nh points in cell-major order (key[hlist[h]] >> 9 is the cell group of
+// Cell groups and runs of nh points in cell-major order (key[hlist[h]] >> gshift is the cell group of
 // point h; hlist NULL: point h is entry h). cwork: flags | positions | group starts, run starts
 // ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of runs, to be evaluated by blocks
 // of *t_out threads; *rstart_out points into cwork. Synchronises the stream twice (group and run
 // counts).
-static int build_runs(GridDev g, SortedKV kv, const int* hlist, int nh, int* cwork, int* tmp,
+static int build_runs(GridDev g, SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp,
                       int** rstart_out, int* kx_out, int* t_out, hipStream_t s) {
     int* flag = cwork;
     int* pos = cwork + (nh + 2);
@@ -394,7 +399,7 @@ static int build_runs(GridDev g, SortedKV kv, const int* hlist, int nh, int* cwo
     *t_out = kRunThreads;
     if (nh <= 0) return 0;
     const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
-    k_cell_flags<<<nbh, 256, 0, s>>>(kv, hlist, nh, flag);
+    k_cell_flags<<<nbh, 256, 0, s>>>(kv, gshift, hlist, nh, flag);
     exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
     k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
     int ng = 0;
@@ -403,7 +408,7 @@ static int build_runs(GridDev g, SortedKV kv, const int* hlist, int nh, int* cwo
     run_shape((double)nh / (double)std::max(ng, 1), &kx, t_out);
     *kx_out = kx;
     const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
-    k_run_flags<<<nbg, 256, 0, s>>>(g, kv, hlist, cstart, ng, kx, flag);
+    k_run_flags<<<nbg, 256, 0, s>>>(g, kv, gshift, hlist, cstart, ng, kx, flag);
     exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
     int* rstart = cstart + (ng + 1);  // (ng + 1) + (nruns + 1) <= 2 (nh + 1)
     k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
@@ -454,7 +459,12 @@ __global__ void k_sample_scatter(const int* __restrict__ list, SortedKV kv, cons
     if (j >= nent) return;
     const int e = kv.val(j);
     const int64_t dst = (int64_t)list[e / 27] * 27 + e % 27;
-    const int64_t src = hpos[rep[j]];
+    // rep[j] is the first equal entry within j's 64-entry window; in a longer run of equal keys
+    // that entry can itself have an earlier copy (its window reaches further back), so follow the
+    // chain to the head (rep[x] <= x, equal only at a head; hpos is written for heads only)
+    int r = rep[j];
+    while (rep[r] != r) r = rep[r];
+    const int64_t src = hpos[r];
     if (rsz == 8) ((double*)samples)[dst] = ((const double*)vals)[src];
     else ((float*)samples)[dst] = ((const float*)vals)[src];
 }
@@ -663,14 +673,14 @@ void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* l
 }
 
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
-                               int vbits, uint64_t* key, int* val, hipStream_t s) {
+                               int lb, int vbits, uint64_t* key, int* val, hipStream_t s) {
     if (nent <= 0) return;
     k_sample_keys<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(g, lv, list, nent, origin[0], origin[1], origin[2],
-                                                                 vbits, key, val);
+                                                                 lb, vbits, key, val);
 }
 
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
-                               const int* list, SortedKV kv, int64_t nent, int* head,
+                               const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
                                hipEvent_t e1, hipStream_t s) {
@@ -690,7 +700,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     *nheads_out = nh;
     int* rstart = nullptr;
     int kx = 0, T = kRunThreads;
-    const int nr = build_runs(g, kv, hlist, nh, cwork, tmp, &rstart, &kx, &T, s);
+    const int nr = build_runs(g, kv, gshift, hlist, nh, cwork, tmp, &rstart, &kx, &T, s);
     *nruns_out = nr;
     *kx_out = kx;
     (void)hipEventRecord(e0, s);
@@ -743,12 +753,12 @@ void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, Fi
     const uint64_t groups = (uint64_t)(g.n[0] + 2) * (uint64_t)(g.n[1] + 2) * (uint64_t)(g.n[2] + 2);
     int bits = 1;
     while (bits < 55 && ((uint64_t)1 << bits) <= groups) bits++;
-    const int vbits = pack_vbits(bits + 9, k);
+    const int vbits = pack_vbits(bits, k);
     k_center_keys<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(g, lv, list, k, vbits, key, val);
-    const SortedKV kv = radix_sort_entries(key, val, key2, val2, k, bits + 9, vbits, tmp, s);
+    const SortedKV kv = radix_sort_entries(key, val, key2, val2, k, bits, vbits, tmp, s);
     int* rstart = nullptr;
     int kx = 0, T = kRunThreads;
-    const int nr = build_runs(g, kv, nullptr, (int)k, cwork, tmp, &rstart, &kx, &T, s);
+    const int nr = build_runs(g, kv, 0, nullptr, (int)k, cwork, tmp, &rstart, &kx, &T, s);
     if (nr <= 0) return;
     auto go = [&](auto tag, auto mode) {
         constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;

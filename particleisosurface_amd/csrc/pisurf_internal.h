@@ -147,14 +147,15 @@ void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec
                          const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
                          int* tmp, void* samples, Counters* ctr, hipStream_t s);
 // sample deduplication: keys for the nent = 27 * listed entries (hash cell group, then the
-// lattice position modulo 8), then (after the key sort) heads, the distinct points grouped into
-// runs of cells, one workgroup per run between events e0 / e1, and the scatter to samples. head
+// lattice position modulo 2^lb per axis; the group starts at bit gshift = 3 lb), then (after the
+// key sort) heads, the distinct points grouped into runs of cells, one workgroup per run between
+// events e0 / e1, and the scatter to samples. head
 // holds 2 (nent / 256 + 2) ints, hpos / rep / hlist nent, cwork 4 (nent + 2); kx_out is the
 // number of cells of a row per run.
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
-                               int vbits, uint64_t* key, int* val, hipStream_t s);
+                               int lb, int vbits, uint64_t* key, int* val, hipStream_t s);
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
-                               const int* list, SortedKV kv, int64_t nent, int* head,
+                               const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
                                hipEvent_t e1, hipStream_t s);
