@@ -1,4 +1,4 @@
-// Input front-end of the reference's frame driver: controlData.json (src/main.cpp:116-213), CSV
+// Input front-end of the reference's frame loop program: controlData.json (src/main.cpp:116-213), CSV
 // particle tables (src/main.cpp:215-279) and ShonDy HDF5 frames (src/main.cpp:281-328,
 // src/hdf5Utils.cpp:50-77), plus raw float32 triples. Header-only C++17 without dependencies: the
 // JSON subset is parsed here, and libhdf5 is loaded at run time (dlopen) only when an .h5 frame is
@@ -27,7 +27,7 @@ struct IoError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
-// The driver globals controlData.json sets, with the reference's defaults (src/main.cpp:33-55).
+// The frame-loop globals controlData.json sets, with the reference's defaults (src/main.cpp:33-55).
 struct Config {
     bool use_ani = true, use_ours = true, use_poly6 = false, calc_p_normal = true, gen_splash = true;
     bool need_record = false;
@@ -96,7 +96,7 @@ inline std::map<std::string, JsonValue> parse_flat_json(const std::string& text)
             const char* p = text.c_str() + i;
             char* end = nullptr;
             v.num = std::strtod(p, &end);
-            if (end == p) fail("unsupported value for " + key + " (the driver reads scalars and strings only)");
+            if (end == p) fail("unsupported value for " + key + " (the frame loop reads scalars and strings only)");
             v.kind = JsonValue::NUM;
             i += (size_t)(end - p);
         }

@@ -234,7 +234,7 @@ SortedKV radix_sort_entries(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_
 
 // ---------------------------------------------------------------------------------------------
 // Small read-backs (counts, bounding box) in the frame's level loop. The wait polls an event
-// instead of sleeping in the driver, so a count that is ready a few microseconds after the copy
+// instead of blocking in the HIP runtime, so a count that is ready a few microseconds after the copy
 // is seen then, not after an interrupt wake-up; the copy lands in pinned memory so it is a plain
 // asynchronous transfer. Each host thread keeps its own staging buffer and event until the
 // process ends.
