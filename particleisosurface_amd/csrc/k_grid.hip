@@ -50,6 +50,29 @@ __global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restri
     }
 }
 
+// Second pass for up to kScanDirect tiles: each tile adds the sum of the tile totals before it,
+// reduced by the block itself, so the totals need no scan of their own (one launch fewer).
+constexpr int kScanDirect = 2048;
+__global__ void __launch_bounds__(kScanT) k_scan_add_prefix(int* __restrict__ out, int64_t n,
+                                                            const int* __restrict__ sums) {
+    __shared__ int wsum[kScanT / kWave];
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0) return;  // uniform over the block
+    int a = 0;
+    for (int k = t; k < (int)blockIdx.x; k += kScanT) a += sums[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, kWave);
+    if ((t & 63) == 0) wsum[t >> 6] = a;
+    __syncthreads();
+    a = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const int64_t i = (int64_t)blockIdx.x * kScanTile + t;
+#pragma unroll
+    for (int k = 0; k < kScanI; k++) {
+        const int64_t j = i + (int64_t)k * kScanT;
+        if (j < n) out[j] += a;
+    }
+}
+
 int64_t scan_tmp_ints(int64_t n) {
     int64_t tot = 0;
     while (n > 1) {
@@ -65,7 +88,9 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_
     int* sums = tmp;
     int* sums_scanned = tmp + nb + 1;
     k_scan_tile<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(in, out, n, sums);
-    if (nb > 1) {
+    if (nb > 1 && nb <= kScanDirect) {
+        k_scan_add_prefix<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums);
+    } else if (nb > 1) {
         exclusive_scan_i32(sums, sums_scanned, nb, tmp + 2 * nb + 2, s);
         k_scan_add<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums_scanned);
     }
