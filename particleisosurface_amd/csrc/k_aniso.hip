@@ -35,19 +35,29 @@ __global__ void __launch_bounds__(256) k_xmean(GridDev g, const float4* __restri
     // compensated FP32 sums the offsets x_j - x_i (exact for nearby floats) instead of x_j
     constexpr bool REL = K && sizeof(R) == 4;
     Acc<R, K> sx, sy, sz, sw;
+    auto cand = [&](const float4 q) {
+        const R dx = R(q.x) - px, dy = R(q.y) - py, dz = R(q.z) - pz;
+        const R d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 >= nI2) return;
+        const R w = wij<R>(dsqrt<R>(d2), nI, inv_h3);
+        sw.add(w);
+        sx.add((REL ? dx : R(q.x)) * w);
+        sy.add((REL ? dy : R(q.y)) * w);
+        sz.add((REL ? dz : R(q.z)) * w);
+    };
     for (int r = 0; r < 9; r++) {
-        for (int j = lo[r]; j < hi[r]; j++) {
-            if (j == (int)p) continue;
-            const float4 q = spos[j];
-            const R dx = R(q.x) - px, dy = R(q.y) - py, dz = R(q.z) - pz;
-            const R d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 >= nI2) continue;
-            const R w = wij<R>(dsqrt<R>(d2), nI, inv_h3);
-            sw.add(w);
-            sx.add((REL ? dx : R(q.x)) * w);
-            sy.add((REL ? dy : R(q.y)) * w);
-            sz.add((REL ? dz : R(q.z)) * w);
+        int j = lo[r];
+        // four candidates' positions are loaded before any is used, so the loads overlap
+        for (; j + 4 <= hi[r]; j += 4) {
+            float4 q[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) q[k] = spos[j + k];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (j + k != (int)p) cand(q[k]);
         }
+        for (; j < hi[r]; j++)
+            if (j != (int)p) cand(spos[j]);
     }
     R ox = px, oy = py, oz = pz;
     const R ws = sw.get();
@@ -108,37 +118,63 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
     const float4 li = REL ? ((const float4*)xm)[n + p] : make_float4(0.f, 0.f, 0.f, 0.f);
     Acc<R, K> c[6], sw;
     const R base = ap.use_poly6 ? R(1.5) : h2;
+    auto cov_add = [&](R wx, R wy, R wz, R w) {
+        sw.add(w);
+        c[0].add(wx * wx * w);
+        c[1].add(wx * wy * w);
+        c[2].add(wx * wz * w);
+        c[3].add(wy * wy * w);
+        c[4].add(wy * wz * w);
+        c[5].add(wz * wz * w);
+    };
+    // default weight: wij of |xMean_j - xMean_i|; (hi_j - hi_i) is exact for nearby floats, and
+    // compensated FP32 adds the residuals
+    auto cand_xm = [&](const typename R4<R>::T xj, const float4 lj) {
+        R wx = xj.x - xi.x, wy = xj.y - xi.y, wz = xj.z - xi.z;
+        if (REL) {
+            wx += R(lj.x - li.x);
+            wy += R(lj.y - li.y);
+            wz += R(lj.z - li.z);
+        }
+        const R d2 = wx * wx + wy * wy + wz * wz;
+        if (d2 >= nI2) return;
+        cov_add(wx, wy, wz, wij<R>(dsqrt<R>(d2), nI, inv_h3));
+    };
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int r = 0; r < 9; r++) {
         int lo, hi;
         if (!row_range(g, ix, iy, iz, r, lo, hi)) continue;
-        for (int j = lo; j < hi; j++) {
+        int j = lo;
+        if (!ap.use_poly6) {
+            // four candidates' means are loaded before any is used, so the loads overlap
+            for (; j + 4 <= hi; j += 4) {
+                typename R4<R>::T x4[4];
+                float4 l4[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    x4[k] = xm[j + k];
+                    l4[k] = REL ? ((const float4*)xm)[n + j + k] : zero4;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) cand_xm(x4[k], l4[k]);
+            }
+            for (; j < hi; j++) cand_xm(xm[j], REL ? ((const float4*)xm)[n + j] : zero4);
+            continue;
+        }
+        for (; j < hi; j++) {
             const auto xj = xm[j];
             R wx = xj.x - xi.x, wy = xj.y - xi.y, wz = xj.z - xi.z;
-            if (REL) {  // (hi_j - hi_i) is exact for nearby floats; add the residuals
+            if (REL) {
                 const float4 lj = ((const float4*)xm)[n + j];
                 wx += R(lj.x - li.x);
                 wy += R(lj.y - li.y);
                 wz += R(lj.z - li.z);
             }
-            R w;
-            if (!ap.use_poly6) {
-                const R d2 = wx * wx + wy * wy + wz * wz;
-                if (d2 >= nI2) continue;
-                w = wij<R>(dsqrt<R>(d2), nI, inv_h3);
-            } else {
-                const float4 q = spos[j];
-                const R ex = R(q.x) - R(pp.x), ey = R(q.y) - R(pp.y), ez = R(q.z) - R(pp.z);
-                if (ex * ex + ey * ey + ez * ez >= nI2) continue;
-                const R fx = R(pp.x) - xj.x, fy = R(pp.y) - xj.y, fz = R(pp.z) - xj.z;
-                w = wij<R>(dsqrt<R>(fx * fx + fy * fy + fz * fz), nI, inv_h3);
-            }
-            sw.add(w);
-            c[0].add(wx * wx * w);
-            c[1].add(wx * wy * w);
-            c[2].add(wx * wz * w);
-            c[3].add(wy * wy * w);
-            c[4].add(wy * wz * w);
-            c[5].add(wz * wz * w);
+            const float4 q = spos[j];
+            const R ex = R(q.x) - R(pp.x), ey = R(q.y) - R(pp.y), ez = R(q.z) - R(pp.z);
+            if (ex * ex + ey * ey + ez * ez >= nI2) continue;
+            const R fx = R(pp.x) - xj.x, fy = R(pp.y) - xj.y, fz = R(pp.z) - xj.z;
+            cov_add(wx, wy, wz, wij<R>(dsqrt<R>(fx * fx + fy * fy + fz * fz), nI, inv_h3));
         }
     }
     const R ws = sw.get();
