@@ -34,6 +34,7 @@ __global__ void __launch_bounds__(256) k_xmean(GridDev g, const float4* __restri
     const R nI = ap.nI, nI2 = ap.nI2, inv_h3 = R(1) / (nI * nI * nI);
     // compensated FP32 sums the offsets x_j - x_i (exact for nearby floats) instead of x_j
     constexpr bool REL = K && sizeof(R) == 4;
+    constexpr int kLoadAhead = sizeof(R) == 4 ? 8 : 4;
     Acc<R, K> sx, sy, sz, sw;
     auto cand = [&](const float4 q) {
         const R dx = R(q.x) - px, dy = R(q.y) - py, dz = R(q.z) - pz;
@@ -47,13 +48,13 @@ __global__ void __launch_bounds__(256) k_xmean(GridDev g, const float4* __restri
     };
     for (int r = 0; r < 9; r++) {
         int j = lo[r];
-        // four candidates' positions are loaded before any is used, so the loads overlap
-        for (; j + 4 <= hi[r]; j += 4) {
-            float4 q[4];
+        // kLoadAhead candidates' positions are loaded before any is used, so the loads overlap
+        for (; j + kLoadAhead <= hi[r]; j += kLoadAhead) {
+            float4 q[kLoadAhead];
 #pragma unroll
-            for (int k = 0; k < 4; k++) q[k] = spos[j + k];
+            for (int k = 0; k < kLoadAhead; k++) q[k] = spos[j + k];
 #pragma unroll
-            for (int k = 0; k < 4; k++)
+            for (int k = 0; k < kLoadAhead; k++)
                 if (j + k != (int)p) cand(q[k]);
         }
         for (; j < hi[r]; j++)
@@ -141,22 +142,23 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
         cov_add(wx, wy, wz, wij<R>(dsqrt<R>(d2), nI, inv_h3));
     };
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    constexpr int kLoadAhead = sizeof(R) == 4 ? 8 : 4;
     for (int r = 0; r < 9; r++) {
         int lo, hi;
         if (!row_range(g, ix, iy, iz, r, lo, hi)) continue;
         int j = lo;
         if (!ap.use_poly6) {
-            // four candidates' means are loaded before any is used, so the loads overlap
-            for (; j + 4 <= hi; j += 4) {
-                typename R4<R>::T x4[4];
-                float4 l4[4];
+            // kLoadAhead candidates' means are loaded before any is used, so the loads overlap
+            for (; j + kLoadAhead <= hi; j += kLoadAhead) {
+                typename R4<R>::T x4[kLoadAhead];
+                float4 l4[kLoadAhead];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
+                for (int k = 0; k < kLoadAhead; k++) {
                     x4[k] = xm[j + k];
                     l4[k] = REL ? ((const float4*)xm)[n + j + k] : zero4;
                 }
 #pragma unroll
-                for (int k = 0; k < 4; k++) cand_xm(x4[k], l4[k]);
+                for (int k = 0; k < kLoadAhead; k++) cand_xm(x4[k], l4[k]);
             }
             for (; j < hi; j++) cand_xm(xm[j], REL ? ((const float4*)xm)[n + j] : zero4);
             continue;
