@@ -110,16 +110,43 @@ __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __r
             const int y = lo[1] + row % ny, z = lo[2] + row / ny;
             const int base = (z * g.n[1] + y) * g.n[0];
             const int j0 = g.off[base + lo[0]], j1 = g.off[base + hi[0] + 1];
-            for (int j = j0; j < j1; j++) {
-                const float4 p = spos[j];
-                if (p.x > lx[0] && p.x < hx[0] && p.y > lx[1] && p.y < hx[1] && p.z > lx[2] && p.z < hx[2]) {
+            auto inside = [&](const float4 p) {
+                return p.x > lx[0] && p.x < hx[0] && p.y > lx[1] && p.y < hx[1] && p.z > lx[2] && p.z < hx[2];
+            };
+            auto add_normal = [&](int j) {
+                const float4 n = nrm[j];
+                sx += n.x;
+                sy += n.y;
+                sz += n.z;
+                area += sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+            };
+            int j = j0;
+            // four positions are loaded before any is tested, so the loads overlap
+            for (; j + 4 <= j1; j += 4) {
+                float4 q[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) q[k] = spos[j + k];
+                bool in[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) in[k] = inside(q[k]);
+                if (!need_curv) {
+                    if (in[0] || in[1] || in[2] || in[3]) {
+                        found = true;
+                        break;
+                    }
+                    continue;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (in[k]) {
+                        found = true;
+                        add_normal(j + k);
+                    }
+            }
+            for (; j < j1 && !(found && !need_curv); j++) {
+                if (inside(spos[j])) {
                     found = true;
-                    if (!need_curv) break;
-                    const float4 n = nrm[j];
-                    sx += n.x;
-                    sy += n.y;
-                    sz += n.z;
-                    area += sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+                    if (need_curv) add_normal(j);
                 }
             }
         }
