@@ -95,17 +95,15 @@ __global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ l
     if ((threadIdx.x & 63) == 0 && valid) atomicAdd(n_valid + 32 * (blockIdx.x & 63), valid);
 }
 
-// leaf containing the finest cell q (descent from the root)
-__device__ __forceinline__ int locate(const TreeView& tv, const int* lidx, int qx, int qy, int qz) {
-    int L = 0;
-    int64_t i = 0;
-    for (int guard = 0; guard < tv.nlev; guard++) {
+// leaf containing the finest cell q, descending from node i of level L (the root: 0, 0)
+__device__ __forceinline__ int locate_from(const TreeView& tv, const int* lidx, int L, int64_t i, int qx, int qy,
+                                           int qz) {
+    for (; L < tv.nlev; L++) {
         const uint8_t t = tv.type[L][i];
         if (is_leaf_type(t)) return lidx[tv.off[L] + i];
         const int sh = tv.dp - L - 1;
         const int b = ((qx >> sh) & 1) | (((qy >> sh) & 1) << 1) | (((qz >> sh) & 1) << 2);
         i = (int64_t)tv.child[L][i] + b;
-        L++;
     }
     return -1;
 }
@@ -121,9 +119,35 @@ __device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx
         n[(v & 7) ^ 7] = v >> 3;  // a leaf with P as corner c lies in octant c ^ 7 of P
     }
     const int px = (int)morton_compact(key), py = (int)morton_compact(key >> 1), pz = (int)morton_compact(key >> 2);
+    bool missing = false;
+    for (int o = 0; o < 8; o++) missing = missing || n[o] < 0;
+    if (missing) {
+        // The finest cells of the 8 octants, P - 1 or P per axis, take the same child of a node
+        // whose children have size 2^sh as long as no coordinate of P is a multiple of 2^sh, so
+        // their descents share the path down to that node and are walked once.
+        int L = 0;
+        int64_t i = 0;
+        int shared = -1;
+        for (; L < tv.nlev; L++) {
+            const uint8_t t = tv.type[L][i];
+            if (is_leaf_type(t)) {
+                shared = lidx[tv.off[L] + i];
+                break;
+            }
+            const int sh = tv.dp - L - 1;
+            if (sh <= 0) break;
+            const int m = (1 << sh) - 1;
+            if ((px & m) == 0 || (py & m) == 0 || (pz & m) == 0) break;
+            const int b = ((px >> sh) & 1) | (((py >> sh) & 1) << 1) | (((pz >> sh) & 1) << 2);
+            i = (int64_t)tv.child[L][i] + b;
+        }
+        for (int o = 0; o < 8; o++)
+            if (n[o] < 0)
+                n[o] = shared >= 0 ? shared
+                                   : locate_from(tv, lidx, L, i, px - 1 + (o & 1), py - 1 + ((o >> 1) & 1),
+                                                 pz - 1 + ((o >> 2) & 1));
+    }
     int any = -1;
-    for (int o = 0; o < 8; o++)
-        if (n[o] < 0) n[o] = locate(tv, lidx, px - 1 + (o & 1), py - 1 + ((o >> 1) & 1), pz - 1 + ((o >> 2) & 1));
     for (int o = 0; o < 8; o++)
         if (n[o] >= 0) any = n[o];
     for (int o = 0; o < 8; o++)
