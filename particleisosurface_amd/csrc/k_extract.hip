@@ -296,30 +296,50 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k) {  // SplitMix64 finaliser
     return k;
 }
 // slot[i] = table slot of key[i]; table[slot] = smallest i with that key
+// An entry whose key equals that of one of the 7 entries before it in the same wave takes that
+// entry's slot without probing (consecutive triangle corners repeat vertices); the table keeps the
+// smallest index of each key, which such an entry never holds.
 __global__ void k_hash_insert(const uint64_t* __restrict__ key, int64_t n, unsigned* table, uint64_t mask,
                               int* __restrict__ slot) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t k = key[i];
-    uint64_t h = mix64(k) & mask;
-    for (uint64_t probe = 0; probe <= mask; probe++) {
-        // a plain read may be stale: an empty slot is claimed only through the exchange, and a
-        // slot's index only decreases among indices of one key
-        unsigned cur = table[h];
-        if (cur == kSlotEmpty) {
-            cur = atomicCAS(&table[h], kSlotEmpty, (unsigned)i);
-            if (cur == kSlotEmpty) {
-                slot[i] = (int)h;
-                return;
-            }
-        }
-        if (key[cur] == k) {
-            if (cur > (unsigned)i) atomicMin(&table[h], (unsigned)i);
-            slot[i] = (int)h;
-            return;
-        }
-        h = (h + 1) & mask;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool act = i < n;
+    const uint64_t k = act ? key[i] : 0;
+    const unsigned klo = (unsigned)k, khi = (unsigned)(k >> 32);
+    // root: the earliest lane among this one and the 7 before it with the same key
+    int root = lane;
+#pragma unroll
+    for (int o = 7; o >= 1; o--) {
+        const unsigned lo = __shfl_up(klo, o, kWave), hi = __shfl_up(khi, o, kWave);
+        if (root == lane && lane >= o && lo == klo && hi == khi) root = lane - o;
     }
+    // follow roots to a lane that is its own root (pointer jumping; roots only point back)
+#pragma unroll
+    for (int it = 0; it < 6; it++) root = __shfl(root, root, kWave);
+    int h_found = -1;
+    if (act && root == lane) {
+        uint64_t h = mix64(k) & mask;
+        for (uint64_t probe = 0; probe <= mask; probe++) {
+            // a plain read may be stale: an empty slot is claimed only through the exchange, and a
+            // slot's index only decreases among indices of one key
+            unsigned cur = table[h];
+            if (cur == kSlotEmpty) {
+                cur = atomicCAS(&table[h], kSlotEmpty, (unsigned)i);
+                if (cur == kSlotEmpty) {
+                    h_found = (int)h;
+                    break;
+                }
+            }
+            if (key[cur] == k) {
+                if (cur > (unsigned)i) atomicMin(&table[h], (unsigned)i);
+                h_found = (int)h;
+                break;
+            }
+            h = (h + 1) & mask;
+        }
+    }
+    const int hs = __shfl(h_found, root, kWave);
+    if (act) slot[i] = hs;
 }
 __global__ void k_hash_first(const int* __restrict__ slot, const unsigned* __restrict__ table, int64_t n,
                              int* __restrict__ first) {
