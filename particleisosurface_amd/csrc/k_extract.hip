@@ -296,9 +296,10 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k) {  // SplitMix64 finaliser
     return k;
 }
 // slot[i] = table slot of key[i]; table[slot] = smallest i with that key
-// An entry whose key equals that of one of the 7 entries before it in the same wave takes that
-// entry's slot without probing (consecutive triangle corners repeat vertices); the table keeps the
-// smallest index of each key, which such an entry never holds.
+// An entry whose key equals that of one of the kDedupeWindow entries before it in the same wave
+// takes that entry's slot without probing (nearby triangle corners repeat vertices); the table keeps
+// the smallest index of each key, which such an entry never holds.
+constexpr int kDedupeWindow = 31;
 __global__ void k_hash_insert(const uint64_t* __restrict__ key, int64_t n, unsigned* table, uint64_t mask,
                               int* __restrict__ slot) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,10 +307,10 @@ __global__ void k_hash_insert(const uint64_t* __restrict__ key, int64_t n, unsig
     const bool act = i < n;
     const uint64_t k = act ? key[i] : 0;
     const unsigned klo = (unsigned)k, khi = (unsigned)(k >> 32);
-    // root: the earliest lane among this one and the 7 before it with the same key
+    // root: the earliest lane among this one and the kDedupeWindow before it with the same key
     int root = lane;
 #pragma unroll
-    for (int o = 7; o >= 1; o--) {
+    for (int o = kDedupeWindow; o >= 1; o--) {
         const unsigned lo = __shfl_up(klo, o, kWave), hi = __shfl_up(khi, o, kWave);
         if (root == lane && lane >= o && lo == klo && hi == khi) root = lane - o;
     }
