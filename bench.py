@@ -106,6 +106,24 @@ def mesh_error(v1, t1, v2, t2, r):
                 frac_within_1e3r=float(np.mean(d <= 1e-3)))
 
 
+def aniso_roofline(ctx, ms_aniso, n):
+    """Algorithmic bytes of the two anisotropy passes (xMean, then covariance and G) per particle,
+    [12 + 216 + 12 C + 12] + [12 + 216 + 12 C + 40] with C the candidates of its 27-cell window,
+    over the anisotropy phase of a timed frame (host clock around both launches)."""
+    g = ctx.hashgrid_build()
+    _, s, e = ctx.hashgrid_download(g["cellnum"])
+    nx, ny, nz = (int(v) for v in g["cellnum3"])
+    cnt = (e - s).astype(np.int64).reshape(nz, ny, nx)
+    p = np.pad(cnt, 1)
+    win = sum(p[z:z + nz, y:y + ny, x:x + nx] for z in range(3) for y in range(3) for x in range(3))
+    cand = int((cnt * win).sum())
+    alg = n * (12 + 216 + 12 + 12 + 216 + 40) + 24 * cand
+    achieved = alg / (ms_aniso * 1e-3) / 1e9
+    return dict(kernels="k_xmean + k_gmat", bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), phase_ms=ms_aniso,
+                candidates_per_particle=cand / max(n, 1), algorithmic_bytes=alg)
+
+
 def cpu_baseline(pts_all, xyz, r, ctx, budget_s, log_every=True):
     from oracle import pyoracle
     o = pyoracle.Oracle(xyz, r, fp64=False)
@@ -334,6 +352,7 @@ def main():
         log("cpu baseline %.3g evals/s on %d threads" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"]))
     else:
         out["cpu_baseline"] = None
+    out["aniso_roofline"] = aniso_roofline(ctx, out["phases_ms"]["aniso"], len(xyz))
     ctx.close()
     print(json.dumps(out), flush=True)
 
