@@ -238,6 +238,7 @@ struct pisurf_ctx {
     DBuf xyz_next;
     PinnedBuf pin;
     hipStream_t cstream = nullptr;
+    hipEvent_t cev = nullptr;  // end of a staged copy on cstream (waited for by the staging thread)
     // mesh download stream and the events around the bisection
     hipStream_t dstream = nullptr;
     hipEvent_t eb0 = nullptr, eb1 = nullptr;
@@ -260,6 +261,7 @@ struct pisurf_ctx {
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
         if (cstream) (void)hipStreamDestroy(cstream);
+        if (cev) (void)hipEventDestroy(cev);
         if (dstream) (void)hipStreamDestroy(dstream);
         if (eb0) (void)hipEventDestroy(eb0);
         if (eb1) (void)hipEventDestroy(eb1);
@@ -1418,6 +1420,18 @@ int pisurf_neighbors(pisurf_ctx* c, const float* q, int64_t nq, int64_t* offsets
     if (nq > 0) HIPCHK(c, hipMemcpyAsync(c->q.p, q, nq * 12, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->out1.as<int>() + nq, 0, 4, c->stream));
     launch_neighbor_count(c->g, c->q.as<float>(), nq, c->out1.as<int>(), c->stream);
+    // the offsets are scanned in 32 bits: refuse totals they cannot hold
+    HIPCHK(c, c->small.ensure(256));
+    unsigned long long* tot_d = (unsigned long long*)(c->small.as<char>() + 192);
+    HIPCHK(c, hipMemsetAsync(tot_d, 0, 8, c->stream));
+    launch_sum_counts(c->out1.as<int>(), nq, tot_d, c->stream);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long tot64 = 0;
+    HIPCHK(c, read_device(&tot64, tot_d, 8, c->stream));
+    if (tot64 >= (unsigned long long)INT32_MAX) {
+        if (total) *total = (int64_t)tot64;
+        return c->fail(PISURF_E_UNSUPPORTED, "neighbour lists total 2^31 ids or more: query fewer points per call");
+    }
     exclusive_scan_i32(c->out1.as<int>(), c->out2.as<int>(), nq + 1, c->tmp.as<int>(), c->stream);
     std::vector<int> off32(nq + 1);
     HIPCHK(c, hipMemcpyAsync(off32.data(), c->out2.p, (nq + 1) * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1852,6 +1866,7 @@ int pisurf_run_frames(pisurf_ctx* c, int32_t nframes, const float* const* xyz, c
     API_BEGIN
     (void)hipSetDevice(c->device);
     if (!c->cstream) HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+    if (!c->cev) HIPCHK(c, hipEventCreateWithFlags(&c->cev, hipEventDisableTiming));
     const size_t bytes_max = (size_t)nmax * 3 * sizeof(float);
     HIPCHK(c, stream_wait(c->stream));
     HIPCHK(c, c->xyz.ensure(bytes_max));
@@ -1869,7 +1884,8 @@ int pisurf_run_frames(pisurf_ctx* c, int32_t nframes, const float* const* xyz, c
             std::memcpy(c->pin.p, xyz[k], b);
             e = hipMemcpyAsync(c->xyz_next.p, c->pin.p, b, hipMemcpyHostToDevice, c->cstream);
         }
-        return e != hipSuccess ? e : stream_wait(c->cstream);
+        // the context's event, not the thread's: the staging threads create no HIP objects
+        return e != hipSuccess ? e : stream_wait_on(c->cstream, c->cev);
     };
     struct Joiner {  // the helper is joined on every exit path, exceptions included
         std::thread t;

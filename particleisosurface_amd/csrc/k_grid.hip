@@ -272,13 +272,18 @@ struct Readback {
 thread_local Readback tl_readback;
 }  // namespace
 
+hipError_t stream_wait_on(hipStream_t s, hipEvent_t ev) {
+    hipError_t e;
+    if ((e = hipEventRecord(ev, s)) != hipSuccess) return e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) __builtin_ia32_pause();
+    return e;
+}
+
 hipError_t stream_wait(hipStream_t s) {
     Readback& r = tl_readback;
     hipError_t e;
     if (!r.ev && (e = hipEventCreateWithFlags(&r.ev, hipEventDisableTiming)) != hipSuccess) return e;
-    if ((e = hipEventRecord(r.ev, s)) != hipSuccess) return e;
-    while ((e = hipEventQuery(r.ev)) == hipErrorNotReady) __builtin_ia32_pause();
-    return e;
+    return stream_wait_on(s, r.ev);
 }
 
 hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s) {
@@ -459,6 +464,21 @@ __global__ void k_nb_write(GridDev g, const float* __restrict__ q, int64_t nq, c
 void launch_neighbor_count(GridDev g, const float* q, int64_t nq, int* counts, hipStream_t s) {
     if (nq <= 0) return;
     k_nb_count<<<(unsigned)((nq + 255) / 256), 256, 0, s>>>(g, q, nq, counts);
+}
+// 64-bit total of n non-negative counts, added to *out (one atomic per wave)
+__global__ void __launch_bounds__(256) k_sum_counts(const int* __restrict__ c, int64_t n,
+                                                    unsigned long long* __restrict__ out) {
+    unsigned long long v = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v += (unsigned)c[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+void launch_sum_counts(const int* counts, int64_t n, unsigned long long* out, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 2048);
+    k_sum_counts<<<(unsigned)blocks, 256, 0, s>>>(counts, n, out);
 }
 void launch_neighbor_write(GridDev g, const float* q, int64_t nq, const int* offsets, const int* sidx, int* ids,
                            hipStream_t s) {

@@ -38,8 +38,10 @@ SortedKV radix_sort_entries(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_
                             hipStream_t s);
 
 // ---- read-backs ----
-// wait for the stream's work so far by polling an event (no driver sleep)
+// wait for the stream's work so far by polling an event (no driver sleep); stream_wait uses an
+// event of the calling thread (kept until the process ends), stream_wait_on the caller's event
 hipError_t stream_wait(hipStream_t s);
+hipError_t stream_wait_on(hipStream_t s, hipEvent_t ev);
 // copy bytes (<= kReadbackBytes) of device memory to host after the stream's earlier work
 constexpr size_t kReadbackBytes = 16384;
 hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s);
@@ -55,6 +57,8 @@ void launch_gather_pos(const float* xyz, const int* sidx, int64_t n, float4* spo
 void launch_neighbor_count(GridDev g, const float* q, int64_t nq, int* counts, hipStream_t s);
 void launch_neighbor_write(GridDev g, const float* q, int64_t nq, const int* offsets, const int* sidx, int* ids,
                            hipStream_t s);
+// *out += sum of n non-negative counts, in 64 bits
+void launch_sum_counts(const int* counts, int64_t n, unsigned long long* out, hipStream_t s);
 void launch_box_count(GridDev g, const int* lo3, const int* n3, int64_t ncells, int* counts, hipStream_t s);
 void launch_box_write(GridDev g, const int* lo3, const int* n3, int64_t ncells, const int* offsets, const int* sidx,
                       int* ids, hipStream_t s);
