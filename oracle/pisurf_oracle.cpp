@@ -249,7 +249,9 @@ template <class T> void jacobi_eig4(const T a_in[4][4], T lam[4], T Q[4][4]) {
 
 // SVD tail of compute_G_Yus / compute_G_ours (src/evaluator.cpp:627-651, :556-580): singular
 // values |lambda| clamped below at max/kr (kr = 4), G = V diag(1/w) U^T * cbrt(w0 w1 w2)
-// (times 0.5 in poly6 mode); U = V = Q for the symmetric positive-definite covariance.
+// (times 0.5 in poly6 mode); U = V = Q for the symmetric positive-definite covariance. G is
+// symmetric in exact arithmetic; the upper triangle is computed and mirrored (Eigen's bit
+// pattern, whose U and V differ by rounding, is not reproducible without Eigen).
 template <class T> void g_from_cov(const T cov[3][3], bool poly6_mode, M3<T>& G) {
     T lam[3], Q[3][3];
     jacobi_eig3<T>(cov, lam, Q);
@@ -261,10 +263,11 @@ template <class T> void g_from_cov(const T cov[3][3], bool poly6_mode, M3<T>& G)
     T inv[3] = {T(1) / w[0], T(1) / w[1], T(1) / w[2]};
     T scale = T(std::pow(double((w[0] * w[1]) * w[2]), 1.0 / 3.0));
     for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) {
+        for (int b = a; b < 3; b++) {
             T s = ((Q[a][0] * inv[0]) * Q[b][0] + (Q[a][1] * inv[1]) * Q[b][1]) + (Q[a][2] * inv[2]) * Q[b][2];
             G(a, b) = s * scale;
             if (poly6_mode) G(a, b) = G(a, b) * T(0.5);
+            G(b, a) = G(a, b);
         }
 }
 

@@ -519,6 +519,7 @@ int do_aniso(pisurf_ctx* c) {
     ap.xmean_delta = 0.9f;
     ap.use_poly6 = c->prm.use_poly6;
     ap.use_ani = c->prm.use_ani;
+    ap.nI3 = std::pow(double(ap.nI), 3);
     launch_xmean(prec, c->g, c->spos.as<float4>(), c->n, c->ginfo.n_invalid, ap, c->xm.p, c->stream);
     unsigned long long* nfp = (unsigned long long*)(c->small.as<char>() + 128);
     launch_gmat(prec, c->g, c->spos.as<float4>(), c->n, c->ginfo.n_invalid, ap, c->xm.p, c->rec_a.p, c->rec_g.p,

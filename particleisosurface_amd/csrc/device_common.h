@@ -142,6 +142,41 @@ struct FieldConst {
     const float4* lo;
 };
 
+// Square roots that parity depends on are written __builtin_sqrtf: on gfx950 it lowers to the
+// hardware square root followed by a one-ulp correction (correctly rounded), while __fsqrt_rn
+// lowers to the bare, scaled v_sqrt_f32.
+//
+// FP32 without compensation (PISURF_FP32) is the reference's float arithmetic, term by term:
+// Bspline_kernel / poly6_kernel return sigma times the shape, so each term carries sigma and the
+// sums are finished without it (out_sigma = 1); the other modes sum shapes and scale once.
+template <class R, bool K> __host__ __device__ constexpr bool ref_float() { return sizeof(R) == 4 && !K; }
+template <class R, bool K> __device__ __forceinline__ R out_sigma(const FieldConst& fc) {
+    return ref_float<R, K>() ? R(1) : R(fc.sigma);
+}
+
+// Bspline_kernel (src/evaluator.cpp:383-396) as the reference evaluates it: the float ratio
+// promoted to double by pow (pow(q, 2) and pow(q, 3) are q*q and (q*q)*q rounded once, q*q being
+// exact for a float q), 2 - ratio taken in float, the result narrowed to float
+__device__ __forceinline__ float ref_bspline(float ratio, float sigma) {
+#pragma clang fp contract(off)
+    if (ratio < 2.0f) {
+        if (ratio <= 1.0f) {
+            const double q = ratio;
+            return float(double(sigma) * (1.0 - 1.5 * (q * q) + 0.75 * (q * q * q)));
+        }
+        const double t = double(2.0f - ratio);
+        return float(double(sigma) * (0.25 * (t * t * t)));
+    }
+    return 0.0f;
+}
+// poly6_kernel (src/evaluator.cpp:377-381): (h2 - d2)^3 in double, narrowed, times sigma in float
+__device__ __forceinline__ float ref_poly6(float d2, float h2, float sigma) {
+#pragma clang fp contract(off)
+    const double t = double(h2 - d2);
+    const float p = (d2 > h2) ? 0.0f : float(t * t * t);
+    return p * sigma;
+}
+
 // squared distance of a candidate offset (one expression shared by every field kernel, so the
 // compiler forms the same operations everywhere)
 template <class R> __device__ __forceinline__ R dist2(R dx, R dy, R dz) { return dx * dx + dy * dy + dz * dz; }
@@ -155,12 +190,42 @@ __host__ __device__ inline int field_mode(const FieldConst& fc) {
 
 // contribution of one candidate that passed the cheap reject test: offset (dx, dy, dz) = q - a.xyz,
 // records g0 / g1 and, in compensated FP32, the residual lo of the particle mean
+// One term of the reference's float sums (AnisotropicInterpolate / IsotropicInterpolate and their
+// gradients, src/evaluator.cpp:408-463), every operation rounded on its own in the reference's
+// order: G d row by row, |G d|^2, sqrt(|G d|^2 / h^2) for the B-spline, sigma inside the kernel,
+// det times the kernel; the poly6-form gradient sigma (-6 G d) (h^2 - |G d|^2)^2 times det.
+template <bool GRAD>
+__device__ __forceinline__ void ref_float_contrib(bool ani, bool poly6, float h2, float sigma, float dx, float dy,
+                                                  float dz, const float4& g0, const float4& g1, Acc<float, false>& acc,
+                                                  Acc<float, false>* ga) {
+#pragma clang fp contract(off)
+    float gx = dx, gy = dy, gz = dz;
+    if (ani) {
+        gx = (g0.x * dx + g0.y * dy) + g0.z * dz;
+        gy = (g0.y * dx + g0.w * dy) + g1.x * dz;
+        gz = (g0.z * dx + g1.x * dy) + g1.y * dz;
+    }
+    const float n2 = (gx * gx + gy * gy) + gz * gz;
+    const float k = (poly6 || !ani) ? ref_poly6(n2, h2, sigma) : ref_bspline(__builtin_sqrtf(__fdiv_rn(n2, h2)), sigma);
+    acc.s = acc.s + (ani ? g1.z * k : k);
+    if (GRAD) {
+        const float w = (n2 > h2) ? 0.0f : (h2 - n2) * (h2 - n2);
+        const float t[3] = {(sigma * (-6.0f * gx)) * w, (sigma * (-6.0f * gy)) * w, (sigma * (-6.0f * gz)) * w};
+#pragma unroll
+        for (int a = 0; a < 3; a++) ga[a].s = ga[a].s + (ani ? g1.z * t[a] : t[a]);
+    }
+}
+
 template <class R, bool KAHAN, bool GRAD, int MODE = kModeAny>
 __device__ __forceinline__ void field_contrib(const FieldConst& fc, R inv_h2, R h2, R dx, R dy, R dz, R d2,
                                               const typename Vec4<R>::T& g0, const typename Vec4<R>::T& g1,
                                               const float4& lo, Acc<R, KAHAN>& acc, Acc<R, KAHAN>* ga) {
     const bool ani = MODE == kModeAniBspline || fc.use_ani;
     const bool poly6 = MODE != kModeAniBspline && fc.use_poly6;
+    if constexpr (ref_float<R, KAHAN>()) {
+        ref_float_contrib<GRAD>(ani, poly6, h2, float(fc.sigma), dx, dy, dz, g0, g1, acc, ga);
+        return;
+    }
     if (KAHAN && sizeof(R) == 4) {
         dx -= R(lo.x);
         dy -= R(lo.y);

@@ -207,13 +207,230 @@ __global__ void __launch_bounds__(256) k_gmat(GridDev g, const float4* __restric
     if (REL) rec_lo[p] = li;
 }
 
+// ---- PISURF_FP32: the reference's float arithmetic, operation by operation ----------------
+// Every operation is rounded on its own (contraction off), divisions and square roots are
+// correctly rounded, and the double promotions of pow() inside wij and the SVD's cube root are
+// reproduced, so the records equal those of a float restatement of compute_Gs_xMeans that keeps
+// the reference's operation order.
+
+// wij (src/evaluator.cpp:751-766): pow(d, 2), pow(h, 3) and pow(h - d, 3) in double from floats;
+// h3 = pow(h, 3) comes from the host's libm, like the reference's
+__device__ __forceinline__ float ref_wij(float d, float h, double h3) {
+#pragma clang fp contract(off)
+    if (d < h) {
+        const double dd = d;
+        if (d < h / 2) return float(1.0 - (6.0 * (dd * dd) / h3 * double(h - d)));
+        const double t = double(h - d);
+        return float(2.0 / h3 * (t * t * t));
+    }
+    return 0.0f;
+}
+
+// symmetric 3x3 eigendecomposition by cyclic Jacobi rotations (Golub & Van Loan Alg. 8.5.3) in
+// float, rotation pairs (0,1), (0,2), (1,2), stopping when the off-diagonal mass is below
+// (eps^2 / 100) of the diagonal's; eigenvalues returned in descending order with their vectors
+__device__ __forceinline__ void ref_jacobi3(const float a_in[3][3], float lam[3], float Q[3][3]) {
+#pragma clang fp contract(off)
+    float a[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            a[i][j] = a_in[i][j];
+            Q[i][j] = (i == j) ? 1.0f : 0.0f;
+        }
+    const float eps = 1.1920928955078125e-07f;  // float epsilon
+    for (int sweep = 0; sweep < 32; sweep++) {
+        const float off = (a[0][1] * a[0][1] + a[0][2] * a[0][2]) + a[1][2] * a[1][2];
+        const float dia = (a[0][0] * a[0][0] + a[1][1] * a[1][1]) + a[2][2] * a[2][2];
+        if (off <= eps * eps * dia * 0.01f || off == 0.0f) break;
+        for (int k = 0; k < 3; k++) {
+            const int p = (k == 2) ? 1 : 0, q = (k == 0) ? 1 : 2;
+            const float apq = a[p][q];
+            if (apq == 0.0f) continue;
+            const float tau = (a[q][q] - a[p][p]) / (2.0f * apq);
+            const float t = (tau >= 0.0f ? 1.0f : -1.0f) / (fabsf(tau) + __builtin_sqrtf(1.0f + tau * tau));
+            const float c = 1.0f / __builtin_sqrtf(1.0f + t * t), sn = t * c;
+            for (int r = 0; r < 3; r++) {
+                const float arp = a[r][p], arq = a[r][q];
+                a[r][p] = c * arp - sn * arq;
+                a[r][q] = sn * arp + c * arq;
+            }
+            for (int r = 0; r < 3; r++) {
+                const float apr = a[p][r], aqr = a[q][r];
+                a[p][r] = c * apr - sn * aqr;
+                a[q][r] = sn * apr + c * aqr;
+            }
+            a[p][q] = a[q][p] = 0.0f;
+            for (int r = 0; r < 3; r++) {
+                const float qrp = Q[r][p], qrq = Q[r][q];
+                Q[r][p] = c * qrp - sn * qrq;
+                Q[r][q] = sn * qrp + c * qrq;
+            }
+        }
+    }
+    for (int i = 0; i < 3; i++) lam[i] = a[i][i];
+    for (int i = 0; i < 2; i++)
+        for (int j = i + 1; j < 3; j++)
+            if (lam[j] > lam[i]) {
+                const float tl = lam[i];
+                lam[i] = lam[j];
+                lam[j] = tl;
+                for (int r = 0; r < 3; r++) {
+                    const float tq = Q[r][i];
+                    Q[r][i] = Q[r][j];
+                    Q[r][j] = tq;
+                }
+            }
+}
+
+// SVD tail of compute_G_Yus / compute_G_ours (src/evaluator.cpp:627-651, :556-580) on the
+// symmetric positive-definite covariance (U = V = Q): |lambda| clamped below at max / 4,
+// G = Q diag(1 / w) Q^T * pow(w0 w1 w2, 1/3) (times 0.5 for compute_G_ours), the upper triangle
+// computed and mirrored; det by cofactors along the first row
+__device__ __forceinline__ void ref_g_from_cov(const float cov[3][3], bool poly6, float G[6], float& det,
+                                               float& gmin) {
+#pragma clang fp contract(off)
+    float lam[3], Q[3][3];
+    ref_jacobi3(cov, lam, Q);
+    float w[3] = {fabsf(lam[0]), fabsf(lam[1]), fabsf(lam[2])};
+    const float mx = (w[0] < ((w[1] < w[2]) ? w[2] : w[1])) ? ((w[1] < w[2]) ? w[2] : w[1]) : w[0];
+    for (int k = 0; k < 3; k++) w[k] = (w[k] < mx / 4.0f) ? mx / 4.0f : w[k];
+    const float inv[3] = {1.0f / w[0], 1.0f / w[1], 1.0f / w[2]};
+    const float scale = float(pow(double((w[0] * w[1]) * w[2]), 1.0 / 3.0));
+    float M[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++) {
+            const float v = ((Q[a][0] * inv[0]) * Q[b][0] + (Q[a][1] * inv[1]) * Q[b][1]) + (Q[a][2] * inv[2]) * Q[b][2];
+            M[a][b] = v * scale;
+            if (poly6) M[a][b] = M[a][b] * 0.5f;
+            M[b][a] = M[a][b];
+        }
+    G[0] = M[0][0];
+    G[1] = M[0][1];
+    G[2] = M[0][2];
+    G[3] = M[1][1];
+    G[4] = M[1][2];
+    G[5] = M[2][2];
+    det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+          M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    gmin = float(double(scale) / double(mx) * (poly6 ? 0.5 : 1.0));
+}
+
+// Pass 1 in the reference's float order: x_j - x_i, |.|^2, sqrt, wij, sum w, sum x_j w, divide,
+// blend x (1 - 0.9f) + mean 0.9f
+__global__ void __launch_bounds__(256) k_xmean_ref(GridDev g, const float4* __restrict__ spos, int64_t n,
+                                                   AnisoParams ap, float4* __restrict__ xm) {
+#pragma clang fp contract(off)
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float4 pp = spos[p];
+    int ix, iy, iz;
+    cell_of(g, pp.x, pp.y, pp.z, ix, iy, iz);
+    int lo[9], hi[9], cnt = 0;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        if (!row_range(g, ix, iy, iz, r, lo[r], hi[r])) lo[r] = hi[r] = 0;
+        cnt += hi[r] - lo[r];
+    }
+    if (cnt <= 2 || ap.use_poly6) {  // fewer than 3 candidates, or no smoothing in poly6 mode
+        xm[p] = make_float4(pp.x, pp.y, pp.z, 0.0f);
+        return;
+    }
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, ws = 0.0f;
+    for (int r = 0; r < 9; r++)
+        for (int j = lo[r]; j < hi[r]; j++) {
+            if (j == (int)p) continue;
+            const float4 q = spos[j];
+            const float dx = q.x - pp.x, dy = q.y - pp.y, dz = q.z - pp.z;
+            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            if (d2 >= ap.nI2) continue;
+            const float w = ref_wij(__builtin_sqrtf(d2), ap.nI, ap.nI3);
+            ws = ws + w;
+            sx = sx + q.x * w;
+            sy = sy + q.y * w;
+            sz = sz + q.z * w;
+        }
+    float ox = pp.x, oy = pp.y, oz = pp.z;
+    if (ws > 0.0f) {
+        const float a = 1.0f - ap.xmean_delta, b = ap.xmean_delta;
+        sx = sx / ws;
+        sy = sy / ws;
+        sz = sz / ws;
+        ox = pp.x * a + sx * b;
+        oy = pp.y * a + sy * b;
+        oz = pp.z * a + sz * b;
+    }
+    xm[p] = make_float4(ox, oy, oz, 0.0f);
+}
+
+// Pass 2 in the reference's float order: the covariance starts at diag(h^2) (compute_G_Yus) or
+// diag(1 + 0.5) (compute_G_ours), each candidate adds (wd_a wd_b) w to all nine entries, then
+// every entry is divided by the weight sum
+__global__ void __launch_bounds__(256) k_gmat_ref(GridDev g, const float4* __restrict__ spos, int64_t n,
+                                                  AnisoParams ap, const float4* __restrict__ xm,
+                                                  float4* __restrict__ rec_a, float4* __restrict__ rec_g) {
+#pragma clang fp contract(off)
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float4 xi = xm[p];
+    const float h2 = float(ap.h2);
+    if (!ap.use_ani) {  // isotropic poly6 support |d|^2 <= h^2
+        rec_a[p] = make_float4(xi.x, xi.y, xi.z, h2 * 1.001f);
+        rec_g[2 * p] = make_float4(1.0f, 0.0f, 0.0f, 1.0f);
+        rec_g[2 * p + 1] = make_float4(0.0f, 1.0f, 1.0f, 0.0f);
+        return;
+    }
+    const float4 pp = spos[p];
+    int ix, iy, iz;
+    cell_of(g, pp.x, pp.y, pp.z, ix, iy, iz);
+    const bool ours = ap.use_poly6 != 0;
+    const float diag = ours ? 1.0f + 0.5f : h2;
+    float cov[3][3] = {{diag, 0.0f, 0.0f}, {0.0f, diag, 0.0f}, {0.0f, 0.0f, diag}};
+    float ws = 0.0f;
+    for (int r = 0; r < 9; r++) {
+        int lo, hi;
+        if (!row_range(g, ix, iy, iz, r, lo, hi)) continue;
+        for (int j = lo; j < hi; j++) {
+            const float4 xj = xm[j];
+            float d;
+            if (ours) {
+                const float4 q = spos[j];
+                const float ex = q.x - pp.x, ey = q.y - pp.y, ez = q.z - pp.z;
+                if ((ex * ex + ey * ey) + ez * ez >= ap.nI2) continue;
+                const float fx = pp.x - xj.x, fy = pp.y - xj.y, fz = pp.z - xj.z;
+                d = __builtin_sqrtf((fx * fx + fy * fy) + fz * fz);
+            } else {
+                const float ex = xj.x - xi.x, ey = xj.y - xi.y, ez = xj.z - xi.z;
+                const float d2 = (ex * ex + ey * ey) + ez * ez;
+                if (d2 >= ap.nI2) continue;
+                d = __builtin_sqrtf(d2);
+            }
+            const float w = ref_wij(d, ap.nI, ap.nI3);
+            ws = ws + w;
+            const float wd[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) cov[a][b] = cov[a][b] + (wd[a] * wd[b]) * w;
+        }
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) cov[a][b] = cov[a][b] / ws;
+    float G[6], det, gmin;
+    ref_g_from_cov(cov, ours, G, det, gmin);
+    // squared world radius outside which |G d|^2 >= (support)^2: 4 h^2 (B-spline) or h^2 (poly6)
+    const double supp = ours ? double(h2) : 4.0 * double(h2);
+    float R2 = float(supp / (double(gmin) * double(gmin)) * 1.001);
+    if (R2 != R2) R2 = INFINITY;  // 0 / 0 covariance: never culled, its NaN reaches every sum
+    rec_a[p] = make_float4(xi.x, xi.y, xi.z, R2);
+    rec_g[2 * p] = make_float4(G[0], G[1], G[2], G[3]);
+    rec_g[2 * p + 1] = make_float4(G[4], G[5], det, 0.0f);
+}
+
 void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, int64_t, AnisoParams ap, void* xm,
                   hipStream_t s) {
     if (n <= 0) return;
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (prec == 2) k_xmean<double, false><<<nb, 256, 0, s>>>(g, spos, n, ap, (double4*)xm);
     else if (prec == 1) k_xmean<float, true><<<nb, 256, 0, s>>>(g, spos, n, ap, (float4*)xm);
-    else k_xmean<float, false><<<nb, 256, 0, s>>>(g, spos, n, ap, (float4*)xm);
+    else k_xmean_ref<<<nb, 256, 0, s>>>(g, spos, n, ap, (float4*)xm);
 }
 
 void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, int64_t, AnisoParams ap, const void* xm,
@@ -227,8 +444,7 @@ void launch_gmat(int prec, GridDev g, const float4* spos, int64_t n, int64_t, An
         k_gmat<float, true><<<nb, 256, 0, s>>>(g, spos, n, ap, (const float4*)xm, (float4*)rec_a, (float4*)rec_g,
                                               (float4*)rec_lo, n_fp64);
     else
-        k_gmat<float, false><<<nb, 256, 0, s>>>(g, spos, n, ap, (const float4*)xm, (float4*)rec_a, (float4*)rec_g,
-                                              (float4*)rec_lo, n_fp64);
+        k_gmat_ref<<<nb, 256, 0, s>>>(g, spos, n, ap, (const float4*)xm, (float4*)rec_a, (float4*)rec_g);
 }
 
 }  // namespace pisurf

@@ -17,7 +17,7 @@ __global__ void __launch_bounds__(256) k_eval_points(GridDev g, Recs<R> rec, Fie
         const float x = q[3 * i], y = q[3 * i + 1], z = q[3 * i + 2];
         R s, gr[3];
         vis = field_sum<R, K, GRAD>(g, rec, fc, x, y, z, R(x), R(y), R(z), s, gr);
-        const R sigma = R(fc.sigma);
+        const R sigma = out_sigma<R, K>(fc);
         f[i] = double(R(fc.iso) - sigma * s);
         if (GRAD) {
             grad[3 * i] = double(sigma * gr[0]);
@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(256) k_normals_pt(GridDev g, Recs<R> rec, Fiel
         const float4 pp = spos[p];
         R s, gr[3];
         vis = field_sum<R, K, true>(g, rec, fc, pp.x, pp.y, pp.z, R(pp.x), R(pp.y), R(pp.z), s, gr);
-        const R sigma = R(fc.sigma);
+        const R sigma = out_sigma<R, K>(fc);
         nrm[p] = make_float4(float(sigma * gr[0]), float(sigma * gr[1]), float(sigma * gr[2]), 0.0f);
     }
     wave_count(&ctr_stripe(ctr)->grad_evals, p < n ? 1ull : 0ull);
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(kRunThreads) k_run_normals(GridDev g, Recs<R> 
     int h0, h1;
     particle_run(g, kx, h0, h1);
     if (h0 >= h1) return;  // uniform over the block
-    ParticleSrc<R> src{spos, nrm, R(fc.sigma)};
+    ParticleSrc<R> src{spos, nrm, out_sigma<R, K>(fc)};
     const unsigned long long vis = run_eval<kRunThreads, R, K, true, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(256) k_uniform_grid(GridDev g, Recs<R> rec, Fi
         }
         R s, gr[3];
         vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), s, gr);
-        out[i] = float(R(fc.iso) - R(fc.sigma) * s);
+        out[i] = float(R(fc.iso) - out_sigma<R, K>(fc) * s);
     }
     wave_count(&ctr_stripe(ctr)->field_evals, i < total ? 1ull : 0ull);
     wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);

@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(T) k_run_centers(GridDev g, Recs<R> rec, Field
                                                              Counters* ctr) {
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
-    CenterSrc<R> src{lv, list, kv, samples, R(fc.iso), R(fc.sigma)};
+    CenterSrc<R> src{lv, list, kv, samples, R(fc.iso), out_sigma<R, K>(fc)};
     const unsigned long long vis = run_eval<T, R, K, false, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(T) k_run_samples(GridDev g, Recs<R> rec, Field
                                                              Counters* ctr) {
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
-    SampleSrc<R> src{lv, list, kv, hlist, out, R(fc.iso), R(fc.sigma)};
+    SampleSrc<R> src{lv, list, kv, hlist, out, R(fc.iso), out_sigma<R, K>(fc)};
     const unsigned long long vis = run_eval<T, R, K, false, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->sample_candidates, vis);
@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(256) k_centers(GridDev g, Recs<R> rec, FieldCo
         const float4 c = lv.c[node];
         R sum, gr[3];
         vis = field_sum<R, K, false>(g, rec, fc, c.x, c.y, c.z, R(c.x), R(c.y), R(c.z), sum, gr);
-        samples[node * 27 + 13] = R(fc.iso) - R(fc.sigma) * sum;
+        samples[node * 27 + 13] = R(fc.iso) - out_sigma<R, K>(fc) * sum;
         did = true;
     }
     wave_count(&ctr_stripe(ctr)->field_evals, did ? 1ull : 0ull);
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(256) k_samples(GridDev g, Recs<R> rec, FieldCo
         sample_point(lv.c[node], s, p);
         R sum, gr[3];
         vis = field_sum<R, K, false>(g, rec, fc, p[0], p[1], p[2], R(p[0]), R(p[1]), R(p[2]), sum, gr);
-        samples[node * 27 + s] = R(fc.iso) - R(fc.sigma) * sum;
+        samples[node * 27 + s] = R(fc.iso) - out_sigma<R, K>(fc) * sum;
         did = true;
     }
     wave_count(&ctr_stripe(ctr)->field_evals, did ? 1ull : 0ull);
@@ -595,7 +595,7 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
                     vis += field_sum<R, K, false>(g, rec, fc, float(vx), float(vy), float(vz), vx, vy, vz, sum, g3);
                     evals++;
                     qefs++;
-                    fval = R(fc.iso) - R(fc.sigma) * sum;
+                    fval = R(fc.iso) - out_sigma<R, K>(fc) * sum;
                     if (err / R(__fmul_rn(2.0f, c.w)) < R(0.0001)) fval = R(0);
                 } else {
                     float p13[3];
@@ -608,7 +608,7 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
                         R sum, g3[3];
                         vis += field_sum<R, K, false>(g, rec, fc, c.x, c.y, c.z, R(c.x), R(c.y), R(c.z), sum, g3);
                         evals++;
-                        fval = R(fc.iso) - R(fc.sigma) * sum;
+                        fval = R(fc.iso) - out_sigma<R, K>(fc) * sum;
                     }
                 }
             }

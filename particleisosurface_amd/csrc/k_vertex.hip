@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(256) k_bisect(GridDev g, Recs<R> rec, FieldCon
         }
         const R ex = v1[0] - v2[0], ey = v1[1] - v2[1], ez = v1[2] - v2[2];
         const R e2 = ex * ex + ey * ey + ez * ez;
-        R d = (sizeof(R) == 4) ? R(__fsqrt_rn(float(e2))) : R(sqrt(double(e2)));
+        R d = (sizeof(R) == 4) ? R(__builtin_sqrtf(float(e2))) : R(sqrt(double(e2)));
         const R rr = R(half_r);
         // the reference loop has no cap; 256 halvings bound it for any finite input
         for (int it = 0; it < 256 && d > rr; it++) {
@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(256) k_bisect(GridDev g, Recs<R> rec, FieldCon
             vis += field_sum<R, K, false, MODE>(g, rec, fc, float(m[0]), float(m[1]), float(m[2]), m[0], m[1], m[2],
                                                 sum, g3);
             nev++;
-            m[3] = R(fc.iso) - R(fc.sigma) * sum;
+            m[3] = R(fc.iso) - out_sigma<R, K>(fc) * sum;
             const int sm = m[3] > R(0) ? 1 : -1, s1 = v1[3] > R(0) ? 1 : -1;
             if (sm == s1)
                 for (int a = 0; a < 4; a++) v1[a] = m[a];
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(64) k_qef(GridDev g, Recs<R> rec, FieldConst f
         }
         R sum, g3[3];
         vis = field_sum<R, K, false>(g, rec, fc, float(vx), float(vy), float(vz), vx, vy, vz, sum, g3);
-        R fv = R(fc.iso) - R(fc.sigma) * sum;
+        R fv = R(fc.iso) - out_sigma<R, K>(fc) * sum;
         if (err / R(__fmul_rn(2.0f, c.w)) < R(0.0001)) fv = R(0);
         out[i] = make_double4(double(vx), double(vy), double(vz), double(fv));
     }

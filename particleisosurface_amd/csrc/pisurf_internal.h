@@ -73,6 +73,7 @@ struct AnisoParams {
     float xmean_delta;     // 0.9 (B-spline) / unused in poly6 mode
     int use_poly6;
     int use_ani;
+    double nI3;            // pow(double(nI), 3) as the host's libm gives it (wij's denominator)
 };
 // one thread per sorted particle (n_invalid: particles outside the grid; these kernels handle any)
 void launch_xmean(int prec, GridDev g, const float4* spos, int64_t n, int64_t n_invalid, AnisoParams ap,

@@ -120,7 +120,8 @@ def test_mode_full_tree_and_mesh(mode):
         # the isotropic poly6 sums stay far below the B-spline iso value (sigma B(0.01)): no surface
         assert len(t) == 0
     else:
-        assert len(t) > 1000
+        # without normals the curvature is 1 everywhere and nothing refines below D_min
+        assert len(t) > (100 if mode == "no_normals" else 1000)
 
 
 def test_batch_24_frames():
