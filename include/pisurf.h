@@ -12,7 +12,8 @@
  *  - every function returns PISURF_OK (0) or a negative PISURF_E_* code; no exception or
  *    abort crosses the ABI; pisurf_last_error() gives the message of the last failure;
  *  - host buffers are caller-owned, device buffers are context-owned;
- *  - a context is single-threaded (one host thread, one HIP stream at default priority);
+ *  - a context is used by one host thread at a time (any thread; calls must not overlap) and
+ *    launches on one HIP stream of default priority;
  *  - positions are 3 packed floats per particle (layout of std::vector<Eigen::Vector3f>),
  *    3x3 matrices are 9 floats/doubles column-major (layout of Eigen::Matrix3f);
  *  - value outputs are double: FP32 results are exact in double, FP64 results are native.
@@ -168,6 +169,13 @@ int pisurf_qef_solve(pisurf_ctx* ctx, const float* centers, const float* halfs, 
 int pisurf_bisect_edges(pisurf_ctx* ctx, const double* ends, int64_t n, double* out_xyz, int32_t* n_evals);
 
 /* ---- frame: extraction and mesh (genIsoOurs state 1, src/surface_reconstructor.cpp:290-324) ---- */
+/* traverse_node<trav_vert> with VisitorExtract::on_vert (include/traverse.h:12-312,
+ * src/visitorextract.cpp:229-257): the dual grids of the built octree in the reference's traversal
+ * order, each as 8 node indices (creation order, as pisurf_tree_download lists them) already
+ * permuted by the procedure table's `trans`, and its sign index (bit i set when corner i of the
+ * traversal has f > 0; the procedure is table[sign_index]). *total receives the number of dual
+ * grids; at most cap are written. */
+int pisurf_dual_cells(pisurf_ctx* ctx, int32_t* nodes8, uint8_t* sign_index, int64_t cap, int64_t* total);
 int pisurf_extract(pisurf_ctx* ctx, int64_t* n_verts, int64_t* n_tris);
 int pisurf_mesh_download(pisurf_ctx* ctx, double* verts, int32_t* tris);
 /* SurfReconstructor::Run (src/surface_reconstructor.cpp:409-479) on the uploaded particles */

@@ -52,6 +52,15 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("facade demo build failed:\n" + r.stdout + r.stderr)
     os.replace(demo + ".tmp", demo)
+    # the reference's extraction stage (calc_vertices on OpenMP threads, generate_mesh) driven
+    # through the C++ mirror; float operations are not contracted, as in the reference's build
+    ext = os.path.join(HERE, "pisurf_extract_demo")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fopenmp", "-ffp-contract=off", os.path.join(CSRC, "facade_extract.cpp"),
+           "-o", ext + ".tmp", "-L" + HERE, "-lpisurf", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("extraction driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(ext + ".tmp", ext)
     # the frame driver over a data directory (controlData.json, CSV / HDF5 / raw frames)
     run = os.path.join(HERE, "pisurf_run")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "pisurf_run.cpp"), "-o", run + ".tmp",

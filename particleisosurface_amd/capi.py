@@ -20,7 +20,8 @@ EXPORTS = [
     "pisurf_upload_particles", "pisurf_hashgrid_build", "pisurf_hashgrid_download", "pisurf_neighbors",
     "pisurf_box_particles", "pisurf_anisotropy", "pisurf_download_aniso", "pisurf_iso_constants",
     "pisurf_particle_normals", "pisurf_eval_points", "pisurf_uniform_grid", "pisurf_sweep_level", "pisurf_build_tree",
-    "pisurf_tree_download", "pisurf_qef_solve", "pisurf_bisect_edges", "pisurf_extract", "pisurf_mesh_download",
+    "pisurf_tree_download", "pisurf_qef_solve", "pisurf_bisect_edges", "pisurf_dual_cells", "pisurf_extract",
+    "pisurf_mesh_download",
     "pisurf_run_frame", "pisurf_run_frames", "pisurf_get_stats", "pisurf_reset_stats", "pisurf_stream",
 ]
 

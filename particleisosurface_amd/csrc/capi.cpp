@@ -257,7 +257,22 @@ struct pisurf_ctx {
     std::vector<int> ev_kind;
     size_t ev_used = 0;
 
+    // read-backs: a polled event and a page-locked staging buffer owned by the context, so calls
+    // made from different threads (one at a time) create no per-thread HIP objects
+    hipEvent_t wev = nullptr;
+    PinnedBuf rb;
+    hipError_t wait() { return stream_wait_on(stream, wev); }
+    hipError_t read(void* host, const void* dptr, size_t bytes) {
+        if (bytes > kReadbackBytes) return hipErrorInvalidValue;
+        hipError_t e;
+        if ((e = hipMemcpyAsync(rb.p, dptr, bytes, hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
+        if ((e = wait()) != hipSuccess) return e;
+        std::memcpy(host, rb.p, bytes);
+        return hipSuccess;
+    }
+
     ~pisurf_ctx() {
+        if (wev) (void)hipEventDestroy(wev);
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
         if (cstream) (void)hipStreamDestroy(cstream);
@@ -429,8 +444,8 @@ int ensure_bbox(pisurf_ctx* c) {
     launch_bbox(c->xyz.as<float>(), c->n, c->part.as<float>(), out6, nanc, c->stream);
     HIPCHK(c, hipGetLastError());
     int nans = 0;
-    HIPCHK(c, read_device(c->bbox, out6, 6 * sizeof(float), c->stream));
-    HIPCHK(c, read_device(&nans, nanc, sizeof(int), c->stream));
+    HIPCHK(c, c->read(c->bbox, out6, 6 * sizeof(float)));
+    HIPCHK(c, c->read(&nans, nanc, sizeof(int)));
     if (nans) return c->fail(PISURF_E_INVALID, "particle positions contain NaN");
     for (int a = 0; a < 6; a++)
         if (!std::isfinite(c->bbox[a])) return c->fail(PISURF_E_INVALID, "particle positions are not finite");
@@ -482,7 +497,7 @@ int do_grid(pisurf_ctx* c, const float* bbox, float inf) {
     launch_gather_pos(c->xyz.as<float>(), c->vals.as<int>(), n, c->spos.as<float4>(), c->stream);
     HIPCHK(c, hipGetLastError());
     int ninv = 0;
-    HIPCHK(c, read_device(&ninv, ninv_d, sizeof(int), c->stream));
+    HIPCHK(c, c->read(&ninv, ninv_d, sizeof(int)));
     c->g = g;
     for (int a = 0; a < 6; a++) c->ginfo.bounding[a] = b[a];
     for (int a = 0; a < 3; a++) c->ginfo.xyz_cell_num[a] = n3[a];
@@ -525,7 +540,7 @@ int do_aniso(pisurf_ctx* c) {
     launch_gmat(prec, c->g, c->spos.as<float4>(), c->n, c->ginfo.n_invalid, ap, c->xm.p, c->rec_a.p, c->rec_g.p,
                 c->rec_lo.p, nfp, c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, read_device(&c->n_fp64_eig, nfp, 8, c->stream));
+    HIPCHK(c, c->read(&c->n_fp64_eig, nfp, 8));
     c->prec_rec = prec;
     c->aniso_done = true;
     c->normals_done = c->tree_done = false;
@@ -547,7 +562,7 @@ int do_normals(pisurf_ctx* c) {
     launch_normals(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->spos.as<float4>(), c->n,
                    c->ginfo.n_invalid, c->nrm.as<float4>(), c->ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     c->normals_done = true;
     c->st.ms_normals += now_ms() - t0;
     return PISURF_OK;
@@ -624,7 +639,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     const NodeLevel v = L->view();
     const int64_t need_tmp = scan_tmp_ints(L->n + 1) + 64;
     if ((size_t)need_tmp * 4 > c->tmp.cap) {
-        HIPCHK(c, stream_wait(c->stream));
+        HIPCHK(c, c->wait());
         HIPCHK(c, c->tmp.ensure(need_tmp * 4));
     }
     // lanes per node ~ (y, z) cell rows of the grown node box: (width / cell + 1)^2 on average
@@ -653,7 +668,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     const bool dedupe = c->prm.sample_dedupe && level_depth >= 0 && level_depth <= 19 && level_depth >= c->dmin;
     if (dedupe) {
         int k = 0;
-        HIPCHK(c, read_device(&k, L->scanned.as<int>() + L->n, 4, c->stream));
+        HIPCHK(c, c->read(&k, L->scanned.as<int>() + L->n, 4));
         const int64_t nent = (int64_t)k * 27;
         if (nent >= INT32_MAX) return c->fail(PISURF_E_UNSUPPORTED, "octree level has too many sample points");
         if (nent > 0) {
@@ -711,7 +726,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
                        c->samples.p, c->ctr(), c->stream);
     } else if (leaf_level && c->prm.qef_mode == 0 && level_depth >= c->dmin) {
         int k = 0;
-        HIPCHK(c, read_device(&k, c->cl_pos.as<int>() + L->n, 4, c->stream));
+        HIPCHK(c, c->read(&k, c->cl_pos.as<int>() + L->n, 4));
         if (k > 0) {
             HIPCHK(c, c->sd_key.ensure((size_t)k * 8));
             HIPCHK(c, c->sd_key2.ensure((size_t)k * 8));
@@ -765,7 +780,7 @@ int do_tree(pisurf_ctx* c) {
         HIPCHK(c, hipMemsetAsync(L->flag.as<int>() + L->n, 0, 4, c->stream));
         exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), L->n + 1, c->tmp.as<int>(), c->stream);
         int n_int = 0;
-        HIPCHK(c, read_device(&n_int, L->scanned.as<int>() + L->n, 4, c->stream));
+        HIPCHK(c, c->read(&n_int, L->scanned.as<int>() + L->n, 4));
         c->harvest_events();
         if (n_int == 0) break;
         if ((int64_t)n_int * 8 > (int64_t)INT32_MAX / 2) return c->fail(PISURF_E_UNSUPPORTED, "octree level too large");
@@ -775,14 +790,14 @@ int do_tree(pisurf_ctx* c) {
         const int64_t need_tmp = scan_tmp_ints(N->n + 1) + 64;
         if ((size_t)need_tmp * 4 > c->tmp.cap) {
             // grow the scan workspace without losing the grid (tmp holds no persistent data)
-            HIPCHK(c, stream_wait(c->stream));
+            HIPCHK(c, c->wait());
             HIPCHK(c, c->tmp.ensure(need_tmp * 4));
         }
         launch_children(L->view(), L->scanned.as<int>(), N->view(), c->stream);
         HIPCHK(c, hipGetLastError());
         li++;
     }
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     c->tree_done = true;
     c->st.n_nodes = 0;
     for (auto* l : c->levels) c->st.n_nodes += l->n;
@@ -829,7 +844,7 @@ int download_tree(pisurf_ctx* c, HostTree& t, bool geometry) {
             HIPCHK(c, hipMemcpyAsync(&t.depth[o], l->depth.p, l->n, hipMemcpyDeviceToHost, c->stream));
         }
     }
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     for (size_t li = 0; li + 1 < t.level_off.size(); li++) {
         const int64_t next = t.level_off[li + 1];
         for (int64_t i = t.level_off[li]; i < next; i++)
@@ -944,7 +959,7 @@ struct Extractor {
 
 int read_int(pisurf_ctx* c, const int* dptr, int64_t& out) {
     int v = 0;
-    HIPCHK(c, read_device(&v, dptr, 4, c->stream));
+    HIPCHK(c, c->read(&v, dptr, 4));
     out = v;
     return PISURF_OK;
 }
@@ -952,7 +967,7 @@ int read_int(pisurf_ctx* c, const int* dptr, int64_t& out) {
 int ensure_tmp(pisurf_ctx* c, int64_t n) {
     const int64_t need = std::max(radix_tmp_ints(n), scan_tmp_ints(n + 1)) + 64;
     if ((size_t)need * 4 > c->tmp.cap) {
-        HIPCHK(c, stream_wait(c->stream));
+        HIPCHK(c, c->wait());
         HIPCHK(c, c->tmp.ensure(need * 4));
     }
     return PISURF_OK;
@@ -1013,7 +1028,7 @@ int do_extract_device(pisurf_ctx* c) {
     int64_t nent = 0;
     {
         int st[64 * 32];
-        HIPCHK(c, read_device(st, E[7].p, sizeof(st), s));
+        HIPCHK(c, c->read(st, E[7].p, sizeof(st)));
         for (int k = 0; k < 64; k++) nent += st[32 * k];
     }
     // points -> dual cells
@@ -1146,8 +1161,8 @@ int do_extract_device(pisurf_ctx* c) {
         v0 = part_end[ci];
     }
     HIPCHK(c, hipEventRecord(c->eb1, s));
-    HIPCHK(c, stream_wait(s));
-    HIPCHK(c, stream_wait(c->dstream));
+    HIPCHK(c, c->wait());
+    HIPCHK(c, stream_wait_on(c->dstream, c->wev));
     {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c->eb0, c->eb1) == hipSuccess) c->st.ms_bisect += ms;
@@ -1240,7 +1255,7 @@ int do_extract_host(pisurf_ctx* c) {
                       c->out1.as<double>(), nullptr, c->ctr(), c->stream);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(c->mesh_v.data(), c->out1.p, nv * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, stream_wait(c->stream));
+        HIPCHK(c, c->wait());
     }
     c->st.ms_bisect += now_ms() - t0;
     c->st.n_verts = nv;
@@ -1252,7 +1267,7 @@ int sync_counters(pisurf_ctx* c) {
     Counters h;
     Counters hs[kCtrStripes];
     HIPCHK(c, hipMemcpyAsync(hs, c->counters.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     std::memset(&h, 0, sizeof(h));
     for (const Counters& k : hs) {
         h.field_evals += k.field_evals;
@@ -1315,6 +1330,10 @@ int pisurf_create(pisurf_ctx** out, int device) {
         delete c;
         return PISURF_E_HIP;
     }
+    if (hipEventCreateWithFlags(&c->wev, hipEventDisableTiming) != hipSuccess || c->rb.ensure(kReadbackBytes) != hipSuccess) {
+        delete c;
+        return PISURF_E_HIP;
+    }
     pisurf_default_params(&c->prm);
     std::memset(&c->st, 0, sizeof(c->st));
     if (c->counters.ensure(kCtrStripes * sizeof(Counters)) != hipSuccess ||
@@ -1329,7 +1348,7 @@ int pisurf_create(pisurf_ctx** out, int device) {
 int pisurf_destroy(pisurf_ctx* c) {
     if (!c) return PISURF_E_INVALID;
     (void)hipSetDevice(c->device);
-    (void)stream_wait(c->stream);
+    (void)c->wait();
     delete c;
     return PISURF_OK;
 }
@@ -1369,7 +1388,7 @@ int pisurf_upload_particles(pisurf_ctx* c, const float* xyz, int64_t n, int32_t 
     if (n > 0)
         HIPCHK(c, hipMemcpyAsync(c->xyz.p, xyz, n * 3 * sizeof(float),
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     c->n = n;
     c->bbox_done = c->grid_done = c->aniso_done = c->normals_done = c->tree_done = c->consts_done = false;
     return PISURF_OK;
@@ -1403,7 +1422,7 @@ int pisurf_hashgrid_download(pisurf_ctx* c, int32_t* index_list, int32_t* cs, in
         if (cs) HIPCHK(c, hipMemcpyAsync(cs, c->out1.p, nc * 4, hipMemcpyDeviceToHost, c->stream));
         if (ce) HIPCHK(c, hipMemcpyAsync(ce, c->out2.p, nc * 4, hipMemcpyDeviceToHost, c->stream));
     }
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     return PISURF_OK;
     API_END(c)
 }
@@ -1428,7 +1447,7 @@ int pisurf_neighbors(pisurf_ctx* c, const float* q, int64_t nq, int64_t* offsets
     launch_sum_counts(c->out1.as<int>(), nq, tot_d, c->stream);
     HIPCHK(c, hipGetLastError());
     unsigned long long tot64 = 0;
-    HIPCHK(c, read_device(&tot64, tot_d, 8, c->stream));
+    HIPCHK(c, c->read(&tot64, tot_d, 8));
     if (tot64 >= (unsigned long long)INT32_MAX) {
         if (total) *total = (int64_t)tot64;
         return c->fail(PISURF_E_UNSUPPORTED, "neighbour lists total 2^31 ids or more: query fewer points per call");
@@ -1436,7 +1455,7 @@ int pisurf_neighbors(pisurf_ctx* c, const float* q, int64_t nq, int64_t* offsets
     exclusive_scan_i32(c->out1.as<int>(), c->out2.as<int>(), nq + 1, c->tmp.as<int>(), c->stream);
     std::vector<int> off32(nq + 1);
     HIPCHK(c, hipMemcpyAsync(off32.data(), c->out2.p, (nq + 1) * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     const int64_t tot = off32[nq];
     if (total) *total = tot;
     if (offsets)
@@ -1447,7 +1466,7 @@ int pisurf_neighbors(pisurf_ctx* c, const float* q, int64_t nq, int64_t* offsets
                               c->stream);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(ids, c->out3.p, std::min(tot, cap) * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, stream_wait(c->stream));
+        HIPCHK(c, c->wait());
     }
     return PISURF_OK;
     API_END(c)
@@ -1480,7 +1499,7 @@ int pisurf_box_particles(pisurf_ctx* c, const float* box1, const float* box2, in
     exclusive_scan_i32(c->out1.as<int>(), c->out2.as<int>(), ncells + 1, c->tmp.as<int>(), c->stream);
     int tot = 0;
     HIPCHK(c, hipMemcpyAsync(&tot, c->out2.as<int>() + ncells, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     if (total) *total = tot;
     if (ids && tot > 0) {
         HIPCHK(c, c->out3.ensure((int64_t)tot * 4));
@@ -1488,7 +1507,7 @@ int pisurf_box_particles(pisurf_ctx* c, const float* box1, const float* box2, in
                          c->out3.as<int>(), c->stream);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(ids, c->out3.p, std::min<int64_t>(tot, cap) * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, stream_wait(c->stream));
+        HIPCHK(c, c->wait());
     }
     return PISURF_OK;
     API_END(c)
@@ -1518,7 +1537,7 @@ int pisurf_download_aniso(pisurf_ctx* c, double* xmean, double* G9, double* det)
         HIPCHK(c, hipMemcpyAsync(lo.data(), c->rec_lo.p, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(g.data(), c->rec_g.p, 2 * n * rb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(sidx.data(), c->vals.p, n * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     auto rd = [&](const std::vector<char>& b, int64_t k) {
         return f64 ? ((const double*)b.data())[k] : double(((const float*)b.data())[k]);
     };
@@ -1566,7 +1585,7 @@ int pisurf_particle_normals(pisurf_ctx* c, double* normals) {
         std::vector<int> sidx(c->n);
         HIPCHK(c, hipMemcpyAsync(h.data(), c->nrm.p, c->n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(sidx.data(), c->vals.p, c->n * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, stream_wait(c->stream));
+        HIPCHK(c, c->wait());
         for (int64_t p = 0; p < c->n; p++) {
             normals[3 * sidx[p]] = h[p].x;
             normals[3 * sidx[p] + 1] = h[p].y;
@@ -1596,7 +1615,7 @@ int pisurf_eval_points(pisurf_ctx* c, const float* q, int64_t n, double* f, doub
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(f, c->out1.p, n * 8, hipMemcpyDeviceToHost, c->stream));
     if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->out2.p, n * 24, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     return PISURF_OK;
     API_END(c)
 }
@@ -1641,7 +1660,7 @@ int pisurf_uniform_grid(pisurf_ctx* c, int32_t dims[3], float box[6], float* spa
                         c->ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(scalars, c->out1.p, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     return PISURF_OK;
     API_END(c)
 }
@@ -1687,7 +1706,7 @@ int pisurf_sweep_level(pisurf_ctx* c, const float* centers, const float* halfs, 
     if (curv) HIPCHK(c, hipMemcpyAsync(curv, L.curv.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     if (signchange) HIPCHK(c, hipMemcpyAsync(signchange, c->out3.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(nd.data(), L.node.p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     c->harvest_events();
     if (f)
         for (int64_t i = 0; i < n; i++) f[i] = nd[i].w;
@@ -1780,7 +1799,7 @@ int pisurf_qef_solve(pisurf_ctx* c, const float* centers, const float* halfs, co
                c->out2.as<double>(), n, mode, c->out3.as<double4>(), c->ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(node4, c->out3.p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     return PISURF_OK;
     API_END(c)
 }
@@ -1806,7 +1825,7 @@ int pisurf_bisect_edges(pisurf_ctx* c, const double* ends, int64_t n, double* ou
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(out_xyz, c->out1.p, n * 24, hipMemcpyDeviceToHost, c->stream));
     if (n_evals) HIPCHK(c, hipMemcpyAsync(n_evals, c->out2.p, n * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     return PISURF_OK;
     API_END(c)
 }
@@ -1819,6 +1838,26 @@ int pisurf_extract(pisurf_ctx* c, int64_t* nv, int64_t* nt) {
     if (rc) return rc;
     if (nv) *nv = (int64_t)c->mesh_v.size() / 3;
     if (nt) *nt = (int64_t)c->mesh_t.size() / 3;
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_dual_cells(pisurf_ctx* c, int32_t* nodes8, uint8_t* sign_index, int64_t cap, int64_t* total) {
+    if (!c) return PISURF_E_INVALID;
+    if (!c->tree_done) return c->fail(PISURF_E_STATE, "octree not built");
+    if (cap < 0 || (cap > 0 && (!nodes8 || !sign_index))) return c->fail(PISURF_E_INVALID, "bad output buffers");
+    API_BEGIN
+    HostTree t;
+    int rc = download_tree(c, t, false);
+    if (rc) return rc;
+    Extractor ex(t);
+    if (!t.type.empty()) ex.node(0);
+    const int64_t nd = (int64_t)ex.duals.size();
+    if (total) *total = nd;
+    for (int64_t i = 0; i < nd && i < cap; i++) {
+        for (int k = 0; k < 8; k++) nodes8[8 * i + k] = ex.duals[i].n[k];
+        sign_index[i] = ex.duals[i].sidx;
+    }
     return PISURF_OK;
     API_END(c)
 }
@@ -1869,7 +1908,7 @@ int pisurf_run_frames(pisurf_ctx* c, int32_t nframes, const float* const* xyz, c
     if (!c->cstream) HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
     if (!c->cev) HIPCHK(c, hipEventCreateWithFlags(&c->cev, hipEventDisableTiming));
     const size_t bytes_max = (size_t)nmax * 3 * sizeof(float);
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     HIPCHK(c, c->xyz.ensure(bytes_max));
     HIPCHK(c, c->xyz_next.ensure(bytes_max));
     if (!on_device) HIPCHK(c, c->pin.ensure(bytes_max));
@@ -1951,7 +1990,7 @@ int pisurf_reset_stats(pisurf_ctx* c) {
     c->host_sample_entries = 0;
     c->n_fp64_eig = 0;
     HIPCHK(c, hipMemsetAsync(c->counters.p, 0, kCtrStripes * sizeof(Counters), c->stream));
-    HIPCHK(c, stream_wait(c->stream));
+    HIPCHK(c, c->wait());
     return PISURF_OK;
 }
 

@@ -41,7 +41,8 @@ int main(int argc, char** argv) {
                 bb[2 * a + 1] = std::max(bb[2 * a + 1], p[a]);
             }
         auto grid = std::make_shared<pisurf::HashGrid>(sr.device(), sr.getGlobalParticles(), bb, r, 4.0f);
-        pisurf::Evaluator ev(grid, sr.getGlobalParticles(), r);
+        std::shared_ptr<pisurf::MultiLevelSearcher> searcher;  // constant radius: none, as in the reference
+        pisurf::Evaluator ev(grid, searcher, sr.getGlobalParticles(), &radiuses, r);
         ev.compute_Gs_xMeans();
         std::vector<int> nb;
         grid->GetPIdxList(particles[0], nb);

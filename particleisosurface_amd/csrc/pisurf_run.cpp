@@ -33,7 +33,7 @@ void on_mesh(void* user, int32_t k, const pisurf_tree_info*, const double* v, in
     m.vertices.resize(nv);
     m.tris.resize(nt);
     for (int64_t i = 0; i < nv; i++) m.vertices[i] = pisurf::Vector3f((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2]);
-    for (int64_t i = 0; i < nt; i++) m.tris[i] = pisurf::Triangle{{t[3 * i], t[3 * i + 1], t[3 * i + 2]}};
+    for (int64_t i = 0; i < nt; i++) m.tris[i] = pisurf::Triangle(t[3 * i], t[3 * i + 1], t[3 * i + 2]);
     m.verticesNum = (unsigned)nv;
     m.trianglesNum = (unsigned)nt;
     const std::string& f = c.names[k];

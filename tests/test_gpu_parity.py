@@ -129,9 +129,10 @@ def _queries(xyz, r, n, seed=1):
 # relative error of S; far outside (S << iso) it bounds the absolute error by a fraction of iso,
 # where the relative error of S is dominated by support-edge terms and is not a meaningful
 # measure (the reference's own FP32 arithmetic reaches |dS|/S ~ 2e-4 there and 1.4e-5 at S >= iso).
-# FP32 against the FP32-faithful oracle therefore allows twice that noise floor, and signs are
-# compared outside |f| <= 5e-5 iso; compensated FP32 and FP64 are held to the FP64 oracle.
-FIELD_TOL = {"fp32": (3e-5, 5e-5), "fp32k": (1e-5, 2e-5), "fp64": (1e-10, 1e-9)}
+# FP32 performs the FP32 oracle's operations in its order (test_gpu_fp32_exact.py holds it to
+# bit equality with counted exceptions) and is held to the 1e-5 bar here; compensated FP32 and
+# FP64 are held to the FP64 oracle.
+FIELD_TOL = {"fp32": (1e-5, 2e-5), "fp32k": (1e-5, 2e-5), "fp64": (1e-10, 1e-9)}
 
 
 @pytest.mark.parametrize("prec,ref", [("fp32", "fp32"), ("fp32k", "fp64"), ("fp64", "fp64")])
@@ -153,12 +154,12 @@ def test_field_values_and_signs(prec, ref):
     gn = np.linalg.norm(g2, axis=1)
     ok = gn > 1e-3 * gn.max()
     # angle from atan2(|a x b|, a . b): accurate for tiny angles, where arccos of the cosine has a
-    # floor of ~1e-6 degrees. FP32 bounds: the reference's own FP32 noise on these points (FP32
-    # oracle vs FP64 oracle) is p99 3.2e-3 and max 2.4e-2 degrees.
+    # floor of ~1e-6 degrees. FP32 follows the FP32 oracle's operations, so its gradients agree to
+    # the last bit apart from libm's rare last-bit differences.
     cr = np.linalg.norm(np.cross(g[ok], g2[ok]), axis=1)
     ang = np.degrees(np.arctan2(cr, np.sum(g[ok] * g2[ok], 1)))
-    assert np.quantile(ang, 0.99) < {"fp32": 5e-3, "fp32k": 2e-4, "fp64": 1e-9}[prec]
-    assert ang.max() < {"fp32": 5e-2, "fp32k": 5e-3, "fp64": 1e-8}[prec]
+    assert np.quantile(ang, 0.99) < {"fp32": 1e-6, "fp32k": 2e-4, "fp64": 1e-9}[prec]
+    assert ang.max() < {"fp32": 1e-3, "fp32k": 5e-3, "fp64": 1e-8}[prec]
 
 
 @pytest.mark.parametrize("prec,ref", [("fp32", "fp32"), ("fp64", "fp64")])
@@ -171,7 +172,9 @@ def test_particle_normals(prec, ref):
     ctx = gpu_context(xyz, r, precision=PREC[prec])
     n = ctx.particle_normals()
     scale = np.linalg.norm(n2, axis=1).max()
-    assert np.max(np.abs(n - n2)) <= (5e-5 if prec == "fp32" else 1e-7) * scale
+    assert np.max(np.abs(n - n2)) <= (1e-5 if prec == "fp32" else 1e-7) * scale
+    if prec == "fp32":
+        assert np.mean(np.all(n == n2, axis=1)) >= 0.999
 
 
 def _tie_nodes(o, centers, halfs, curv, iso, eps):
@@ -322,7 +325,10 @@ def test_full_tree_and_mesh(case, prec):
     if prec == "fp64":
         assert d.max() < 1e-6
     else:
-        assert np.mean(d <= 1e-3) >= 0.995, np.mean(d <= 1e-3)
+        # the same float operations as the oracle: vertices agree bit for bit, apart from a
+        # counted handful fed by libm's last-bit differences
+        assert np.mean(np.all(v == v2, axis=1)) >= 0.999
+        assert np.sum(d > 1e-3) <= max(2, 1e-4 * len(v)), int(np.sum(d > 1e-3))
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
@@ -348,7 +354,11 @@ def test_bisection_on_oracle_edges(prec):
     if prec == "fp64":
         assert np.array_equal(nev, nev2) and d.max() < 1e-9
     else:
-        assert np.mean(d <= 1e-3) > 0.995
+        # a vertex beyond 1e-3 r must have taken a different bisection path (a midpoint value on the
+        # other side of zero), and such paths are counted
+        assert np.all(same_path[d > 1e-3] == False)  # noqa: E712
+        assert np.sum(~same_path) <= max(2, 1e-3 * len(d))
+        assert np.mean(np.all(out == out2, axis=1)) >= 0.999
 
 
 @pytest.mark.parametrize("mode", [1, 2])
