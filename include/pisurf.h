@@ -123,6 +123,17 @@ int pisurf_hashgrid_download(pisurf_ctx* ctx, int32_t* index_list, int32_t* cell
  * *total receives the number of ids (ids beyond cap are not written). */
 int pisurf_neighbors(pisurf_ctx* ctx, const float* q, int64_t nq, int64_t* offsets, int32_t* ids, int64_t cap,
                      int64_t* total);
+/* No counterpart in the reference: pisurf_neighbors for callers that cannot allocate device or
+ * host memory during the call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). q (nq x 3 floats), offsets (nq + 1 int64) and ids (cap int32) are DEVICE
+ * pointers; the per-query counts, 32-bit offsets and scan workspace live in a fixed device array
+ * of the library (one per GPU, held until the process ends, about 8.4 MB), so nq is at most
+ * PISURF_NOALLOC_MAX_QUERIES. *total (host, may be NULL) receives the number of ids; when it
+ * exceeds cap nothing is written and PISURF_E_INVALID is returned. Calls that overlap on one GPU
+ * race on the shared array; the call returns after its results are written. */
+#define PISURF_NOALLOC_MAX_QUERIES (1 << 20)
+int pisurf_neighbors_noalloc(pisurf_ctx* ctx, const float* q, int64_t nq, int64_t* offsets, int32_t* ids,
+                             int64_t cap, int64_t* total);
 /* HashGrid::GetInBoxParticles (src/hash_grid.cpp:164-194) */
 int pisurf_box_particles(pisurf_ctx* ctx, const float* box1, const float* box2, int32_t* ids, int64_t cap,
                          int64_t* total);

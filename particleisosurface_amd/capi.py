@@ -18,6 +18,7 @@ EMPTY, INTERNAL, LEAF = 0, 1, 2
 EXPORTS = [
     "pisurf_create", "pisurf_destroy", "pisurf_last_error", "pisurf_default_params", "pisurf_set_params",
     "pisurf_upload_particles", "pisurf_hashgrid_build", "pisurf_hashgrid_download", "pisurf_neighbors",
+    "pisurf_neighbors_noalloc",
     "pisurf_box_particles", "pisurf_anisotropy", "pisurf_download_aniso", "pisurf_iso_constants",
     "pisurf_particle_normals", "pisurf_eval_points", "pisurf_uniform_grid", "pisurf_sweep_level", "pisurf_build_tree",
     "pisurf_tree_download", "pisurf_qef_solve", "pisurf_bisect_edges", "pisurf_dual_cells", "pisurf_extract",

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -197,8 +198,24 @@ struct KeyMap {
 
 }  // namespace
 
+// ---- working memory of pisurf_neighbors_noalloc -------------------------------------------
+// A fixed device array (each GPU the library runs on holds its own copy until the process ends):
+// per-query counts and 32-bit offsets for at most PISURF_NOALLOC_MAX_QUERIES queries, the scan's
+// workspace and a 64-bit total. Host code never uses the array's name as an address: its device
+// address is looked up once per device (hipGetSymbolAddress, in pisurf_create) and kept here.
+namespace {
+constexpr int64_t kNoAllocQ = PISURF_NOALLOC_MAX_QUERIES;
+constexpr int64_t kNoAllocScanInts = 4096;  // scan workspace for kNoAllocQ + 1 counts: 1032 ints
+constexpr int64_t kNoAllocInts = 2 * (kNoAllocQ + 1) + kNoAllocScanInts + 4;
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_m;
+void* g_scratch[kNoAllocMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
 struct pisurf_ctx {
     int device = 0;
+    void* scratch_ptr = nullptr;  // this device's copy of the no-allocation working array
     hipStream_t stream = nullptr;
     pisurf_params prm;
     std::string err;
@@ -1334,6 +1351,18 @@ int pisurf_create(pisurf_ctx** out, int device) {
         delete c;
         return PISURF_E_HIP;
     }
+    if (device < kNoAllocMaxDevices) {  // the no-allocation array's address on this device, once
+        std::lock_guard<std::mutex> lk(g_scratch_m);
+        if (!g_scratch[device]) {
+            void* p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+                delete c;
+                return PISURF_E_HIP;
+            }
+            g_scratch[device] = p;
+        }
+        c->scratch_ptr = g_scratch[device];
+    }
     pisurf_default_params(&c->prm);
     std::memset(&c->st, 0, sizeof(c->st));
     if (c->counters.ensure(kCtrStripes * sizeof(Counters)) != hipSuccess ||
@@ -1468,6 +1497,43 @@ int pisurf_neighbors(pisurf_ctx* c, const float* q, int64_t nq, int64_t* offsets
         HIPCHK(c, hipMemcpyAsync(ids, c->out3.p, std::min(tot, cap) * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, c->wait());
     }
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_neighbors_noalloc(pisurf_ctx* c, const float* q, int64_t nq, int64_t* offsets, int32_t* ids, int64_t cap,
+                             int64_t* total) {
+    if (!c) return PISURF_E_INVALID;
+    if (!c->grid_done) return c->fail(PISURF_E_STATE, "hash grid not built");
+    if (!c->scratch_ptr) return c->fail(PISURF_E_UNSUPPORTED, "no working array on this device");
+    if (nq < 0 || (nq > 0 && !q) || !offsets || cap < 0 || (cap > 0 && !ids))
+        return c->fail(PISURF_E_INVALID, "bad query or output buffer");
+    if (nq > kNoAllocQ)
+        return c->fail(PISURF_E_UNSUPPORTED, "at most " + std::to_string(kNoAllocQ) + " queries per call");
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    int* counts = (int*)c->scratch_ptr;  // nq + 1
+    int* off32 = counts + (kNoAllocQ + 1);  // nq + 1
+    int* scan_tmp = off32 + (kNoAllocQ + 1);
+    unsigned long long* tot_d = (unsigned long long*)(scan_tmp + kNoAllocScanInts);
+    if (scan_tmp_ints(nq + 1) > kNoAllocScanInts) return c->fail(PISURF_E_UNSUPPORTED, "scan workspace too small");
+    // the array keeps the previous call's contents: the sentinel count and the total start at zero
+    launch_fill_i32(counts + nq, 1, 0, c->stream);
+    launch_fill_i32((int*)tot_d, 2, 0, c->stream);
+    launch_neighbor_count(c->g, q, nq, counts, c->stream);
+    launch_sum_counts(counts, nq, tot_d, c->stream);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long tot64 = 0;
+    HIPCHK(c, c->read(&tot64, tot_d, 8));
+    if (total) *total = (int64_t)tot64;
+    if (tot64 >= (unsigned long long)INT32_MAX)
+        return c->fail(PISURF_E_UNSUPPORTED, "neighbour lists total 2^31 ids or more: query fewer points per call");
+    if ((int64_t)tot64 > cap) return c->fail(PISURF_E_INVALID, "ids capacity below the total (see *total)");
+    exclusive_scan_i32(counts, off32, nq + 1, scan_tmp, c->stream);
+    launch_widen_i64(off32, nq + 1, offsets, c->stream);
+    if (tot64 > 0) launch_neighbor_write(c->g, q, nq, off32, c->vals.as<int>(), ids, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, c->wait());
     return PISURF_OK;
     API_END(c)
 }

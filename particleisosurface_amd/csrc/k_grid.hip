@@ -475,6 +475,22 @@ __global__ void __launch_bounds__(256) k_sum_counts(const int* __restrict__ c, i
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
 }
+__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {
+    if (n <= 0) return;
+    k_fill_i32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+}
+__global__ void k_widen_i64(const int* __restrict__ in, int64_t n, int64_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+void launch_widen_i64(const int* in, int64_t n, int64_t* out, hipStream_t s) {
+    if (n <= 0) return;
+    k_widen_i64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(in, n, out);
+}
 void launch_sum_counts(const int* counts, int64_t n, unsigned long long* out, hipStream_t s) {
     if (n <= 0) return;
     const int64_t blocks = std::min<int64_t>((n + 255) / 256, 2048);

@@ -59,6 +59,10 @@ void launch_neighbor_write(GridDev g, const float* q, int64_t nq, const int* off
                            hipStream_t s);
 // *out += sum of n non-negative counts, in 64 bits
 void launch_sum_counts(const int* counts, int64_t n, unsigned long long* out, hipStream_t s);
+// p[0 .. n) = v (a kernel, not a memset, so the work can be captured in a graph)
+void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
+// out[i] = in[i] widened to 64 bits
+void launch_widen_i64(const int* in, int64_t n, int64_t* out, hipStream_t s);
 void launch_box_count(GridDev g, const int* lo3, const int* n3, int64_t ncells, int* counts, hipStream_t s);
 void launch_box_write(GridDev g, const int* lo3, const int* n3, int64_t ncells, const int* offsets, const int* sidx,
                       int* ids, hipStream_t s);
