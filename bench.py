@@ -24,6 +24,11 @@ sys.path.insert(0, HERE)
 from particleisosurface_amd import capi, synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# MI355X FP32 vector peak as published (packed FP32) and the rate without packed instructions,
+# which this code keeps out: one FP32 FMA wave64 instruction per CU per cycle, 256 CUs at 2.4 GHz
+FP32_PEAK_TFLOPS = 157.3
+FP32_UNPACKED_TFLOPS = 78.6
+N_CU, CLOCK_HZ = 256, 2.4e9
 BYTES_PER_CAND = {capi.FP32: 52, capi.FP32_KAHAN: 52, capi.FP64: 104}
 BYTES_FIXED = 16 + 27 * 8
 
@@ -41,6 +46,30 @@ def measured_traffic(kernel, workload_tag):
             if m and m.group(1) == kernel:
                 return int(m.group(2)), os.path.relpath(f, HERE)
     return None, None
+
+
+def measured_issue(kernel, workload_tag):
+    """VALU wave instructions per launch of `kernel` on the workload from the committed rocprofv3 SQ
+    counter summary profiles/<kernel>_<workload_tag>_issue.md (line
+    `valu_instr_per_launch <kernel> = <count>`), or (None, None)."""
+    import re
+    f = os.path.join(HERE, "profiles", "%s_%s_issue.md" % (kernel, workload_tag))
+    if os.path.exists(f):
+        for line in open(f):
+            m = re.match(r"valu_instr_per_launch (\w+) = (\d+)", line.strip())
+            if m and m.group(1) == kernel:
+                return int(m.group(2)), os.path.relpath(f, HERE)
+    return None, None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def device_memory_in_use():
@@ -106,10 +135,13 @@ def mesh_error(v1, t1, v2, t2, r):
                 frac_within_1e3r=float(np.mean(d <= 1e-3)))
 
 
-def aniso_roofline(ctx, ms_aniso, n):
-    """Algorithmic bytes of the two anisotropy passes (xMean, then covariance and G) per particle,
-    [12 + 216 + 12 C + 12] + [12 + 216 + 12 C + 40] with C the candidates of its 27-cell window,
-    over the anisotropy phase of a timed frame (host clock around both launches)."""
+def aniso_roofline(ctx, ms_aniso, n, wtag):
+    """The two anisotropy passes (xMean, then covariance and G): their algorithmic bytes per
+    particle, [12 + 216 + 12 C + 12] + [12 + 216 + 12 C + 40] with C the candidates of its 27-cell
+    window, over the anisotropy phase of a timed frame (host clock around both launches), and their
+    VALU issue from the committed SQ summary. The candidates' positions and means are re-read from
+    L1 / L2 by neighbouring threads, so the algorithmic rate is not an HBM rate; the limiter is
+    instruction issue."""
     g = ctx.hashgrid_build()
     _, s, e = ctx.hashgrid_download(g["cellnum"])
     nx, ny, nz = (int(v) for v in g["cellnum3"])
@@ -118,13 +150,23 @@ def aniso_roofline(ctx, ms_aniso, n):
     win = sum(p[z:z + nz, y:y + ny, x:x + nx] for z in range(3) for y in range(3) for x in range(3))
     cand = int((cnt * win).sum())
     alg = n * (12 + 216 + 12 + 12 + 216 + 40) + 24 * cand
-    achieved = alg / (ms_aniso * 1e-3) / 1e9
-    return dict(kernels="k_xmean + k_gmat", bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), phase_ms=ms_aniso,
-                candidates_per_particle=cand / max(n, 1), algorithmic_bytes=alg)
+    out = dict(kernels="k_xmean + k_gmat", phase_ms=ms_aniso, candidates_per_particle=cand / max(n, 1),
+               algorithmic_bytes=alg, algorithmic_GBps=round(alg / (ms_aniso * 1e-3) / 1e9, 1), bound="valu")
+    ix, src = measured_issue("k_xmean", wtag)
+    ig, _ = measured_issue("k_gmat", wtag)
+    if ix and ig:
+        achieved = (ix + ig) * 64 * 2 / (ms_aniso * 1e-3) / 1e12
+        out.update(valu_wave_instr=ix + ig, achieved=round(achieved, 2), unit="TFLOP/s", peak=FP32_PEAK_TFLOPS,
+                   frac=round(achieved / FP32_PEAK_TFLOPS, 4), frac_unpacked=round(achieved / FP32_UNPACKED_TFLOPS, 4),
+                   issue_source=src)
+    return out
 
 
-def cpu_baseline(pts_all, xyz, r, ctx, budget_s, log_every=True):
+def cpu_baseline(pts_all, xyz, r, ctx, budget_s, box, gpu_subset_ms):
+    """The oracle on the host cores: (1) SingleEval throughput on a bounded sample of the frame's
+    own octree sample points (the bench's unit), (2) a whole frame (grid, anisotropy, constants,
+    normals, octree, extraction) on a stated spatial subset of the workload, with the full-frame
+    time extrapolated by particle count."""
     from oracle import pyoracle
     o = pyoracle.Oracle(xyz, r, fp64=False)
     t = time.perf_counter()
@@ -141,9 +183,25 @@ def cpu_baseline(pts_all, xyz, r, ctx, budget_s, log_every=True):
         o.eval_points(q)
         spent += time.perf_counter() - t
         done += len(q)
+    del o
+    sel = ((xyz[:, 0] >= box[0]) & (xyz[:, 0] < box[1]) & (xyz[:, 1] >= box[2]) & (xyz[:, 1] < box[3]))
+    sub = xyz[sel]
+    os_ = pyoracle.Oracle(sub, r, fp64=False)
+    t = time.perf_counter()
+    ph = os_.run()
+    cpu_frame_s = time.perf_counter() - t
     return dict(value=done / spent, unit="field evals/s", cores=pyoracle.num_threads(), kind="port",
+                cpu_model=cpu_model(),
                 sample="%d octree sample points of the same frame (SingleEval with std::map cell tables, "
-                       "OpenMP over points), %.1f s; setup %.1f s not timed" % (done, spent, setup))
+                       "OpenMP over points), %.1f s; setup %.1f s not timed" % (done, spent, setup),
+                frame={"subset": "particles with x in [%g, %g) and y in [%g, %g): the crown, the droplets and "
+                                 "the pool beneath them" % tuple(box),
+                       "particles": int(len(sub)), "cpu_frame_s": round(cpu_frame_s, 3),
+                       "cpu_phases_s": {k: round(v, 3) for k, v in ph.items()},
+                       "gpu_frame_ms_same_subset": gpu_subset_ms,
+                       "cpu_over_gpu_same_subset": round(1e3 * cpu_frame_s / gpu_subset_ms, 1) if gpu_subset_ms else None,
+                       "extrapolated_full_frame_s": round(cpu_frame_s * len(xyz) / max(len(sub), 1), 2),
+                       "extrapolation": "subset frame time x (frame particles / subset particles)"})
 
 
 def copy_calibration(mib):
@@ -206,6 +264,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-qef", action="store_true", help="skip the FP64-QEF leg")
     ap.add_argument("--precision", default="fp32k", choices=["fp32", "fp32k", "fp64"])
     ap.add_argument("--no-dedupe", action="store_true", help="evaluate all 27 samples of every node")
     ap.add_argument("--qef", type=int, default=0, choices=[0, 1, 2],
@@ -261,28 +320,56 @@ def main():
                                             st["ms_grid"] / args.steps, st["ms_aniso"] / args.steps,
                                             st["ms_normals"] / args.steps, st["ms_tree"] / args.steps,
                                             st["ms_extract_host"] / args.steps, st["ms_bisect"] / args.steps))
-    value = evals_per_frame / (dt / args.steps) / args.steps
-    # roofline of the dominant field-evaluation kernel, the node-sampling kernel: algorithmic bytes of
-    # the evaluations it performs (one per distinct sample point when deduplicated)
+    performed = st["sample_unique"] if st["sample_unique"] > 0 else st["sample_evals"]
+    performed_per_frame = (evals_per_frame - st["sample_evals"] + performed) // args.steps
+    # the headline: field evaluations the device performed per second of frame time (node samples
+    # shared by neighbouring nodes are evaluated once); the reference's count (27 per sampled node)
+    # is reported beside it
+    value = performed_per_frame / (dt / args.steps)
+
+    # roofline of the dominant field-evaluation kernel, the node-sampling kernel. Its limiter is VALU
+    # instruction issue (SQ counters): achieved = its VALU wave instructions per launch (committed
+    # rocprofv3 SQ summary) x 64 lanes x 2 flops, as FMA-equivalent TFLOP/s, over the launch time
+    # measured here with HIP events. Measured HBM traffic and the algorithmic bytes (16 + 27*8 + R*C
+    # per evaluation, C candidates) are reported beside it.
     def kernel_roof(name, evals, cands, launches, ms):
         k_ms = ms / max(launches, 1)
+        k_s = k_ms * 1e-3
         alg = evals * BYTES_FIXED + cands * BYTES_PER_CAND[prec]
-        achieved = alg / max(launches, 1) / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
         traffic, source = measured_traffic(name, wtag)
-        return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=source, kernel=name,
+        instr, isource = measured_issue(name, wtag)
+        roof = dict(bound="valu", unit="TFLOP/s", peak=FP32_PEAK_TFLOPS, kernel=name,
                     launches_per_frame=launches / args.steps, avg_launch_ms=round(k_ms, 4),
                     evals_per_launch=evals / max(launches, 1), candidates_per_eval=cands / max(evals, 1),
-                    algorithmic_bytes_per_launch=alg / max(launches, 1),
                     evals_per_s=evals / (ms * 1e-3) if ms > 0 else None)
+        if instr and k_s > 0:
+            achieved = instr * 64 * 2 / k_s / 1e12
+            roof.update(achieved=round(achieved, 2), frac=round(achieved / FP32_PEAK_TFLOPS, 4),
+                        peak_unpacked=FP32_UNPACKED_TFLOPS, frac_unpacked=round(achieved / FP32_UNPACKED_TFLOPS, 4),
+                        valu_wave_instr_per_launch=instr,
+                        valu_wave_instr_per_cu_cycle=round(instr / (N_CU * CLOCK_HZ * k_s), 3),
+                        issue_source=isource)
+        else:
+            roof.update(achieved=None, frac=None, issue_source=None)
+        roof.update(traffic=traffic, traffic_source=source,
+                    hbm_achieved_GBps=round(traffic / k_s / 1e9, 1) if traffic and k_s > 0 else None,
+                    hbm_frac=round(traffic / k_s / 1e9 / HBM_PEAK_GBS, 4) if traffic and k_s > 0 else None,
+                    algorithmic_bytes_per_launch=alg / max(launches, 1),
+                    algorithmic_GBps=round(alg / max(launches, 1) / k_s / 1e9, 1) if k_s > 0 else None)
+        return roof
     if st["run_launches"] > 0:
         roof = kernel_roof("k_run_samples", st["run_evals"], st["run_candidates"], st["run_launches"],
                            st["ms_run_kernel"])
     else:
         roof = kernel_roof("k_samples", st["sample_evals"], st["sample_candidates"], st["sample_launches"],
                            st["ms_sample_kernel"])
-    performed = st["sample_unique"] if st["sample_unique"] > 0 else st["sample_evals"]
     v32, t32 = ctx.mesh()
+    # the same frame from host particle arrays to host mesh arrays (PCIe upload included)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.upload(xyz)
+        ctx.run_frame()
+    host_ms = 1e3 * (time.perf_counter() - t) / args.steps
     out = {
         "metric": "field evals/s + frame mesh time @4M particles, FP32-comp vs FP64 at stated err",
         "value": round(value, 1), "unit": "field evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -294,8 +381,10 @@ def main():
         "qef_solves_per_frame": st["qef_solves"] // args.steps,
         "device_memory_bytes": device_memory_in_use() - mem0 if mem0 is not None else None,
         "frame_ms": round(ms, 3),
-        "field_evals_per_frame": evals_per_frame // args.steps,
-        "field_evals_performed_per_frame": (evals_per_frame - st["sample_evals"] + performed) // args.steps,
+        "frame_ms_from_host_particles": round(host_ms, 3),
+        "field_evals_performed_per_frame": performed_per_frame,
+        "field_evals_reference_count_per_frame": evals_per_frame // args.steps,
+        "field_evals_reference_count_per_s": round(evals_per_frame / dt, 1),
         "phases_ms": {k[3:]: round(st[k] / args.steps, 3) for k in
                       ["ms_grid", "ms_aniso", "ms_normals", "ms_tree", "ms_extract_host", "ms_bisect"]},
         "mesh": {"verts": len(v32), "tris": len(t32), "nodes": info["n_nodes"]},
@@ -342,17 +431,40 @@ def main():
         set_params(prec)
         info, st, dt = frame(ctx, 1, 0)
 
+    if not args.no_qef and args.qef == 0:
+        # the leaf vertex by the FP64 truncated-SVD QEF (TNode::NodeCalcNode, dormant in the
+        # reference) in the same precision mode: solves per frame and per second of frame time
+        ctx.set_params(r, precision=prec, sample_dedupe=not args.no_dedupe, qef_mode=capi.QEF_SVD64,
+                       depth_max_override=depth)
+        infoq, stq, dtq = frame(ctx, args.steps, 1)
+        msq = 1e3 * dtq / args.steps
+        out["qef"] = {"mode": "fp64 truncated-SVD pseudo-inverse (tol 1e-3)", "frame_ms": round(msq, 3),
+                      "solves_per_frame": stq["qef_solves"] // args.steps,
+                      "solves_per_s": round(stq["qef_solves"] / dtq, 1), "nodes": infoq["n_nodes"]}
+        log("qef frame %.2f ms, %d solves" % (msq, out["qef"]["solves_per_frame"]))
+        set_params(prec)
+        info, st, dt = frame(ctx, 1, 0)
+
     if args.batch:
         out["batch"] = batch_leg(prec, not args.no_dedupe, args.qef)
         log("batch: %s" % out["batch"])
 
     if not args.no_cpu:
         pts = sample_points_of_tree(ctx, info, r, limit=20_000_000)
-        out["cpu_baseline"] = cpu_baseline(pts, xyz, r, ctx, args.cpu_seconds)
-        log("cpu baseline %.3g evals/s on %d threads" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"]))
+        box = (0.2, 0.4, 0.2, 0.4) if args.workload == "crown" else (0.2, 0.3, 0.2, 0.3)
+        sel = (xyz[:, 0] >= box[0]) & (xyz[:, 0] < box[1]) & (xyz[:, 1] >= box[2]) & (xyz[:, 1] < box[3])
+        sub_ctx = capi.Context(0)
+        sub_ctx.set_params(r, precision=capi.FP32, sample_dedupe=not args.no_dedupe, depth_max_override=depth)
+        sub_ctx.upload(xyz[sel])
+        _, _, dts = frame(sub_ctx, 3, 1)
+        sub_ctx.close()
+        out["cpu_baseline"] = cpu_baseline(pts, xyz, r, ctx, args.cpu_seconds, box, round(1e3 * dts / 3, 3))
+        log("cpu baseline %.3g evals/s on %d threads; subset frame %.1f s (%d particles)" % (
+            out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"], out["cpu_baseline"]["frame"]["cpu_frame_s"],
+            out["cpu_baseline"]["frame"]["particles"]))
     else:
         out["cpu_baseline"] = None
-    out["aniso_roofline"] = aniso_roofline(ctx, out["phases_ms"]["aniso"], len(xyz))
+    out["aniso_roofline"] = aniso_roofline(ctx, out["phases_ms"]["aniso"], len(xyz), wtag)
     ctx.close()
     print(json.dumps(out), flush=True)
 
