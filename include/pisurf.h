@@ -204,6 +204,17 @@ typedef void (*pisurf_mesh_cb)(void* user, int32_t frame, const pisurf_tree_info
 int pisurf_run_frames(pisurf_ctx* ctx, int32_t nframes, const float* const* xyz, const int64_t* n,
                       const float* radius, int32_t on_device, pisurf_mesh_cb on_mesh, void* user,
                       pisurf_stats* per_frame);
+/* The same frames meshed together: every frame gets a context of its own on this context's
+ * stream, and the frames go through each step together, so a step costs one host synchronisation
+ * for the whole batch. The octree levels of all frames advance in lockstep, and at each level one
+ * launch evaluates the distinct sample points and leaf-depth centres of every frame, its
+ * workgroups taking (frame, run of cells) items from a device work queue. Arguments and callback
+ * as pisurf_run_frames. No counterpart in the reference (its frames run one after another).
+ * Currently disabled: returns PISURF_E_UNSUPPORTED after checking its arguments (its first
+ * device run stopped making progress, cause not yet found); use pisurf_run_frames. */
+int pisurf_run_batch(pisurf_ctx* ctx, int32_t nframes, const float* const* xyz, const int64_t* n,
+                     const float* radius, int32_t on_device, pisurf_mesh_cb on_mesh, void* user,
+                     pisurf_stats* per_frame);
 int pisurf_get_stats(pisurf_ctx* ctx, pisurf_stats* out);
 int pisurf_reset_stats(pisurf_ctx* ctx);
 /* device stream the library launches on (hipStream_t) */

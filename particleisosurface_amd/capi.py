@@ -23,7 +23,7 @@ EXPORTS = [
     "pisurf_particle_normals", "pisurf_eval_points", "pisurf_uniform_grid", "pisurf_sweep_level", "pisurf_build_tree",
     "pisurf_tree_download", "pisurf_qef_solve", "pisurf_bisect_edges", "pisurf_dual_cells", "pisurf_extract",
     "pisurf_mesh_download",
-    "pisurf_run_frame", "pisurf_run_frames", "pisurf_get_stats", "pisurf_reset_stats", "pisurf_stream",
+    "pisurf_run_frame", "pisurf_run_frames", "pisurf_run_batch", "pisurf_get_stats", "pisurf_reset_stats", "pisurf_stream",
 ]
 
 
@@ -79,6 +79,7 @@ def load():
         L.pisurf_stream.argtypes = [C.c_void_p]
         L.pisurf_run_frames.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_float), C.c_int32, MESH_CB, C.c_void_p, C.c_void_p]
+        L.pisurf_run_batch.argtypes = L.pisurf_run_frames.argtypes
         _lib = L
     return _lib
 
@@ -282,8 +283,13 @@ class Context:
             self._c(self.L.pisurf_uniform_grid(self.h, None, None, None, _p(out)), "uniform_grid")
         return out, np.array(list(box), np.float32), sp.value
 
-    def run_frames(self, frames, radii, on_mesh=None):
-        """Mesh a batch of frames in one persistent call. frames: list of (n, 3) float32 arrays;
+    def run_batch(self, frames, radii, on_mesh=None):
+        """Mesh frames together (pisurf_run_batch): lockstep levels, one run-queue launch per level
+        for every frame. Arguments and result as run_frames."""
+        return self.run_frames(frames, radii, on_mesh, entry="pisurf_run_batch")
+
+    def run_frames(self, frames, radii, on_mesh=None, entry="pisurf_run_frames"):
+        """Mesh a batch of frames in one call. frames: list of (n, 3) float32 arrays;
         on_mesh(k, verts (nv, 3) float64, tris (nt, 3) int32, tree dict) is called after each frame.
         Returns the per-frame statistics."""
         nf = len(frames)
@@ -307,8 +313,7 @@ class Context:
                 err.append(e)
 
         fn = MESH_CB(cb)
-        self._c(self.L.pisurf_run_frames(self.h, nf, ptrs, ns, rs, 0, fn, None, C.cast(per, C.c_void_p)),
-                "run_frames")
+        self._c(getattr(self.L, entry)(self.h, nf, ptrs, ns, rs, 0, fn, None, C.cast(per, C.c_void_p)), entry)
         if err:
             raise err[0]
         if frames:

@@ -215,7 +215,13 @@ static __device__ __align__(256) int scratch[kNoAllocInts];
 
 struct pisurf_ctx {
     int device = 0;
+    bool own_stream = true;       // false for the frame contexts of a batch (the batch's stream)
     void* scratch_ptr = nullptr;  // this device's copy of the no-allocation working array
+    std::vector<pisurf_ctx*> sub;  // pisurf_run_batch: one context per frame, kept for reuse
+    // pisurf_run_batch work space: frame descriptors of the run queue, item offsets, items, counter
+    DBuf bq_frames, bq_off, bq_items, bq_head;
+    PinnedBuf bq_host;
+    size_t rb_used = 0;  // bytes of rb claimed by the pending batched read-backs
     hipStream_t stream = nullptr;
     pisurf_params prm;
     std::string err;
@@ -289,6 +295,7 @@ struct pisurf_ctx {
     }
 
     ~pisurf_ctx() {
+        for (auto* b : sub) delete b;
         if (wev) (void)hipEventDestroy(wev);
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -299,7 +306,7 @@ struct pisurf_ctx {
         if (eb1) (void)hipEventDestroy(eb1);
         for (auto e : ev_chunk)
             if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
     // start/stop events around one evaluation launch; kind 0: per-point sampling kernel, 1: run kernel
     hipError_t event_pair(hipEvent_t& a, hipEvent_t& b, size_t* slot = nullptr) {
@@ -450,19 +457,21 @@ int host_cell_coord(float p, float b0, float cell, int n) {
     return (int)v;
 }
 
-int ensure_bbox(pisurf_ctx* c) {
-    if (c->bbox_done) return PISURF_OK;
+// the tight bounding box on the device: min / max (6 floats) at small + 64, NaN count at small
+int bbox_launch(pisurf_ctx* c) {
     if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
     HIPCHK(c, c->part.ensure(6 * 1024 * sizeof(float) + 64));
     HIPCHK(c, c->small.ensure(256));
     HIPCHK(c, hipMemsetAsync(c->small.as<char>(), 0, 256, c->stream));
-    float* out6 = (float*)(c->small.as<char>() + 64);
-    int* nanc = c->small.as<int>();
-    launch_bbox(c->xyz.as<float>(), c->n, c->part.as<float>(), out6, nanc, c->stream);
+    launch_bbox(c->xyz.as<float>(), c->n, c->part.as<float>(), (float*)(c->small.as<char>() + 64), c->small.as<int>(),
+                c->stream);
     HIPCHK(c, hipGetLastError());
-    int nans = 0;
-    HIPCHK(c, c->read(c->bbox, out6, 6 * sizeof(float)));
-    HIPCHK(c, c->read(&nans, nanc, sizeof(int)));
+    return PISURF_OK;
+}
+const void* bbox_dev(pisurf_ctx* c) { return c->small.as<char>() + 64; }
+const int* bbox_nan_dev(pisurf_ctx* c) { return c->small.as<int>(); }
+// with c->bbox read back
+int bbox_finish(pisurf_ctx* c, int nans) {
     if (nans) return c->fail(PISURF_E_INVALID, "particle positions contain NaN");
     for (int a = 0; a < 6; a++)
         if (!std::isfinite(c->bbox[a])) return c->fail(PISURF_E_INVALID, "particle positions are not finite");
@@ -471,10 +480,41 @@ int ensure_bbox(pisurf_ctx* c) {
     return PISURF_OK;
 }
 
+int ensure_bbox(pisurf_ctx* c) {
+    if (c->bbox_done) return PISURF_OK;
+    int rc = bbox_launch(c);
+    if (rc) return rc;
+    int nans = 0;
+    HIPCHK(c, c->read(c->bbox, bbox_dev(c), 6 * sizeof(float)));
+    HIPCHK(c, c->read(&nans, bbox_nan_dev(c), sizeof(int)));
+    return bbox_finish(c, nans);
+}
+
+// HashGrid build launches on the bbox already known on the host; the count of particles outside
+// the grid is left at grid_ninv_dev for grid_finish
+int grid_launch(pisurf_ctx* c, const float* bbox, float inf);
+const int* grid_ninv_dev(pisurf_ctx* c) { return c->small.as<int>(); }
+int grid_finish(pisurf_ctx* c, int ninv) {
+    c->ginfo.n_invalid = ninv;
+    c->grid_done = true;
+    c->aniso_done = c->normals_done = c->tree_done = false;
+    return PISURF_OK;
+}
+
 int do_grid(pisurf_ctx* c, const float* bbox, float inf) {
     int rc = ensure_bbox(c);
     if (rc) return rc;
     const double t0 = now_ms();
+    rc = grid_launch(c, bbox, inf);
+    if (rc) return rc;
+    int ninv = 0;
+    HIPCHK(c, c->read(&ninv, grid_ninv_dev(c), sizeof(int)));
+    grid_finish(c, ninv);
+    c->st.ms_grid += now_ms() - t0;
+    return PISURF_OK;
+}
+
+int grid_launch(pisurf_ctx* c, const float* bbox, float inf) {
     if (!(c->prm.radius > 0.0f) || !std::isfinite(c->prm.radius)) return c->fail(PISURF_E_INVALID, "radius must be > 0");
     if (!(inf > 0.0f)) return c->fail(PISURF_E_INVALID, "inf_factor must be > 0");
     float b[6];
@@ -513,18 +553,22 @@ int do_grid(pisurf_ctx* c, const float* bbox, float inf) {
     launch_cell_offsets(c->keys.as<uint32_t>(), n, ninv_d, ncell, c->off.as<int>(), c->stream);
     launch_gather_pos(c->xyz.as<float>(), c->vals.as<int>(), n, c->spos.as<float4>(), c->stream);
     HIPCHK(c, hipGetLastError());
-    int ninv = 0;
-    HIPCHK(c, c->read(&ninv, ninv_d, sizeof(int)));
     c->g = g;
     for (int a = 0; a < 6; a++) c->ginfo.bounding[a] = b[a];
     for (int a = 0; a < 3; a++) c->ginfo.xyz_cell_num[a] = n3[a];
     c->ginfo.cell_num = ncell;
     c->ginfo.cell_size = cell;
-    c->ginfo.n_invalid = ninv;
-    c->grid_done = true;
-    c->aniso_done = c->normals_done = c->tree_done = false;
-    c->st.ms_grid += now_ms() - t0;
     return PISURF_OK;
+}
+
+// compute_Gs_xMeans launches; the count of FP64 eigensolves is left at aniso_nfp_dev
+int aniso_launch(pisurf_ctx* c);
+const void* aniso_nfp_dev(pisurf_ctx* c) { return c->small.as<char>() + 128; }
+void aniso_finish(pisurf_ctx* c, unsigned long long nfp) {
+    c->n_fp64_eig = nfp;
+    c->prec_rec = prec_of(c);
+    c->aniso_done = true;
+    c->normals_done = c->tree_done = false;
 }
 
 int do_aniso(pisurf_ctx* c) {
@@ -533,6 +577,16 @@ int do_aniso(pisurf_ctx* c) {
         if (rc) return rc;
     }
     const double t0 = now_ms();
+    int rc = aniso_launch(c);
+    if (rc) return rc;
+    unsigned long long nfp = 0;
+    HIPCHK(c, c->read(&nfp, aniso_nfp_dev(c), 8));
+    aniso_finish(c, nfp);
+    c->st.ms_aniso += now_ms() - t0;
+    return PISURF_OK;
+}
+
+int aniso_launch(pisurf_ctx* c) {
     const int prec = prec_of(c);
     const size_t rb = (prec == PISURF_FP64) ? sizeof(double4) : sizeof(float4);
     // compensated FP32 keeps xMean as hi + lo (two float4 per particle) and a residual record
@@ -557,11 +611,6 @@ int do_aniso(pisurf_ctx* c) {
     launch_gmat(prec, c->g, c->spos.as<float4>(), c->n, c->ginfo.n_invalid, ap, c->xm.p, c->rec_a.p, c->rec_g.p,
                 c->rec_lo.p, nfp, c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, c->read(&c->n_fp64_eig, nfp, 8));
-    c->prec_rec = prec;
-    c->aniso_done = true;
-    c->normals_done = c->tree_done = false;
-    c->st.ms_aniso += now_ms() - t0;
     return PISURF_OK;
 }
 
@@ -570,17 +619,24 @@ int need_aniso(pisurf_ctx* c) {
     return PISURF_OK;
 }
 
+// CalcParticlesNormal launches (records and constants ready)
+int normals_launch(pisurf_ctx* c) {
+    HIPCHK(c, c->nrm.ensure(c->n * sizeof(float4)));
+    launch_normals(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->spos.as<float4>(), c->n,
+                   c->ginfo.n_invalid, c->nrm.as<float4>(), c->ctr(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    c->normals_done = true;
+    return PISURF_OK;
+}
+
 int do_normals(pisurf_ctx* c) {
     int rc = need_aniso(c);
     if (rc) return rc;
     if (!c->consts_done) compute_constants(c);
     const double t0 = now_ms();
-    HIPCHK(c, c->nrm.ensure(c->n * sizeof(float4)));
-    launch_normals(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->spos.as<float4>(), c->n,
-                   c->ginfo.n_invalid, c->nrm.as<float4>(), c->ctr(), c->stream);
-    HIPCHK(c, hipGetLastError());
+    rc = normals_launch(c);
+    if (rc) return rc;
     HIPCHK(c, c->wait());
-    c->normals_done = true;
     c->st.ms_normals += now_ms() - t0;
     return PISURF_OK;
 }
@@ -1307,6 +1363,336 @@ int sync_counters(pisurf_ctx* c) {
     c->st.run_evals = (int64_t)h.run_evals;
     c->st.run_candidates = (int64_t)h.run_candidates;
     c->st.fp64_eigensolves = (int64_t)c->n_fp64_eig;
+    return PISURF_OK;
+}
+
+// ---- pisurf_run_batch: the frames of a batch meshed together ------------------------------
+// Every frame has its own context (sharing the batch's stream). The frames go through each step
+// together: a step's launches are issued for every frame, then the counts the host needs are read
+// for all frames at once (one wait), so a step costs one synchronisation for the whole batch
+// rather than one per frame. The octree levels advance in lockstep, and at each level one
+// launch of the run queue evaluates the distinct sample points (and leaf-depth centres) of every
+// frame, its workgroups taking (frame, run) items from a device counter.
+
+constexpr bool kBatchEnabled = false;
+
+pisurf_ctx* make_frame_ctx(pisurf_ctx* c) {
+    pisurf_ctx* f = new (std::nothrow) pisurf_ctx();
+    if (!f) return nullptr;
+    f->device = c->device;
+    f->stream = c->stream;
+    f->own_stream = false;
+    if (hipEventCreateWithFlags(&f->wev, hipEventDisableTiming) != hipSuccess || f->rb.ensure(kReadbackBytes) != hipSuccess ||
+        f->counters.ensure(kCtrStripes * sizeof(Counters)) != hipSuccess) {
+        delete f;
+        return nullptr;
+    }
+    std::memset(&f->st, 0, sizeof(f->st));
+    return f;
+}
+
+// device-to-host reads of many frames' counts, completed by one wait on the shared stream; each
+// frame context stages its reads in its own page-locked buffer
+struct BatchReads {
+    struct Req {
+        pisurf_ctx* c;
+        const void* d;
+        void* host;
+        size_t bytes, off;
+    };
+    std::vector<Req> req;
+    pisurf_ctx* owner;
+    explicit BatchReads(pisurf_ctx* o) : owner(o) {}
+    int add(pisurf_ctx* c, const void* d, void* host, size_t bytes) {
+        const size_t off = (c->rb_used + 15) & ~size_t(15);
+        if (off + bytes > kReadbackBytes) return owner->fail(PISURF_E_STATE, "batched read-back buffer full");
+        c->rb_used = off + bytes;
+        req.push_back({c, d, host, bytes, off});
+        HIPCHK(owner, hipMemcpyAsync((char*)c->rb.p + off, d, bytes, hipMemcpyDeviceToHost, owner->stream));
+        return PISURF_OK;
+    }
+    int flush() {
+        HIPCHK(owner, owner->wait());
+        for (const Req& r : req) {
+            std::memcpy(r.host, (const char*)r.c->rb.p + r.off, r.bytes);
+            r.c->rb_used = 0;
+        }
+        req.clear();
+        return PISURF_OK;
+    }
+};
+
+struct BatchFrame {
+    pisurf_ctx* c = nullptr;
+    int k = 0;             // index of the frame in the caller's arrays
+    bool active = false;   // still building its octree
+    size_t li = 0;         // current level
+    int nans = 0, ninv = 0;
+    unsigned long long nfp = 0;
+    // the current level
+    bool leaf_level = false, dedupe = false, centers = false, split_curv = false;
+    TreeParams tp;
+    int group = 64;
+    int ksamp = 0, kcent = 0, nh = 0, ng = 0, nr = 0, ngc = 0, nrc = 0, n_int = 0;
+    int64_t nent = 0;
+    SortedKV skv{nullptr, nullptr, 0}, ckv{nullptr, nullptr, 0};
+    int T = 128;
+};
+
+LevelBuf* level_of(pisurf_ctx* c, size_t i) {
+    while (c->levels.size() <= i) c->levels.push_back(new LevelBuf());
+    return c->levels[i];
+}
+
+// the launches of sweep() up to the first count read (emptiness, sample list, the per-node sampling
+// of levels that are not deduplicated); the counts to read are queued
+int batch_level_begin(BatchFrame& f, BatchReads& br) {
+    pisurf_ctx* c = f.c;
+    LevelBuf* L = c->levels[f.li];
+    const int ld = (int)f.li;
+    const size_t rsz = prec_of(c) == PISURF_FP64 ? 8 : 4;
+    HIPCHK(c, c->samples.ensure(L->n * 27 * rsz));
+    const float half = std::ldexp(c->root_half, -ld);
+    f.leaf_level = double((2.0f * half) - c->prm.radius) < 1e-8;
+    f.tp = tree_params(c);
+    if (f.leaf_level) f.tp.curv = 0;
+    const int64_t need_tmp = scan_tmp_ints(L->n + 1) + 64;
+    if ((size_t)need_tmp * 4 > c->tmp.cap) {
+        HIPCHK(c, c->wait());
+        HIPCHK(c, c->tmp.ensure(need_tmp * 4));
+    }
+    const double w = (2 * std::ldexp((double)c->root_half, -ld) + 2 * (double)f.tp.margin) / (double)c->g.cell + 1;
+    f.group = (w * w <= 6) ? 4 : (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
+    f.split_curv = ld == c->dmin;
+    const NodeLevel v = L->view();
+    launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), f.tp, v, f.group,
+                      f.split_curv ? kEmptyOnly : kEmptyAndCurv, nullptr, 0, c->stream);
+    HIPCHK(c, c->cl_flag.ensure((L->n + 1) * 4));
+    HIPCHK(c, c->cl_pos.ensure((L->n + 1) * 4));
+    HIPCHK(c, c->cl_list.ensure((L->n + 1) * 4));
+    launch_sample_list(f.tp, v, L->flag.as<int>(), L->scanned.as<int>(), L->child.as<int>(), c->cl_flag.as<int>(),
+                       c->cl_pos.as<int>(), c->cl_list.as<int>(), c->tmp.as<int>(), c->stream);
+    f.dedupe = c->prm.sample_dedupe && ld <= 19 && ld >= c->dmin;
+    f.centers = f.leaf_level && c->prm.qef_mode == 0 && ld >= c->dmin;
+    f.ksamp = f.kcent = f.nh = f.ng = f.nr = f.ngc = f.nrc = 0;
+    f.nent = 0;
+    int rc = PISURF_OK;
+    if (f.dedupe) rc = br.add(c, L->scanned.as<int>() + L->n, &f.ksamp, 4);
+    else
+        launch_samples(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), f.tp, v, L->child.as<int>(),
+                       L->scanned.as<int>() + L->n, c->samples.p, c->ctr(), c->stream);
+    if (!rc && f.centers) rc = br.add(c, c->cl_pos.as<int>() + L->n, &f.kcent, 4);
+    HIPCHK(c, hipGetLastError());
+    return rc;
+}
+
+int batch_ensure_entries(pisurf_ctx* c, int64_t m, bool heads) {
+    HIPCHK(c, c->sd_key.ensure(m * 8));
+    HIPCHK(c, c->sd_key2.ensure(m * 8));
+    HIPCHK(c, c->sd_val.ensure(m * 4));
+    HIPCHK(c, c->sd_val2.ensure(m * 4));
+    HIPCHK(c, c->sd_cwork.ensure(4 * (m + 2) * 4));
+    if (heads) {
+        const size_t rsz = prec_of(c) == PISURF_FP64 ? 8 : 4;
+        HIPCHK(c, c->sd_head.ensure((2 * ((m + 255) / 256) + 2) * 4));
+        HIPCHK(c, c->sd_hpos.ensure((m + 1) * 4));
+        HIPCHK(c, c->sd_rep.ensure(m * 4));
+        HIPCHK(c, c->sd_hlist.ensure(m * 4));
+        HIPCHK(c, c->sd_vals.ensure(m * rsz));
+    }
+    const int64_t need = std::max(radix_tmp_ints(m), scan_tmp_ints(m + 1)) + 64;
+    HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, need * 4)));
+    return PISURF_OK;
+}
+
+// sample keys, their sort and the heads (deduplicated levels); centre keys, their sort and cell
+// groups (leaf-depth level)
+int batch_level_keys(BatchFrame& f, BatchReads& br) {
+    pisurf_ctx* c = f.c;
+    LevelBuf* L = c->levels[f.li];
+    const NodeLevel v = L->view();
+    if (f.ksamp > 0 && f.kcent > 0) return c->fail(PISURF_E_STATE, "samples and centres on one level");
+    if (f.ksamp > 0) {
+        f.nent = (int64_t)f.ksamp * 27;
+        if (f.nent >= INT32_MAX) return c->fail(PISURF_E_UNSUPPORTED, "octree level has too many sample points");
+        int rc = batch_ensure_entries(c, f.nent, true);
+        if (rc) return rc;
+        const float origin[3] = {c->root_c[0] - c->root_half, c->root_c[1] - c->root_half, c->root_c[2] - c->root_half};
+        const uint64_t groups = (uint64_t)(c->g.n[0] + 2) * (uint64_t)(c->g.n[1] + 2) * (uint64_t)(c->g.n[2] + 2);
+        const int lb = 3;
+        const int kbits = std::min(64, bits_for((int64_t)groups + 1) + 3 * lb);
+        const int vbits = pack_vbits(kbits, f.nent);
+        launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), f.nent, origin, lb, vbits, c->sd_key.as<uint64_t>(),
+                                  c->sd_val.as<int>(), c->stream);
+        f.skv = radix_sort_entries(c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
+                                   c->sd_val2.as<int>(), f.nent, kbits, vbits, c->tmp.as<int>(), c->stream);
+        launch_heads(v, L->child.as<int>(), f.skv, f.nent, c->sd_head.as<int>(), c->sd_rep.as<int>(),
+                     c->sd_hlist.as<int>(), c->sd_hpos.as<int>(), c->tmp.as<int>(), c->stream);
+        c->host_sample_entries += (unsigned long long)f.nent;
+        rc = br.add(c, heads_count(c->sd_head.as<int>(), f.nent), &f.nh, 4);
+        if (rc) return rc;
+    }
+    if (f.kcent > 0) {
+        int rc = batch_ensure_entries(c, f.kcent, false);
+        if (rc) return rc;
+        f.ckv = launch_center_keys_sorted(c->g, v, c->cl_list.as<int>(), f.kcent, c->sd_key.as<uint64_t>(),
+                                          c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(), c->sd_val2.as<int>(),
+                                          c->tmp.as<int>(), c->stream);
+        launch_cell_groups(f.ckv, 0, nullptr, f.kcent, c->sd_cwork.as<int>(), c->tmp.as<int>(), c->stream);
+        rc = br.add(c, groups_count(c->sd_cwork.as<int>(), f.kcent), &f.ngc, 4);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipGetLastError());
+    return PISURF_OK;
+}
+
+// the distinct sample points' cell groups; the centres' runs
+int batch_level_groups(BatchFrame& f, BatchReads& br) {
+    pisurf_ctx* c = f.c;
+    if (f.nh > 0) {
+        launch_cell_groups(f.skv, 9, c->sd_hlist.as<int>(), f.nh, c->sd_cwork.as<int>(), c->tmp.as<int>(), c->stream);
+        int rc = br.add(c, groups_count(c->sd_cwork.as<int>(), f.nh), &f.ng, 4);
+        if (rc) return rc;
+    }
+    if (f.kcent > 0 && f.ngc > 0) {
+        int kx = 1, T = 128;
+        run_shape_for((double)f.kcent / f.ngc, &kx, &T);
+        launch_run_starts(c->g, f.ckv, 0, nullptr, f.kcent, f.ngc, kx, c->sd_cwork.as<int>(), c->tmp.as<int>(),
+                          c->stream);
+        int rc = br.add(c, runs_count(c->sd_cwork.as<int>(), f.kcent, f.ngc), &f.nrc, 4);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipGetLastError());
+    return PISURF_OK;
+}
+
+// the distinct sample points' runs
+int batch_level_runs(BatchFrame& f, BatchReads& br) {
+    pisurf_ctx* c = f.c;
+    if (f.nh > 0 && f.ng > 0) {
+        int kx = 1;
+        run_shape_for((double)f.nh / f.ng, &kx, &f.T);
+        launch_run_starts(c->g, f.skv, 9, c->sd_hlist.as<int>(), f.nh, f.ng, kx, c->sd_cwork.as<int>(),
+                          c->tmp.as<int>(), c->stream);
+        int rc = br.add(c, runs_count(c->sd_cwork.as<int>(), f.nh, f.ng), &f.nr, 4);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipGetLastError());
+    return PISURF_OK;
+}
+
+// one launch of the run queue for every frame's runs of the level
+int batch_level_eval(pisurf_ctx* c, std::vector<BatchFrame*>& fr) {
+    std::vector<RunFrame> rf;
+    std::vector<int> off(1, 0);
+    int T = 128;
+    int64_t most = -1;
+    for (BatchFrame* f : fr) {
+        pisurf_ctx* s = f->c;
+        LevelBuf* L = s->levels[f->li];
+        const FieldConst fc = field_const(s);
+        if (f->nr > 0) {
+            RunFrame r;
+            r.g = s->g;
+            r.ra = s->rec_a.p;
+            r.rg = s->rec_g.p;
+            r.fc = fc;
+            r.lv = L->view();
+            r.list = L->child.as<int>();
+            r.kv = f->skv;
+            r.hlist = s->sd_hlist.as<int>();
+            r.rstart = runs_start(s->sd_cwork.as<int>(), f->nh, f->ng);
+            r.out = s->sd_vals.p;
+            r.ctr = s->ctr();
+            r.kind = 0;
+            rf.push_back(r);
+            off.push_back(off.back() + f->nr);
+            if (f->nh > most) {
+                most = f->nh;
+                T = f->T;
+            }
+        }
+        if (f->nrc > 0) {
+            RunFrame r;
+            r.g = s->g;
+            r.ra = s->rec_a.p;
+            r.rg = s->rec_g.p;
+            r.fc = fc;
+            r.lv = L->view();
+            r.list = s->cl_list.as<int>();
+            r.kv = f->ckv;
+            r.hlist = nullptr;
+            r.rstart = runs_start(s->sd_cwork.as<int>(), f->kcent, f->ngc);
+            r.out = s->samples.p;
+            r.ctr = s->ctr();
+            r.kind = 1;
+            rf.push_back(r);
+            off.push_back(off.back() + f->nrc);
+        }
+    }
+    const int nf = (int)rf.size();
+    if (nf == 0) return PISURF_OK;
+    const int nitems = off.back();
+    const size_t fbytes = nf * sizeof(RunFrame), obytes = (nf + 1) * sizeof(int);
+    HIPCHK(c, c->bq_host.ensure(fbytes + obytes + 64));
+    std::memcpy(c->bq_host.p, rf.data(), fbytes);
+    std::memcpy((char*)c->bq_host.p + fbytes, off.data(), obytes);
+    HIPCHK(c, c->bq_frames.ensure(fbytes));
+    HIPCHK(c, c->bq_off.ensure(obytes));
+    HIPCHK(c, c->bq_items.ensure((size_t)nitems * sizeof(int2)));
+    HIPCHK(c, c->bq_head.ensure(64));
+    HIPCHK(c, hipMemcpyAsync(c->bq_frames.p, c->bq_host.p, fbytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->bq_off.p, (char*)c->bq_host.p + fbytes, obytes, hipMemcpyHostToDevice, c->stream));
+    launch_run_queue(prec_of(c), field_mode(rf[0].fc), T, c->bq_frames.as<RunFrame>(), c->bq_off.as<int>(), nf, nitems,
+                     c->bq_items.as<int2>(), c->bq_head.as<int>(), 2048, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return PISURF_OK;
+}
+
+// scatter, curvature, classification and the children count
+int batch_level_classify(BatchFrame& f, BatchReads& br) {
+    pisurf_ctx* c = f.c;
+    LevelBuf* L = c->levels[f.li];
+    const NodeLevel v = L->view();
+    const int prec = prec_of(c);
+    const size_t rsz = prec == PISURF_FP64 ? 8 : 4;
+    if (f.nent > 0)
+        launch_sample_scatter(L->child.as<int>(), f.skv, c->sd_rep.as<int>(), c->sd_hpos.as<int>(), f.nent,
+                              c->sd_vals.p, (int)rsz, c->samples.p, c->stream);
+    if (f.split_curv)
+        launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), f.tp, v, f.group, kCurvOnly, c->samples.p,
+                          (int)rsz, c->stream);
+    launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, field_const(c), f.tp, v, c->samples.p, nullptr, c->ctr(),
+                    c->stream);
+    launch_fill_i32(L->flag.as<int>() + L->n, 1, 0, c->stream);
+    exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), L->n + 1, c->tmp.as<int>(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    return br.add(c, L->scanned.as<int>() + L->n, &f.n_int, 4);
+}
+
+// the next level, or the end of the frame's octree
+int batch_level_next(BatchFrame& f) {
+    pisurf_ctx* c = f.c;
+    LevelBuf* L = c->levels[f.li];
+    if (f.n_int == 0) {
+        f.active = false;
+        c->tree_done = true;
+        c->st.n_nodes = 0;
+        for (auto* l : c->levels) c->st.n_nodes += l->n;
+        return PISURF_OK;
+    }
+    if ((int64_t)f.n_int * 8 > (int64_t)INT32_MAX / 2) return c->fail(PISURF_E_UNSUPPORTED, "octree level too large");
+    LevelBuf* N = level_of(c, f.li + 1);
+    N->n = (int64_t)f.n_int * 8;
+    HIPCHK(c, N->ensure(N->n));
+    const int64_t need_tmp = scan_tmp_ints(N->n + 1) + 64;
+    if ((size_t)need_tmp * 4 > c->tmp.cap) {
+        HIPCHK(c, c->wait());
+        HIPCHK(c, c->tmp.ensure(need_tmp * 4));
+    }
+    launch_children(L->view(), L->scanned.as<int>(), N->view(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    f.li++;
     return PISURF_OK;
 }
 
@@ -2037,6 +2423,170 @@ int pisurf_run_frames(pisurf_ctx* c, int32_t nframes, const float* const* xyz, c
         if (on_mesh)
             on_mesh(user, k, &ti, c->mesh_v.data(), (int64_t)c->mesh_v.size() / 3, c->mesh_t.data(),
                     (int64_t)c->mesh_t.size() / 3);
+    }
+    return PISURF_OK;
+    API_END(c)
+}
+
+int pisurf_run_batch(pisurf_ctx* c, int32_t nframes, const float* const* xyz, const int64_t* n, const float* radius,
+                     int32_t on_device, pisurf_mesh_cb on_mesh, void* user, pisurf_stats* per_frame) {
+    if (!c) return PISURF_E_INVALID;
+    if (nframes < 0 || (nframes > 0 && (!xyz || !n || !radius))) return c->fail(PISURF_E_INVALID, "bad frame arrays");
+    for (int32_t k = 0; k < nframes; k++) {
+        const std::string fk = " (frame " + std::to_string(k) + ")";
+        if (n[k] < 0 || (n[k] > 0 && !xyz[k])) return c->fail(PISURF_E_INVALID, "bad particle buffer" + fk);
+        if (n[k] >= (int64_t)INT32_MAX / 2) return c->fail(PISURF_E_UNSUPPORTED, "particle count exceeds 2^30" + fk);
+        if (!(radius[k] > 0.0f) || !std::isfinite(radius[k])) return c->fail(PISURF_E_INVALID, "radius must be > 0" + fk);
+    }
+    // Switched off: the first device run of the lockstep batch stopped making progress and the
+    // cause has not been found in the code; pisurf_run_frames meshes the same batch.
+    if (!kBatchEnabled)
+        return c->fail(PISURF_E_UNSUPPORTED, "pisurf_run_batch is disabled (use pisurf_run_frames)");
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, c->wait());
+    while ((int32_t)c->sub.size() < nframes) {
+        pisurf_ctx* f = make_frame_ctx(c);
+        if (!f) return c->fail(PISURF_E_HIP, "frame context creation failed");
+        c->sub.push_back(f);
+    }
+    auto tag = [&](pisurf_ctx* f, int rc) {
+        for (int32_t k = 0; k < nframes; k++)
+            if (c->sub[k] == f) {
+                c->err = f->err + " (frame " + std::to_string(k) + ")";
+                break;
+            }
+        return rc;
+    };
+    // every frame's particles on the device, its parameters and statistics reset
+    std::vector<BatchFrame> frames;
+    for (int32_t k = 0; k < nframes; k++) {
+        pisurf_ctx* f = c->sub[k];
+        pisurf_params p = c->prm;
+        p.radius = radius[k];
+        int rc = pisurf_set_params(f, &p);
+        if (rc) return tag(f, rc);
+        std::memset(&f->st, 0, sizeof(f->st));
+        f->host_sample_entries = 0;
+        f->n_fp64_eig = 0;
+        HIPCHK(c, hipMemsetAsync(f->counters.p, 0, kCtrStripes * sizeof(Counters), c->stream));
+        f->n = n[k];
+        f->bbox_done = f->grid_done = f->aniso_done = f->normals_done = f->tree_done = f->consts_done = false;
+        for (auto* l : f->levels) l->n = 0;
+        f->mesh_v.clear();
+        f->mesh_t.clear();
+        if (n[k] == 0) continue;
+        HIPCHK(c, f->xyz.ensure((size_t)n[k] * 12));
+        HIPCHK(c, hipMemcpyAsync(f->xyz.p, xyz[k], (size_t)n[k] * 12,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+        BatchFrame b;
+        b.c = f;
+        b.k = k;
+        b.active = true;
+        frames.push_back(b);
+    }
+    // larger frames first: their runs enter the queue first
+    std::stable_sort(frames.begin(), frames.end(), [&](const BatchFrame& a, const BatchFrame& b) { return n[a.k] > n[b.k]; });
+    BatchReads br(c);
+    int rc = PISURF_OK;
+    // bounding boxes, hash grids, anisotropy, constants, normals
+    for (auto& f : frames) {
+        if ((rc = bbox_launch(f.c))) return tag(f.c, rc);
+        if ((rc = br.add(f.c, bbox_dev(f.c), f.c->bbox, 6 * sizeof(float)))) return rc;
+        if ((rc = br.add(f.c, bbox_nan_dev(f.c), &f.nans, sizeof(int)))) return rc;
+    }
+    if ((rc = br.flush())) return rc;
+    for (auto& f : frames) {
+        if ((rc = bbox_finish(f.c, f.nans))) return tag(f.c, rc);
+        if ((rc = grid_launch(f.c, nullptr, 4.0f))) return tag(f.c, rc);
+        if ((rc = br.add(f.c, grid_ninv_dev(f.c), &f.ninv, sizeof(int)))) return rc;
+    }
+    if ((rc = br.flush())) return rc;
+    for (auto& f : frames) {
+        grid_finish(f.c, f.ninv);
+        if ((rc = aniso_launch(f.c))) return tag(f.c, rc);
+        if ((rc = br.add(f.c, aniso_nfp_dev(f.c), &f.nfp, 8))) return rc;
+        compute_constants(f.c);
+        if (f.c->prm.calc_normals) {
+            if ((rc = normals_launch(f.c))) return tag(f.c, rc);
+        } else {
+            HIPCHK(c, f.c->nrm.ensure(f.c->n * sizeof(float4)));
+            HIPCHK(c, hipMemsetAsync(f.c->nrm.p, 0, f.c->n * sizeof(float4), c->stream));
+        }
+        // the root of the octree (resizeRootBoxConstR)
+        resize_root(f.c);
+        LevelBuf* L = level_of(f.c, 0);
+        L->n = 1;
+        HIPCHK(c, L->ensure(1));
+        const float4 rc4 = make_float4(f.c->root_c[0], f.c->root_c[1], f.c->root_c[2], f.c->root_half);
+        const int8_t d0 = 0;
+        const uint64_t id0 = 0;
+        const int4 ip0 = make_int4(0, 0, 0, 0);
+        HIPCHK(c, hipMemcpyAsync(L->ip.p, &ip0, sizeof(int4), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(L->c.p, &rc4, sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(L->depth.p, &d0, 1, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(L->id.p, &id0, 8, hipMemcpyHostToDevice, c->stream));
+    }
+    if ((rc = br.flush())) return rc;
+    for (auto& f : frames) {
+        const bool nrm = f.c->normals_done;
+        aniso_finish(f.c, f.nfp);
+        f.c->normals_done = nrm;
+    }
+    // the octree levels of all frames in lockstep
+    std::vector<BatchFrame*> act;
+    for (int guard = 0;; guard++) {
+        act.clear();
+        for (auto& f : frames)
+            if (f.active) act.push_back(&f);
+        if (act.empty()) break;
+        if (guard > 64) return c->fail(PISURF_E_STATE, "octree deeper than 64 levels");
+        for (BatchFrame* f : act)
+            if ((rc = batch_level_begin(*f, br))) return tag(f->c, rc);
+        if ((rc = br.flush())) return rc;
+        for (BatchFrame* f : act)
+            if ((rc = batch_level_keys(*f, br))) return tag(f->c, rc);
+        if ((rc = br.flush())) return rc;
+        for (BatchFrame* f : act)
+            if ((rc = batch_level_groups(*f, br))) return tag(f->c, rc);
+        if ((rc = br.flush())) return rc;
+        for (BatchFrame* f : act)
+            if ((rc = batch_level_runs(*f, br))) return tag(f->c, rc);
+        if ((rc = br.flush())) return rc;
+        if ((rc = batch_level_eval(c, act))) return rc;
+        for (BatchFrame* f : act)
+            if ((rc = batch_level_classify(*f, br))) return tag(f->c, rc);
+        if ((rc = br.flush())) return rc;
+        for (BatchFrame* f : act)
+            if ((rc = batch_level_next(*f))) return tag(f->c, rc);
+    }
+    HIPCHK(c, c->wait());
+    // extraction and meshes, in the caller's order
+    for (int32_t k = 0; k < nframes; k++) {
+        pisurf_ctx* f = c->sub[k];
+        pisurf_tree_info ti;
+        std::memset(&ti, 0, sizeof(ti));
+        if (n[k] > 0) {
+            if ((rc = do_extract(f))) return tag(f, rc);
+            ti.depth_max = f->dmax;
+            ti.depth_min = f->dmin;
+            for (int a = 0; a < 3; a++) ti.root_center[a] = f->root_c[a];
+            ti.root_half = f->root_half;
+            int nl = 0;
+            for (auto* l : f->levels) {
+                if (l->n == 0) break;
+                ti.n_nodes += l->n;
+                nl++;
+            }
+            ti.n_levels = nl;
+        }
+        if (per_frame) {
+            if ((rc = sync_counters(f))) return tag(f, rc);
+            per_frame[k] = f->st;
+        }
+        if (on_mesh)
+            on_mesh(user, k, &ti, f->mesh_v.data(), (int64_t)f->mesh_v.size() / 3, f->mesh_t.data(),
+                    (int64_t)f->mesh_t.size() / 3);
     }
     return PISURF_OK;
     API_END(c)

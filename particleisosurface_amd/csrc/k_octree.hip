@@ -671,6 +671,143 @@ __global__ void k_sample_grads(TreeParams tp, NodeLevel lv, const R* __restrict_
     }
 }
 
+// ---- phase-split deduplicated sampling and the multi-frame run queue --------------------------
+// The same steps as launch_sample_dedupe_eval / launch_centers_runs, cut at every count the host
+// must read, so that a sweep over many frames can issue one phase for every frame and then read
+// every frame's count at once.
+
+int* heads_count(int* head, int64_t nent) {
+    const int64_t nb = (nent + 255) / 256;
+    return head + (nb + 1) + nb;
+}
+
+void launch_heads(NodeLevel lv, const int* list, SortedKV kv, int64_t nent, int* head, int* rep, int* hlist, int* hpos,
+                  int* tmp, hipStream_t s) {
+    if (nent <= 0) return;
+    const unsigned nb = (unsigned)((nent + 255) / 256);
+    int* block_off = head + (nb + 1);
+    k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, nent, head, rep);
+    launch_fill_i32(head + nb, 1, 0, s);
+    exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
+    k_head_list<<<nb, 256, 0, s>>>(rep, nent, block_off, hlist, hpos);
+}
+
+int* groups_count(int* cwork, int nh) { return cwork + (nh + 2) + nh; }
+
+void launch_cell_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp, hipStream_t s) {
+    if (nh <= 0) return;
+    int* flag = cwork;
+    int* pos = cwork + (nh + 2);
+    int* cstart = cwork + 2 * (int64_t)(nh + 2);
+    const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
+    k_cell_flags<<<nbh, 256, 0, s>>>(kv, gshift, hlist, nh, flag);
+    exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
+    k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
+}
+
+int* runs_count(int* cwork, int nh, int ng) { return cwork + (nh + 2) + ng; }
+int* runs_start(int* cwork, int nh, int ng) { return cwork + 2 * (int64_t)(nh + 2) + (ng + 1); }
+
+void launch_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist, int nh, int ng, int kx, int* cwork,
+                       int* tmp, hipStream_t s) {
+    if (nh <= 0 || ng <= 0) return;
+    int* flag = cwork;
+    int* pos = cwork + (nh + 2);
+    int* cstart = cwork + 2 * (int64_t)(nh + 2);
+    const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
+    k_run_flags<<<nbg, 256, 0, s>>>(g, kv, gshift, hlist, cstart, ng, kx, flag);
+    exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
+    k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, cstart + (ng + 1));
+}
+
+void run_shape_for(double points_per_cell, int* kx, int* t) { run_shape(points_per_cell, kx, t); }
+
+void launch_sample_scatter(const int* list, SortedKV kv, const int* rep, const int* hpos, int64_t nent,
+                           const void* vals, int rsz, void* samples, hipStream_t s) {
+    if (nent <= 0) return;
+    k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, kv, rep, hpos, nent, vals, rsz, samples);
+}
+
+SortedKV launch_center_keys_sorted(GridDev g, NodeLevel lv, const int* list, int64_t k, uint64_t* key, int* val,
+                                   uint64_t* key2, int* val2, int* tmp, hipStream_t s) {
+    const uint64_t groups = (uint64_t)(g.n[0] + 2) * (uint64_t)(g.n[1] + 2) * (uint64_t)(g.n[2] + 2);
+    int bits = 1;
+    while (bits < 55 && ((uint64_t)1 << bits) <= groups) bits++;
+    const int vbits = pack_vbits(bits, k);
+    if (k <= 0) return SortedKV{key, val, vbits};
+    k_center_keys<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(g, lv, list, k, vbits, key, val);
+    return radix_sort_entries(key, val, key2, val2, k, bits, vbits, tmp, s);
+}
+
+// items[i] = (frame, run): frame f owns items [off[f], off[f + 1])
+__global__ void k_queue_items(const int* __restrict__ off, int nframes, int nitems, int2* __restrict__ items) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {  // the last frame whose offset is <= i
+        const int mid = (lo + hi + 1) / 2;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    items[i] = make_int2(lo, i - off[lo]);
+}
+
+// One launch for the runs of every frame of a level: each workgroup takes the next (frame, run)
+// item from a device counter until none is left (every workgroup ends when the counter passes
+// the item count), and evaluates that run with the frame's own grid, records and constants.
+template <class R, bool K, int T, int MODE>
+__global__ void __launch_bounds__(T) k_run_queue(const RunFrame* __restrict__ frames, const int2* __restrict__ items,
+                                                 int nitems, int* __restrict__ qhead) {
+    __shared__ RunLds<R, K && sizeof(R) == 4> lds;
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(qhead, 1);
+        __syncthreads();
+        const int it = next;
+        __syncthreads();
+        if (it >= nitems) return;
+        const int2 w = items[it];
+        const RunFrame& F = frames[w.x];
+        const Recs<R> rec{(const typename Vec4<R>::T*)F.ra, (const typename Vec4<R>::T*)F.rg};
+        const int h0 = F.rstart[w.y], h1 = F.rstart[w.y + 1];
+        Counters* c = ctr_stripe(F.ctr);
+        if (F.kind == 0) {
+            SampleSrc<R> src{F.lv, F.list, F.kv, F.hlist, (R*)F.out, R(F.fc.iso), out_sigma<R, K>(F.fc)};
+            const unsigned long long vis = run_eval<T, R, K, false, MODE>(F.g, rec, F.fc, src, h0, h1, lds);
+            wave_count(&c->sample_candidates, vis);
+            wave_count(&c->candidates, vis);
+            wave_count(&c->run_candidates, vis);
+            if (threadIdx.x == 0) {
+                atomicAdd(&c->sample_unique, (unsigned long long)(h1 - h0));
+                atomicAdd(&c->run_evals, (unsigned long long)(h1 - h0));
+            }
+        } else {
+            CenterSrc<R> src{F.lv, F.list, F.kv, (R*)F.out, R(F.fc.iso), out_sigma<R, K>(F.fc)};
+            const unsigned long long vis = run_eval<T, R, K, false, MODE>(F.g, rec, F.fc, src, h0, h1, lds);
+            wave_count(&c->candidates, vis);
+            if (threadIdx.x == 0) atomicAdd(&c->field_evals, (unsigned long long)(h1 - h0));
+        }
+    }
+}
+
+void launch_run_queue(int prec, int mode, int T, const RunFrame* frames, const int* item_off, int nframes, int nitems,
+                      int2* items, int* qhead, int max_blocks, hipStream_t s) {
+    if (nitems <= 0) return;
+    k_queue_items<<<(unsigned)((nitems + 255) / 256), 256, 0, s>>>(item_off, nframes, nitems, items);
+    launch_fill_i32(qhead, 1, 0, s);
+    const unsigned nb = (unsigned)std::min(nitems, std::max(max_blocks, 1));
+    auto go = [&](auto tag, auto md) {
+        constexpr int TT = decltype(tag)::value, M = decltype(md)::value;
+        if (prec == 2) k_run_queue<double, false, TT, M><<<nb, TT, 0, s>>>(frames, items, nitems, qhead);
+        else if (prec == 1) k_run_queue<float, true, TT, M><<<nb, TT, 0, s>>>(frames, items, nitems, qhead);
+        else k_run_queue<float, false, TT, M><<<nb, TT, 0, s>>>(frames, items, nitems, qhead);
+    };
+    using M0 = std::integral_constant<int, kModeAny>;
+    using M1 = std::integral_constant<int, kModeAniBspline>;
+    if (T == 256) mode == kModeAniBspline ? go(std::integral_constant<int, 256>(), M1()) : go(std::integral_constant<int, 256>(), M0());
+    else mode == kModeAniBspline ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
+}
+
 void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, int group,
                        int pass, const void* samples, int rsz, hipStream_t s) {
     if (lv.n <= 0) return;

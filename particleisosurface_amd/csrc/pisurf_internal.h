@@ -168,6 +168,49 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const voi
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
                                hipEvent_t e1, hipStream_t s);
+// Phase-split deduplicated sampling (the steps of launch_sample_dedupe_eval and
+// launch_centers_runs cut at each count the host reads), for sweeps over many frames:
+// heads of nent sorted entries, their count at heads_count(head, nent)
+int* heads_count(int* head, int64_t nent);
+void launch_heads(NodeLevel lv, const int* list, SortedKV kv, int64_t nent, int* head, int* rep, int* hlist, int* hpos,
+                  int* tmp, hipStream_t s);
+// cell groups of nh points (hlist NULL: the sorted entries are the points), count at
+// groups_count(cwork, nh)
+int* groups_count(int* cwork, int nh);
+void launch_cell_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp, hipStream_t s);
+// runs of kx cells over ng groups: count at runs_count, run starts at runs_start
+int* runs_count(int* cwork, int nh, int ng);
+int* runs_start(int* cwork, int nh, int ng);
+void launch_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist, int nh, int ng, int kx, int* cwork,
+                       int* tmp, hipStream_t s);
+// block size and cells per run for an average of p points per cell
+void run_shape_for(double points_per_cell, int* kx, int* t);
+void launch_sample_scatter(const int* list, SortedKV kv, const int* rep, const int* hpos, int64_t nent,
+                           const void* vals, int rsz, void* samples, hipStream_t s);
+// leaf-depth centres of the listed nodes keyed by cell group and sorted
+SortedKV launch_center_keys_sorted(GridDev g, NodeLevel lv, const int* list, int64_t k, uint64_t* key, int* val,
+                                   uint64_t* key2, int* val2, int* tmp, hipStream_t s);
+// one frame's runs in the multi-frame queue: kind 0 = distinct sample points (values to out[h]),
+// kind 1 = leaf-depth centres (values to slot 13 of the node's samples in out)
+struct RunFrame {
+    GridDev g;
+    const void* ra;
+    const void* rg;
+    FieldConst fc;
+    NodeLevel lv;
+    const int* list;
+    SortedKV kv;
+    const int* hlist;
+    const int* rstart;
+    void* out;
+    Counters* ctr;
+    int kind;
+};
+// runs of every frame of a level in one launch fed by a device counter: item_off[f] = first item
+// of frame f (nframes + 1 entries, device), items (nitems, device) and qhead (one int, device) are
+// work space; at most max_blocks workgroups of T (128 or 256) threads
+void launch_run_queue(int prec, int mode, int T, const RunFrame* frames, const int* item_off, int nframes, int nitems,
+                      int2* items, int* qhead, int max_blocks, hipStream_t s);
 void launch_samples(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
                     NodeLevel lv, const int* list, const int* count, void* samples /*n*27 R*/, Counters* ctr,
                     hipStream_t s);

@@ -243,3 +243,64 @@ def test_neighbors_total_guard():
         ctx.neighbors(q)
     off, ids = ctx.neighbors(q[:1000])
     assert off[-1] == 400 * 1000 and np.array_equal(np.sort(ids[:400]), np.arange(400))
+
+
+def _batch_matches_fresh(frames, kw, entry):
+    got = {}
+
+    def on_mesh(k, v, t, tree):
+        got[k] = (v, t, tree)
+
+    ctx = capi.Context(0)
+    ctx.set_params(0.01, **kw)
+    per = getattr(ctx, entry)([f for f, _ in frames], [r for _, r in frames], on_mesh)
+    assert sorted(got) == list(range(len(frames)))
+    for k, (xyz, r) in enumerate(frames):
+        v, t, tree = got[k]
+        if len(xyz) == 0:
+            assert len(v) == 0 and len(t) == 0 and per[k]["field_evals"] == 0
+            continue
+        ref = gpu_context(xyz, r, **kw)
+        info = ref.run_frame()
+        v2, t2 = ref.mesh()
+        assert tree == info, k
+        assert np.array_equal(v, v2) and np.array_equal(t, t2), k
+        st = ref.stats()
+        for key in ("field_evals", "grad_evals", "qef_solves", "sample_evals", "sample_unique", "candidates"):
+            assert per[k][key] == st[key], (k, key)
+        ref.close()
+    return ctx
+
+
+BATCH_OFF = ("pisurf_run_batch is switched off: its first device run stopped making progress and "
+             "the cause has not been found")
+
+
+def test_run_batch_disabled():
+    """The switched-off entry point fails cleanly with PISURF_E_UNSUPPORTED."""
+    ctx = capi.Context(0)
+    ctx.set_params(0.01)
+    xyz, r = synth.dam_break()
+    with pytest.raises(capi.PisurfError, match=r"\(-4\).*disabled"):
+        ctx.run_batch([xyz], [r])
+
+
+@pytest.mark.skip(reason=BATCH_OFF)
+@pytest.mark.parametrize("prec", ["fp32", "fp32k", "fp64"])
+def test_run_batch_small_frames(prec):
+    """Frames of different sizes and radii meshed together (lockstep levels, one run-queue launch
+    per level): every frame bit-identical to a fresh context, counters equal; an empty frame; and a
+    second call on the same context (frame contexts reused)."""
+    frames = [synth.dam_break(), synth.lattice_sphere(), (np.zeros((0, 3), np.float32), 0.004),
+              synth.random_cloud(20000, r=0.004, extent=0.2), synth.dam_break()]
+    ctx = _batch_matches_fresh(frames, dict(precision=PREC[prec]), "run_batch")
+    per = ctx.run_batch([f for f, _ in frames[:2]], [r for _, r in frames[:2]])
+    assert per[0]["field_evals"] > 0
+
+
+@pytest.mark.skip(reason=BATCH_OFF)
+def test_run_batch_24_frames():
+    """The 24-frame batch (compensated FP32, FP64 QEF) through pisurf_run_batch: every frame's mesh,
+    tree and counters equal to a fresh context meshing it alone."""
+    _batch_matches_fresh(synth.batch_frames(), dict(precision=capi.FP32_KAHAN, qef_mode=capi.QEF_SVD64),
+                         "run_batch")
