@@ -295,30 +295,32 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
         field_contrib<R, KAHAN, GRAD, MODE>(fc, inv_h2, h2, dx, dy, dz, d2, rec.g[2 * j], rec.g[2 * j + 1], lo, acc,
                                             ga);
     };
+    // Candidates are culled in groups of up to 32 of one row (four records requested before any
+    // is used), then only the ones that pass are summed, in order: the lanes of a wave hold
+    // different points, and a per-candidate branch would run the contribution whenever any lane
+    // needs it. The cull is written !(d2 >= R^2) so that a NaN record passes and poisons the sum,
+    // as the reference's kernels do for it.
     for (int r = 0; r < 9; r++) {
-        int j = rlo[r];
         const int e = rhi[r];
-        // four candidates per step: their records are requested before any is used. The culls
-        // are written !(d2 >= R^2) so that a NaN record is visited and poisons the sum, as the
-        // reference's kernels do for it.
-        for (; j + 4 <= e; j += 4) {
-            const auto a0 = rec.a[j], a1 = rec.a[j + 1], a2 = rec.a[j + 2], a3 = rec.a[j + 3];
-            const R x0 = px - a0.x, y0 = py - a0.y, z0 = pz - a0.z;
-            const R x1 = px - a1.x, y1 = py - a1.y, z1 = pz - a1.z;
-            const R x2 = px - a2.x, y2 = py - a2.y, z2 = pz - a2.z;
-            const R x3 = px - a3.x, y3 = py - a3.y, z3 = pz - a3.z;
-            const R q0 = dist2(x0, y0, z0), q1 = dist2(x1, y1, z1);
-            const R q2 = dist2(x2, y2, z2), q3 = dist2(x3, y3, z3);
-            if (!(q0 >= a0.w)) contrib(j, x0, y0, z0, q0);
-            if (!(q1 >= a1.w)) contrib(j + 1, x1, y1, z1, q1);
-            if (!(q2 >= a2.w)) contrib(j + 2, x2, y2, z2, q2);
-            if (!(q3 >= a3.w)) contrib(j + 3, x3, y3, z3, q3);
-        }
-        for (; j < e; j++) {
-            const auto a = rec.a[j];
-            const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-            const R d2 = dist2(dx, dy, dz);
-            if (!(d2 >= a.w)) contrib(j, dx, dy, dz, d2);
+        for (int j = rlo[r]; j < e; j += 32) {
+            const int jn = min(e, j + 32);
+            unsigned m = 0;
+            for (int u = j; u < jn; u += 4) {
+                typename Vec4<R>::T a[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) a[k] = rec.a[min(u + k, jn - 1)];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    m |= unsigned(!(dist2(px - a[k].x, py - a[k].y, pz - a[k].z) >= a[k].w)) << (u - j + k);
+            }
+            if (jn - j < 32) m &= (1u << (jn - j)) - 1u;
+            while (m) {
+                const int jj = j + __builtin_ctz(m);
+                m &= m - 1;
+                const auto a = rec.a[jj];
+                const R dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+                contrib(jj, dx, dy, dz, dist2(dx, dy, dz));
+            }
         }
     }
     sum = acc.get();

@@ -15,18 +15,30 @@ namespace pisurf {
 
 constexpr int kRunThreads = 128;  // block size of the particle runs (normals)
 constexpr int kRunChunk = 256;
+constexpr int kCullStep = 4;     // candidates tested per cull call (the staging arrays are padded by it)
+constexpr int kCullGroup = 32;   // candidates per pass mask
 
 // Policy interface of run_visit:
-//   typename Pol::Lds     staging buffers for kRunChunk candidates
+//   typename Pol::Lds     staging buffers for kRunChunk (+ kCullStep padding) candidates
 //   typename Pol::State   per-point accumulators
 //   void point(int h, float p[3]) const                 query point h
 //   void stage(Lds&, int t, int j) const                candidate j -> slot t
 //   void begin(State&, int h, const float p[3]) const   (inactive threads too, with h = -1)
-//   void visit(State&, const Lds&, int t, int j) const  candidate in slot t (global index j)
-//   void visitn<N>(State&, const Lds&, int t) const    candidates t .. t + N - 1, in that order
+//   unsigned cull<N>(const State&, const Lds&, int t) const
+//                                                       bit k set: slot t + k can contribute (slots
+//                                                       past the row are dropped by the caller)
+//   void take(State&, const Lds&, int t) const          contribution of slot t (one that passed)
 //   void end(State&, int h, int ncand) const            active points: ncand = candidates in its
 //                                                       27-cell window
 // T is the block size. Returns the candidates visited by this thread's points.
+//
+// Candidates are culled in groups of up to kCullGroup slots of one row, and only the ones that
+// pass are then summed, in slot order. The lanes of a wave hold different points, so a per-
+// candidate branch would run the contribution whenever any lane needs it, which is nearly always
+// even though most candidates lie outside the kernel support; the group's pass mask lets each lane
+// step through its own passing candidates, and the wave runs as many contribution steps as its
+// busiest lane needs. Every point still adds exactly its passing candidates in the reference's
+// order, so the sums are the same as with one test per candidate.
 template <int T, class Pol>
 __device__ __forceinline__ unsigned long long run_visit(const GridDev& g, const Pol& pol, int h0, int h1,
                                                         typename Pol::Lds& lds) {
@@ -110,10 +122,18 @@ __device__ __forceinline__ unsigned long long run_visit(const GridDev& g, const 
 #pragma unroll
             for (int r = 0; r < 9; r++) {
                 const int t0 = max(clo[r], cb) - cb, t1 = min(chi[r], cb + n) - cb;
-                const int jr = w0[r] - roff[r] + cb;  // global index of slot t in row r: jr + t
-                int t = t0;
-                for (; t + 4 <= t1; t += 4) pol.template visitn<4>(st, lds, t);
-                for (; t < t1; t++) pol.visit(st, lds, t, jr + t);
+                for (int t = t0; t < t1; t += kCullGroup) {
+                    const int tn = min(t1, t + kCullGroup);
+                    unsigned m = 0;
+                    for (int u = t; u < tn; u += kCullStep) m |= pol.template cull<kCullStep>(st, lds, u) << (u - t);
+                    // slots from tn on (other rows, or not staged) are tested too and dropped here
+                    if (tn - t < 32) m &= (1u << (tn - t)) - 1u;
+                    while (m) {
+                        const int k = __builtin_ctz(m);
+                        m &= m - 1;
+                        pol.take(st, lds, t + k);
+                    }
+                }
             }
         }
         if (act) pol.end(st, h, ncand);
@@ -123,7 +143,8 @@ __device__ __forceinline__ unsigned long long run_visit(const GridDev& g, const 
 
 // ---- field sums (SingleEval / SingleEvalWithGrad): records of device_common.h ----
 template <class R, bool LO> struct RunLds {
-    typename Vec4<R>::T a[kRunChunk], g0[kRunChunk], g1[kRunChunk];
+    typename Vec4<R>::T a[kRunChunk + kCullStep];  // the padding is read by a cull step at a row's end
+    typename Vec4<R>::T g0[kRunChunk], g1[kRunChunk];
     float4 lo[LO ? kRunChunk : 1];
 };
 
@@ -152,36 +173,27 @@ template <class R, bool K, bool GRAD, int MODE, class Src> struct FieldPol {
         s.py = R(p[1]);
         s.pz = R(p[2]);
     }
-    __device__ __forceinline__ void visit(State& s, const Lds& l, int t, int) const {
+    // The cull is written !(d2 >= R^2) so that a NaN record passes and poisons the sum, as the
+    // reference's kernels do for it. The N positions are read before any is used, so their LDS
+    // reads overlap.
+    template <int N> __device__ __forceinline__ unsigned cull(const State& s, const Lds& l, int t) const {
+        typename Vec4<R>::T a[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) a[k] = l.a[t + k];
+        unsigned m = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const R dx = s.px - a[k].x, dy = s.py - a[k].y, dz = s.pz - a[k].z;
+            m |= unsigned(!(dist2(dx, dy, dz) >= a[k].w)) << k;
+        }
+        return m;
+    }
+    __device__ __forceinline__ void take(State& s, const Lds& l, int t) const {
         const typename Vec4<R>::T a = l.a[t];
         const R dx = s.px - a.x, dy = s.py - a.y, dz = s.pz - a.z;
         const R d2 = dist2(dx, dy, dz);
-        if (!(d2 >= a.w)) {
-            const float4 lo = LO ? l.lo[LO ? t : 0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            field_contrib<R, K, GRAD, MODE>(fc, inv_h2, h2, dx, dy, dz, d2, l.g0[t], l.g1[t], lo, s.acc, s.ga);
-        }
-    }
-    // the N positions are read before any is used, so their LDS reads overlap
-    template <int N> __device__ __forceinline__ void visitn(State& s, const Lds& l, int t) const {
-        typename Vec4<R>::T a[N];
-        R dx[N], dy[N], dz[N], q[N];
-#pragma unroll
-        for (int k = 0; k < N; k++) a[k] = l.a[t + k];
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            dx[k] = s.px - a[k].x;
-            dy[k] = s.py - a[k].y;
-            dz[k] = s.pz - a[k].z;
-            q[k] = dist2(dx[k], dy[k], dz[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            if (!(q[k] >= a[k].w)) {
-                const float4 lo = LO ? l.lo[LO ? t + k : 0] : make_float4(0.f, 0.f, 0.f, 0.f);
-                field_contrib<R, K, GRAD, MODE>(fc, inv_h2, h2, dx[k], dy[k], dz[k], q[k], l.g0[t + k], l.g1[t + k],
-                                                lo, s.acc, s.ga);
-            }
-        }
+        const float4 lo = LO ? l.lo[LO ? t : 0] : make_float4(0.f, 0.f, 0.f, 0.f);
+        field_contrib<R, K, GRAD, MODE>(fc, inv_h2, h2, dx, dy, dz, d2, l.g0[t], l.g1[t], lo, s.acc, s.ga);
     }
     __device__ __forceinline__ void end(State& s, int h, int) const {
         R gr[3] = {s.ga[0].get(), s.ga[1].get(), s.ga[2].get()};
