@@ -252,6 +252,7 @@ struct pisurf_ctx {
     float root_c[3] = {0, 0, 0}, root_half = 0;
     std::vector<LevelBuf*> levels;
     DBuf samples, cl_flag, cl_pos, cl_list;
+    DBuf lvcnt;  // node counts of the levels built from device-resident counts
     // sample deduplication: keys / entries (ping-pong), heads, scanned heads, representatives,
     // head list, per-point values
     DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals, sd_cwork;
@@ -822,6 +823,80 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     return PISURF_OK;
 }
 
+// The levels above D_min take no samples: a node there is INTERNAL exactly when it is not empty
+// (afterSampleEval, src/surface_reconstructor.cpp:237-273, with depth < D_min). They are built
+// with their node counts kept in device memory and their arrays and grids sized by an upper bound,
+// so the host issues every launch without waiting; the counts are read once, together with the
+// count of the first level built the ordinary way. Level 0 (the root) is in place. Returns, in
+// *next, the index of that first ordinary level.
+constexpr int64_t kTopMaxNodes = int64_t(1) << 22;
+
+int top_levels(pisurf_ctx* c, size_t* next) {
+    *next = 0;
+    const int top = c->dmin;
+    if (top <= 0) return PISURF_OK;
+    // bounds: a level-l node that is not empty meets the particles' box grown by the margin, so
+    // level l holds at most geo(l) non-empty nodes and level l + 1 at most 8 geo(l) nodes
+    std::vector<int64_t> ub(1, 1);
+    const double m = double(c->prm.radius) * double(c->prm.smooth_factor);
+    for (int l = 1; l <= top; l++) {
+        const double size = std::ldexp(2.0 * (double)c->root_half, -(l - 1));
+        double geo = 1;
+        for (int a = 0; a < 3; a++)
+            geo *= std::floor(((double)c->ginfo.bounding[2 * a + 1] - (double)c->ginfo.bounding[2 * a] + 2 * m) / size) + 3;
+        const double b = 8 * std::min((double)ub.back(), geo);
+        if (b > (double)kTopMaxNodes) break;
+        ub.push_back((int64_t)b);
+    }
+    const int t = (int)ub.size() - 1;  // levels 0 .. t - 1 are built here, level t's nodes written
+    if (t <= 0) return PISURF_OK;
+    auto level = [&](size_t i) {
+        while (c->levels.size() <= i) c->levels.push_back(new LevelBuf());
+        return c->levels[i];
+    };
+    // every buffer first: the launches below run while the host goes on
+    int64_t tmp_need = 0;
+    for (int l = 0; l <= t; l++) {
+        LevelBuf* L = level(l);
+        HIPCHK(c, L->ensure(ub[l]));
+        L->n = ub[l];
+        tmp_need = std::max(tmp_need, scan_tmp_ints(ub[l] + 1) + 64);
+    }
+    if ((size_t)tmp_need * 4 > c->tmp.cap) {
+        HIPCHK(c, c->wait());
+        HIPCHK(c, c->tmp.ensure(tmp_need * 4));
+    }
+    HIPCHK(c, c->lvcnt.ensure(64 * sizeof(int)));
+    int* cnt = c->lvcnt.as<int>();
+    launch_fill_i32(cnt, 1, 1, c->stream);
+    const int prec = prec_of(c);
+    const FieldConst fc = field_const(c);
+    const TreeParams tp = tree_params(c);
+    for (int l = 0; l < t; l++) {
+        LevelBuf* L = c->levels[l];
+        NodeLevel v = L->view();
+        v.ndev = cnt + l;
+        const double half = std::ldexp((double)c->root_half, -l);
+        const double w = (2 * half + 2 * (double)tp.margin) / (double)c->g.cell + 1;
+        const int group = (w * w <= 6) ? 4 : (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
+        launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group, kEmptyAndCurv, nullptr, 0,
+                          c->stream);
+        launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, nullptr, c->ctr(), c->stream);
+        launch_fill_i32(L->flag.as<int>() + ub[l], 1, 0, c->stream);
+        exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), ub[l] + 1, c->tmp.as<int>(), c->stream);
+        launch_children_counted(v, L->scanned.as<int>(), c->levels[l + 1]->view(), cnt + l + 1, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    std::vector<int> n(t + 1);
+    HIPCHK(c, c->read(n.data(), cnt, (t + 1) * sizeof(int)));
+    for (int l = 0; l <= t; l++) {
+        if (n[l] < 0 || n[l] > ub[l]) return c->fail(PISURF_E_STATE, "octree level count out of its bound");
+        c->levels[l]->n = n[l];
+    }
+    *next = (size_t)t;
+    return PISURF_OK;
+}
+
 int do_tree(pisurf_ctx* c) {
     int rc = frame_ready(c);
     if (rc) return rc;
@@ -846,8 +921,10 @@ int do_tree(pisurf_ctx* c) {
         HIPCHK(c, hipMemcpyAsync(L->depth.p, &d0, 1, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(L->id.p, &id0, 8, hipMemcpyHostToDevice, c->stream));
     }
+    if ((rc = top_levels(c, &li))) return rc;
     while (true) {
         L = level(li);
+        if (L->n == 0) break;  // every node above D_min empty
         rc = sweep(c, L, nullptr, (int)li);
         if (rc) return rc;
         HIPCHK(c, hipMemsetAsync(L->flag.as<int>() + L->n, 0, 4, c->stream));

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) k_empty_curv(GridDev g, const float4* __r
     const int64_t node = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const int lane = threadIdx.x % G;
     const int gbase = (threadIdx.x & 63) - lane;  // first lane of this group within the wave
-    if (node >= lv.n) return;
+    if (node >= lv_count(lv)) return;
     if (PASS == kCurvOnly) {
         // classification reads curv only for sampled nodes whose samples change sign (the exits
         // below are uniform over the node's lanes)
@@ -555,7 +555,8 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int vis = 0;
     unsigned long long evals = 0, qefs = 0;
-    if (node < lv.n) {
+    const int64_t count = lv_count(lv);
+    if (node < count) {
         const float4 c = lv.c[node];
         const int depth = lv.depth[node];
         uint8_t type;
@@ -618,6 +619,8 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
         lv.node[node] = make_double4(double(vx), double(vy), double(vz), double(fval));
         lv.child[node] = -1;
         if (sc_out) sc_out[node] = sc ? 1 : 0;
+    } else if (node < lv.n) {
+        lv.flag[node] = 0;  // past a device count: nothing to scan
     }
     wave_count(&ctr_stripe(ctr)->field_evals, evals);
     wave_count(&ctr_stripe(ctr)->qef_solves, qefs);
@@ -626,10 +629,12 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
 
 // Children of INTERNAL nodes (TNode ctor, src/iso_method_ours.cpp:25-38): child t has
 // centre parent +/- half_child per axis (bit0 = x, bit1 = y, bit2 = z) and id = parent * 8 + t.
-__global__ void k_children(NodeLevel par, const int* __restrict__ scanned, NodeLevel nxt) {
+__global__ void k_children(NodeLevel par, const int* __restrict__ scanned, NodeLevel nxt, int* __restrict__ next_count) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= par.n || !par.flag[i]) return;
+    if (next_count && i == par.n) *next_count = 8 * scanned[par.n];
+    if (i >= lv_count(par) || !par.flag[i]) return;
     const int64_t base = (int64_t)scanned[i] * 8;
+    if (base + 8 > nxt.n) return;  // past the next level's arrays (its count is then out of bound)
     par.child[i] = (int)base;
     const float4 c = par.c[i];
     const float hc = c.w / 2.0f;
@@ -986,7 +991,12 @@ void launch_classify(int prec, GridDev g, const void* ra, const void* rg, FieldC
 
 void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s) {
     if (parent.n <= 0) return;
-    k_children<<<(unsigned)((parent.n + 255) / 256), 256, 0, s>>>(parent, scanned, next);
+    k_children<<<(unsigned)((parent.n + 255) / 256), 256, 0, s>>>(parent, scanned, next, nullptr);
+}
+
+void launch_children_counted(NodeLevel parent, const int* scanned, NodeLevel next, int* next_count, hipStream_t s) {
+    if (parent.n <= 0) return;
+    k_children<<<(unsigned)((parent.n + 1 + 255) / 256), 256, 0, s>>>(parent, scanned, next, next_count);
 }
 
 void launch_sample_grads(int prec, TreeParams tp, NodeLevel lv, const void* samples, double* os, double* og,

@@ -125,8 +125,11 @@ struct NodeLevel {
     int* flag;         // INTERNAL -> 1 (scanned into child offsets)
     int* child;        // level-local index of the first child in the next level, -1
     int4* ip;          // integer cell coordinates at the node's depth (x, y, z, depth)
-    int64_t n;
+    int64_t n;         // node count, or with ndev an upper bound that sizes the arrays and grids
+    const int* ndev = nullptr;  // node count in device memory (levels built without a host read)
 };
+// node count of a level as the kernels see it
+__device__ __forceinline__ int64_t lv_count(const NodeLevel& lv) { return lv.ndev ? (int64_t)*lv.ndev : lv.n; }
 struct TreeParams {
     float radius;
     float margin;      // r * smooth_factor (box margin of checkEmptyAndCalcCurv)
@@ -217,6 +220,9 @@ void launch_samples(int prec, GridDev g, const void* rec_a, const void* rec_g, F
 void launch_classify(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
                      NodeLevel lv, const void* samples, int32_t* signchange, Counters* ctr, hipStream_t s);
 void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s);
+// the same for a parent level with a device count (parent.ndev, parent.n its upper bound): the
+// children's count 8 * scanned[parent.n] is written to *next_count
+void launch_children_counted(NodeLevel parent, const int* scanned, NodeLevel next, int* next_count, hipStream_t s);
 void launch_sample_grads(int prec, TreeParams tp, NodeLevel lv, const void* samples, double* out_samples,
                          double* out_grads, hipStream_t s);
 
