@@ -219,6 +219,13 @@ int pisurf_get_stats(pisurf_ctx* ctx, pisurf_stats* out);
 int pisurf_reset_stats(pisurf_ctx* ctx);
 /* device stream the library launches on (hipStream_t) */
 void* pisurf_stream(pisurf_ctx* ctx);
+/* Shape autotuning of the run launches (node samples, leaf-depth centres, particle normals): the
+ * first launch of a (precision, launch kind, points-per-cell bucket) times the candidate
+ * (threads per workgroup, cells per run) shapes on its own data and keeps the fastest for the
+ * process; every shape gives the same results. enabled = 0 takes the heuristic shape instead
+ * (also PISURF_RUN_TUNING=0 in the environment); clear = 1 forgets the kept shapes. Returns the
+ * number of kept shapes. No counterpart in the reference (it has no device launches). */
+int pisurf_run_tuning(int32_t enabled, int32_t clear);
 
 #ifdef __cplusplus
 }

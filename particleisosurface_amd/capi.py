@@ -24,6 +24,7 @@ EXPORTS = [
     "pisurf_tree_download", "pisurf_qef_solve", "pisurf_bisect_edges", "pisurf_dual_cells", "pisurf_extract",
     "pisurf_mesh_download",
     "pisurf_run_frame", "pisurf_run_frames", "pisurf_run_batch", "pisurf_get_stats", "pisurf_reset_stats", "pisurf_stream",
+    "pisurf_run_tuning",
 ]
 
 
@@ -80,6 +81,7 @@ def load():
         L.pisurf_run_frames.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_float), C.c_int32, MESH_CB, C.c_void_p, C.c_void_p]
         L.pisurf_run_batch.argtypes = L.pisurf_run_frames.argtypes
+        L.pisurf_run_tuning.argtypes = [C.c_int32, C.c_int32]
         _lib = L
     return _lib
 

@@ -18,6 +18,7 @@
 #include "../../include/pisurf.h"
 #include "dmc_tables.h"
 #include "pisurf_internal.h"
+#include "run_tuner.h"
 
 using namespace pisurf;
 
@@ -349,6 +350,12 @@ struct pisurf_ctx {
         return PISURF_E_HIP;
     }
     Counters* ctr() { return counters.as<Counters>(); }
+    // counters of the timed trial launches of run-shape tuning (never read)
+    DBuf trial_counters;
+    Counters* trial_ctr() {
+        if (!trial_counters.p && trial_counters.ensure(kCtrStripes * sizeof(Counters)) != hipSuccess) return nullptr;
+        return trial_counters.as<Counters>();
+    }
 };
 
 #define HIPCHK(ctx, call)                                   \
@@ -624,7 +631,7 @@ int need_aniso(pisurf_ctx* c) {
 int normals_launch(pisurf_ctx* c) {
     HIPCHK(c, c->nrm.ensure(c->n * sizeof(float4)));
     launch_normals(prec_of(c), c->g, c->rec_a.p, c->rec_g.p, field_const(c), c->spos.as<float4>(), c->n,
-                   c->ginfo.n_invalid, c->nrm.as<float4>(), c->ctr(), c->stream);
+                   c->ginfo.n_invalid, c->nrm.as<float4>(), c->ctr(), c->trial_ctr(), c->stream);
     HIPCHK(c, hipGetLastError());
     c->normals_done = true;
     return PISURF_OK;
@@ -782,7 +789,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
                                       c->sd_head.as<int>(),
                                       c->sd_hpos.as<int>(), c->sd_rep.as<int>(), c->sd_hlist.as<int>(),
                                       c->sd_cwork.as<int>(), &nheads, &nruns, &kx, c->sd_vals.p, c->samples.p,
-                                      c->tmp.as<int>(), c->ctr(), e0, e1, c->stream);
+                                      c->tmp.as<int>(), c->ctr(), c->trial_ctr(), e0, e1, c->stream);
             c->host_sample_entries += (unsigned long long)nent;
             c->ev_kind[eslot] = nruns > 0 ? 1 : 0;
         }
@@ -812,7 +819,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             launch_centers_runs(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), k,
                                 c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
                                 c->sd_val2.as<int>(), c->sd_cwork.as<int>(), c->tmp.as<int>(), c->samples.p, c->ctr(),
-                                c->stream);
+                                c->trial_ctr(), c->stream);
         }
     }
     if (split_curv)
@@ -1848,6 +1855,13 @@ int pisurf_destroy(pisurf_ctx* c) {
 const char* pisurf_last_error(const pisurf_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 void* pisurf_stream(pisurf_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int pisurf_run_tuning(int32_t enabled, int32_t clear) {
+    RunTuner& t = RunTuner::get();
+    if (enabled >= 0) t.set_enabled(enabled != 0);
+    if (clear) t.clear();
+    return (int)t.size();
+}
 
 int pisurf_set_params(pisurf_ctx* c, const pisurf_params* p) {
     if (!c || !p) return PISURF_E_INVALID;

@@ -2,6 +2,7 @@
 // src/evaluator.cpp:60-125) and particle normals (CalcParticlesNormal, :363-375).
 #include "pisurf_internal.h"
 #include "run_eval.h"
+#include "run_tuner.h"
 
 #include <type_traits>
 
@@ -63,16 +64,16 @@ template <class R> struct ParticleSrc {
     }
 };
 
-template <class R, bool K, int MODE>
-__global__ void __launch_bounds__(kRunThreads) k_run_normals(GridDev g, Recs<R> rec, FieldConst fc,
-                                                             const float4* __restrict__ spos, int kx,
-                                                             float4* __restrict__ nrm, Counters* ctr) {
+template <class R, bool K, int T, int MODE>
+__global__ void __launch_bounds__(T) k_run_normals(GridDev g, Recs<R> rec, FieldConst fc,
+                                                   const float4* __restrict__ spos, int kx,
+                                                   float4* __restrict__ nrm, Counters* ctr) {
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     int h0, h1;
     particle_run(g, kx, h0, h1);
     if (h0 >= h1) return;  // uniform over the block
     ParticleSrc<R> src{spos, nrm, out_sigma<R, K>(fc)};
-    const unsigned long long vis = run_eval<kRunThreads, R, K, true, MODE>(g, rec, fc, src, h0, h1, lds);
+    const unsigned long long vis = run_eval<T, R, K, true, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->candidates, vis);
     if (threadIdx.x == 0) atomicAdd(&c->grad_evals, (unsigned long long)(h1 - h0));
@@ -99,7 +100,7 @@ void launch_eval_points(int prec, GridDev g, const void* ra, const void* rg, Fie
 }
 
 void launch_normals(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, const float4* spos, int64_t n,
-                    int64_t n_invalid, float4* nrm, Counters* ctr, hipStream_t s) {
+                    int64_t n_invalid, float4* nrm, Counters* ctr, Counters* trial_ctr, hipStream_t s) {
     if (n <= 0) return;
     if (n_invalid > 0) {
         const unsigned nb = (unsigned)((n + 255) / 256);
@@ -113,26 +114,32 @@ void launch_normals(int prec, GridDev g, const void* ra, const void* rg, FieldCo
         }
         return;
     }
-    // kx cells per run: about kRunThreads particles in bulk fluid, where particles sit on a lattice of
-    // spacing 2 r (8 per cell of 4 r)
-    const int kx = 16;
-    const int per_row = (g.n[0] + kx - 1) / kx;
-    const int64_t nb64 = (int64_t)per_row * g.n[1] * g.n[2];
-    if (nb64 > INT32_MAX) return;  // grids this large are refused when the grid is built
-    const unsigned nb = (unsigned)nb64;
-    auto go = [&](auto mode) {
-        constexpr int M = decltype(mode)::value;
-        if (prec == 2) {
-            Recs<double> r{(const double4*)ra, (const double4*)rg};
-            k_run_normals<double, false, M><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
-        } else {
-            Recs<float> r{(const float4*)ra, (const float4*)rg};
-            if (prec == 1) k_run_normals<float, true, M><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
-            else k_run_normals<float, false, M><<<nb, kRunThreads, 0, s>>>(g, r, fc, spos, kx, nrm, ctr);
-        }
-    };
-    if (field_mode(fc) == kModeAniBspline) go(std::integral_constant<int, kModeAniBspline>());
-    else go(std::integral_constant<int, kModeAny>());
+    // runs of kx cells of a grid row, one workgroup each (empty runs end at once); the shape is
+    // tuned per precision and average particles per cell
+    const double p = (double)n / std::max<int64_t>((int64_t)g.n[0] * g.n[1] * g.n[2], 1);
+    tuned_run_launch(
+        prec, kRunNormals, p, trial_ctr, s,
+        [&](RunShape c) { return (int64_t)((g.n[0] + c.kx - 1) / c.kx) * g.n[1] * g.n[2] <= INT32_MAX; },
+        [&](RunShape c, bool trial) {
+            Counters* cc = trial ? trial_ctr : ctr;
+            const unsigned nb = (unsigned)((int64_t)((g.n[0] + c.kx - 1) / c.kx) * g.n[1] * g.n[2]);
+            auto go = [&](auto tag, auto mode) {
+                constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
+                if (prec == 2) {
+                    Recs<double> r{(const double4*)ra, (const double4*)rg};
+                    k_run_normals<double, false, TT, M><<<nb, TT, 0, s>>>(g, r, fc, spos, c.kx, nrm, cc);
+                } else {
+                    Recs<float> r{(const float4*)ra, (const float4*)rg};
+                    if (prec == 1) k_run_normals<float, true, TT, M><<<nb, TT, 0, s>>>(g, r, fc, spos, c.kx, nrm, cc);
+                    else k_run_normals<float, false, TT, M><<<nb, TT, 0, s>>>(g, r, fc, spos, c.kx, nrm, cc);
+                }
+            };
+            using M0 = std::integral_constant<int, kModeAny>;
+            using M1 = std::integral_constant<int, kModeAniBspline>;
+            const bool m1 = field_mode(fc) == kModeAniBspline;
+            if (c.T == 256) m1 ? go(std::integral_constant<int, 256>(), M1()) : go(std::integral_constant<int, 256>(), M0());
+            else m1 ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
+        });
 }
 
 // UniformGrid::gridSampling (src/marching.cpp:72-91): point i = x + dims0 * (y + dims1 * z) at

@@ -97,45 +97,54 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_
 }
 
 // ---------------------------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits. Per tile of 4096 keys: digit histogram, a global
-// digit-major exclusive scan of the per-tile histograms, then a stable scatter (below).
+// Stable LSD radix sort, 8-bit digits. Per tile of 4096 keys: a digit histogram, a global
+// digit-major exclusive scan of the per-tile histograms, then a stable scatter (below). (10-bit
+// digits would save a pass on 21-30-bit keys, but a tile of 4096 keys then averages 4 keys per
+// digit and the scatter's writes break into 32-byte pieces: a 10-bit pass of the 23 M sample
+// entries took 255 us against 130 us for an 8-bit one.)
 // ---------------------------------------------------------------------------------------------
-constexpr int kRsT = 256, kRsRounds = 16, kRsTile = kRsT * kRsRounds;
+constexpr int kRsT = 256, kRsRounds = 16, kRsTile = kRsT * kRsRounds, kRsDigits = 256;
 
 template <class KT>
 __global__ void __launch_bounds__(kRsT) k_rs_hist(const KT* __restrict__ keys, int64_t n, int shift,
                                                   int* __restrict__ bh, int nblocks) {
-    __shared__ int h[256];
-    h[threadIdx.x] = 0;
+    __shared__ int h[4][kRsDigits];  // one histogram per wave: atomics of different waves do not collide
+    const int t = threadIdx.x, w = t >> 6;
+    h[0][t] = h[1][t] = h[2][t] = h[3][t] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kRsTile;
+#pragma unroll 4
     for (int r = 0; r < kRsRounds; r++) {
-        const int64_t i = base + r * kRsT + threadIdx.x;
-        if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255u)], 1);
+        const int64_t i = base + r * kRsT + t;
+        if (i < n) atomicAdd(&h[w][(int)((keys[i] >> shift) & 255u)], 1);
     }
     __syncthreads();
-    bh[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    bh[(int64_t)t * nblocks + blockIdx.x] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
 }
 
 // Scatter of one tile: wave w takes the tile's keys [w * 1024, (w + 1) * 1024) in 16 rounds of 64.
 // Within a round, lanes of equal digit find each other with 8 ballots; a wave's running count per
 // digit lives in LDS (the one leader lane per digit advances it), so a key's offset among its
 // wave's keys of that digit needs no barrier. The per-wave counts then give every key its place
-// in the tile sorted by digit; the tile is staged in LDS in that order and written out by
-// consecutive threads, so each digit's run of keys goes to consecutive addresses.
+// in the tile sorted by digit (thread t owns consecutive digits); the tile is staged in LDS in
+// that order and written out by consecutive threads, so each digit's run of keys goes to
+// consecutive addresses.
 template <class KT, bool VAL>
 __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin, const int* __restrict__ vin,
                                                      KT* __restrict__ kout, int* __restrict__ vout, int64_t n,
                                                      int shift, const int* __restrict__ bscan, int nblocks) {
-    __shared__ int base[256];
-    __shared__ int cnt[4][256];
-    __shared__ int dstart[256];
+    constexpr int DB = 8, ND = 1 << DB, DPT = ND / kRsT;
+    __shared__ int base[ND];
+    __shared__ int cnt[4][ND];
+    __shared__ int dstart[ND];
     __shared__ int wsum[4];
     __shared__ KT sk[kRsTile];
     __shared__ int sv[VAL ? kRsTile : 1];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    base[t] = bscan[(int64_t)t * nblocks + blockIdx.x];
-    cnt[0][t] = cnt[1][t] = cnt[2][t] = cnt[3][t] = 0;
+    for (int d = t; d < ND; d += kRsT) {
+        base[d] = bscan[(int64_t)d * nblocks + blockIdx.x];
+        cnt[0][d] = cnt[1][d] = cnt[2][d] = cnt[3][d] = 0;
+    }
     __syncthreads();
     const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     const int64_t tile0 = (int64_t)blockIdx.x * kRsTile;
@@ -148,10 +157,10 @@ __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin,
         const bool valid = i < n;
         k[r] = valid ? kin[i] : KT(0);
         v[r] = (VAL && valid) ? vin[i] : 0;
-        const int d = (int)((k[r] >> shift) & 255u);
+        const int d = (int)((k[r] >> shift) & (ND - 1));
         unsigned long long m = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < DB; b++) {
             const unsigned long long bb = __ballot((d >> b) & 1);
             m &= ((d >> b) & 1) ? bb : ~bb;
         }
@@ -161,25 +170,37 @@ __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin,
         if (valid && rank == 0) cnt[w][d] = before + __popcll(m);
     }
     __syncthreads();
-    // thread t owns digit t: per-wave offsets, then the digit's start in the sorted tile
-    int tot = 0;
+    // thread t owns digits [t DPT, (t + 1) DPT): per-wave offsets, then each digit's start in the
+    // sorted tile
+    int dtot[DPT], tsum = 0;
 #pragma unroll
-    for (int ww = 0; ww < 4; ww++) {
-        const int c = cnt[ww][t];
-        cnt[ww][t] = tot;
-        tot += c;
+    for (int i = 0; i < DPT; i++) {
+        const int d = t * DPT + i;
+        int tot = 0;
+#pragma unroll
+        for (int ww = 0; ww < 4; ww++) {
+            const int c = cnt[ww][d];
+            cnt[ww][d] = tot;
+            tot += c;
+        }
+        dtot[i] = tot;
+        tsum += tot;
     }
-    const int inc = wave_incl_scan(tot, lane);
+    const int inc = wave_incl_scan(tsum, lane);
     if (lane == kWave - 1) wsum[w] = inc;
     __syncthreads();
-    int wpre = 0;
-    for (int ww = 0; ww < w; ww++) wpre += wsum[ww];
-    dstart[t] = wpre + inc - tot;
+    int run = inc - tsum;
+    for (int ww = 0; ww < w; ww++) run += wsum[ww];
+#pragma unroll
+    for (int i = 0; i < DPT; i++) {
+        dstart[t * DPT + i] = run;
+        run += dtot[i];
+    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kRsRounds; r++) {
         if (off[r] < 0) continue;
-        const int d = (int)((k[r] >> shift) & 255u);
+        const int d = (int)((k[r] >> shift) & (ND - 1));
         const int tp = dstart[d] + cnt[w][d] + off[r];
         sk[tp] = k[r];
         if (VAL) sv[tp] = v[r];
@@ -188,7 +209,7 @@ __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin,
     const int nt = (int)min((int64_t)kRsTile, n - tile0);
     for (int e = t; e < nt; e += kRsT) {
         const KT key = sk[e];
-        const int d = (int)((key >> shift) & 255u);
+        const int d = (int)((key >> shift) & (ND - 1));
         const int pos = base[d] + (e - dstart[d]);
         kout[pos] = key;
         if (VAL) vout[pos] = sv[e];
@@ -197,7 +218,7 @@ __global__ void __launch_bounds__(kRsT) k_rs_scatter(const KT* __restrict__ kin,
 
 int64_t radix_tmp_ints(int64_t n) {
     const int64_t nb = (n + kRsTile - 1) / kRsTile;
-    return 2 * 256 * nb + scan_tmp_ints(256 * nb) + 16;
+    return 2 * kRsDigits * nb + scan_tmp_ints(kRsDigits * nb) + 16;
 }
 
 // Sorts by bits [lo_bit, lo_bit + bits) of the keys; v0 / v1 NULL: keys only. Returns the buffer
@@ -206,16 +227,16 @@ template <class KT>
 static KT* radix_passes(KT* k0, int* v0, KT* k1, int* v1, int64_t n, int lo_bit, int bits, int* tmp, hipStream_t s) {
     const int nb = (int)((n + kRsTile - 1) / kRsTile);
     int* bh = tmp;
-    int* bscan = tmp + 256 * (int64_t)nb;
-    int* stmp = bscan + 256 * (int64_t)nb;
+    int* bscan = tmp + kRsDigits * (int64_t)nb;
+    int* stmp = bscan + kRsDigits * (int64_t)nb;
     const int passes = (bits + 7) / 8;
     KT *ks = k0, *kd = k1;
     int *vs = v0, *vd = v1;
     for (int p = 0; p < passes; p++) {
         const int shift = lo_bit + 8 * p;
         k_rs_hist<KT><<<nb, kRsT, 0, s>>>(ks, n, shift, bh, nb);
-        exclusive_scan_i32(bh, bscan, 256 * (int64_t)nb, stmp, s);
-        if (v0) k_rs_scatter<KT, true><<<nb, kRsT, 0, s>>>(ks, vs, kd, vd, n, shift, bscan, nb);
+        exclusive_scan_i32(bh, bscan, kRsDigits * (int64_t)nb, stmp, s);
+        if (vs) k_rs_scatter<KT, true><<<nb, kRsT, 0, s>>>(ks, vs, kd, vd, n, shift, bscan, nb);
         else k_rs_scatter<KT, false><<<nb, kRsT, 0, s>>>(ks, nullptr, kd, nullptr, n, shift, bscan, nb);
         KT* tk = ks; ks = kd; kd = tk;
         int* tv = vs; vs = vd; vd = tv;

@@ -4,6 +4,7 @@
 #include "pisurf_internal.h"
 #include "qef_device.h"
 #include "run_eval.h"
+#include "run_tuner.h"
 
 #include <cmath>
 #include <type_traits>
@@ -394,54 +395,39 @@ __global__ void __launch_bounds__(T) k_run_centers(GridDev g, Recs<R> rec, Field
     if (threadIdx.x == 0) atomicAdd(&c->field_evals, (unsigned long long)(h1 - h0));
 }
 
-// Block size T (128 or 256) and cells per run kx for an average of p points per cell: kx p is
-// about 90 % of T (a run may hold more points than the average), and the shape that keeps more of
-// the block's threads busy wins; a cell of more than T points is taken in several passes.
-static void run_shape(double p, int* kx_out, int* t_out) {
-    double best = -1.0;
-    for (int t : {128, 256}) {
-        const int kx = std::max(1, std::min(128, (int)(0.9 * t / std::max(p, 1e-9))));
-        const double pts = kx * p;
-        const double util = pts / (t * std::ceil(pts / t));
-        if (util > best + 0.02) {
-            best = util;
-            *kx_out = kx;
-            *t_out = t;
-        }
-    }
-}
-
-// Cell groups and runs of nh points in cell-major order (key[hlist[h]] >> gshift is the cell group of
-// point h; hlist NULL: point h is entry h). cwork: flags | positions | group starts, run starts
-// ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of runs, to be evaluated by blocks
-// of *t_out threads; *rstart_out points into cwork. Synchronises the stream twice (group and run
-// counts).
-static int build_runs(GridDev g, SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp,
-                      int** rstart_out, int* kx_out, int* t_out, hipStream_t s) {
+// Cell groups of nh points in cell-major order (key[hlist[h]] >> gshift is the cell group of point
+// h; hlist NULL: point h is entry h). cwork: flags | positions | group starts, run starts
+// ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of groups (one read-back).
+static int build_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp, hipStream_t s) {
+    if (nh <= 0) return 0;
     int* flag = cwork;
     int* pos = cwork + (nh + 2);
     int* cstart = cwork + 2 * (int64_t)(nh + 2);
-    *rstart_out = cstart;
-    *kx_out = 0;
-    *t_out = kRunThreads;
-    if (nh <= 0) return 0;
     const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
     k_cell_flags<<<nbh, 256, 0, s>>>(kv, gshift, hlist, nh, flag);
     exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
     k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
     int ng = 0;
     (void)read_device(&ng, pos + nh, 4, s);
-    int kx = 1;
-    run_shape((double)nh / (double)std::max(ng, 1), &kx, t_out);
-    *kx_out = kx;
+    return ng;
+}
+
+// Runs of at most kx cells of one grid row over the ng groups built by build_groups: returns the
+// number of runs (one read-back) and in *rstart_out their starts (nruns + 1 entries in cwork).
+static int build_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist, int nh, int ng, int kx, int* cwork,
+                            int* tmp, int** rstart_out, hipStream_t s) {
+    int* flag = cwork;
+    int* pos = cwork + (nh + 2);
+    int* cstart = cwork + 2 * (int64_t)(nh + 2);
+    int* rstart = cstart + (ng + 1);  // (ng + 1) + (nruns + 1) <= 2 (nh + 1)
+    *rstart_out = rstart;
+    if (ng <= 0) return 0;
     const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
     k_run_flags<<<nbg, 256, 0, s>>>(g, kv, gshift, hlist, cstart, ng, kx, flag);
     exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
-    int* rstart = cstart + (ng + 1);  // (ng + 1) + (nruns + 1) <= 2 (nh + 1)
     k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
     int nr = 0;
     (void)read_device(&nr, pos + ng, 4, s);
-    *rstart_out = rstart;
     return nr;
 }
 
@@ -725,7 +711,11 @@ void launch_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist, int
     k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, cstart + (ng + 1));
 }
 
-void run_shape_for(double points_per_cell, int* kx, int* t) { run_shape(points_per_cell, kx, t); }
+void run_shape_for(double points_per_cell, int* kx, int* t) {
+    const RunShape sh = run_shape_guess(points_per_cell);
+    *kx = sh.kx;
+    *t = sh.T;
+}
 
 void launch_sample_scatter(const int* list, SortedKV kv, const int* rep, const int* hpos, int64_t nent,
                            const void* vals, int rsz, void* samples, hipStream_t s) {
@@ -848,11 +838,38 @@ void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t
                                                                  lb, vbits, key, val);
 }
 
+// one launch of the sample runs with a given block size (every run one workgroup)
+static void launch_run_samples(int prec, int T, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
+                               const int* list, SortedKV kv, const int* hlist, const int* rstart, void* vals, int nr,
+                               Counters* ctr, hipStream_t s) {
+    auto go = [&](auto tag, auto mode) {
+        constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
+        if (prec == 2) {
+            Recs<double> r{(const double4*)ra, (const double4*)rg};
+            k_run_samples<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart, (double*)vals,
+                                                                  ctr);
+        } else {
+            Recs<float> r{(const float4*)ra, (const float4*)rg};
+            if (prec == 1)
+                k_run_samples<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart, (float*)vals,
+                                                                    ctr);
+            else
+                k_run_samples<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart,
+                                                                     (float*)vals, ctr);
+        }
+    };
+    using M0 = std::integral_constant<int, kModeAny>;
+    using M1 = std::integral_constant<int, kModeAniBspline>;
+    const bool m1 = field_mode(fc) == kModeAniBspline;
+    if (T == 256) m1 ? go(std::integral_constant<int, 256>(), M1()) : go(std::integral_constant<int, 256>(), M0());
+    else m1 ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
+}
+
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
                                const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
-                               int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
-                               hipEvent_t e1, hipStream_t s) {
+                               int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr,
+                               hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
     *nheads_out = 0;
     *nruns_out = 0;
     *kx_out = 0;
@@ -861,41 +878,30 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     const unsigned nb = (unsigned)((nent + 255) / 256);
     int* block_off = head + (nb + 1);
     k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, nent, head, rep);
-    (void)hipMemsetAsync(head + nb, 0, sizeof(int), s);
+    launch_fill_i32(head + nb, 1, 0, s);
     exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
     k_head_list<<<nb, 256, 0, s>>>(rep, nent, block_off, hlist, hpos);
     int nh = 0;
     (void)read_device(&nh, block_off + nb, 4, s);
     *nheads_out = nh;
+    const int ng = build_groups(kv, gshift, hlist, nh, cwork, tmp, s);
     int* rstart = nullptr;
-    int kx = 0, T = kRunThreads;
-    const int nr = build_runs(g, kv, gshift, hlist, nh, cwork, tmp, &rstart, &kx, &T, s);
-    *nruns_out = nr;
-    *kx_out = kx;
+    int nr = 0;
     (void)hipEventRecord(e0, s);
-    if (nr > 0) {
-        auto go = [&](auto tag, auto mode) {
-            constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
-            if (prec == 2) {
-                Recs<double> r{(const double4*)ra, (const double4*)rg};
-                k_run_samples<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart,
-                                                                      (double*)vals, ctr);
-            } else {
-                Recs<float> r{(const float4*)ra, (const float4*)rg};
-                if (prec == 1)
-                    k_run_samples<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart,
-                                                                        (float*)vals, ctr);
-                else
-                    k_run_samples<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart,
-                                                                         (float*)vals, ctr);
-            }
-        };
-        using M0 = std::integral_constant<int, kModeAny>;
-        using M1 = std::integral_constant<int, kModeAniBspline>;
-        const bool m1 = field_mode(fc) == kModeAniBspline;
-        if (T == 256) m1 ? go(std::integral_constant<int, 256>(), M1()) : go(std::integral_constant<int, 256>(), M0());
-        else m1 ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
+    if (ng > 0) {
+        const RunShape sh = tuned_run_launch(
+            prec, kRunSamples, (double)nh / ng, trial_ctr, s,
+            [&](RunShape c) {
+                nr = build_run_starts(g, kv, gshift, hlist, nh, ng, c.kx, cwork, tmp, &rstart, s);
+                return nr > 0;
+            },
+            [&](RunShape c, bool trial) {
+                launch_run_samples(prec, c.T, g, ra, rg, fc, lv, list, kv, hlist, rstart, vals, nr,
+                                   trial ? trial_ctr : ctr, s);
+            });
+        *kx_out = sh.kx;
     }
+    *nruns_out = nr;
     (void)hipEventRecord(e1, s);
     k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, kv, rep, hpos, nent, vals,
                                                                     prec == 2 ? 8 : 4, samples);
@@ -917,7 +923,7 @@ void launch_centers(int prec, GridDev g, const void* ra, const void* rg, FieldCo
 
 void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
                          const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
-                         int* tmp, void* samples, Counters* ctr, hipStream_t s) {
+                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, hipStream_t s) {
     if (k <= 0) return;
     const uint64_t groups = (uint64_t)(g.n[0] + 2) * (uint64_t)(g.n[1] + 2) * (uint64_t)(g.n[2] + 2);
     int bits = 1;
@@ -925,31 +931,41 @@ void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, Fi
     const int vbits = pack_vbits(bits, k);
     k_center_keys<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(g, lv, list, k, vbits, key, val);
     const SortedKV kv = radix_sort_entries(key, val, key2, val2, k, bits, vbits, tmp, s);
+    const int ng = build_groups(kv, 0, nullptr, (int)k, cwork, tmp, s);
+    if (ng <= 0) return;
     int* rstart = nullptr;
-    int kx = 0, T = kRunThreads;
-    const int nr = build_runs(g, kv, 0, nullptr, (int)k, cwork, tmp, &rstart, &kx, &T, s);
-    if (nr <= 0) return;
-    auto go = [&](auto tag, auto mode) {
-        constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
-        if (prec == 2) {
-            Recs<double> r{(const double4*)ra, (const double4*)rg};
-            k_run_centers<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
-                                                                  (double*)samples, ctr);
-        } else {
-            Recs<float> r{(const float4*)ra, (const float4*)rg};
-            if (prec == 1)
-                k_run_centers<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
-                                                                    (float*)samples, ctr);
-            else
-                k_run_centers<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
-                                                                     (float*)samples, ctr);
-        }
+    int nr = 0;
+    auto launch = [&](RunShape c, bool trial) {
+        Counters* cc = trial ? trial_ctr : ctr;
+        auto go = [&](auto tag, auto mode) {
+            constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
+            if (prec == 2) {
+                Recs<double> r{(const double4*)ra, (const double4*)rg};
+                k_run_centers<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
+                                                                      (double*)samples, cc);
+            } else {
+                Recs<float> r{(const float4*)ra, (const float4*)rg};
+                if (prec == 1)
+                    k_run_centers<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
+                                                                        (float*)samples, cc);
+                else
+                    k_run_centers<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, rstart,
+                                                                         (float*)samples, cc);
+            }
+        };
+        using M0 = std::integral_constant<int, kModeAny>;
+        using M1 = std::integral_constant<int, kModeAniBspline>;
+        const bool m1 = field_mode(fc) == kModeAniBspline;
+        if (c.T == 256) m1 ? go(std::integral_constant<int, 256>(), M1()) : go(std::integral_constant<int, 256>(), M0());
+        else m1 ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
     };
-    using M0 = std::integral_constant<int, kModeAny>;
-    using M1 = std::integral_constant<int, kModeAniBspline>;
-    const bool m1 = field_mode(fc) == kModeAniBspline;
-    if (T == 256) m1 ? go(std::integral_constant<int, 256>(), M1()) : go(std::integral_constant<int, 256>(), M0());
-    else m1 ? go(std::integral_constant<int, 128>(), M1()) : go(std::integral_constant<int, 128>(), M0());
+    tuned_run_launch(
+        prec, kRunCenters, (double)k / ng, trial_ctr, s,
+        [&](RunShape c) {
+            nr = build_run_starts(g, kv, 0, nullptr, (int)k, ng, c.kx, cwork, tmp, &rstart, s);
+            return nr > 0;
+        },
+        launch);
 }
 
 void launch_samples(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, TreeParams tp, NodeLevel lv,

@@ -103,8 +103,9 @@ __device__ __forceinline__ void wave_count(unsigned long long* ctr, unsigned lon
 }
 void launch_eval_points(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float* q,
                         int64_t n, double* f, double* grad, Counters* ctr, hipStream_t s);
+// trial_ctr: counter block for the timed trial launches of shape tuning (their counts are not kept)
 void launch_normals(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, const float4* spos,
-                    int64_t n, int64_t n_invalid, float4* nrm, Counters* ctr, hipStream_t s);
+                    int64_t n, int64_t n_invalid, float4* nrm, Counters* ctr, Counters* trial_ctr, hipStream_t s);
 // dense sampling box: lo / hi corners, steps (intervals) and dims = steps + 1 points per axis
 struct UniformBox {
     float lo[3], hi[3];
@@ -157,7 +158,7 @@ void launch_centers(int prec, GridDev g, const void* rec_a, const void* rec_g, F
 // runs of cells; key / val / key2 / val2 hold k entries, cwork 4 (k + 2) ints
 void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                          const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
-                         int* tmp, void* samples, Counters* ctr, hipStream_t s);
+                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, hipStream_t s);
 // sample deduplication: keys for the nent = 27 * listed entries (hash cell group, then the
 // lattice position modulo 2^lb per axis; the group starts at bit gshift = 3 lb), then (after the
 // key sort) heads, the distinct points grouped into runs of cells, one workgroup per run between
@@ -169,8 +170,8 @@ void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                                const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
-                               int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, hipEvent_t e0,
-                               hipEvent_t e1, hipStream_t s);
+                               int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr,
+                               hipEvent_t e0, hipEvent_t e1, hipStream_t s);
 // Phase-split deduplicated sampling (the steps of launch_sample_dedupe_eval and
 // launch_centers_runs cut at each count the host reads), for sweeps over many frames:
 // heads of nent sorted entries, their count at heads_count(head, nent)

@@ -304,3 +304,30 @@ def test_run_batch_24_frames():
     tree and counters equal to a fresh context meshing it alone."""
     _batch_matches_fresh(synth.batch_frames(), dict(precision=capi.FP32_KAHAN, qef_mode=capi.QEF_SVD64),
                          "run_batch")
+
+
+def test_run_shape_tuning_keeps_results():
+    """Run-shape autotuning: a frame meshed while the shapes are being timed (every candidate launch
+    on the same data), again with the kept shapes, and with tuning off gives the same mesh, tree and
+    counters each time; the tuner keeps one shape per (precision, launch kind, density bucket)."""
+    L = capi.load()
+    xyz, r = synth.dam_break()
+    out = []
+    try:
+        for enabled, clear in ((1, 1), (1, 0), (0, 0)):
+            L.pisurf_run_tuning(enabled, clear)
+            ctx = gpu_context(xyz, r, precision=capi.FP32_KAHAN)
+            info = ctx.run_frame()
+            v, t = ctx.mesh()
+            out.append((info, v, t, ctx.stats(), L.pisurf_run_tuning(-1, 0)))
+            ctx.close()
+    finally:
+        L.pisurf_run_tuning(1, 0)
+    kept = out[0][4]
+    assert kept >= 3  # samples of the sampled levels, leaf-depth centres, normals
+    assert out[1][4] == kept  # nothing new to time on the second frame
+    for info, v, t, st, _ in out[1:]:
+        assert info == out[0][0]
+        assert np.array_equal(v, out[0][1]) and np.array_equal(t, out[0][2])
+        for key in ("field_evals", "grad_evals", "sample_unique", "sample_evals", "candidates", "run_evals"):
+            assert st[key] == out[0][3][key], key
