@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pisurf {
 
 constexpr int kWave = 64;
@@ -78,7 +80,6 @@ template <class R> __device__ __forceinline__ R dsqrt(R x);
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
 
-// Compensated (Kahan) accumulator; K = false degenerates to a plain sum.
 // inclusive prefix sum over the 64 lanes of a wave
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 #pragma unroll
@@ -89,32 +90,21 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     return v;
 }
 
+// Sum accumulator. The compensated FP32 mode (K) adds its float terms into a double: on gfx950 an
+// unpacked FP32 add and an FP64 add issue at the same rate, so a conversion and one FP64 add
+// replace the four float operations of a Kahan step, and the sum is at least as accurate. K =
+// false: a plain sum in R.
 template <class R, bool K> struct Acc {
-    R s = R(0), c = R(0);
-    __device__ __forceinline__ void add(R v) {
-        if (K) {
-            const R y = v - c;
-            const R t = s + y;
-            c = (t - s) - y;
-            s = t;
-        } else {
-            s += v;
-        }
-    }
+    typedef typename std::conditional<K && sizeof(R) == 4, double, R>::type S;
+    S s = S(0);
+    __device__ __forceinline__ void add(R v) { s += S(v); }
     // the same, never fused with the product that made v: the field sums of every kernel then
     // round each contribution the same way, whatever code surrounds them
     __device__ __forceinline__ void add_unfused(R v) {
 #pragma clang fp contract(off)
-        if (K) {
-            const R y = v - c;
-            const R t = s + y;
-            c = (t - s) - y;
-            s = t;
-        } else {
-            s += v;
-        }
+        s += S(v);
     }
-    __device__ __forceinline__ R get() const { return s; }
+    __device__ __forceinline__ R get() const { return R(s); }
 };
 
 // ---- per-particle field records (sorted order) ---------------------------------------------

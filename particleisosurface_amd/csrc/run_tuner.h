@@ -5,9 +5,9 @@
 // and on the launch. The first launch of a (precision, launch kind, density bucket) times every
 // candidate shape on its own data with HIP events and keeps the fastest; later launches of that
 // bucket take the kept shape at once. Every shape gives the same sums (a point's candidates, and
-// their order, do not depend on the run that holds it), so tuning changes only the time. Trial
-// launches other than the first count their events into a separate counter block, so the
-// statistics equal those of one launch. PISURF_RUN_TUNING=0 switches tuning off (the shape then
+// their order, do not depend on the run that holds it), so tuning changes only the time. Timed
+// trial launches count their events into a separate counter block, so the statistics equal those
+// of one launch. PISURF_RUN_TUNING=0 switches tuning off (the shape then
 // comes from the occupancy heuristic of run_shape_guess).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -105,10 +105,10 @@ class RunTuner {
     std::atomic<bool> enabled_{true};
 };
 
-// Runs `launch(shape, trial_counters)` for a bucket: with a kept shape once, otherwise once per
-// candidate, timed; returns the shape whose launch's results stand (the last one run). `prepare`
-// builds the runs of a shape (it may read counts back) and returns false to skip the launch.
-// scratch_ctr: counter block for the trial launches after the first.
+// Runs `launch(shape, trial)` for a bucket: with a kept shape once, otherwise once per candidate,
+// timed (trial = true: count into the scratch block); returns the shape whose launch's results
+// stand (the last one run). `prepare` builds the runs of a shape (it may read counts back) and
+// returns false to skip the launch.
 template <class Prepare, class Launch>
 RunShape tuned_run_launch(int prec, int kind, double p, void* scratch_ctr, hipStream_t s, Prepare&& prepare,
                           Launch&& launch) {
@@ -126,17 +126,22 @@ RunShape tuned_run_launch(int prec, int kind, double p, void* scratch_ctr, hipSt
         if (prepare(sh)) launch(sh, false);
         return sh;
     }
+    // the heuristic shape (tried first) stays unless a candidate is clearly faster, so that shapes
+    // within timing noise of each other do not change from one process to the next
     RunShape best = sh, last = sh;
     float best_ms = INFINITY;
     bool first = true, any = false;
     for (const RunShape& c : cand) {
         if (!prepare(c)) continue;
+        // the first shape also runs once untimed (its counts are the ones kept), so that loading
+        // the code and warming the caches is not charged to it
+        if (first) launch(c, false);
         (void)hipEventRecord(e0, s);
-        launch(c, !first);
+        launch(c, true);
         (void)hipEventRecord(e1, s);
         float ms = INFINITY;
         if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best_ms) {
+        if (first ? ms < best_ms : ms < 0.97f * best_ms) {
             best_ms = ms;
             best = c;
         }
