@@ -270,15 +270,15 @@ __device__ __forceinline__ int field_rows(const GridDev& g, float fx, float fy, 
 // sigma), optionally with the poly6-form gradient sum (src/evaluator.cpp:408-415,437-463).
 // Candidates are taken in the reference's order (GetPIdxList: rows z-outer, then y, each row's
 // cells x-ascending, sorted particles within a cell). Returns the number of candidates visited.
-template <class R, bool KAHAN, bool GRAD, int MODE = kModeAny>
+// LEAN: the row windows are looked up one row at a time in a loop that is not unrolled, which
+// keeps far fewer registers live (for kernels whose occupancy the 9 windows and 9 unrolled row
+// loops would cut); the candidates and their order are the same.
+template <class R, bool KAHAN, bool GRAD, int MODE = kModeAny, bool LEAN = false>
 __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, const FieldConst& fc, float fx,
                                          float fy, float fz, R px, R py, R pz, R& sum, R grad[3]) {
     Acc<R, KAHAN> acc, ga[3];
     const R inv_h2 = R(fc.inv_h2);
     const R h2 = R(fc.h2);
-    // all 9 row windows first (independent loads in flight together)
-    int rlo[9], rhi[9];
-    const int visited = field_rows(g, fx, fy, fz, rlo, rhi);
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     auto contrib = [&](int j, R dx, R dy, R dz, R d2) {
         const float4 lo = (KAHAN && sizeof(R) == 4) ? fc.lo[j] : zero4;
@@ -290,9 +290,8 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
     // different points, and a per-candidate branch would run the contribution whenever any lane
     // needs it. The cull is written !(d2 >= R^2) so that a NaN record passes and poisons the sum,
     // as the reference's kernels do for it.
-    for (int r = 0; r < 9; r++) {
-        const int e = rhi[r];
-        for (int j = rlo[r]; j < e; j += 32) {
+    auto row = [&](int j0, int e) {
+        for (int j = j0; j < e; j += 32) {
             const int jn = min(e, j + 32);
             unsigned m = 0;
             for (int u = j; u < jn; u += 4) {
@@ -312,6 +311,23 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
                 contrib(jj, dx, dy, dz, dist2(dx, dy, dz));
             }
         }
+    };
+    int visited = 0;
+    if constexpr (LEAN) {
+        int ix, iy, iz;
+        cell_of(g, fx, fy, fz, ix, iy, iz);
+#pragma unroll 1
+        for (int r = 0; r < 9; r++) {
+            int lo, hi;
+            if (!row_range(g, ix, iy, iz, r, lo, hi)) continue;
+            visited += hi - lo;
+            row(lo, hi);
+        }
+    } else {
+        // all 9 row windows first (independent loads in flight together)
+        int rlo[9], rhi[9];
+        visited = field_rows(g, fx, fy, fz, rlo, rhi);
+        for (int r = 0; r < 9; r++) row(rlo[r], rhi[r]);
     }
     sum = acc.get();
     if (GRAD) {

@@ -34,8 +34,8 @@ __global__ void __launch_bounds__(256) k_bisect(GridDev g, Recs<R> rec, FieldCon
             R m[4];
             for (int a = 0; a < 3; a++) m[a] = (v1[a] + v2[a]) / R(2);
             R sum, g3[3];
-            vis += field_sum<R, K, false, MODE>(g, rec, fc, float(m[0]), float(m[1]), float(m[2]), m[0], m[1], m[2],
-                                                sum, g3);
+            vis += field_sum<R, K, false, MODE, true>(g, rec, fc, float(m[0]), float(m[1]), float(m[2]), m[0], m[1],
+                                                      m[2], sum, g3);
             nev++;
             m[3] = R(fc.iso) - out_sigma<R, K>(fc) * sum;
             const int sm = m[3] > R(0) ? 1 : -1, s1 = v1[3] > R(0) ? 1 : -1;
