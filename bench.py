@@ -3,8 +3,8 @@
 
 One step = one full frame through the hot path (hash grid, anisotropy, iso constants, particle
 normals, octree level sweeps, dual-cell extraction, bisection vertex placement) on particles
-already resident in HBM, in the FP32-compensated mode (Kahan sums, FP64 eigensolve for clamped
-spectra). The same frame is also meshed by the all-FP64 GPU path, and the two are compared.
+already resident in HBM, in the compensated FP32 mode (float terms summed in FP64, FP64 eigensolve
+for clamped spectra). The same frame is also meshed by the all-FP64 GPU path, and the two are compared.
 
 Prints one JSON line (see the keys below). The CPU baseline is the oracle (the reference's CPU
 algorithm with its std::map cell tables and OpenMP loops) evaluating a bounded sample of the
@@ -50,10 +50,10 @@ def measured_traffic(kernel, workload_tag):
 
 def measured_issue(kernel, workload_tag):
     """VALU wave instructions per launch of `kernel` on the workload from the committed rocprofv3 SQ
-    counter summary profiles/<kernel>_<workload_tag>_issue.md (line
+    counter summary profiles/field_kernels_issue_<workload_tag>.md (line
     `valu_instr_per_launch <kernel> = <count>`), or (None, None)."""
     import re
-    f = os.path.join(HERE, "profiles", "%s_%s_issue.md" % (kernel, workload_tag))
+    f = os.path.join(HERE, "profiles", "field_kernels_issue_%s.md" % workload_tag)
     if os.path.exists(f):
         for line in open(f):
             m = re.match(r"valu_instr_per_launch (\w+) = (\d+)", line.strip())
@@ -140,8 +140,9 @@ def aniso_roofline(ctx, ms_aniso, n, wtag):
     particle, [12 + 216 + 12 C + 12] + [12 + 216 + 12 C + 40] with C the candidates of its 27-cell
     window, over the anisotropy phase of a timed frame (host clock around both launches), and their
     VALU issue from the committed SQ summary. The candidates' positions and means are re-read from
-    L1 / L2 by neighbouring threads, so the algorithmic rate is not an HBM rate; the limiter is
-    instruction issue."""
+    L1 / L2 by neighbouring threads, so the algorithmic rate is not an HBM rate. They issue about one
+    VALU instruction per CU and cycle, but cutting their instructions by 15 % did not shorten them:
+    the latency of their candidate loads limits them as much as issue does."""
     g = ctx.hashgrid_build()
     _, s, e = ctx.hashgrid_download(g["cellnum"])
     nx, ny, nz = (int(v) for v in g["cellnum3"])
@@ -218,8 +219,9 @@ def copy_calibration(mib):
 
 
 def batch_leg(prec, dedupe, qef):
-    """24 frames of uneven size: one persistent run_frames call vs. a loop of the plain path (a
-    fresh context per frame, as the reference builds a new SurfReconstructor per frame)."""
+    """24 frames of uneven size: one run_frames call (one context, buffers reused, the next frame
+    staged while the current one is meshed) vs. a loop of the plain path (a fresh context per
+    frame, as the reference builds a new SurfReconstructor per frame)."""
     t = time.perf_counter()
     frames = synth.batch_frames()
     log("batch: %d frames, %d particles (%.1f s to generate)" % (len(frames), sum(len(f) for f, _ in frames),
@@ -232,13 +234,13 @@ def batch_leg(prec, dedupe, qef):
 
     ctx = capi.Context(0)
     ctx.set_params(rs[0], precision=prec, sample_dedupe=dedupe, qef_mode=qef)
-    ctx.run_frames(xs[:2], rs[:2])  # warm-up: buffers, code objects
+    ctx.run_frames(xs, rs)  # warm-up: buffers, code objects, run shapes of every frame's densities
     tris[0] = 0
     t = time.perf_counter()
     ctx.run_frames(xs, rs, on_mesh)
-    dt_persistent = time.perf_counter() - t
+    dt_one = time.perf_counter() - t
     ctx.close()
-    tris_p = tris[0]
+    tris_one = tris[0]
     t = time.perf_counter()
     tris_l = 0
     for x, r in zip(xs, rs):
@@ -250,9 +252,9 @@ def batch_leg(prec, dedupe, qef):
         c.close()
     dt_loop = time.perf_counter() - t
     return dict(frames=len(xs), particles=int(sum(len(x) for x in xs)),
-                persistent_frames_per_s=round(len(xs) / dt_persistent, 3),
-                loop_frames_per_s=round(len(xs) / dt_loop, 3), speedup=round(dt_loop / dt_persistent, 3),
-                same_triangle_count=bool(tris_p == tris_l), host_input=True)
+                run_frames_frames_per_s=round(len(xs) / dt_one, 3),
+                loop_frames_per_s=round(len(xs) / dt_loop, 3), speedup=round(dt_loop / dt_one, 3),
+                same_triangle_count=bool(tris_one == tris_l), host_input=True)
 
 
 def main():
@@ -274,7 +276,7 @@ def main():
     ap.add_argument("--depth-max", type=int, default=-1, help="octree depth override (-1: 12 for pour, else derived)")
     ap.add_argument("--batch", action="store_true",
                     help="also mesh the 24-frame batch (50 k - 3 M particles per frame, radii by frame) "
-                         "through the persistent multi-frame call and through a loop of fresh contexts")
+                         "through one pisurf_run_frames call and through a loop of fresh contexts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) 5 times, to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings, and exit")
@@ -374,7 +376,7 @@ def main():
         "metric": "field evals/s + frame mesh time @4M particles, FP32-comp vs FP64 at stated err",
         "value": round(value, 1), "unit": "field evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp32": "f32", "fp32k": "f32 (compensated; f64 eigensolve when clamped)", "fp64": "f64"}[args.precision],
+        "dtype": {"fp32": "f32", "fp32k": "f32 (sums in f64; f64 eigensolve when clamped)", "fp64": "f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": wname, "precision": args.precision, "qef_mode": args.qef,
                    "depth_max": info["depth_max"], "depth_min": info["depth_min"]},

@@ -35,7 +35,7 @@ extern "C" {
 
 /* arithmetic of the field path */
 #define PISURF_FP32 0        /* float storage and sums */
-#define PISURF_FP32_KAHAN 1  /* float storage, compensated sums, FP64 eigensolve for clamped spectra */
+#define PISURF_FP32_KAHAN 1  /* float storage and terms, sums accumulated in FP64, FP64 eigensolve for clamped spectra */
 #define PISURF_FP64 2        /* double everywhere after the float positions */
 
 /* leaf vertex placement */
@@ -192,7 +192,7 @@ int pisurf_mesh_download(pisurf_ctx* ctx, double* verts, int32_t* tris);
 /* SurfReconstructor::Run (src/surface_reconstructor.cpp:409-479) on the uploaded particles */
 int pisurf_run_frame(pisurf_ctx* ctx, pisurf_tree_info* tree);
 /* runOurs' frame loop (src/main.cpp:330-405: one SurfReconstructor per frame in DATA_PATHES)
- * as one persistent call: device buffers are kept across frames and grow to the largest, and
+ * in one call on one context: device buffers are kept across frames and grow to the largest, and
  * frame k+1's particles are copied to the device (pinned staging, a second stream) while frame k
  * is meshed. Frame k has n[k] particles at xyz[k] (host, or device with on_device != 0) and radius
  * radius[k]; the other parameters are the context's. After each frame `on_mesh` receives the mesh

@@ -551,7 +551,7 @@ def test_sample_dedupe_is_bit_identical(case, prec):
 
 
 def test_run_frames_matches_fresh_contexts():
-    """The persistent multi-frame call meshes each frame exactly as a fresh context does, with
+    """pisurf_run_frames (one context, buffers reused) meshes each frame exactly as a fresh context does, with
     frames of different sizes and radii in one batch (nothing carries over), an empty frame
     included."""
     frames = [synth.dam_break(), synth.lattice_sphere(), (np.zeros((0, 3), np.float32), 0.004),
