@@ -565,25 +565,19 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
             } else {
                 type = kLeaf;
                 if (QEF && tp.qef_mode != 0 && has_samples) {
-                    R f[27], gr[27][3];
-                    float pts[27][3];
-                    for (int s = 0; s < 27; s++) {
-                        f[s] = samples[node * 27 + s];
-                        sample_point(c, s, pts[s]);
-                    }
-                    fd_grads27<R>(f, __fmul_rn(2.0f, c.w), gr);
+                    R f[27];
+#pragma unroll
+                    for (int s = 0; s < 27; s++) f[s] = samples[node * 27 + s];
                     R pc[4], err;
-                    if (qef_solve<R>(c, pts, f, gr, tp.qef_mode, tp.qef_svd_tol, pc, err)) {
+                    if (qef_solve_samples<R>(c, f, tp.qef_mode, tp.qef_svd_tol, pc, err)) {
                         vx = pc[0];
                         vy = pc[1];
                         vz = pc[2];
                     }
-                    R sum, g3[3];
-                    vis += field_sum<R, K, false>(g, rec, fc, float(vx), float(vy), float(vz), vx, vy, vz, sum, g3);
-                    evals++;
                     qefs++;
-                    fval = R(fc.iso) - out_sigma<R, K>(fc) * sum;
-                    if (err / R(__fmul_rn(2.0f, c.w)) < R(0.0001)) fval = R(0);
+                    // the field at the vertex is summed by k_qef_values (fewer registers than the
+                    // solve needs); until then the value slot says whether the DMC error zeroes it
+                    fval = (err / R(__fmul_rn(2.0f, c.w)) < R(0.0001)) ? R(0) : R(1);
                 } else {
                     float p13[3];
                     sample_point(c, 13, p13);
@@ -593,7 +587,8 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
                         fval = samples[node * 27 + 13];  // the centre sample is the node centre
                     } else {
                         R sum, g3[3];
-                        vis += field_sum<R, K, false>(g, rec, fc, c.x, c.y, c.z, R(c.x), R(c.y), R(c.z), sum, g3);
+                        vis += field_sum<R, K, false, kModeAny, true>(g, rec, fc, c.x, c.y, c.z, R(c.x), R(c.y),
+                                                                     R(c.z), sum, g3);
                         evals++;
                         fval = R(fc.iso) - out_sigma<R, K>(fc) * sum;
                     }
@@ -610,6 +605,27 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
     }
     wave_count(&ctr_stripe(ctr)->field_evals, evals);
     wave_count(&ctr_stripe(ctr)->qef_solves, qefs);
+    wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
+}
+
+// Field values at the QEF vertices of a level's sampled leaves (k_classify<QEF> placed them and
+// left 0 in the value slot where the DMC error zeroes the value, 1 elsewhere)
+template <class R, bool K>
+__global__ void __launch_bounds__(256) k_qef_values(GridDev g, Recs<R> rec, FieldConst fc, TreeParams tp, NodeLevel lv,
+                                                    Counters* ctr) {
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vis = 0;
+    unsigned long long evals = 0;
+    if (node < lv_count(lv) && lv.type[node] == kLeaf && wants_samples(lv, tp, node)) {
+        const double4 nd = lv.node[node];
+        const R vx = R(nd.x), vy = R(nd.y), vz = R(nd.z);
+        R sum, g3[3];
+        vis = field_sum<R, K, false, kModeAny, true>(g, rec, fc, float(vx), float(vy), float(vz), vx, vy, vz, sum, g3);
+        evals = 1;
+        const R fval = nd.w == 0.0 ? R(0) : R(fc.iso) - out_sigma<R, K>(fc) * sum;
+        lv.node[node].w = double(fval);
+    }
+    wave_count(&ctr_stripe(ctr)->field_evals, evals);
     wave_count(&ctr_stripe(ctr)->candidates, (unsigned long long)vis);
 }
 
@@ -1001,8 +1017,19 @@ void launch_classify(int prec, GridDev g, const void* ra, const void* rg, FieldC
                 k_classify<float, false, Q><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, (const float*)samples, sc, ctr);
         }
     };
-    if (tp.qef_mode != 0) go(std::true_type());
-    else go(std::false_type());
+    if (tp.qef_mode != 0) {
+        go(std::true_type());
+        if (prec == 2) {
+            Recs<double> r{(const double4*)ra, (const double4*)rg};
+            k_qef_values<double, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, ctr);
+        } else {
+            Recs<float> r{(const float4*)ra, (const float4*)rg};
+            if (prec == 1) k_qef_values<float, true><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, ctr);
+            else k_qef_values<float, false><<<nb, 256, 0, s>>>(g, r, fc, tp, lv, ctr);
+        }
+    } else {
+        go(std::false_type());
+    }
 }
 
 void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s) {

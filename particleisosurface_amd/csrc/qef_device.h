@@ -109,26 +109,33 @@ __device__ __forceinline__ void jacobi4d(double a[4][4], double lam[4], double Q
     for (int i = 0; i < 4; i++) lam[i] = a[i][i];
 }
 
-// Solve the leaf QEF from 27 samples (xyz float, f in R) and their gradients. Returns the
-// candidate vertex pc (4 values) and whether it lies inside the node box; err is the DMC error
-// (calcErrorDMC, src/iso_method_ours.cpp:40-50) when inside.
-template <class R>
-__device__ __forceinline__ bool qef_solve(float4 c, const float pts[27][3], const R f[27], const R g[27][3], int mode,
-                                          float svd_tol, R pc[4], R& err) {
+// Solve the leaf QEF from 27 samples. Returns the candidate vertex pc (4 values) and whether it
+// lies inside the node box; err is the DMC error (calcErrorDMC, src/iso_method_ours.cpp:40-50)
+// when inside. The samples come through accessors (unrolled, so every index is a constant):
+// f_at(i) the value, p_at(i, p) the position (float), g_at(i, g) the gradient. A caller that
+// computes positions and gradients in the accessors keeps only the 27 values live.
+template <class R, class FA, class PA, class GA>
+__device__ __forceinline__ bool qef_core(float4 c, FA f_at, PA p_at, GA g_at, int mode, float svd_tol, R pc[4],
+                                         R& err) {
     if (mode == 1) {
 #pragma clang fp contract(off)
         float q[15];
 #pragma unroll
         for (int k = 0; k < 15; k++) q[k] = 0.0f;
+#pragma unroll
         for (int i = 0; i < 27; i++) {
+            float pt[3];
+            R g[3];
+            p_at(i, pt);
+            g_at(i, g);
             float pl[5];
-            pl[0] = float(g[i][0]);
-            pl[1] = float(g[i][1]);
-            pl[2] = float(g[i][2]);
+            pl[0] = float(g[0]);
+            pl[1] = float(g[1]);
+            pl[2] = float(g[2]);
             pl[3] = -1.0f;
             // plain operators: contraction is off in this scope (the *_rn helpers would be
             // inlined as contractable operations)
-            pl[4] = -((pts[i][0] * pl[0] + pts[i][1] * pl[1]) + pts[i][2] * pl[2]) + float(f[i]);
+            pl[4] = -((pt[0] * pl[0] + pt[1] * pl[1]) + pt[2] * pl[2]) + float(f_at(i));
             int k = 0;
 #pragma unroll
             for (int a = 0; a < 5; a++)
@@ -159,14 +166,19 @@ __device__ __forceinline__ bool qef_solve(float4 c, const float pts[27][3], cons
         double q[5][5];
         for (int a = 0; a < 5; a++)
             for (int b = 0; b < 5; b++) q[a][b] = 0.0;
+#pragma unroll
         for (int i = 0; i < 27; i++) {
-            double pl[5] = {double(g[i][0]), double(g[i][1]), double(g[i][2]), -1.0, 0.0};
-            pl[4] = -(double(pts[i][0]) * pl[0] + double(pts[i][1]) * pl[1] + double(pts[i][2]) * pl[2]) + double(f[i]);
+            float pt[3];
+            R g[3];
+            p_at(i, pt);
+            g_at(i, g);
+            double pl[5] = {double(g[0]), double(g[1]), double(g[2]), -1.0, 0.0};
+            pl[4] = -(double(pt[0]) * pl[0] + double(pt[1]) * pl[1] + double(pt[2]) * pl[2]) + double(f_at(i));
             for (int a = 0; a < 5; a++)
                 for (int b = 0; b < 5; b++) q[a][b] += pl[a] * pl[b];
         }
         double A[4][4], rhs[4];
-        const double m[4] = {double(c.x), double(c.y), double(c.z), double(f[13])};
+        const double m[4] = {double(c.x), double(c.y), double(c.z), double(f_at(13))};
         for (int a = 0; a < 4; a++) {
             rhs[a] = -q[a][4];
             for (int b = 0; b < 4; b++) A[a][b] = q[a][b];
@@ -192,13 +204,57 @@ __device__ __forceinline__ bool qef_solve(float4 c, const float pts[27][3], cons
     err = R(1e30);
     if (inside) {
         err = R(0);
+#pragma unroll
         for (int i = 0; i < 27; i++) {
-            const R dp = (g[i][0] * (pc[0] - R(pts[i][0])) + g[i][1] * (pc[1] - R(pts[i][1]))) + g[i][2] * (pc[2] - R(pts[i][2]));
+            float pt[3];
+            R g[3];
+            p_at(i, pt);
+            g_at(i, g);
+            const R dp = (g[0] * (pc[0] - R(pt[0])) + g[1] * (pc[1] - R(pt[1]))) + g[2] * (pc[2] - R(pt[2]));
             const R e = dp - pc[3];
-            err += (e * e) / (R(1) + (g[i][0] * g[i][0] + g[i][1] * g[i][1]) + g[i][2] * g[i][2]);
+            err += (e * e) / (R(1) + (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
         }
     }
     return inside;
+}
+
+// the same from given sample positions, values and gradients
+template <class R>
+__device__ __forceinline__ bool qef_solve(float4 c, const float pts[27][3], const R f[27], const R g[27][3], int mode,
+                                          float svd_tol, R pc[4], R& err) {
+    return qef_core<R>(
+        c, [&](int i) { return f[i]; },
+        [&](int i, float p[3]) {
+            for (int a = 0; a < 3; a++) p[a] = pts[i][a];
+        },
+        [&](int i, R gg[3]) {
+            for (int a = 0; a < 3; a++) gg[a] = g[i][a];
+        },
+        mode, svd_tol, pc, err);
+}
+
+// Finite-difference (negated) gradient of sample i of a node's 27 (as fd_grads27)
+template <class R> __device__ __forceinline__ void fd_grad_at(const R f[27], int i, float cellsize, R g[3]) {
+    const float step = cellsize / 2.0f;
+    const R s1 = R(step), s2 = R(step * 2.0f);
+    const int t[3] = {i % 3, (i / 3) % 3, i / 9};
+    const int st[3] = {1, 3, 9};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (t[a] == 0) g[a] = (f[i] - f[i + st[a]]) / s1;
+        else if (t[a] == 2) g[a] = (f[i - st[a]] - f[i]) / s1;
+        else g[a] = (f[i - st[a]] - f[i + st[a]]) / s2;
+    }
+}
+
+// the leaf QEF of a node from its 27 sample values: positions (sample_point) and gradients
+// (fd_grads27) are computed where they are used, so only the values stay in registers
+template <class R>
+__device__ __forceinline__ bool qef_solve_samples(float4 c, const R f[27], int mode, float svd_tol, R pc[4], R& err) {
+    const float cellsize = __fmul_rn(2.0f, c.w);
+    return qef_core<R>(
+        c, [&](int i) { return f[i]; }, [&](int i, float p[3]) { sample_point(c, i, p); },
+        [&](int i, R gg[3]) { fd_grad_at<R>(f, i, cellsize, gg); }, mode, svd_tol, pc, err);
 }
 
 }  // namespace pisurf
