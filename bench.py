@@ -218,9 +218,10 @@ def copy_calibration(mib):
                       "launches": 5}), flush=True)
 
 
-def batch_leg(prec, dedupe, qef):
-    """24 frames of uneven size: one run_frames call (one context, buffers reused, the next frame
-    staged while the current one is meshed) vs. a loop of the plain path (a fresh context per
+def batch_leg(prec, dedupe, qef, lanes=(1, 2, 3, 4)):
+    """24 frames of uneven size: one run_frames call per number of frames in flight (1: one
+    context, buffers reused, the next frame staged while the current one is meshed; K > 1: K frames
+    meshed at once on K contexts and streams), vs. a loop of the plain path (a fresh context per
     frame, as the reference builds a new SurfReconstructor per frame)."""
     t = time.perf_counter()
     frames = synth.batch_frames()
@@ -234,13 +235,19 @@ def batch_leg(prec, dedupe, qef):
 
     ctx = capi.Context(0)
     ctx.set_params(rs[0], precision=prec, sample_dedupe=dedupe, qef_mode=qef)
-    ctx.run_frames(xs, rs)  # warm-up: buffers, code objects, run shapes of every frame's densities
-    tris[0] = 0
-    t = time.perf_counter()
-    ctx.run_frames(xs, rs, on_mesh)
-    dt_one = time.perf_counter() - t
+    per_lanes, tri_counts = {}, set()
+    for k in lanes:
+        ctx.run_frames(xs, rs, frames_in_flight=k)  # warm-up: buffers, lane contexts, code objects, run shapes
+        tris[0] = 0
+        t = time.perf_counter()
+        ctx.run_frames(xs, rs, on_mesh, frames_in_flight=k)
+        per_lanes[k] = len(xs) / (time.perf_counter() - t)
+        tri_counts.add(tris[0])
+        log("batch: %d frame(s) in flight: %.1f frames/s" % (k, per_lanes[k]))
     ctx.close()
-    tris_one = tris[0]
+    best = max(per_lanes, key=per_lanes.get)
+    dt_one = len(xs) / per_lanes[best]
+    tris_one = tris[0] if len(tri_counts) == 1 else -1
     t = time.perf_counter()
     tris_l = 0
     for x, r in zip(xs, rs):
@@ -252,7 +259,8 @@ def batch_leg(prec, dedupe, qef):
         c.close()
     dt_loop = time.perf_counter() - t
     return dict(frames=len(xs), particles=int(sum(len(x) for x in xs)),
-                run_frames_frames_per_s=round(len(xs) / dt_one, 3),
+                run_frames_frames_per_s=round(len(xs) / dt_one, 3), frames_in_flight=best,
+                run_frames_frames_per_s_by_frames_in_flight={str(k): round(v, 3) for k, v in per_lanes.items()},
                 loop_frames_per_s=round(len(xs) / dt_loop, 3), speedup=round(dt_loop / dt_one, 3),
                 same_triangle_count=bool(tris_one == tris_l), host_input=True)
 
@@ -276,7 +284,7 @@ def main():
     ap.add_argument("--depth-max", type=int, default=-1, help="octree depth override (-1: 12 for pour, else derived)")
     ap.add_argument("--batch", action="store_true",
                     help="also mesh the 24-frame batch (50 k - 3 M particles per frame, radii by frame) "
-                         "through one pisurf_run_frames call and through a loop of fresh contexts")
+                         "through pisurf_run_frames with 1-4 frames in flight and through a loop of fresh contexts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) 5 times, to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings, and exit")

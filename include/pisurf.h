@@ -61,7 +61,9 @@ typedef struct pisurf_params {
                                   1: host traversal in the reference's order (byte-identical OBJ) */
     int32_t sample_dedupe;     /* 1 (default): node samples shared by neighbouring nodes of a level
                                   (bitwise-identical coordinates) are evaluated once; 0: 27 per node */
-    int32_t reserved[3];
+    int32_t frames_in_flight;  /* pisurf_run_frames: frames meshed at once, each on a context and
+                                  stream of its own (0: the library's choice, 1: one after another) */
+    int32_t reserved[2];
 } pisurf_params;
 
 typedef struct pisurf_grid_info {
@@ -192,13 +194,21 @@ int pisurf_mesh_download(pisurf_ctx* ctx, double* verts, int32_t* tris);
 /* SurfReconstructor::Run (src/surface_reconstructor.cpp:409-479) on the uploaded particles */
 int pisurf_run_frame(pisurf_ctx* ctx, pisurf_tree_info* tree);
 /* runOurs' frame loop (src/main.cpp:330-405: one SurfReconstructor per frame in DATA_PATHES)
- * in one call on one context: device buffers are kept across frames and grow to the largest, and
- * frame k+1's particles are copied to the device (pinned staging, a second stream) while frame k
- * is meshed. Frame k has n[k] particles at xyz[k] (host, or device with on_device != 0) and radius
- * radius[k]; the other parameters are the context's. After each frame `on_mesh` receives the mesh
- * (vertices x,y,z as double, 1-based triangles, valid only during the call); `per_frame`
+ * in one call. Frame k has n[k] particles at xyz[k] (host, or device with on_device != 0) and
+ * radius radius[k]; the other parameters are the context's. After each frame `on_mesh` receives
+ * the mesh (vertices x,y,z as double, 1-based triangles, valid only during the call); `per_frame`
  * (nullable, nframes entries) receives each frame's statistics. Nothing carries over between
- * frames. Afterwards pisurf_get_stats reports the last frame. */
+ * frames, and every mesh is the one a fresh context gives.
+ * frames_in_flight = 1: the frames run one after another on this context; its device buffers are
+ * kept and grow to the largest frame, and frame k+1's particles are copied to the device (pinned
+ * staging, a second stream) while frame k is meshed.
+ * frames_in_flight = K > 1 (0 takes 2): K frames are meshed at once, frame k on lane k mod K;
+ * lane 0 is this context and the others are contexts (each with its own stream and buffers) that
+ * this context creates on first use and keeps. One lane's kernels fill the time another spends
+ * waiting for a count or launching small kernels. Callbacks come one at a time and in frame order,
+ * from the lanes' threads (lane 0's is the caller's). When a frame fails, the frames before it
+ * have been delivered, none after it is, and the error names the frame.
+ * Afterwards pisurf_get_stats reports the last frame meshed on this context. */
 typedef void (*pisurf_mesh_cb)(void* user, int32_t frame, const pisurf_tree_info* tree, const double* verts,
                                int64_t n_verts, const int32_t* tris, int64_t n_tris);
 int pisurf_run_frames(pisurf_ctx* ctx, int32_t nframes, const float* const* xyz, const int64_t* n,

@@ -34,7 +34,7 @@ class Params(C.Structure):
         ("use_ani", C.c_int32), ("use_poly6", C.c_int32), ("calc_normals", C.c_int32), ("oversample", C.c_int32),
         ("precision", C.c_int32), ("qef_mode", C.c_int32), ("qef_svd_tol", C.c_float),
         ("depth_max_override", C.c_int32), ("mesh_order", C.c_int32), ("sample_dedupe", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("frames_in_flight", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -121,12 +121,14 @@ class Context:
             raise PisurfError("%s failed (%d): %s" % (what, rc, self.L.pisurf_last_error(self.h).decode()))
 
     def set_params(self, radius, precision=FP32, use_ani=True, use_poly6=False, calc_normals=True, qef_mode=QEF_OFF,
-                   qef_svd_tol=1e-3, depth_max_override=0, smooth_factor=2.0, iso_factor=1.9, mesh_order=0, sample_dedupe=True):
+                   qef_svd_tol=1e-3, depth_max_override=0, smooth_factor=2.0, iso_factor=1.9, mesh_order=0, sample_dedupe=True,
+                   frames_in_flight=0):
         p = Params()
         self.L.pisurf_default_params(C.byref(p))
         p.radius = radius
         p.mesh_order = mesh_order
         p.sample_dedupe = int(sample_dedupe)
+        p.frames_in_flight = int(frames_in_flight)
         p.precision = precision
         p.use_ani = int(use_ani)
         p.use_poly6 = int(use_poly6)
@@ -290,10 +292,14 @@ class Context:
         for every frame. Arguments and result as run_frames."""
         return self.run_frames(frames, radii, on_mesh, entry="pisurf_run_batch")
 
-    def run_frames(self, frames, radii, on_mesh=None, entry="pisurf_run_frames"):
+    def run_frames(self, frames, radii, on_mesh=None, entry="pisurf_run_frames", frames_in_flight=None):
         """Mesh a batch of frames in one call. frames: list of (n, 3) float32 arrays;
-        on_mesh(k, verts (nv, 3) float64, tris (nt, 3) int32, tree dict) is called after each frame.
-        Returns the per-frame statistics."""
+        on_mesh(k, verts (nv, 3) float64, tris (nt, 3) int32, tree dict) is called after each frame,
+        in frame order. frames_in_flight (None: the context's parameter) sets how many frames are
+        meshed at once. Returns the per-frame statistics."""
+        if frames_in_flight is not None:
+            self.params.frames_in_flight = int(frames_in_flight)
+            self._c(self.L.pisurf_set_params(self.h, C.byref(self.params)), "set_params")
         nf = len(frames)
         frames = [np.ascontiguousarray(f, dtype=np.float32).reshape(-1, 3) for f in frames]
         ptrs = (C.c_void_p * max(nf, 1))(*[f.ctypes.data for f in frames])

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -219,6 +220,7 @@ struct pisurf_ctx {
     bool own_stream = true;       // false for the frame contexts of a batch (the batch's stream)
     void* scratch_ptr = nullptr;  // this device's copy of the no-allocation working array
     std::vector<pisurf_ctx*> sub;  // pisurf_run_batch: one context per frame, kept for reuse
+    std::vector<pisurf_ctx*> lanes;  // pisurf_run_frames: contexts of lanes 1 .. K - 1, kept for reuse
     // pisurf_run_batch work space: frame descriptors of the run queue, item offsets, items, counter
     DBuf bq_frames, bq_off, bq_items, bq_head;
     PinnedBuf bq_host;
@@ -287,6 +289,7 @@ struct pisurf_ctx {
     hipEvent_t wev = nullptr;
     PinnedBuf rb;
     hipError_t wait() { return stream_wait_on(stream, wev); }
+    HostRead hread() { return HostRead{wev, rb.p}; }
     hipError_t read(void* host, const void* dptr, size_t bytes) {
         if (bytes > kReadbackBytes) return hipErrorInvalidValue;
         hipError_t e;
@@ -298,6 +301,11 @@ struct pisurf_ctx {
 
     ~pisurf_ctx() {
         for (auto* b : sub) delete b;
+        for (auto* l : lanes) {
+            (void)hipSetDevice(l->device);
+            (void)l->wait();
+            delete l;
+        }
         if (wev) (void)hipEventDestroy(wev);
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -785,11 +793,14 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             HIPCHK(c, c->event_pair(e0, e1, &eslot));
             int64_t nheads = 0, nruns = 0;
             int kx = 0;
+            hipError_t herr = hipSuccess;
             launch_sample_dedupe_eval(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, L->child.as<int>(), skv, 3 * lb, nent,
                                       c->sd_head.as<int>(),
                                       c->sd_hpos.as<int>(), c->sd_rep.as<int>(), c->sd_hlist.as<int>(),
                                       c->sd_cwork.as<int>(), &nheads, &nruns, &kx, c->sd_vals.p, c->samples.p,
-                                      c->tmp.as<int>(), c->ctr(), c->trial_ctr(), e0, e1, c->stream);
+                                      c->tmp.as<int>(), c->ctr(), c->trial_ctr(), e0, e1, c->hread(), c->stream,
+                                      &herr);
+            HIPCHK(c, herr);
             c->host_sample_entries += (unsigned long long)nent;
             c->ev_kind[eslot] = nruns > 0 ? 1 : 0;
         }
@@ -816,10 +827,12 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             HIPCHK(c, c->sd_cwork.ensure(4 * ((size_t)k + 2) * 4));
             const int64_t need = std::max(radix_tmp_ints(k), scan_tmp_ints((int64_t)k + 1)) + 64;
             HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, need * 4)));
+            hipError_t herr = hipSuccess;
             launch_centers_runs(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), k,
                                 c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
                                 c->sd_val2.as<int>(), c->sd_cwork.as<int>(), c->tmp.as<int>(), c->samples.p, c->ctr(),
-                                c->trial_ctr(), c->stream);
+                                c->trial_ctr(), c->hread(), c->stream, &herr);
+            HIPCHK(c, herr);
         }
     }
     if (split_curv)
@@ -1459,6 +1472,8 @@ int sync_counters(pisurf_ctx* c) {
 // frame, its workgroups taking (frame, run) items from a device counter.
 
 constexpr bool kBatchEnabled = false;
+constexpr int kMaxFramesInFlight = 16;
+constexpr int kDefaultFramesInFlight = 2;
 
 pisurf_ctx* make_frame_ctx(pisurf_ctx* c) {
     pisurf_ctx* f = new (std::nothrow) pisurf_ctx();
@@ -1872,6 +1887,8 @@ int pisurf_set_params(pisurf_ctx* c, const pisurf_params* p) {
     if (p->depth_max_override < 0 || p->depth_max_override > 21) return c->fail(PISURF_E_INVALID, "bad depth override");
     if (p->mesh_order < 0 || p->mesh_order > 1) return c->fail(PISURF_E_INVALID, "bad mesh_order");
     if (p->sample_dedupe < 0 || p->sample_dedupe > 1) return c->fail(PISURF_E_INVALID, "bad sample_dedupe");
+    if (p->frames_in_flight < 0 || p->frames_in_flight > kMaxFramesInFlight)
+        return c->fail(PISURF_E_INVALID, "frames_in_flight must be in 0 .. 16");
     const bool geo = p->radius != c->prm.radius || p->use_ani != c->prm.use_ani || p->use_poly6 != c->prm.use_poly6 ||
                      p->precision != c->prm.precision;
     c->prm = *p;
@@ -2433,6 +2450,127 @@ int pisurf_run_frame(pisurf_ctx* c, pisurf_tree_info* tree) {
     API_END(c)
 }
 
+// pisurf_run_frames with K lanes: frame k is meshed on lane k mod K (lane 0 is c, on the calling
+// thread; the others are contexts of their own, each on a thread of its own for this call), and the
+// meshes are handed to the callback one at a time in frame order.
+static int run_frames_lanes(pisurf_ctx* c, int K, int32_t nframes, const float* const* xyz, const int64_t* n,
+                            const float* radius, int32_t on_device, pisurf_mesh_cb on_mesh, void* user,
+                            pisurf_stats* per_frame) {
+    API_BEGIN
+    (void)hipSetDevice(c->device);
+    while ((int)c->lanes.size() < K - 1) {
+        pisurf_ctx* l = nullptr;
+        const int rc = pisurf_create(&l, c->device);
+        if (rc) return c->fail(rc, "lane context creation failed");
+        c->lanes.push_back(l);
+    }
+    std::vector<pisurf_ctx*> lane(K);
+    lane[0] = c;
+    for (int i = 1; i < K; i++) lane[i] = c->lanes[i - 1];
+    const pisurf_params base = c->prm;
+    // one frame on lane L: its particles copied on the lane's stream (the frame's kernels follow
+    // in stream order, and the frame's own read-backs have completed the copy before the staging
+    // buffer is reused), then the whole frame
+    auto mesh_frame = [&](pisurf_ctx* L, int32_t k, pisurf_tree_info* ti) -> int {
+        pisurf_params p = base;
+        p.radius = radius[k];
+        int rc = pisurf_set_params(L, &p);
+        if (!rc) rc = pisurf_reset_stats(L);
+        if (rc) return rc;
+        L->n = n[k];
+        L->bbox_done = L->grid_done = L->aniso_done = L->normals_done = L->tree_done = L->consts_done = false;
+        if (n[k] > 0) {
+            const size_t b = (size_t)n[k] * 3 * sizeof(float);
+            HIPCHK(L, L->xyz.ensure(b));
+            if (on_device) {
+                HIPCHK(L, hipMemcpyAsync(L->xyz.p, xyz[k], b, hipMemcpyDeviceToDevice, L->stream));
+            } else {
+                HIPCHK(L, L->pin.ensure(b));
+                std::memcpy(L->pin.p, xyz[k], b);
+                HIPCHK(L, hipMemcpyAsync(L->xyz.p, L->pin.p, b, hipMemcpyHostToDevice, L->stream));
+            }
+            rc = pisurf_run_frame(L, ti);
+        } else {
+            L->mesh_v.clear();
+            L->mesh_t.clear();
+        }
+        if (!rc && per_frame) rc = sync_counters(L);
+        return rc;
+    };
+    std::mutex m;
+    std::condition_variable cv;
+    int32_t delivered = 0;  // frames handed to the callback
+    bool stop = false;      // a frame failed: no later frame is delivered
+    int fail_rc = PISURF_OK;
+    std::string fail_msg;
+    auto lane_frames = [&](int li) {
+        pisurf_ctx* L = lane[li];
+        if (li > 0) (void)hipSetDevice(c->device);
+        for (int32_t k = li; k < nframes; k += K) {
+            {
+                std::lock_guard<std::mutex> lk(m);
+                if (stop) return;
+            }
+            pisurf_tree_info ti;
+            std::memset(&ti, 0, sizeof(ti));
+            int rc;
+            try {
+                rc = mesh_frame(L, k, &ti);
+            } catch (const std::bad_alloc&) {
+                rc = L->fail(PISURF_E_OOM, "out of host memory");
+            } catch (...) {
+                rc = L->fail(PISURF_E_STATE, "unexpected exception");
+            }
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return stop || delivered == k; });
+            if (stop) return;
+            if (rc) {
+                fail_rc = rc;
+                fail_msg = L->err + " (frame " + std::to_string(k) + ")";
+                stop = true;
+                cv.notify_all();
+                return;
+            }
+            if (per_frame) per_frame[k] = L->st;
+            if (on_mesh)
+                on_mesh(user, k, &ti, L->mesh_v.data(), (int64_t)L->mesh_v.size() / 3, L->mesh_t.data(),
+                        (int64_t)L->mesh_t.size() / 3);
+            delivered = k + 1;
+            cv.notify_all();
+        }
+    };
+    // a lane that ends by an exception stops the others, which may be waiting for its frame
+    auto lane_main = [&](int li) {
+        try {
+            lane_frames(li);
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(m);
+            if (!fail_rc) {
+                fail_rc = PISURF_E_STATE;
+                fail_msg = "unexpected exception in lane " + std::to_string(li);
+            }
+            stop = true;
+            cv.notify_all();
+        }
+    };
+    struct Joiner {  // the lane threads are joined on every exit path
+        std::vector<std::thread> t;
+        ~Joiner() {
+            for (auto& x : t)
+                if (x.joinable()) x.join();
+        }
+    } threads;
+    for (int i = 1; i < K; i++) threads.t.emplace_back(lane_main, i);
+    lane_main(0);
+    for (auto& x : threads.t) x.join();
+    if (fail_rc) {
+        c->err = fail_msg;
+        return fail_rc;
+    }
+    return PISURF_OK;
+    API_END(c)
+}
+
 int pisurf_run_frames(pisurf_ctx* c, int32_t nframes, const float* const* xyz, const int64_t* n,
                       const float* radius, int32_t on_device, pisurf_mesh_cb on_mesh, void* user,
                       pisurf_stats* per_frame) {
@@ -2446,6 +2584,10 @@ int pisurf_run_frames(pisurf_ctx* c, int32_t nframes, const float* const* xyz, c
         if (!(radius[k] > 0.0f) || !std::isfinite(radius[k])) return c->fail(PISURF_E_INVALID, "radius must be > 0" + fk);
         nmax = std::max(nmax, n[k]);
     }
+    const int lanes_req = c->prm.frames_in_flight == 0 ? kDefaultFramesInFlight : c->prm.frames_in_flight;
+    if (std::min<int>(lanes_req, nframes) > 1)
+        return run_frames_lanes(c, std::min<int>(lanes_req, nframes), nframes, xyz, n, radius, on_device, on_mesh, user,
+                                per_frame);
     API_BEGIN
     (void)hipSetDevice(c->device);
     if (!c->cstream) HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));

@@ -281,18 +281,9 @@ SortedKV radix_sort_entries(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_
 // ---------------------------------------------------------------------------------------------
 // Small read-backs (counts, bounding box) in the frame's level loop. The wait polls an event
 // instead of blocking in the HIP runtime, so a count that is ready a few microseconds after the copy
-// is seen then, not after an interrupt wake-up; the copy lands in pinned memory so it is a plain
-// asynchronous transfer. Each host thread keeps its own staging buffer and event until the
-// process ends.
+// is seen then, not after an interrupt wake-up; the copy lands in the context's page-locked buffer
+// so it is a plain asynchronous transfer.
 // ---------------------------------------------------------------------------------------------
-namespace {
-struct Readback {
-    void* pin = nullptr;
-    hipEvent_t ev = nullptr;
-};
-thread_local Readback tl_readback;
-}  // namespace
-
 hipError_t stream_wait_on(hipStream_t s, hipEvent_t ev) {
     hipError_t e;
     if ((e = hipEventRecord(ev, s)) != hipSuccess) return e;
@@ -300,21 +291,12 @@ hipError_t stream_wait_on(hipStream_t s, hipEvent_t ev) {
     return e;
 }
 
-hipError_t stream_wait(hipStream_t s) {
-    Readback& r = tl_readback;
+hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s, const HostRead& rd) {
+    if (bytes > kReadbackBytes || !rd.pin || !rd.ev) return hipErrorInvalidValue;
     hipError_t e;
-    if (!r.ev && (e = hipEventCreateWithFlags(&r.ev, hipEventDisableTiming)) != hipSuccess) return e;
-    return stream_wait_on(s, r.ev);
-}
-
-hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s) {
-    if (bytes > kReadbackBytes) return hipErrorInvalidValue;
-    Readback& r = tl_readback;
-    hipError_t e;
-    if (!r.pin && (e = hipHostMalloc(&r.pin, kReadbackBytes, hipHostMallocDefault)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(r.pin, dptr, bytes, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-    if ((e = stream_wait(s)) != hipSuccess) return e;
-    std::memcpy(host, r.pin, bytes);
+    if ((e = hipMemcpyAsync(rd.pin, dptr, bytes, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = stream_wait_on(s, rd.ev)) != hipSuccess) return e;
+    std::memcpy(host, rd.pin, bytes);
     return hipSuccess;
 }
 

@@ -397,8 +397,10 @@ __global__ void __launch_bounds__(T) k_run_centers(GridDev g, Recs<R> rec, Field
 
 // Cell groups of nh points in cell-major order (key[hlist[h]] >> gshift is the cell group of point
 // h; hlist NULL: point h is entry h). cwork: flags | positions | group starts, run starts
-// ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of groups (one read-back).
-static int build_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp, hipStream_t s) {
+// ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of groups (one read-back; 0 with *err
+// set when the read fails).
+static int build_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp, const HostRead& rd,
+                        hipStream_t s, hipError_t* err) {
     if (nh <= 0) return 0;
     int* flag = cwork;
     int* pos = cwork + (nh + 2);
@@ -408,14 +410,19 @@ static int build_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* 
     exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
     k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
     int ng = 0;
-    (void)read_device(&ng, pos + nh, 4, s);
+    const hipError_t e = read_device(&ng, pos + nh, 4, s, rd);
+    if (e != hipSuccess) {
+        *err = e;
+        return 0;
+    }
     return ng;
 }
 
 // Runs of at most kx cells of one grid row over the ng groups built by build_groups: returns the
-// number of runs (one read-back) and in *rstart_out their starts (nruns + 1 entries in cwork).
+// number of runs (one read-back; 0 with *err set when the read fails) and in *rstart_out their
+// starts (nruns + 1 entries in cwork).
 static int build_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist, int nh, int ng, int kx, int* cwork,
-                            int* tmp, int** rstart_out, hipStream_t s) {
+                            int* tmp, int** rstart_out, const HostRead& rd, hipStream_t s, hipError_t* err) {
     int* flag = cwork;
     int* pos = cwork + (nh + 2);
     int* cstart = cwork + 2 * (int64_t)(nh + 2);
@@ -427,7 +434,11 @@ static int build_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist
     exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
     k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
     int nr = 0;
-    (void)read_device(&nr, pos + ng, 4, s);
+    const hipError_t e = read_device(&nr, pos + ng, 4, s, rd);
+    if (e != hipSuccess) {
+        *err = e;
+        return 0;
+    }
     return nr;
 }
 
@@ -885,7 +896,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
                                const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr,
-                               hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+                               hipEvent_t e0, hipEvent_t e1, const HostRead& rd, hipStream_t s, hipError_t* err) {
     *nheads_out = 0;
     *nruns_out = 0;
     *kx_out = 0;
@@ -898,9 +909,13 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
     k_head_list<<<nb, 256, 0, s>>>(rep, nent, block_off, hlist, hpos);
     int nh = 0;
-    (void)read_device(&nh, block_off + nb, 4, s);
+    const hipError_t e = read_device(&nh, block_off + nb, 4, s, rd);
+    if (e != hipSuccess) {
+        *err = e;
+        return;
+    }
     *nheads_out = nh;
-    const int ng = build_groups(kv, gshift, hlist, nh, cwork, tmp, s);
+    const int ng = build_groups(kv, gshift, hlist, nh, cwork, tmp, rd, s, err);
     int* rstart = nullptr;
     int nr = 0;
     (void)hipEventRecord(e0, s);
@@ -908,7 +923,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
         const RunShape sh = tuned_run_launch(
             prec, kRunSamples, (double)nh / ng, trial_ctr, s,
             [&](RunShape c) {
-                nr = build_run_starts(g, kv, gshift, hlist, nh, ng, c.kx, cwork, tmp, &rstart, s);
+                nr = build_run_starts(g, kv, gshift, hlist, nh, ng, c.kx, cwork, tmp, &rstart, rd, s, err);
                 return nr > 0;
             },
             [&](RunShape c, bool trial) {
@@ -939,7 +954,8 @@ void launch_centers(int prec, GridDev g, const void* ra, const void* rg, FieldCo
 
 void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
                          const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
-                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, hipStream_t s) {
+                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, const HostRead& rd,
+                         hipStream_t s, hipError_t* err) {
     if (k <= 0) return;
     const uint64_t groups = (uint64_t)(g.n[0] + 2) * (uint64_t)(g.n[1] + 2) * (uint64_t)(g.n[2] + 2);
     int bits = 1;
@@ -947,7 +963,7 @@ void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, Fi
     const int vbits = pack_vbits(bits, k);
     k_center_keys<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(g, lv, list, k, vbits, key, val);
     const SortedKV kv = radix_sort_entries(key, val, key2, val2, k, bits, vbits, tmp, s);
-    const int ng = build_groups(kv, 0, nullptr, (int)k, cwork, tmp, s);
+    const int ng = build_groups(kv, 0, nullptr, (int)k, cwork, tmp, rd, s, err);
     if (ng <= 0) return;
     int* rstart = nullptr;
     int nr = 0;
@@ -978,7 +994,7 @@ void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, Fi
     tuned_run_launch(
         prec, kRunCenters, (double)k / ng, trial_ctr, s,
         [&](RunShape c) {
-            nr = build_run_starts(g, kv, 0, nullptr, (int)k, ng, c.kx, cwork, tmp, &rstart, s);
+            nr = build_run_starts(g, kv, 0, nullptr, (int)k, ng, c.kx, cwork, tmp, &rstart, rd, s, err);
             return nr > 0;
         },
         launch);

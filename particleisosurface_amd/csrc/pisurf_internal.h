@@ -38,13 +38,16 @@ SortedKV radix_sort_entries(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_
                             hipStream_t s);
 
 // ---- read-backs ----
-// wait for the stream's work so far by polling an event (no driver sleep); stream_wait uses an
-// event of the calling thread (kept until the process ends), stream_wait_on the caller's event
-hipError_t stream_wait(hipStream_t s);
+// wait for the stream's work so far by polling the caller's event (no driver sleep)
 hipError_t stream_wait_on(hipStream_t s, hipEvent_t ev);
-// copy bytes (<= kReadbackBytes) of device memory to host after the stream's earlier work
+// copy bytes (<= kReadbackBytes) of device memory to host after the stream's earlier work, through
+// the caller's page-locked staging buffer and event (those of the context that owns the stream)
 constexpr size_t kReadbackBytes = 16384;
-hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s);
+struct HostRead {
+    hipEvent_t ev;
+    void* pin;  // kReadbackBytes of page-locked memory
+};
+hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s, const HostRead& rd);
 
 // ---- grid ----
 void launch_bbox(const float* xyz, int64_t n, float* partial /*>= 6*1024*/, float* out6, int* nan_count,
@@ -155,23 +158,25 @@ void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* l
 void launch_centers(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                     const int* list, const int* count, void* samples, Counters* ctr, hipStream_t s);
 // the same for k listed centres (count known on the host), grouped by hash cell and evaluated in
-// runs of cells; key / val / key2 / val2 hold k entries, cwork 4 (k + 2) ints
+// runs of cells; key / val / key2 / val2 hold k entries, cwork 4 (k + 2) ints. The counts of groups
+// and runs are read back through rd; a failed read sets *err (left untouched otherwise).
 void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                          const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
-                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, hipStream_t s);
+                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, const HostRead& rd,
+                         hipStream_t s, hipError_t* err);
 // sample deduplication: keys for the nent = 27 * listed entries (hash cell group, then the
 // lattice position modulo 2^lb per axis; the group starts at bit gshift = 3 lb), then (after the
 // key sort) heads, the distinct points grouped into runs of cells, one workgroup per run between
 // events e0 / e1, and the scatter to samples. head
 // holds 2 (nent / 256 + 2) ints, hpos / rep / hlist nent, cwork 4 (nent + 2); kx_out is the
-// number of cells of a row per run.
+// number of cells of a row per run. Counts are read back through rd; a failed read sets *err.
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
                                int lb, int vbits, uint64_t* key, int* val, hipStream_t s);
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                                const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr,
-                               hipEvent_t e0, hipEvent_t e1, hipStream_t s);
+                               hipEvent_t e0, hipEvent_t e1, const HostRead& rd, hipStream_t s, hipError_t* err);
 // Phase-split deduplicated sampling (the steps of launch_sample_dedupe_eval and
 // launch_centers_runs cut at each count the host reads), for sweeps over many frames:
 // heads of nent sorted entries, their count at heads_count(head, nent)

@@ -124,22 +124,26 @@ def test_mode_full_tree_and_mesh(mode):
         assert len(t) > (100 if mode == "no_normals" else 1000)
 
 
-def test_batch_24_frames():
-    """The 24-frame batch (compensated FP32, FP64 QEF) through one pisurf_run_frames call: every
-    frame's mesh and tree are bit-identical to a fresh context meshing that frame alone; the
-    smallest frame's cell table is bit-exact against the oracle and its field within the
-    compensated tolerance of the FP64 oracle at 50 k points."""
+@pytest.mark.parametrize("fif", [1, 4])
+def test_batch_24_frames(fif):
+    """The 24-frame batch (compensated FP32, FP64 QEF) through one pisurf_run_frames call, one
+    frame at a time and four at once: every frame's mesh and tree are bit-identical to a fresh
+    context meshing that frame alone, and the meshes arrive in frame order; the smallest frame's
+    cell table is bit-exact against the oracle and its field within the compensated tolerance of
+    the FP64 oracle at 50 k points."""
     frames = synth.batch_frames()
     got = {}
+    order = []
 
     def on_mesh(k, v, t, tree):
+        order.append(k)
         got[k] = (v, t, tree)
 
     kw = dict(precision=capi.FP32_KAHAN, qef_mode=capi.QEF_SVD64)
     ctx = capi.Context(0)
-    ctx.set_params(0.01, **kw)
+    ctx.set_params(0.01, frames_in_flight=fif, **kw)
     per = ctx.run_frames([f for f, _ in frames], [r for _, r in frames], on_mesh)
-    assert sorted(got) == list(range(24))
+    assert order == list(range(24))
     for k, (xyz, r) in enumerate(frames):
         v, t, tree = got[k]
         ref = gpu_context(xyz, r, **kw)

@@ -550,21 +550,25 @@ def test_sample_dedupe_is_bit_identical(case, prec):
     assert 0 < sa["sample_unique"] < sa["sample_evals"]
 
 
-def test_run_frames_matches_fresh_contexts():
-    """pisurf_run_frames (one context, buffers reused) meshes each frame exactly as a fresh context does, with
-    frames of different sizes and radii in one batch (nothing carries over), an empty frame
-    included."""
+@pytest.mark.parametrize("fif", [1, 3])
+def test_run_frames_matches_fresh_contexts(fif):
+    """pisurf_run_frames meshes each frame exactly as a fresh context does, with frames of different
+    sizes and radii in one batch (nothing carries over), an empty frame included: one after another
+    on one context (buffers reused), and three frames at once on three lanes, whose meshes still
+    arrive one at a time in frame order."""
     frames = [synth.dam_break(), synth.lattice_sphere(), (np.zeros((0, 3), np.float32), 0.004),
               synth.random_cloud(20000, r=0.004, extent=0.2), synth.dam_break()]
     got = {}
+    order = []
 
     def on_mesh(k, v, t, tree):
+        order.append(k)
         got[k] = (v, t, tree)
 
     ctx = capi.Context(0)
     ctx.set_params(0.01, precision=capi.FP32_KAHAN)
-    per = ctx.run_frames([f for f, _ in frames], [r for _, r in frames], on_mesh)
-    assert sorted(got) == list(range(len(frames)))
+    per = ctx.run_frames([f for f, _ in frames], [r for _, r in frames], on_mesh, frames_in_flight=fif)
+    assert order == list(range(len(frames)))
     for k, (xyz, r) in enumerate(frames):
         v, t, tree = got[k]
         if len(xyz) == 0:
@@ -576,6 +580,31 @@ def test_run_frames_matches_fresh_contexts():
         assert tree == info, k
         assert np.array_equal(v, v2) and np.array_equal(t, t2), k
         assert per[k]["field_evals"] == ref.stats()["field_evals"], k
+
+
+def test_run_frames_lanes_stop_at_failed_frame():
+    """Frames on two lanes with a NaN particle in frame 2: frames 0 and 1 are delivered, no later
+    frame is, and the error names frame 2; the same context then meshes a batch again, each frame as
+    a fresh context does."""
+    bad = synth.dam_break()[0].copy()
+    bad[7, 2] = np.nan
+    frames = [synth.dam_break(), synth.lattice_sphere(), (bad, 0.01), synth.dam_break(), synth.lattice_sphere()]
+    order = []
+    ctx = capi.Context(0)
+    ctx.set_params(0.01, precision=capi.FP32_KAHAN, frames_in_flight=2)
+    with pytest.raises(capi.PisurfError, match=r"NaN.*\(frame 2\)"):
+        ctx.run_frames([f for f, _ in frames], [r for _, r in frames], lambda k, v, t, tree: order.append(k))
+    assert order == [0, 1]
+    got = {}
+    ok = [frames[1], frames[0], frames[4]]
+    ctx.run_frames([f for f, _ in ok], [r for _, r in ok], lambda k, v, t, tree: got.__setitem__(k, (v, t)))
+    assert sorted(got) == [0, 1, 2]
+    for k, (xyz, r) in enumerate(ok):
+        ref = gpu_context(xyz, r, precision=capi.FP32_KAHAN)
+        ref.run_frame()
+        v2, t2 = ref.mesh()
+        assert np.array_equal(got[k][0], v2) and np.array_equal(got[k][1], t2), k
+        ref.close()
 
 
 def _uniform_points(box, dims, idx):
