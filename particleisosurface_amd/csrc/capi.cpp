@@ -288,16 +288,12 @@ struct pisurf_ctx {
     // made from different threads (one at a time) create no per-thread HIP objects
     hipEvent_t wev = nullptr;
     PinnedBuf rb;
-    hipError_t wait() { return stream_wait_on(stream, wev); }
-    HostRead hread() { return HostRead{wev, rb.p}; }
-    hipError_t read(void* host, const void* dptr, size_t bytes) {
-        if (bytes > kReadbackBytes) return hipErrorInvalidValue;
-        hipError_t e;
-        if ((e = hipMemcpyAsync(rb.p, dptr, bytes, hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
-        if ((e = wait()) != hipSuccess) return e;
-        std::memcpy(host, rb.p, bytes);
-        return hipSuccess;
-    }
+    // counts and waits on the context's stream go through its read-back slot (host-mapped memory
+    // written by a small kernel, polled by the host)
+    HostRead* hr = nullptr;
+    hipError_t wait() { return hr ? stream_fence(stream, hr) : stream_wait_on(stream, wev); }
+    HostRead* hread() { return hr; }
+    hipError_t read(void* host, const void* dptr, size_t bytes) { return read_device(host, dptr, bytes, stream, hr); }
 
     ~pisurf_ctx() {
         for (auto* b : sub) delete b;
@@ -306,6 +302,7 @@ struct pisurf_ctx {
             (void)l->wait();
             delete l;
         }
+        host_read_release(hr);
         if (wev) (void)hipEventDestroy(wev);
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -1482,7 +1479,7 @@ pisurf_ctx* make_frame_ctx(pisurf_ctx* c) {
     f->stream = c->stream;
     f->own_stream = false;
     if (hipEventCreateWithFlags(&f->wev, hipEventDisableTiming) != hipSuccess || f->rb.ensure(kReadbackBytes) != hipSuccess ||
-        f->counters.ensure(kCtrStripes * sizeof(Counters)) != hipSuccess) {
+        f->counters.ensure(kCtrStripes * sizeof(Counters)) != hipSuccess || !(f->hr = host_read_acquire())) {
         delete f;
         return nullptr;
     }
@@ -1832,7 +1829,8 @@ int pisurf_create(pisurf_ctx** out, int device) {
         delete c;
         return PISURF_E_HIP;
     }
-    if (hipEventCreateWithFlags(&c->wev, hipEventDisableTiming) != hipSuccess || c->rb.ensure(kReadbackBytes) != hipSuccess) {
+    if (hipEventCreateWithFlags(&c->wev, hipEventDisableTiming) != hipSuccess || c->rb.ensure(kReadbackBytes) != hipSuccess ||
+        !(c->hr = host_read_acquire())) {
         delete c;
         return PISURF_E_HIP;
     }

@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(T) k_run_centers(GridDev g, Recs<R> rec, Field
 // h; hlist NULL: point h is entry h). cwork: flags | positions | group starts, run starts
 // ((nh + 2) + (nh + 2) + 2 (nh + 2) ints). Returns the number of groups (one read-back; 0 with *err
 // set when the read fails).
-static int build_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp, const HostRead& rd,
+static int build_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* cwork, int* tmp, HostRead* rd,
                         hipStream_t s, hipError_t* err) {
     if (nh <= 0) return 0;
     int* flag = cwork;
@@ -422,7 +422,7 @@ static int build_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* 
 // number of runs (one read-back; 0 with *err set when the read fails) and in *rstart_out their
 // starts (nruns + 1 entries in cwork).
 static int build_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist, int nh, int ng, int kx, int* cwork,
-                            int* tmp, int** rstart_out, const HostRead& rd, hipStream_t s, hipError_t* err) {
+                            int* tmp, int** rstart_out, HostRead* rd, hipStream_t s, hipError_t* err) {
     int* flag = cwork;
     int* pos = cwork + (nh + 2);
     int* cstart = cwork + 2 * (int64_t)(nh + 2);
@@ -896,7 +896,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
                                const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr,
-                               hipEvent_t e0, hipEvent_t e1, const HostRead& rd, hipStream_t s, hipError_t* err) {
+                               hipEvent_t e0, hipEvent_t e1, HostRead* rd, hipStream_t s, hipError_t* err) {
     *nheads_out = 0;
     *nruns_out = 0;
     *kx_out = 0;
@@ -954,7 +954,7 @@ void launch_centers(int prec, GridDev g, const void* ra, const void* rg, FieldCo
 
 void launch_centers_runs(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
                          const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
-                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, const HostRead& rd,
+                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, HostRead* rd,
                          hipStream_t s, hipError_t* err) {
     if (k <= 0) return;
     const uint64_t groups = (uint64_t)(g.n[0] + 2) * (uint64_t)(g.n[1] + 2) * (uint64_t)(g.n[2] + 2);

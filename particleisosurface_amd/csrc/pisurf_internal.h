@@ -38,16 +38,24 @@ SortedKV radix_sort_entries(uint64_t* k0, int* v0, uint64_t* k1, int* v1, int64_
                             hipStream_t s);
 
 // ---- read-backs ----
-// wait for the stream's work so far by polling the caller's event (no driver sleep)
+// wait for the stream's work so far by polling an event (no driver sleep)
 hipError_t stream_wait_on(hipStream_t s, hipEvent_t ev);
-// copy bytes (<= kReadbackBytes) of device memory to host after the stream's earlier work, through
-// the caller's page-locked staging buffer and event (those of the context that owns the stream)
+// A read-back slot: kReadbackBytes of host memory mapped into the device's address space, then a
+// sequence word. A one-workgroup kernel on the stream copies the bytes into the slot and then
+// writes the next sequence number; the host polls that word. Slots come from a process-wide pool
+// (a context holds one; they are never freed, only returned to the pool).
 constexpr size_t kReadbackBytes = 16384;
 struct HostRead {
-    hipEvent_t ev;
-    void* pin;  // kReadbackBytes of page-locked memory
+    char* host;     // host address of the slot
+    char* dev;      // its device address
+    uint32_t next;  // last sequence number written to the slot
 };
-hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s, const HostRead& rd);
+HostRead* host_read_acquire();  // nullptr when mapped memory cannot be allocated
+void host_read_release(HostRead* rd);
+// copy bytes (<= kReadbackBytes) of device memory to host after the stream's earlier work
+hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s, HostRead* rd);
+// wait for the stream's earlier work
+hipError_t stream_fence(hipStream_t s, HostRead* rd);
 
 // ---- grid ----
 void launch_bbox(const float* xyz, int64_t n, float* partial /*>= 6*1024*/, float* out6, int* nan_count,
@@ -162,7 +170,7 @@ void launch_centers(int prec, GridDev g, const void* rec_a, const void* rec_g, F
 // and runs are read back through rd; a failed read sets *err (left untouched otherwise).
 void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                          const int* list, int64_t k, uint64_t* key, int* val, uint64_t* key2, int* val2, int* cwork,
-                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, const HostRead& rd,
+                         int* tmp, void* samples, Counters* ctr, Counters* trial_ctr, HostRead* rd,
                          hipStream_t s, hipError_t* err);
 // sample deduplication: keys for the nent = 27 * listed entries (hash cell group, then the
 // lattice position modulo 2^lb per axis; the group starts at bit gshift = 3 lb), then (after the
@@ -176,7 +184,7 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const voi
                                const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
                                int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
                                int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr,
-                               hipEvent_t e0, hipEvent_t e1, const HostRead& rd, hipStream_t s, hipError_t* err);
+                               hipEvent_t e0, hipEvent_t e1, HostRead* rd, hipStream_t s, hipError_t* err);
 // Phase-split deduplicated sampling (the steps of launch_sample_dedupe_eval and
 // launch_centers_runs cut at each count the host reads), for sweeps over many frames:
 // heads of nent sorted entries, their count at heads_count(head, nent)
