@@ -271,7 +271,7 @@ struct pisurf_ctx {
     hipEvent_t eb0 = nullptr, eb1 = nullptr;
     std::vector<hipEvent_t> ev_chunk;  // ends of the bisection chunks
     // device extraction workspace
-    DBuf eb[29];
+    DBuf eb[30];
     // mesh
     PinnedVec<double> mesh_v;
     PinnedVec<int32_t> mesh_t;
@@ -1171,35 +1171,34 @@ int do_extract_device(pisurf_ctx* c) {
     int64_t nleaf;
     if ((rc = read_int(c, E[1].as<int>() + T, nleaf))) return rc;
     const int* lidx = E[1].as<int>();
-    // corner keys
-    const int64_t nent_all = 8 * nleaf;
+    // distinct corner points: per tile of 128 nodes of a level, one record per point (its key and
+    // the tile's leaves around it); then the records sorted by key
+    const int64_t nrec_max = 8 * nleaf;
     HIPCHK(c, E[2].ensure(std::max<int64_t>(nleaf, 1) * sizeof(double4)));
-    HIPCHK(c, E[3].ensure(std::max<int64_t>(nent_all, 1) * 8));
-    HIPCHK(c, E[4].ensure(std::max<int64_t>(nent_all, 1) * 4));
-    HIPCHK(c, E[5].ensure(std::max<int64_t>(nent_all, 1) * 8));
-    HIPCHK(c, E[6].ensure(std::max<int64_t>(nent_all, 1) * 4));
-    HIPCHK(c, E[7].ensure(64 * 128));
-    HIPCHK(c, hipMemsetAsync(E[7].p, 0, 64 * 128, s));
+    HIPCHK(c, E[3].ensure(std::max<int64_t>(nrec_max, 1) * 8));
+    HIPCHK(c, E[4].ensure(std::max<int64_t>(nrec_max, 1) * 4));
+    HIPCHK(c, E[5].ensure(std::max<int64_t>(nrec_max, 1) * 8));
+    HIPCHK(c, E[6].ensure(std::max<int64_t>(nrec_max, 1) * 4));
+    HIPCHK(c, E[29].ensure(std::max<int64_t>(nrec_max, 1) * 8 * 4));
+    HIPCHK(c, E[7].ensure(256));
+    launch_fill_i32(E[7].as<int>(), 1, 0, s);
     double4* leaf_node = E[2].as<double4>();
     HIPCHK(c, E[28].ensure(std::max<int64_t>(nleaf, 1)));
-    // corners outside the finest lattice carry all-ones keys and sort after the nent valid ones
-    const int ckbits = std::min(64, 3 * tv.dp + 1);
-    const int cvbits = pack_vbits(ckbits, nent_all);
+    const int ckbits = std::max(1, 3 * tv.dp);
+    const int cvbits = pack_vbits(ckbits, nrec_max);
     for (int L = 0; L < tv.nlev; L++)
-        launch_leaf_emit(c->levels[L]->view(), tv.off[L], lidx, tv.dp, ckbits, cvbits, leaf_node, E[28].as<uint8_t>(),
-                         E[3].as<uint64_t>(), E[4].as<int>(), E[7].as<int>(), s);
-    if ((rc = ensure_tmp(c, nent_all + 1))) return rc;
-    const SortedKV ckv = radix_sort_entries(E[3].as<uint64_t>(), E[4].as<int>(), E[5].as<uint64_t>(), E[6].as<int>(),
-                                            nent_all, ckbits, cvbits, c->tmp.as<int>(), s);
+        launch_corner_tiles(c->levels[L]->view(), tv.off[L], lidx, tv.dp, cvbits, leaf_node, E[28].as<uint8_t>(),
+                            E[3].as<uint64_t>(), E[4].as<int>(), E[29].as<int>(), E[7].as<int>(), s);
     HIPCHK(c, hipGetLastError());
     int64_t nent = 0;
-    {
-        int st[64 * 32];
-        HIPCHK(c, c->read(st, E[7].p, sizeof(st)));
-        for (int k = 0; k < 64; k++) nent += st[32 * k];
-    }
+    if ((rc = read_int(c, E[7].as<int>(), nent))) return rc;
+    if (nent > nrec_max) return c->fail(PISURF_E_STATE, "corner records exceed their bound");
+    if ((rc = ensure_tmp(c, nent + 1))) return rc;
+    const SortedKV ckv = radix_sort_entries(E[3].as<uint64_t>(), E[4].as<int>(), E[5].as<uint64_t>(), E[6].as<int>(),
+                                            nent, ckbits, cvbits, c->tmp.as<int>(), s);
+    HIPCHK(c, hipGetLastError());
     // points -> dual cells
-    // distinct points: run heads of the sorted corner keys, one thread each
+    // distinct points: run heads of the sorted records, one thread each
     HIPCHK(c, E[25].ensure(std::max<int64_t>(nent, 1) * 4));
     HIPCHK(c, E[9].ensure(head_list_tmp_ints(nent) * 4));
     launch_head_list(ckv, nent, E[25].as<int>(), E[9].as<int>(), c->tmp.as<int>(), s);
@@ -1208,8 +1207,8 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[26].ensure((npts + 1) * 4));
     HIPCHK(c, E[27].ensure((npts + 1) * 4));
     HIPCHK(c, E[24].ensure(std::max<int64_t>(npts, 1) * 9 * 4));
-    launch_points(tv, lidx, E[28].as<uint8_t>(), ckv, nent, E[25].as<int>(), npts, E[26].as<int>(), E[24].as<int>(),
-                  s);
+    launch_points(tv, lidx, E[28].as<uint8_t>(), ckv, nent, E[29].as<int>(), E[25].as<int>(), npts, E[26].as<int>(),
+                  E[24].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[26].as<int>() + npts, 0, 4, s));
     exclusive_scan_i32(E[26].as<int>(), E[27].as<int>(), npts + 1, c->tmp.as<int>(), s);
     int64_t nd;

@@ -45,23 +45,53 @@ __global__ void k_leaf_flag(NodeLevel lv, int64_t off, int* __restrict__ flag) {
     flag[off + i] = is_leaf_type(lv.type[i]) ? 1 : 0;
 }
 
-// leaf records and the 8 corner keys of every leaf (corners on the root boundary get the
-// sentinel key ~0 and sort last; valid keys use 3 * dp bits)
-__global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp, int kbits, int vbits,
-                            double4* __restrict__ leaf_node, uint8_t* __restrict__ leaf_pos,
-                            uint64_t* __restrict__ ckey, int* __restrict__ cval, int* __restrict__ n_valid) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int valid = 0;
+// The distinct corner points of the leaves, in two steps. Each workgroup takes a tile of 128
+// consecutive nodes of one level (a level's nodes are in Morton order, so a tile covers a compact
+// piece of space and its leaves share most corners) and merges the tile's corner keys in an LDS
+// hash table: one record per distinct point of the tile, holding its Morton key and, for each of
+// the point's 8 octants, the tile's leaf in that octant (-1 if none). A point shared by several
+// tiles gets one record in each; the global sort of the records' keys then groups them, and the
+// point pass merges a point's records. A leaf with P as corner c lies in octant c ^ 7 of P, and no
+// two leaves lie in the same octant of P, so the merge is a plain union.
+//
+// Keys: Morton interleave of the finest-level corner coordinates (3 dp bits); corners on the root
+// boundary are not points of the traversal and are skipped. Records are numbered by a global
+// counter (tiles claim consecutive ranges), so their order varies from run to run; every result
+// taken from them does not.
+constexpr int kCtT = 128;               // nodes per tile
+constexpr int kCtKeys = 8 * kCtT;       // corner keys per tile at most
+constexpr int kCtSlots = 2 * kCtKeys;   // hash slots: load factor at most 1/2
+constexpr int kCtSlotBits = 11;
+static_assert((1 << kCtSlotBits) == kCtSlots, "slot bits");
+constexpr uint64_t kCtEmpty = ~0ull;     // above every key (3 dp <= 63 bits)
+
+__global__ void __launch_bounds__(kCtT) k_corner_tiles(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp,
+                                                       int vbits, double4* __restrict__ leaf_node,
+                                                       uint8_t* __restrict__ leaf_pos, uint64_t* __restrict__ rkey,
+                                                       int* __restrict__ rval, int* __restrict__ rleaf,
+                                                       int* __restrict__ rcount) {
+    __shared__ uint64_t tab[kCtSlots];
+    __shared__ short uid[kCtSlots];  // record of a slot within the tile (< kCtKeys)
+    __shared__ uint64_t ukey[kCtKeys];
+    __shared__ int un8[kCtKeys * 8];
+    __shared__ int wsum[kCtT / kWave];
+    __shared__ int gbase;
+    const int t = threadIdx.x;
+    for (int k = t; k < kCtSlots; k += kCtT) tab[k] = kCtEmpty;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * kCtT + t;
+    int L = -1;
+    int slot[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) slot[c] = -1;
     if (i < lv.n && is_leaf_type(lv.type[i])) {
-        const int L = lidx[off + i];
+        L = lidx[off + i];
         const double4 nd = lv.node[i];
         leaf_node[L] = nd;
         leaf_pos[L] = nd.w > 0.0 ? 1 : 0;  // sign of the leaf value, read by the point pass
         const int4 p = lv.ip[i];
         const int sh = dp - p.w;
         const int64_t hi = ((int64_t)1 << dp) - 1;
-        const uint64_t kmax = kbits >= 64 ? ~0ull : ((1ull << kbits) - 1);
-        // the two coordinates per axis of the 8 corners, and their spread bits
         int64_t cx[2], cy[2], cz[2];
         uint64_t mx[2], my[2], mz[2];
 #pragma unroll
@@ -77,22 +107,66 @@ __global__ void k_leaf_emit(NodeLevel lv, int64_t off, const int* __restrict__ l
         for (int c = 0; c < 8; c++) {
             const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
             const int64_t X = cx[ox], Y = cy[oy], Z = cz[oz];
-            const bool in = X >= 1 && X <= hi && Y >= 1 && Y <= hi && Z >= 1 && Z <= hi;
-            const uint64_t k = in ? (mx[ox] | my[oy] | mz[oz]) : kmax;
-            const int64_t e = 8 * (int64_t)L + c;
-            if (vbits) {
-                ckey[e] = (k << vbits) | (uint64_t)e;
-            } else {
-                ckey[e] = k;
-                cval[e] = (int)e;
+            if (!(X >= 1 && X <= hi && Y >= 1 && Y <= hi && Z >= 1 && Z <= hi)) continue;
+            const uint64_t k = mx[ox] | my[oy] | mz[oz];
+            // at most kCtKeys keys in kCtSlots slots: a free slot is always reached
+            unsigned h = (unsigned)((k * 0x9E3779B97F4A7C15ull) >> (64 - kCtSlotBits));
+            for (;;) {
+                const uint64_t prev = atomicCAS((unsigned long long*)&tab[h], (unsigned long long)kCtEmpty,
+                                                (unsigned long long)k);
+                if (prev == kCtEmpty || prev == k) break;
+                h = (h + 1) & (kCtSlots - 1);
             }
-            valid += in;
+            slot[c] = (int)h;
         }
     }
+    __syncthreads();
+    // number the occupied slots in slot order
+    constexpr int per = kCtSlots / kCtT;
+    int cnt = 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) valid += __shfl_xor(valid, o, kWave);
-    // striped like the event counters: 64 copies, 128 bytes apart
-    if ((threadIdx.x & 63) == 0 && valid) atomicAdd(n_valid + 32 * (blockIdx.x & 63), valid);
+    for (int k = 0; k < per; k++) cnt += tab[t * per + k] != kCtEmpty ? 1 : 0;
+    const int lane = t & (kWave - 1), w = t / kWave;
+    const int incl = wave_incl_scan(cnt, lane);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int u = incl - cnt, total = 0;
+#pragma unroll
+    for (int k = 0; k < kCtT / kWave; k++) {
+        if (k < w) u += wsum[k];
+        total += wsum[k];
+    }
+#pragma unroll
+    for (int k = 0; k < per; k++) {
+        const int sl = t * per + k;
+        const uint64_t kk = tab[sl];
+        if (kk != kCtEmpty) {
+            uid[sl] = (short)u;
+            ukey[u] = kk;
+            u++;
+        }
+    }
+    for (int e = t; e < total * 8; e += kCtT) un8[e] = -1;
+    if (t == 0 && total > 0) gbase = atomicAdd(rcount, total);
+    __syncthreads();
+    if (L >= 0) {
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            if (slot[c] >= 0) un8[uid[slot[c]] * 8 + (c ^ 7)] = L;
+    }
+    __syncthreads();
+    if (total == 0) return;
+    const int64_t gb = gbase;
+    for (int k = t; k < total; k += kCtT) {
+        const int64_t r = gb + k;
+        if (vbits) {
+            rkey[r] = (ukey[k] << vbits) | (uint64_t)r;
+        } else {
+            rkey[r] = ukey[k];
+            rval[r] = (int)r;
+        }
+    }
+    for (int e = t; e < total * 8; e += kCtT) rleaf[gb * 8 + e] = un8[e];
 }
 
 // leaf containing the finest cell q, descending from node i of level L (the root: 0, 0)
@@ -108,15 +182,24 @@ __device__ __forceinline__ int locate_from(const TreeView& tv, const int* lidx, 
     return -1;
 }
 
-// the 8 leaves around the point whose corner entries start at sorted position j
+// the 8 leaves around the point whose records start at sorted position j (at most one record per
+// tile, and a point lies in at most 8 tiles' leaves, so its run is at most 8 long)
 __device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx, const SortedKV& kv, int64_t nent,
-                                             int64_t j, int n[8]) {
+                                             const int* rleaf, int64_t j, int n[8]) {
     const uint64_t key = kv.key(j);
 #pragma unroll
     for (int o = 0; o < 8; o++) n[o] = -1;
     for (int64_t k = j; k < nent && k < j + 8 && kv.key(k) == key; k++) {
-        const int v = kv.val(k);
-        n[(v & 7) ^ 7] = v >> 3;  // a leaf with P as corner c lies in octant c ^ 7 of P
+        const int64_t r = kv.val(k);
+        const int4 a = ((const int4*)rleaf)[2 * r], b = ((const int4*)rleaf)[2 * r + 1];
+        n[0] = max(n[0], a.x);
+        n[1] = max(n[1], a.y);
+        n[2] = max(n[2], a.z);
+        n[3] = max(n[3], a.w);
+        n[4] = max(n[4], b.x);
+        n[5] = max(n[5], b.y);
+        n[6] = max(n[6], b.z);
+        n[7] = max(n[7], b.w);
     }
     const int px = (int)morton_compact(key), py = (int)morton_compact(key >> 1), pz = (int)morton_compact(key >> 2);
     bool missing = false;
@@ -214,12 +297,12 @@ __device__ __forceinline__ int sign_index(const uint8_t* leaf_pos, const int n[8
 // is non-trivial and stage the cell (8 leaves in case order, sign configuration) at slot t;
 // k_points_compact then packs the staged cells
 __global__ void k_points(TreeView tv, const int* __restrict__ lidx, const uint8_t* __restrict__ leaf_pos,
-                         SortedKV kv, int64_t nent, const int* __restrict__ heads, int64_t npts,
-                         int* __restrict__ dflag, int* __restrict__ stage) {
+                         SortedKV kv, int64_t nent, const int* __restrict__ rleaf, const int* __restrict__ heads,
+                         int64_t npts, int* __restrict__ dflag, int* __restrict__ stage) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npts) return;
     int n[8];
-    gather_point(tv, lidx, kv, nent, heads[t], n);
+    gather_point(tv, lidx, kv, nent, rleaf, heads[t], n);
     const int s = sign_index(leaf_pos, n);
     const uint8_t code = d_kDmcCase[s];
     const bool keep = code != 0 && code != 255;
@@ -427,11 +510,11 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s) {
     if (lv.n > 0) k_leaf_flag<<<nblk(lv.n), 256, 0, s>>>(lv, off, flag);
 }
-void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, int kbits, int vbits, double4* leaf_node,
-                      uint8_t* leaf_pos, uint64_t* ckey, int* cval, int* n_valid, hipStream_t s) {
+void launch_corner_tiles(NodeLevel lv, int64_t off, const int* lidx, int dp, int vbits, double4* leaf_node,
+                         uint8_t* leaf_pos, uint64_t* rkey, int* rval, int* rleaf, int* rcount, hipStream_t s) {
     if (lv.n > 0)
-        k_leaf_emit<<<nblk(lv.n), 256, 0, s>>>(lv, off, lidx, dp, kbits, vbits, leaf_node, leaf_pos, ckey, cval,
-                                               n_valid);
+        k_corner_tiles<<<(unsigned)((lv.n + kCtT - 1) / kCtT), kCtT, 0, s>>>(lv, off, lidx, dp, vbits, leaf_node,
+                                                                           leaf_pos, rkey, rval, rleaf, rcount);
 }
 void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned* table, int64_t tsize, int* slot, int* first,
                         hipStream_t s) {
@@ -462,8 +545,9 @@ void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_t
     k_head_write<<<(unsigned)nb, kHeadT, 0, s>>>(kv, n, tile_off, list);
 }
 void launch_points(const TreeView& tv, const int* lidx, const uint8_t* leaf_pos, SortedKV kv, int64_t nent,
-                   const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s) {
-    if (npts > 0) k_points<<<nblk(npts), 256, 0, s>>>(tv, lidx, leaf_pos, kv, nent, heads, npts, dflag, stage);
+                   const int* rleaf, const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s) {
+    if (npts > 0)
+        k_points<<<nblk(npts), 256, 0, s>>>(tv, lidx, leaf_pos, kv, nent, rleaf, heads, npts, dflag, stage);
 }
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s) {

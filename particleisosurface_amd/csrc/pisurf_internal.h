@@ -250,12 +250,12 @@ struct TreeView {
     const int* child[kMaxLevels];
 };
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
-// corner keys: Morton code of the corner (kbits bits; all ones outside the finest lattice), packed
-// with the entry index when vbits > 0
-// leaf_node[L] = vertex and value of leaf L, leaf_pos[L] = (value > 0)
-void launch_leaf_emit(NodeLevel lv, int64_t off, const int* lidx, int dp, int kbits, int vbits, double4* leaf_node,
-                      uint8_t* leaf_pos, uint64_t* ckey, int* cval, int* n_valid /* 64 stripes, 32 ints apart */,
-                      hipStream_t s);
+// leaf_node[L] = vertex and value of leaf L, leaf_pos[L] = (value > 0); and the distinct corner
+// points of every tile of 128 nodes of the level: record r (claimed through *rcount) has the
+// point's Morton key (3 dp bits) in rkey[r], packed with r when vbits > 0 (else r in rval[r]), and
+// in rleaf[8 r + o] the tile's leaf in octant o of the point, or -1. At most 8 records per leaf.
+void launch_corner_tiles(NodeLevel lv, int64_t off, const int* lidx, int dp, int vbits, double4* leaf_node,
+                         uint8_t* leaf_pos, uint64_t* rkey, int* rval, int* rleaf, int* rcount, hipStream_t s);
 // first[i] = 1 when i is the smallest index with key[i] (open-addressing table of tsize slots, a
 // power of two above the number of distinct keys, with room to spare; slot[i] = slot of key[i],
 // table[slot] = smallest index of the key; n < 2^32 - 1)
@@ -264,10 +264,11 @@ void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned* table, int64_t
 void launch_vertex_ids_hashed(const int* slot, const unsigned* table, const int* rank, int64_t n, const int* vend,
                               const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s);
 void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s);
-// point t (first sorted entry heads[t]): dflag[t] = 1 when its dual cell is non-trivial, the cell
-// goes to stage[9 t ..]
+// point t (first sorted record heads[t]; the run of its key's records, rleaf as written by
+// launch_corner_tiles): dflag[t] = 1 when its dual cell is non-trivial, the cell goes to
+// stage[9 t ..]
 void launch_points(const TreeView& tv, const int* lidx, const uint8_t* leaf_pos, SortedKV kv, int64_t nent,
-                   const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s);
+                   const int* rleaf, const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s);
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s);
 void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s);
