@@ -271,7 +271,7 @@ struct pisurf_ctx {
     hipEvent_t eb0 = nullptr, eb1 = nullptr;
     std::vector<hipEvent_t> ev_chunk;  // ends of the bisection chunks
     // device extraction workspace
-    DBuf eb[30];
+    DBuf eb[32];
     // mesh
     PinnedVec<double> mesh_v;
     PinnedVec<int32_t> mesh_t;
@@ -1172,27 +1172,37 @@ int do_extract_device(pisurf_ctx* c) {
     if ((rc = read_int(c, E[1].as<int>() + T, nleaf))) return rc;
     const int* lidx = E[1].as<int>();
     // distinct corner points: per tile of 128 nodes of a level, one record per point (its key and
-    // the tile's leaves around it); then the records sorted by key
-    const int64_t nrec_max = 8 * nleaf;
+    // the tile's leaves around it) in the tile's own region; then the keys of all records in one
+    // array, sorted
+    int64_t tile0[kMaxLevels + 1];
+    tile0[0] = 0;
+    for (int L = 0; L < tv.nlev; L++) tile0[L + 1] = tile0[L] + corner_tiles(c->levels[L]->n);
+    const int64_t ntiles = tile0[tv.nlev];
+    const int64_t nslots = ntiles * corner_tile_records();
     HIPCHK(c, E[2].ensure(std::max<int64_t>(nleaf, 1) * sizeof(double4)));
-    HIPCHK(c, E[3].ensure(std::max<int64_t>(nrec_max, 1) * 8));
-    HIPCHK(c, E[4].ensure(std::max<int64_t>(nrec_max, 1) * 4));
-    HIPCHK(c, E[5].ensure(std::max<int64_t>(nrec_max, 1) * 8));
-    HIPCHK(c, E[6].ensure(std::max<int64_t>(nrec_max, 1) * 4));
-    HIPCHK(c, E[29].ensure(std::max<int64_t>(nrec_max, 1) * 8 * 4));
-    HIPCHK(c, E[7].ensure(256));
-    launch_fill_i32(E[7].as<int>(), 1, 0, s);
+    HIPCHK(c, E[7].ensure(std::max<int64_t>(nslots, 1) * 8));
+    HIPCHK(c, E[29].ensure(std::max<int64_t>(nslots, 1) * 8 * 4));
+    HIPCHK(c, E[30].ensure((ntiles + 1) * 4));
+    HIPCHK(c, E[31].ensure((ntiles + 1) * 4));
     double4* leaf_node = E[2].as<double4>();
     HIPCHK(c, E[28].ensure(std::max<int64_t>(nleaf, 1)));
-    const int ckbits = std::max(1, 3 * tv.dp);
-    const int cvbits = pack_vbits(ckbits, nrec_max);
     for (int L = 0; L < tv.nlev; L++)
-        launch_corner_tiles(c->levels[L]->view(), tv.off[L], lidx, tv.dp, cvbits, leaf_node, E[28].as<uint8_t>(),
-                            E[3].as<uint64_t>(), E[4].as<int>(), E[29].as<int>(), E[7].as<int>(), s);
+        launch_corner_tiles(c->levels[L]->view(), tv.off[L], lidx, tv.dp, tile0[L], leaf_node, E[28].as<uint8_t>(),
+                            E[7].as<uint64_t>(), E[29].as<int>(), E[30].as<int>(), s);
+    launch_fill_i32(E[30].as<int>() + ntiles, 1, 0, s);
+    if ((rc = ensure_tmp(c, ntiles + 1))) return rc;
+    exclusive_scan_i32(E[30].as<int>(), E[31].as<int>(), ntiles + 1, c->tmp.as<int>(), s);
     HIPCHK(c, hipGetLastError());
     int64_t nent = 0;
-    if ((rc = read_int(c, E[7].as<int>(), nent))) return rc;
-    if (nent > nrec_max) return c->fail(PISURF_E_STATE, "corner records exceed their bound");
+    if ((rc = read_int(c, E[31].as<int>() + ntiles, nent))) return rc;
+    const int ckbits = std::max(1, 3 * tv.dp);
+    const int cvbits = pack_vbits(ckbits, nslots);
+    HIPCHK(c, E[3].ensure(std::max<int64_t>(nent, 1) * 8));
+    HIPCHK(c, E[4].ensure(std::max<int64_t>(nent, 1) * 4));
+    HIPCHK(c, E[5].ensure(std::max<int64_t>(nent, 1) * 8));
+    HIPCHK(c, E[6].ensure(std::max<int64_t>(nent, 1) * 4));
+    launch_records_compact(E[30].as<int>(), E[31].as<int>(), ntiles, E[7].as<uint64_t>(), cvbits, E[3].as<uint64_t>(),
+                           E[4].as<int>(), s);
     if ((rc = ensure_tmp(c, nent + 1))) return rc;
     const SortedKV ckv = radix_sort_entries(E[3].as<uint64_t>(), E[4].as<int>(), E[5].as<uint64_t>(), E[6].as<int>(),
                                             nent, ckbits, cvbits, c->tmp.as<int>(), s);

@@ -45,37 +45,35 @@ __global__ void k_leaf_flag(NodeLevel lv, int64_t off, int* __restrict__ flag) {
     flag[off + i] = is_leaf_type(lv.type[i]) ? 1 : 0;
 }
 
-// The distinct corner points of the leaves, in two steps. Each workgroup takes a tile of 128
+// The distinct corner points of the leaves, in three steps. Each workgroup takes a tile of 128
 // consecutive nodes of one level (a level's nodes are in Morton order, so a tile covers a compact
 // piece of space and its leaves share most corners) and merges the tile's corner keys in an LDS
 // hash table: one record per distinct point of the tile, holding its Morton key and, for each of
-// the point's 8 octants, the tile's leaf in that octant (-1 if none). A point shared by several
-// tiles gets one record in each; the global sort of the records' keys then groups them, and the
-// point pass merges a point's records. A leaf with P as corner c lies in octant c ^ 7 of P, and no
-// two leaves lie in the same octant of P, so the merge is a plain union.
+// the point's 8 octants, the tile's leaf in that octant (-1 if none). Tile b writes its records to
+// its own region, slots b * kCtRecs .. b * kCtRecs + count - 1 (no shared counter: tiles of a
+// frame number in the tens of thousands, and one device-wide atomic per tile serialises them),
+// and its count to tcount[b]. A scan of the counts then places each tile's keys in one contiguous
+// array, packed with the record's slot, for the global sort. A point shared by several tiles has a
+// record in each; the sort groups them and the point pass merges them. A leaf with P as corner c
+// lies in octant c ^ 7 of P, and no two leaves lie in the same octant of P, so the merge is a plain
+// union, and every result taken from the records is independent of the tiles.
 //
 // Keys: Morton interleave of the finest-level corner coordinates (3 dp bits); corners on the root
-// boundary are not points of the traversal and are skipped. Records are numbered by a global
-// counter (tiles claim consecutive ranges), so their order varies from run to run; every result
-// taken from them does not.
+// boundary are not points of the traversal and are skipped.
 constexpr int kCtT = 128;               // nodes per tile
-constexpr int kCtKeys = 8 * kCtT;       // corner keys per tile at most
-constexpr int kCtSlots = 2 * kCtKeys;   // hash slots: load factor at most 1/2
+constexpr int kCtRecs = 8 * kCtT;       // corner keys, hence records, per tile at most
+constexpr int kCtSlots = 2 * kCtRecs;   // hash slots: load factor at most 1/2
 constexpr int kCtSlotBits = 11;
 static_assert((1 << kCtSlotBits) == kCtSlots, "slot bits");
 constexpr uint64_t kCtEmpty = ~0ull;     // above every key (3 dp <= 63 bits)
 
 __global__ void __launch_bounds__(kCtT) k_corner_tiles(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp,
-                                                       int vbits, double4* __restrict__ leaf_node,
-                                                       uint8_t* __restrict__ leaf_pos, uint64_t* __restrict__ rkey,
-                                                       int* __restrict__ rval, int* __restrict__ rleaf,
-                                                       int* __restrict__ rcount) {
+                                                       int64_t tile0, double4* __restrict__ leaf_node,
+                                                       uint8_t* __restrict__ leaf_pos, uint64_t* __restrict__ fkey,
+                                                       int* __restrict__ rleaf, int* __restrict__ tcount) {
     __shared__ uint64_t tab[kCtSlots];
-    __shared__ short uid[kCtSlots];  // record of a slot within the tile (< kCtKeys)
-    __shared__ uint64_t ukey[kCtKeys];
-    __shared__ int un8[kCtKeys * 8];
+    __shared__ short uid[kCtSlots];  // record of a slot within the tile (< kCtRecs)
     __shared__ int wsum[kCtT / kWave];
-    __shared__ int gbase;
     const int t = threadIdx.x;
     for (int k = t; k < kCtSlots; k += kCtT) tab[k] = kCtEmpty;
     __syncthreads();
@@ -109,12 +107,16 @@ __global__ void __launch_bounds__(kCtT) k_corner_tiles(NodeLevel lv, int64_t off
             const int64_t X = cx[ox], Y = cy[oy], Z = cz[oz];
             if (!(X >= 1 && X <= hi && Y >= 1 && Y <= hi && Z >= 1 && Z <= hi)) continue;
             const uint64_t k = mx[ox] | my[oy] | mz[oz];
-            // at most kCtKeys keys in kCtSlots slots: a free slot is always reached
+            // at most kCtRecs keys in kCtSlots slots: a free slot is always reached
             unsigned h = (unsigned)((k * 0x9E3779B97F4A7C15ull) >> (64 - kCtSlotBits));
             for (;;) {
-                const uint64_t prev = atomicCAS((unsigned long long*)&tab[h], (unsigned long long)kCtEmpty,
-                                                (unsigned long long)k);
-                if (prev == kCtEmpty || prev == k) break;
+                const uint64_t cur = tab[h];
+                if (cur == k) break;
+                if (cur == kCtEmpty) {
+                    const uint64_t prev = atomicCAS((unsigned long long*)&tab[h], (unsigned long long)kCtEmpty,
+                                                    (unsigned long long)k);
+                    if (prev == kCtEmpty || prev == k) break;
+                }
                 h = (h + 1) & (kCtSlots - 1);
             }
             slot[c] = (int)h;
@@ -123,9 +125,13 @@ __global__ void __launch_bounds__(kCtT) k_corner_tiles(NodeLevel lv, int64_t off
     __syncthreads();
     // number the occupied slots in slot order
     constexpr int per = kCtSlots / kCtT;
+    uint64_t mine[per];
     int cnt = 0;
 #pragma unroll
-    for (int k = 0; k < per; k++) cnt += tab[t * per + k] != kCtEmpty ? 1 : 0;
+    for (int k = 0; k < per; k++) {
+        mine[k] = tab[t * per + k];
+        cnt += mine[k] != kCtEmpty ? 1 : 0;
+    }
     const int lane = t & (kWave - 1), w = t / kWave;
     const int incl = wave_incl_scan(cnt, lane);
     if (lane == kWave - 1) wsum[w] = incl;
@@ -136,37 +142,47 @@ __global__ void __launch_bounds__(kCtT) k_corner_tiles(NodeLevel lv, int64_t off
         if (k < w) u += wsum[k];
         total += wsum[k];
     }
+    const int64_t b = tile0 + blockIdx.x;
+    if (t == 0) tcount[b] = total;
+    // each record: its key, and no leaf in any octant yet
+    const int4 none = make_int4(-1, -1, -1, -1);
 #pragma unroll
     for (int k = 0; k < per; k++) {
-        const int sl = t * per + k;
-        const uint64_t kk = tab[sl];
-        if (kk != kCtEmpty) {
-            uid[sl] = (short)u;
-            ukey[u] = kk;
-            u++;
-        }
+        if (mine[k] == kCtEmpty) continue;
+        const int64_t r = b * kCtRecs + u;
+        uid[t * per + k] = (short)u;
+        fkey[r] = mine[k];
+        ((int4*)rleaf)[2 * r] = none;
+        ((int4*)rleaf)[2 * r + 1] = none;
+        u++;
     }
-    for (int e = t; e < total * 8; e += kCtT) un8[e] = -1;
-    if (t == 0 && total > 0) gbase = atomicAdd(rcount, total);
+    // the record stores above are complete before any thread of the tile writes a leaf into them
+    __threadfence_block();
     __syncthreads();
     if (L >= 0) {
 #pragma unroll
         for (int c = 0; c < 8; c++)
-            if (slot[c] >= 0) un8[uid[slot[c]] * 8 + (c ^ 7)] = L;
+            if (slot[c] >= 0) rleaf[(b * kCtRecs + uid[slot[c]]) * 8 + (c ^ 7)] = L;
     }
-    __syncthreads();
-    if (total == 0) return;
-    const int64_t gb = gbase;
-    for (int k = t; k < total; k += kCtT) {
-        const int64_t r = gb + k;
+}
+
+// the tiles' keys in one array (tile b from tbase[b] on), each packed with its record slot when
+// vbits > 0 (else the slot in rval)
+__global__ void __launch_bounds__(kCtT) k_records_compact(const int* __restrict__ tcount, const int* __restrict__ tbase,
+                                                          const uint64_t* __restrict__ fkey, int vbits,
+                                                          uint64_t* __restrict__ rkey, int* __restrict__ rval) {
+    const int64_t b = blockIdx.x;
+    const int n = tcount[b];
+    const int64_t d0 = tbase[b];
+    for (int u = threadIdx.x; u < n; u += kCtT) {
+        const int64_t r = b * kCtRecs + u;
         if (vbits) {
-            rkey[r] = (ukey[k] << vbits) | (uint64_t)r;
+            rkey[d0 + u] = (fkey[r] << vbits) | (uint64_t)r;
         } else {
-            rkey[r] = ukey[k];
-            rval[r] = (int)r;
+            rkey[d0 + u] = fkey[r];
+            rval[d0 + u] = (int)r;
         }
     }
-    for (int e = t; e < total * 8; e += kCtT) rleaf[gb * 8 + e] = un8[e];
 }
 
 // leaf containing the finest cell q, descending from node i of level L (the root: 0, 0)
@@ -510,11 +526,17 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s) {
     if (lv.n > 0) k_leaf_flag<<<nblk(lv.n), 256, 0, s>>>(lv, off, flag);
 }
-void launch_corner_tiles(NodeLevel lv, int64_t off, const int* lidx, int dp, int vbits, double4* leaf_node,
-                         uint8_t* leaf_pos, uint64_t* rkey, int* rval, int* rleaf, int* rcount, hipStream_t s) {
+int64_t corner_tiles(int64_t nodes) { return (nodes + kCtT - 1) / kCtT; }
+int64_t corner_tile_records() { return kCtRecs; }
+void launch_corner_tiles(NodeLevel lv, int64_t off, const int* lidx, int dp, int64_t tile0, double4* leaf_node,
+                         uint8_t* leaf_pos, uint64_t* fkey, int* rleaf, int* tcount, hipStream_t s) {
     if (lv.n > 0)
-        k_corner_tiles<<<(unsigned)((lv.n + kCtT - 1) / kCtT), kCtT, 0, s>>>(lv, off, lidx, dp, vbits, leaf_node,
-                                                                           leaf_pos, rkey, rval, rleaf, rcount);
+        k_corner_tiles<<<(unsigned)corner_tiles(lv.n), kCtT, 0, s>>>(lv, off, lidx, dp, tile0, leaf_node, leaf_pos,
+                                                                     fkey, rleaf, tcount);
+}
+void launch_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey, int vbits,
+                            uint64_t* rkey, int* rval, hipStream_t s) {
+    if (ntiles > 0) k_records_compact<<<(unsigned)ntiles, kCtT, 0, s>>>(tcount, tbase, fkey, vbits, rkey, rval);
 }
 void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned* table, int64_t tsize, int* slot, int* first,
                         hipStream_t s) {

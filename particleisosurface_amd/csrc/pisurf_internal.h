@@ -250,12 +250,18 @@ struct TreeView {
     const int* child[kMaxLevels];
 };
 void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
-// leaf_node[L] = vertex and value of leaf L, leaf_pos[L] = (value > 0); and the distinct corner
-// points of every tile of 128 nodes of the level: record r (claimed through *rcount) has the
-// point's Morton key (3 dp bits) in rkey[r], packed with r when vbits > 0 (else r in rval[r]), and
-// in rleaf[8 r + o] the tile's leaf in octant o of the point, or -1. At most 8 records per leaf.
-void launch_corner_tiles(NodeLevel lv, int64_t off, const int* lidx, int dp, int vbits, double4* leaf_node,
-                         uint8_t* leaf_pos, uint64_t* rkey, int* rval, int* rleaf, int* rcount, hipStream_t s);
+// Distinct corner points per tile of corner_tile_records() / 8 nodes of a level. Tile b (tile0 +
+// the level's tile) writes its records to slots b * corner_tile_records() + u, u < tcount[b]: the
+// point's Morton key (3 dp bits) in fkey, and in rleaf[8 slot + o] the tile's leaf in octant o of
+// the point, or -1. Also leaf_node[L] = vertex and value of leaf L, leaf_pos[L] = (value > 0).
+int64_t corner_tiles(int64_t nodes);
+int64_t corner_tile_records();
+void launch_corner_tiles(NodeLevel lv, int64_t off, const int* lidx, int dp, int64_t tile0, double4* leaf_node,
+                         uint8_t* leaf_pos, uint64_t* fkey, int* rleaf, int* tcount, hipStream_t s);
+// the records' keys, tile b's from tbase[b] on (the scanned counts), packed with their slots when
+// vbits > 0 (else the slots in rval)
+void launch_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey, int vbits,
+                            uint64_t* rkey, int* rval, hipStream_t s);
 // first[i] = 1 when i is the smallest index with key[i] (open-addressing table of tsize slots, a
 // power of two above the number of distinct keys, with room to spare; slot[i] = slot of key[i],
 // table[slot] = smallest index of the key; n < 2^32 - 1)
