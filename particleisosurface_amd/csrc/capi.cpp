@@ -1164,7 +1164,20 @@ int do_extract_device(pisurf_ctx* c) {
     // leaves
     HIPCHK(c, E[0].ensure((T + 1) * 4));
     HIPCHK(c, E[1].ensure((T + 1) * 4));
-    for (int L = 0; L < tv.nlev; L++) launch_leaf_flag(c->levels[L]->view(), tv.off[L], E[0].as<int>(), s);
+    LeafLevels ll;
+    std::memset(&ll, 0, sizeof(ll));
+    ll.nlev = tv.nlev;
+    ll.dp = tv.dp;
+    for (int L = 0; L < tv.nlev; L++) {
+        LevelBuf* lb = c->levels[L];
+        ll.off[L] = tv.off[L];
+        ll.type[L] = lb->type.as<uint8_t>();
+        ll.node[L] = lb->node.as<double4>();
+        ll.ip[L] = lb->ip.as<int4>();
+        ll.tile0[L + 1] = ll.tile0[L] + corner_tiles(lb->n);
+    }
+    ll.off[tv.nlev] = T;
+    launch_leaf_flags(ll, E[0].as<int>(), s);
     HIPCHK(c, hipMemsetAsync(E[0].as<int>() + T, 0, 4, s));
     if ((rc = ensure_tmp(c, T + 1))) return rc;
     exclusive_scan_i32(E[0].as<int>(), E[1].as<int>(), T + 1, c->tmp.as<int>(), s);
@@ -1174,10 +1187,7 @@ int do_extract_device(pisurf_ctx* c) {
     // distinct corner points: per tile of 128 nodes of a level, one record per point (its key and
     // the tile's leaves around it) in the tile's own region; then the keys of all records in one
     // array, sorted
-    int64_t tile0[kMaxLevels + 1];
-    tile0[0] = 0;
-    for (int L = 0; L < tv.nlev; L++) tile0[L + 1] = tile0[L] + corner_tiles(c->levels[L]->n);
-    const int64_t ntiles = tile0[tv.nlev];
+    const int64_t ntiles = ll.tile0[tv.nlev];
     const int64_t nslots = ntiles * corner_tile_records();
     HIPCHK(c, E[2].ensure(std::max<int64_t>(nleaf, 1) * sizeof(double4)));
     HIPCHK(c, E[7].ensure(std::max<int64_t>(nslots, 1) * 8));
@@ -1186,9 +1196,8 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[31].ensure((ntiles + 1) * 4));
     double4* leaf_node = E[2].as<double4>();
     HIPCHK(c, E[28].ensure(std::max<int64_t>(nleaf, 1)));
-    for (int L = 0; L < tv.nlev; L++)
-        launch_corner_tiles(c->levels[L]->view(), tv.off[L], lidx, tv.dp, tile0[L], leaf_node, E[28].as<uint8_t>(),
-                            E[7].as<uint64_t>(), E[29].as<int>(), E[30].as<int>(), s);
+    launch_corner_tiles(ll, lidx, leaf_node, E[28].as<uint8_t>(), E[7].as<uint64_t>(), E[29].as<int>(), E[30].as<int>(),
+                        s);
     launch_fill_i32(E[30].as<int>() + ntiles, 1, 0, s);
     if ((rc = ensure_tmp(c, ntiles + 1))) return rc;
     exclusive_scan_i32(E[30].as<int>(), E[31].as<int>(), ntiles + 1, c->tmp.as<int>(), s);

@@ -39,10 +39,12 @@ __device__ __forceinline__ uint32_t morton_compact(uint64_t v) {
     return (uint32_t)v;
 }
 
-__global__ void k_leaf_flag(NodeLevel lv, int64_t off, int* __restrict__ flag) {
+__global__ void k_leaf_flags(LeafLevels lv, int* __restrict__ flag) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= lv.n) return;
-    flag[off + i] = is_leaf_type(lv.type[i]) ? 1 : 0;
+    if (i >= lv.off[lv.nlev]) return;
+    int L = 0;
+    while (L + 1 < lv.nlev && i >= lv.off[L + 1]) L++;
+    flag[i] = is_leaf_type(lv.type[L][i - lv.off[L]]) ? 1 : 0;
 }
 
 // The distinct corner points of the leaves, in three steps. Each workgroup takes a tile of 128
@@ -67,27 +69,32 @@ constexpr int kCtSlotBits = 11;
 static_assert((1 << kCtSlotBits) == kCtSlots, "slot bits");
 constexpr uint64_t kCtEmpty = ~0ull;     // above every key (3 dp <= 63 bits)
 
-__global__ void __launch_bounds__(kCtT) k_corner_tiles(NodeLevel lv, int64_t off, const int* __restrict__ lidx, int dp,
-                                                       int64_t tile0, double4* __restrict__ leaf_node,
-                                                       uint8_t* __restrict__ leaf_pos, uint64_t* __restrict__ fkey,
-                                                       int* __restrict__ rleaf, int* __restrict__ tcount) {
+__global__ void __launch_bounds__(kCtT) k_corner_tiles(LeafLevels lv, const int* __restrict__ lidx,
+                                                       double4* __restrict__ leaf_node, uint8_t* __restrict__ leaf_pos,
+                                                       uint64_t* __restrict__ fkey, int* __restrict__ rleaf,
+                                                       int* __restrict__ tcount) {
     __shared__ uint64_t tab[kCtSlots];
     __shared__ short uid[kCtSlots];  // record of a slot within the tile (< kCtRecs)
     __shared__ int wsum[kCtT / kWave];
     const int t = threadIdx.x;
     for (int k = t; k < kCtSlots; k += kCtT) tab[k] = kCtEmpty;
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * kCtT + t;
+    // this tile's level (uniform over the workgroup) and node
+    const int64_t b = blockIdx.x;
+    int lev = 0;
+    while (lev + 1 < lv.nlev && b >= lv.tile0[lev + 1]) lev++;
+    const int dp = lv.dp;
+    const int64_t i = (b - lv.tile0[lev]) * kCtT + t;
     int L = -1;
     int slot[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) slot[c] = -1;
-    if (i < lv.n && is_leaf_type(lv.type[i])) {
-        L = lidx[off + i];
-        const double4 nd = lv.node[i];
+    if (i < lv.off[lev + 1] - lv.off[lev] && is_leaf_type(lv.type[lev][i])) {
+        L = lidx[lv.off[lev] + i];
+        const double4 nd = lv.node[lev][i];
         leaf_node[L] = nd;
         leaf_pos[L] = nd.w > 0.0 ? 1 : 0;  // sign of the leaf value, read by the point pass
-        const int4 p = lv.ip[i];
+        const int4 p = lv.ip[lev][i];
         const int sh = dp - p.w;
         const int64_t hi = ((int64_t)1 << dp) - 1;
         int64_t cx[2], cy[2], cz[2];
@@ -142,7 +149,6 @@ __global__ void __launch_bounds__(kCtT) k_corner_tiles(NodeLevel lv, int64_t off
         if (k < w) u += wsum[k];
         total += wsum[k];
     }
-    const int64_t b = tile0 + blockIdx.x;
     if (t == 0) tcount[b] = total;
     // each record: its key, and no leaf in any octant yet
     const int4 none = make_int4(-1, -1, -1, -1);
@@ -523,16 +529,16 @@ __global__ void k_tri_out(const int* __restrict__ cv, const int* __restrict__ ke
 
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s) {
-    if (lv.n > 0) k_leaf_flag<<<nblk(lv.n), 256, 0, s>>>(lv, off, flag);
+void launch_leaf_flags(const LeafLevels& lv, int* flag, hipStream_t s) {
+    if (lv.off[lv.nlev] > 0) k_leaf_flags<<<nblk(lv.off[lv.nlev]), 256, 0, s>>>(lv, flag);
 }
 int64_t corner_tiles(int64_t nodes) { return (nodes + kCtT - 1) / kCtT; }
 int64_t corner_tile_records() { return kCtRecs; }
-void launch_corner_tiles(NodeLevel lv, int64_t off, const int* lidx, int dp, int64_t tile0, double4* leaf_node,
-                         uint8_t* leaf_pos, uint64_t* fkey, int* rleaf, int* tcount, hipStream_t s) {
-    if (lv.n > 0)
-        k_corner_tiles<<<(unsigned)corner_tiles(lv.n), kCtT, 0, s>>>(lv, off, lidx, dp, tile0, leaf_node, leaf_pos,
-                                                                     fkey, rleaf, tcount);
+void launch_corner_tiles(const LeafLevels& lv, const int* lidx, double4* leaf_node, uint8_t* leaf_pos, uint64_t* fkey,
+                         int* rleaf, int* tcount, hipStream_t s) {
+    if (lv.tile0[lv.nlev] > 0)
+        k_corner_tiles<<<(unsigned)lv.tile0[lv.nlev], kCtT, 0, s>>>(lv, lidx, leaf_node, leaf_pos, fkey, rleaf,
+                                                                    tcount);
 }
 void launch_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey, int vbits,
                             uint64_t* rkey, int* rval, hipStream_t s) {

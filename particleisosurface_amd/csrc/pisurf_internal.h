@@ -249,15 +249,27 @@ struct TreeView {
     const uint8_t* type[kMaxLevels];
     const int* child[kMaxLevels];
 };
-void launch_leaf_flag(NodeLevel lv, int64_t off, int* flag, hipStream_t s);
-// Distinct corner points per tile of corner_tile_records() / 8 nodes of a level. Tile b (tile0 +
-// the level's tile) writes its records to slots b * corner_tile_records() + u, u < tcount[b]: the
-// point's Morton key (3 dp bits) in fkey, and in rleaf[8 slot + o] the tile's leaf in octant o of
-// the point, or -1. Also leaf_node[L] = vertex and value of leaf L, leaf_pos[L] = (value > 0).
+// the levels of a built tree as the extraction's per-node passes read them
+struct LeafLevels {
+    int nlev;                          // levels present
+    int dp;                            // deepest level (finest integer resolution 2^dp)
+    int64_t off[kMaxLevels + 1];       // global index of each level's first node; off[nlev] = nodes
+    int64_t tile0[kMaxLevels + 1];     // first corner tile of each level; tile0[nlev] = tiles
+    const uint8_t* type[kMaxLevels];
+    const double4* node[kMaxLevels];
+    const int4* ip[kMaxLevels];
+};
+// flag[i] = 1 when global node i is a leaf, one launch for every level
+void launch_leaf_flags(const LeafLevels& lv, int* flag, hipStream_t s);
+// Distinct corner points per tile of corner_tile_records() / 8 nodes of a level (lv.tile0 from
+// corner_tiles() of each level's node count), one launch for every level. Tile b writes its
+// records to slots b * corner_tile_records() + u, u < tcount[b]: the point's Morton key (3 dp
+// bits) in fkey, and in rleaf[8 slot + o] the tile's leaf in octant o of the point, or -1. Also
+// leaf_node[L] = vertex and value of leaf L, leaf_pos[L] = (value > 0).
 int64_t corner_tiles(int64_t nodes);
 int64_t corner_tile_records();
-void launch_corner_tiles(NodeLevel lv, int64_t off, const int* lidx, int dp, int64_t tile0, double4* leaf_node,
-                         uint8_t* leaf_pos, uint64_t* fkey, int* rleaf, int* tcount, hipStream_t s);
+void launch_corner_tiles(const LeafLevels& lv, const int* lidx, double4* leaf_node, uint8_t* leaf_pos, uint64_t* fkey,
+                         int* rleaf, int* tcount, hipStream_t s);
 // the records' keys, tile b's from tbase[b] on (the scanned counts), packed with their slots when
 // vbits > 0 (else the slots in rval)
 void launch_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey, int vbits,
