@@ -1178,7 +1178,6 @@ int do_extract_device(pisurf_ctx* c) {
     }
     ll.off[tv.nlev] = T;
     launch_leaf_flags(ll, E[0].as<int>(), s);
-    HIPCHK(c, hipMemsetAsync(E[0].as<int>() + T, 0, 4, s));
     if ((rc = ensure_tmp(c, T + 1))) return rc;
     exclusive_scan_i32(E[0].as<int>(), E[1].as<int>(), T + 1, c->tmp.as<int>(), s);
     int64_t nleaf;
@@ -1228,7 +1227,6 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[24].ensure(std::max<int64_t>(npts, 1) * 9 * 4));
     launch_points(tv, lidx, E[28].as<uint8_t>(), ckv, nent, E[29].as<int>(), E[25].as<int>(), npts, E[26].as<int>(),
                   E[24].as<int>(), s);
-    HIPCHK(c, hipMemsetAsync(E[26].as<int>() + npts, 0, 4, s));
     exclusive_scan_i32(E[26].as<int>(), E[27].as<int>(), npts + 1, c->tmp.as<int>(), s);
     int64_t nd;
     if ((rc = read_int(c, E[27].as<int>() + npts, nd))) return rc;
@@ -1238,7 +1236,6 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[11].ensure((nd + 1) * 4));
     HIPCHK(c, E[12].ensure((nd + 1) * 4));
     launch_tri_count(E[10].as<int>(), nd, E[11].as<int>(), s);
-    HIPCHK(c, hipMemsetAsync(E[11].as<int>() + nd, 0, 4, s));
     exclusive_scan_i32(E[11].as<int>(), E[12].as<int>(), nd + 1, c->tmp.as<int>(), s);
     int64_t nt;
     if ((rc = read_int(c, E[12].as<int>() + nd, nt))) return rc;
@@ -1259,7 +1256,6 @@ int do_extract_device(pisurf_ctx* c) {
     if ((rc = ensure_tmp(c, nc + 1))) return rc;
     int* vslot = E[14].as<int>();  // the corner indices written there are implicit
     launch_hash_dedupe(E[13].as<uint64_t>(), nc, E[15].as<unsigned>(), vtab, vslot, E[18].as<int>(), s);
-    HIPCHK(c, hipMemsetAsync(E[18].as<int>() + nc, 0, 4, s));
     exclusive_scan_i32(E[18].as<int>(), E[19].as<int>(), nc + 1, c->tmp.as<int>(), s);
     int64_t nv;
     if ((rc = read_int(c, E[19].as<int>() + nc, nv))) return rc;
@@ -1303,7 +1299,6 @@ int do_extract_device(pisurf_ctx* c) {
                            32 + bits_for(nv), c->tmp.as<int>(), s);
         launch_tri_keep(cvid, E[6].as<int>(), nt, E[8].as<int>(), s);
     }
-    HIPCHK(c, hipMemsetAsync(E[8].as<int>() + nt, 0, 4, s));
     exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nt + 1, c->tmp.as<int>(), s);
     int64_t ntf;
     if ((rc = read_int(c, E[9].as<int>() + nt, ntf))) return rc;

@@ -39,8 +39,10 @@ __device__ __forceinline__ uint32_t morton_compact(uint64_t v) {
     return (uint32_t)v;
 }
 
+// (flag[nodes] = 0 closes the array for its scan)
 __global__ void k_leaf_flags(LeafLevels lv, int* __restrict__ flag) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == lv.off[lv.nlev]) flag[i] = 0;
     if (i >= lv.off[lv.nlev]) return;
     int L = 0;
     while (L + 1 < lv.nlev && i >= lv.off[L + 1]) L++;
@@ -322,6 +324,7 @@ __global__ void k_points(TreeView tv, const int* __restrict__ lidx, const uint8_
                          SortedKV kv, int64_t nent, const int* __restrict__ rleaf, const int* __restrict__ heads,
                          int64_t npts, int* __restrict__ dflag, int* __restrict__ stage) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == npts) dflag[t] = 0;  // closes the flags for their scan
     if (t >= npts) return;
     int n[8];
     gather_point(tv, lidx, kv, nent, rleaf, heads[t], n);
@@ -355,6 +358,7 @@ __device__ __forceinline__ int canon_slot(uint8_t code) {
 
 __global__ void k_tri_count(const int* __restrict__ duals, int64_t nd, int* __restrict__ ntri) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d == nd) ntri[d] = 0;  // closes the counts for their scan
     if (d >= nd) return;
     const int slot = canon_slot(d_kDmcCase[duals[9 * d + 8]]);
     ntri[d] = slot < 0 ? 0 : d_kDmcNTri[slot];
@@ -451,6 +455,7 @@ __global__ void k_hash_first(const int* __restrict__ slot, const unsigned* __res
                              int* __restrict__ first) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) first[i] = (table[slot[i]] == (unsigned)i) ? 1 : 0;
+    if (i == n) first[i] = 0;  // closes the flags for their scan
 }
 // vertex id of every corner (rank of its key's first corner) and the bisection edge of each vertex,
 // oriented as its first corner's dual cell has it
@@ -506,6 +511,7 @@ __device__ __forceinline__ void sorted3(const int* cv, int64_t t, int& lo, int& 
 __global__ void k_tri_keep(const int* __restrict__ cv, const int* __restrict__ order, int64_t nt,
                            int* __restrict__ keep) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == nt) keep[j] = 0;  // closes the flags for their scan
     if (j >= nt) return;
     const int t = order[j];
     int k = 1;
@@ -530,7 +536,7 @@ __global__ void k_tri_out(const int* __restrict__ cv, const int* __restrict__ ke
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 void launch_leaf_flags(const LeafLevels& lv, int* flag, hipStream_t s) {
-    if (lv.off[lv.nlev] > 0) k_leaf_flags<<<nblk(lv.off[lv.nlev]), 256, 0, s>>>(lv, flag);
+    k_leaf_flags<<<nblk(lv.off[lv.nlev] + 1), 256, 0, s>>>(lv, flag);
 }
 int64_t corner_tiles(int64_t nodes) { return (nodes + kCtT - 1) / kCtT; }
 int64_t corner_tile_records() { return kCtRecs; }
@@ -546,10 +552,13 @@ void launch_records_compact(const int* tcount, const int* tbase, int64_t ntiles,
 }
 void launch_hash_dedupe(const uint64_t* key, int64_t n, unsigned* table, int64_t tsize, int* slot, int* first,
                         hipStream_t s) {
-    if (n <= 0) return;
+    if (n <= 0) {
+        launch_fill_i32(first, 1, 0, s);
+        return;
+    }
     (void)hipMemsetAsync(table, 0xff, tsize * sizeof(unsigned), s);  // every slot empty
     k_hash_insert<<<nblk(n), 256, 0, s>>>(key, n, table, (uint64_t)(tsize - 1), slot);
-    k_hash_first<<<nblk(n), 256, 0, s>>>(slot, table, n, first);
+    k_hash_first<<<nblk(n + 1), 256, 0, s>>>(slot, table, n, first);
 }
 void launch_vertex_ids_hashed(const int* slot, const unsigned* table, const int* rank, int64_t n, const int* vend,
                               const double4* leaf_node, int* corner_vid, double* ends, hipStream_t s) {
@@ -574,15 +583,14 @@ void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_t
 }
 void launch_points(const TreeView& tv, const int* lidx, const uint8_t* leaf_pos, SortedKV kv, int64_t nent,
                    const int* rleaf, const int* heads, int64_t npts, int* dflag, int* stage, hipStream_t s) {
-    if (npts > 0)
-        k_points<<<nblk(npts), 256, 0, s>>>(tv, lidx, leaf_pos, kv, nent, rleaf, heads, npts, dflag, stage);
+    k_points<<<nblk(npts + 1), 256, 0, s>>>(tv, lidx, leaf_pos, kv, nent, rleaf, heads, npts, dflag, stage);
 }
 void launch_points_compact(const int* dflag, const int* dpos, const int* stage, int64_t nent, int* duals,
                            hipStream_t s) {
     if (nent > 0) k_points_compact<<<nblk(nent), 256, 0, s>>>(dflag, dpos, stage, nent, duals);
 }
 void launch_tri_count(const int* duals, int64_t nd, int* ntri, hipStream_t s) {
-    if (nd > 0) k_tri_count<<<nblk(nd), 256, 0, s>>>(duals, nd, ntri);
+    k_tri_count<<<nblk(nd + 1), 256, 0, s>>>(duals, nd, ntri);
 }
 void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t* vkey, int* vval, int* vend,
                         hipStream_t s) {
@@ -595,7 +603,7 @@ void launch_tri_key_lo(const int* cv, const int* perm, int64_t nt, uint64_t* k, 
     if (nt > 0) k_tri_key_lo<<<nblk(nt), 256, 0, s>>>(cv, perm, nt, k, v);
 }
 void launch_tri_keep(const int* cv, const int* order, int64_t nt, int* keep, hipStream_t s) {
-    if (nt > 0) k_tri_keep<<<nblk(nt), 256, 0, s>>>(cv, order, nt, keep);
+    k_tri_keep<<<nblk(nt + 1), 256, 0, s>>>(cv, order, nt, keep);
 }
 void launch_tri_out(const int* cv, const int* keep, const int* kpos, int64_t nt, int* tris, hipStream_t s) {
     if (nt > 0) k_tri_out<<<nblk(nt), 256, 0, s>>>(cv, keep, kpos, nt, tris);
