@@ -63,7 +63,10 @@ typedef struct pisurf_params {
                                   (bitwise-identical coordinates) are evaluated once; 0: 27 per node */
     int32_t frames_in_flight;  /* pisurf_run_frames: frames meshed at once, each on a context and
                                   stream of its own (0: the library's choice, 1: one after another) */
-    int32_t reserved[2];
+    int32_t level_graph;       /* 1: the launches of the octree levels above D_min are captured as a
+                                  HIP graph, instantiated once and updated in place per frame
+                                  (default 0: launched one by one, which measured 0.1 ms faster) */
+    int32_t reserved[1];
 } pisurf_params;
 
 typedef struct pisurf_grid_info {

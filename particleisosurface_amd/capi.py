@@ -34,7 +34,7 @@ class Params(C.Structure):
         ("use_ani", C.c_int32), ("use_poly6", C.c_int32), ("calc_normals", C.c_int32), ("oversample", C.c_int32),
         ("precision", C.c_int32), ("qef_mode", C.c_int32), ("qef_svd_tol", C.c_float),
         ("depth_max_override", C.c_int32), ("mesh_order", C.c_int32), ("sample_dedupe", C.c_int32),
-        ("frames_in_flight", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("frames_in_flight", C.c_int32), ("level_graph", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -122,13 +122,14 @@ class Context:
 
     def set_params(self, radius, precision=FP32, use_ani=True, use_poly6=False, calc_normals=True, qef_mode=QEF_OFF,
                    qef_svd_tol=1e-3, depth_max_override=0, smooth_factor=2.0, iso_factor=1.9, mesh_order=0, sample_dedupe=True,
-                   frames_in_flight=0):
+                   frames_in_flight=0, level_graph=False):
         p = Params()
         self.L.pisurf_default_params(C.byref(p))
         p.radius = radius
         p.mesh_order = mesh_order
         p.sample_dedupe = int(sample_dedupe)
         p.frames_in_flight = int(frames_in_flight)
+        p.level_graph = int(level_graph)
         p.precision = precision
         p.use_ani = int(use_ani)
         p.use_poly6 = int(use_poly6)

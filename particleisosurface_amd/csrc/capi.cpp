@@ -256,6 +256,7 @@ struct pisurf_ctx {
     std::vector<LevelBuf*> levels;
     DBuf samples, cl_flag, cl_pos, cl_list;
     DBuf lvcnt;  // node counts of the levels built from device-resident counts
+    hipGraphExec_t top_exec = nullptr;  // those levels' launches as an executable graph
     // sample deduplication: keys / entries (ping-pong), heads, scanned heads, representatives,
     // head list, per-point values
     DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals, sd_cwork;
@@ -303,6 +304,7 @@ struct pisurf_ctx {
             delete l;
         }
         host_read_release(hr);
+        if (top_exec) (void)hipGraphExecDestroy(top_exec);
         if (wev) (void)hipEventDestroy(wev);
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -885,24 +887,60 @@ int top_levels(pisurf_ctx* c, size_t* next) {
     }
     HIPCHK(c, c->lvcnt.ensure(64 * sizeof(int)));
     int* cnt = c->lvcnt.as<int>();
-    launch_fill_i32(cnt, 1, 1, c->stream);
     const int prec = prec_of(c);
     const FieldConst fc = field_const(c);
     const TreeParams tp = tree_params(c);
-    for (int l = 0; l < t; l++) {
-        LevelBuf* L = c->levels[l];
-        NodeLevel v = L->view();
-        v.ndev = cnt + l;
-        const double half = std::ldexp((double)c->root_half, -l);
-        const double w = (2 * half + 2 * (double)tp.margin) / (double)c->g.cell + 1;
-        const int group = (w * w <= 6) ? 4 : (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
-        launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group, kEmptyAndCurv, nullptr, 0,
-                          c->stream);
-        launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, nullptr, c->ctr(), c->stream);
-        launch_fill_i32(L->flag.as<int>() + ub[l], 1, 0, c->stream);
-        exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), ub[l] + 1, c->tmp.as<int>(), c->stream);
-        launch_children_counted(v, L->scanned.as<int>(), c->levels[l + 1]->view(), cnt + l + 1, c->stream);
+    auto issue = [&]() {
+        launch_fill_i32(cnt, 1, 1, c->stream);
+        for (int l = 0; l < t; l++) {
+            LevelBuf* L = c->levels[l];
+            NodeLevel v = L->view();
+            v.ndev = cnt + l;
+            const double half = std::ldexp((double)c->root_half, -l);
+            const double w = (2 * half + 2 * (double)tp.margin) / (double)c->g.cell + 1;
+            const int group = (w * w <= 6) ? 4 : (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
+            launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group, kEmptyAndCurv, nullptr, 0,
+                              c->stream);
+            launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, nullptr, c->ctr(),
+                            c->stream);
+            launch_fill_i32(L->flag.as<int>() + ub[l], 1, 0, c->stream);
+            exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), ub[l] + 1, c->tmp.as<int>(), c->stream);
+            launch_children_counted(v, L->scanned.as<int>(), c->levels[l + 1]->view(), cnt + l + 1, c->stream);
+        }
+    };
+    // With level_graph the launches (kernels only: fills are kernels too) are captured as a HIP
+    // graph: the first frame instantiates it, later frames update its parameters in place (the same
+    // sequence of kernels whenever the level count and the scans' tile counts repeat) or
+    // instantiate anew. The update and the graph launch cost about as much host time as the 40-odd
+    // launches they replace, so the launches go to the stream directly by default.
+    bool graphed = false;
+    if (c->prm.level_graph && hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        issue();
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &gr);
+        if (ec == hipSuccess && gr) {
+            hipError_t e = hipErrorNotInitialized;
+            if (c->top_exec) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res;
+                e = hipGraphExecUpdate(c->top_exec, gr, &bad, &res);
+                if (e != hipSuccess) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(c->top_exec);
+                    c->top_exec = nullptr;
+                }
+            }
+            if (!c->top_exec) e = hipGraphInstantiate(&c->top_exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) return c->hipfail(e, "octree level graph");
+            HIPCHK(c, hipGraphLaunch(c->top_exec, c->stream));
+            graphed = true;
+        } else {
+            if (gr) (void)hipGraphDestroy(gr);
+            (void)hipGetLastError();  // capture refused: the launches go to the stream directly
+        }
     }
+    if (!graphed) issue();
     HIPCHK(c, hipGetLastError());
     std::vector<int> n(t + 1);
     HIPCHK(c, c->read(n.data(), cnt, (t + 1) * sizeof(int)));
@@ -1900,6 +1938,7 @@ int pisurf_set_params(pisurf_ctx* c, const pisurf_params* p) {
     if (p->sample_dedupe < 0 || p->sample_dedupe > 1) return c->fail(PISURF_E_INVALID, "bad sample_dedupe");
     if (p->frames_in_flight < 0 || p->frames_in_flight > kMaxFramesInFlight)
         return c->fail(PISURF_E_INVALID, "frames_in_flight must be in 0 .. 16");
+    if (p->level_graph < 0 || p->level_graph > 1) return c->fail(PISURF_E_INVALID, "bad level_graph");
     const bool geo = p->radius != c->prm.radius || p->use_ani != c->prm.use_ani || p->use_poly6 != c->prm.use_poly6 ||
                      p->precision != c->prm.precision;
     c->prm = *p;

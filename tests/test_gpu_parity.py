@@ -607,6 +607,35 @@ def test_run_frames_lanes_stop_at_failed_frame():
         ref.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32k", "fp64"])
+def test_level_graph_matches_eager(prec):
+    """The levels above D_min as a HIP graph (instantiated on the first frame, updated in place on
+    later ones, instantiated anew when the kernel sequence changes) build the same tree and mesh as
+    the same launches issued one by one: replayed several times on one context, with a frame of
+    another geometry in between and eager frames on a second context interleaved."""
+    frames = [synth.dam_break(), synth.dam_break(), synth.lattice_sphere(), synth.dam_break(),
+              synth.random_cloud(20000, r=0.004, extent=0.2), synth.dam_break()]
+    g = capi.Context(0)
+    e = capi.Context(0)
+    for k, (xyz, r) in enumerate(frames):
+        out = []
+        for ctx, graph in ((g, True), (e, False)):
+            ctx.set_params(r, precision=PREC[prec], level_graph=graph)
+            ctx.upload(xyz)
+            info = ctx.run_frame()
+            nd = ctx.tree_nodes(info["n_nodes"])
+            v, t = ctx.mesh()
+            out.append((info, nd, v, t, ctx.stats()["field_evals"]))
+        (ia, na, va, ta, fa), (ib, nb, vb, tb, fb) = out
+        assert ia == ib, k
+        for key in ("type", "id", "depth", "child", "node"):
+            assert np.array_equal(na[key], nb[key]), (k, key)
+        assert np.array_equal(va, vb) and np.array_equal(ta, tb), k
+        assert fa == fb, k
+    g.close()
+    e.close()
+
+
 def _uniform_points(box, dims, idx):
     """UniformGrid::gridSampling's point of linear index idx (x fastest), each float op rounded."""
     lo, hi = box[0::2], box[1::2]
