@@ -1226,6 +1226,7 @@ int do_extract_device(pisurf_ctx* c) {
     // array, sorted
     const int64_t ntiles = ll.tile0[tv.nlev];
     const int64_t nslots = ntiles * corner_tile_records();
+    if (nslots >= INT32_MAX) return c->fail(PISURF_E_UNSUPPORTED, "octree too large for the corner records");
     HIPCHK(c, E[2].ensure(std::max<int64_t>(nleaf, 1) * sizeof(double4)));
     HIPCHK(c, E[7].ensure(std::max<int64_t>(nslots, 1) * 8));
     HIPCHK(c, E[29].ensure(std::max<int64_t>(nslots, 1) * 8 * 4));
