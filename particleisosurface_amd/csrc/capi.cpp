@@ -272,7 +272,7 @@ struct pisurf_ctx {
     hipEvent_t eb0 = nullptr, eb1 = nullptr;
     std::vector<hipEvent_t> ev_chunk;  // ends of the bisection chunks
     // device extraction workspace
-    DBuf eb[32];
+    DBuf eb[37];
     // mesh
     PinnedVec<double> mesh_v;
     PinnedVec<int32_t> mesh_t;
@@ -1198,6 +1198,11 @@ int do_extract_device(pisurf_ctx* c) {
     tv.dp = tv.nlev - 1;
     if (tv.dp > 21) return c->fail(PISURF_E_UNSUPPORTED, "octree deeper than 21 levels");
     DBuf* E = c->eb;
+    // the tree index from which the point pass's descents start
+    tv.glev = tree_index_level(tv.dp);
+    HIPCHK(c, E[36].ensure(((size_t)1 << (3 * tv.glev)) * 4));
+    tv.gidx = E[36].as<int>();
+    launch_tree_index(tv, E[36].as<int>(), s);
     int rc;
     // leaves
     HIPCHK(c, E[0].ensure((T + 1) * 4));

@@ -206,6 +206,37 @@ __device__ __forceinline__ int locate_from(const TreeView& tv, const int* lidx, 
     return -1;
 }
 
+// the tree index: cell c of level glev (x fastest) descended from the root until level glev or a
+// leaf above it
+__global__ void k_tree_index(TreeView tv, int* __restrict__ gidx) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = tv.glev;
+    if (c >= ((int64_t)1 << (3 * g))) return;
+    const int x = (int)(c & ((1 << g) - 1)), y = (int)((c >> g) & ((1 << g) - 1)), z = (int)(c >> (2 * g));
+    int L = 0;
+    int64_t i = 0;
+    for (; L < g; L++) {
+        if (is_leaf_type(tv.type[L][i])) break;
+        const int sh = g - L - 1;
+        const int b = ((x >> sh) & 1) | (((y >> sh) & 1) << 1) | (((z >> sh) & 1) << 2);
+        i = (int64_t)tv.child[L][i] + b;
+    }
+    gidx[c] = (L << 27) | (int)i;
+}
+int tree_index_level(int dp) { return min(dp, 7); }
+void launch_tree_index(TreeView tv, int* gidx, hipStream_t s) {
+    const int64_t n = (int64_t)1 << (3 * tv.glev);
+    k_tree_index<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tv, gidx);
+}
+// where the descent to finest cell (qx, qy, qz) resumes: its index cell's node
+__device__ __forceinline__ void index_start(const TreeView& tv, int qx, int qy, int qz, int& L, int64_t& i) {
+    const int gs = tv.dp - tv.glev, g = tv.glev;
+    const int64_t c = (int64_t)(qx >> gs) | ((int64_t)(qy >> gs) << g) | ((int64_t)(qz >> gs) << (2 * g));
+    const int e = tv.gidx[c];
+    L = e >> 27;
+    i = e & ((1 << 27) - 1);
+}
+
 // the 8 leaves around the point whose records start at sorted position j (at most one record per
 // tile, and a point lies in at most 8 tiles' leaves, so its run is at most 8 long)
 __device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx, const SortedKV& kv, int64_t nent,
@@ -229,11 +260,29 @@ __device__ __forceinline__ void gather_point(const TreeView& tv, const int* lidx
     bool missing = false;
     for (int o = 0; o < 8; o++) missing = missing || n[o] < 0;
     if (missing) {
+        // The descents start from the tree index, not the root. When a coordinate of P is a
+        // multiple of the index cell size the 8 octant cells lie in different index cells, and
+        // each missing octant descends from its own.
+        const int gm = (1 << (tv.dp - tv.glev)) - 1;
+        if ((px & gm) == 0 || (py & gm) == 0 || (pz & gm) == 0) {
+            for (int o = 0; o < 8; o++)
+                if (n[o] < 0) {
+                    const int qx = px - 1 + (o & 1), qy = py - 1 + ((o >> 1) & 1), qz = pz - 1 + ((o >> 2) & 1);
+                    int L0;
+                    int64_t i0;
+                    index_start(tv, qx, qy, qz, L0, i0);
+                    n[o] = locate_from(tv, lidx, L0, i0, qx, qy, qz);
+                }
+            missing = false;
+        }
+    }
+    if (missing) {
         // The finest cells of the 8 octants, P - 1 or P per axis, take the same child of a node
         // whose children have size 2^sh as long as no coordinate of P is a multiple of 2^sh, so
         // their descents share the path down to that node and are walked once.
-        int L = 0;
-        int64_t i = 0;
+        int L;
+        int64_t i;
+        index_start(tv, px, py, pz, L, i);
         int shared = -1;
         for (; L < tv.nlev; L++) {
             const uint8_t t = tv.type[L][i];

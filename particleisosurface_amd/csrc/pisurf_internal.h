@@ -248,7 +248,14 @@ struct TreeView {
     int64_t off[kMaxLevels];           // global node offset of each level
     const uint8_t* type[kMaxLevels];
     const int* child[kMaxLevels];
+    // index of the tree at level glev (<= dp): for each of the (2^glev)^3 cells, the node of level
+    // glev containing it, or the leaf above glev that does, as (level << 27) | level-local index
+    const int* gidx = nullptr;
+    int glev = 0;
 };
+// levels of the tree index of a tree of deepest level dp, and its cell count
+int tree_index_level(int dp);
+void launch_tree_index(TreeView tv, int* gidx, hipStream_t s);
 // the levels of a built tree as the extraction's per-node passes read them
 struct LeafLevels {
     int nlev;                          // levels present
