@@ -209,8 +209,10 @@ int pisurf_run_frame(pisurf_ctx* ctx, pisurf_tree_info* tree);
  * lane 0 is this context and the others are contexts (each with its own stream and buffers) that
  * this context creates on first use and keeps. One lane's kernels fill the time another spends
  * waiting for a count or launching small kernels. Callbacks come one at a time and in frame order,
- * from the lanes' threads (lane 0's is the caller's). When a frame fails, the frames before it
- * have been delivered, none after it is, and the error names the frame.
+ * from the lanes' threads (lane 0's is the caller's); while one runs, other lanes, this context
+ * among them, may still be meshing, so a callback must not call the library on this context.
+ * When a frame fails, the frames before it have been delivered, none after it is, and the error
+ * names the frame.
  * Afterwards pisurf_get_stats reports the last frame meshed on this context. */
 typedef void (*pisurf_mesh_cb)(void* user, int32_t frame, const pisurf_tree_info* tree, const double* verts,
                                int64_t n_verts, const int32_t* tris, int64_t n_tris);
