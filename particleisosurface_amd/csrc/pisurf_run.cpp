@@ -1,6 +1,6 @@
 // Frame driver over a data directory, as the reference's runOurs (src/main.cpp:330-405) with
 // loadConfigJson (src/main.cpp:116-213): reads <data_dir>/controlData.json, loads every frame
-// (.csv, .h5, or raw float32), meshes all frames in one persistent pisurf_run_frames call and
+// (.csv, .h5, or raw float32), meshes all frames in one pisurf_run_frames call and
 // writes <out_dir>/<frame name>.obj or .ply (OUTPUT_TYPE).
 //   pisurf_run <data_dir> [out_dir] [--precision 0|1|2] [--load-only]
 // out_dir defaults to <data_dir>/out (src/main.cpp:590). --load-only parses the inputs and prints
