@@ -257,6 +257,15 @@ struct pisurf_ctx {
     DBuf samples, cl_flag, cl_pos, cl_list;
     DBuf lvcnt;  // node counts of the levels built from device-resident counts
     hipGraphExec_t top_exec = nullptr;  // those levels' launches as an executable graph
+    // a frame issues those levels on the mesh-download stream (dstream, idle until the bisection;
+    // a stream of their own would take a hardware queue from the lanes of a multi-frame call),
+    // with their own scan workspace, right after the grid, so they run beside the anisotropy
+    hipEvent_t tev_grid = nullptr, tev_top = nullptr;
+    DBuf tmp_top;
+    bool top_pending = false;  // issued on dstream, counts not read yet
+    bool lane = false;         // meshing as a lane of pisurf_run_frames (whole frames overlap instead)
+    int top_t = 0;             // levels 0 .. top_t - 1 built there
+    std::vector<int64_t> top_ub;
     // sample deduplication: keys / entries (ping-pong), heads, scanned heads, representatives,
     // head list, per-point values
     DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals, sd_cwork;
@@ -305,6 +314,8 @@ struct pisurf_ctx {
         }
         host_read_release(hr);
         if (top_exec) (void)hipGraphExecDestroy(top_exec);
+        if (tev_grid) (void)hipEventDestroy(tev_grid);
+        if (tev_top) (void)hipEventDestroy(tev_top);
         if (wev) (void)hipEventDestroy(wev);
         for (auto* l : levels) delete l;
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -842,21 +853,38 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     return PISURF_OK;
 }
 
-// The levels above D_min take no samples: a node there is INTERNAL exactly when it is not empty
-// (afterSampleEval, src/surface_reconstructor.cpp:237-273, with depth < D_min). They are built
-// with their node counts kept in device memory and their arrays and grids sized by an upper bound,
-// so the host issues every launch without waiting; the counts are read once, together with the
-// count of the first level built the ordinary way. Level 0 (the root) is in place. Returns, in
-// *next, the index of that first ordinary level.
+// bound on the nodes of a level built from device-resident counts
 constexpr int64_t kTopMaxNodes = int64_t(1) << 22;
 
-int top_levels(pisurf_ctx* c, size_t* next) {
-    *next = 0;
+// the root node (resizeRootBoxConstR) as level 0, every level's count reset
+int tree_root(pisurf_ctx* c, hipStream_t s) {
+    resize_root(c);
+    for (auto* l : c->levels) l->n = 0;
+    while (c->levels.empty()) c->levels.push_back(new LevelBuf());
+    LevelBuf* L = c->levels[0];
+    L->n = 1;
+    HIPCHK(c, L->ensure(1));
+    const float4 rc4 = make_float4(c->root_c[0], c->root_c[1], c->root_c[2], c->root_half);
+    const int8_t d0 = 0;
+    const uint64_t id0 = 0;
+    const int4 ip0 = make_int4(0, 0, 0, 0);
+    HIPCHK(c, hipMemcpyAsync(L->ip.p, &ip0, sizeof(int4), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(L->c.p, &rc4, sizeof(float4), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(L->depth.p, &d0, 1, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(L->id.p, &id0, 8, hipMemcpyHostToDevice, s));
+    return PISURF_OK;
+}
+
+// The launches of the levels above D_min on stream s with scan workspace tmp (counts stay on the
+// device until top_levels_finish reads them).
+int top_levels_issue(pisurf_ctx* c, hipStream_t s, DBuf& tmp) {
+    c->top_t = 0;
+    c->top_ub.assign(1, 1);
     const int top = c->dmin;
     if (top <= 0) return PISURF_OK;
     // bounds: a level-l node that is not empty meets the particles' box grown by the margin, so
     // level l holds at most geo(l) non-empty nodes and level l + 1 at most 8 geo(l) nodes
-    std::vector<int64_t> ub(1, 1);
+    std::vector<int64_t>& ub = c->top_ub;
     const double m = double(c->prm.radius) * double(c->prm.smooth_factor);
     for (int l = 1; l <= top; l++) {
         const double size = std::ldexp(2.0 * (double)c->root_half, -(l - 1));
@@ -881,9 +909,9 @@ int top_levels(pisurf_ctx* c, size_t* next) {
         L->n = ub[l];
         tmp_need = std::max(tmp_need, scan_tmp_ints(ub[l] + 1) + 64);
     }
-    if ((size_t)tmp_need * 4 > c->tmp.cap) {
-        HIPCHK(c, c->wait());
-        HIPCHK(c, c->tmp.ensure(tmp_need * 4));
+    if ((size_t)tmp_need * 4 > tmp.cap) {
+        HIPCHK(c, hipStreamSynchronize(s));
+        HIPCHK(c, tmp.ensure(tmp_need * 4));
     }
     HIPCHK(c, c->lvcnt.ensure(64 * sizeof(int)));
     int* cnt = c->lvcnt.as<int>();
@@ -891,7 +919,7 @@ int top_levels(pisurf_ctx* c, size_t* next) {
     const FieldConst fc = field_const(c);
     const TreeParams tp = tree_params(c);
     auto issue = [&]() {
-        launch_fill_i32(cnt, 1, 1, c->stream);
+        launch_fill_i32(cnt, 1, 1, s);
         for (int l = 0; l < t; l++) {
             LevelBuf* L = c->levels[l];
             NodeLevel v = L->view();
@@ -900,12 +928,11 @@ int top_levels(pisurf_ctx* c, size_t* next) {
             const double w = (2 * half + 2 * (double)tp.margin) / (double)c->g.cell + 1;
             const int group = (w * w <= 6) ? 4 : (w * w <= 8) ? 8 : (w * w <= 16) ? 16 : 64;
             launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group, kEmptyAndCurv, nullptr, 0,
-                              c->stream);
-            launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, nullptr, c->ctr(),
-                            c->stream);
-            launch_fill_i32(L->flag.as<int>() + ub[l], 1, 0, c->stream);
-            exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), ub[l] + 1, c->tmp.as<int>(), c->stream);
-            launch_children_counted(v, L->scanned.as<int>(), c->levels[l + 1]->view(), cnt + l + 1, c->stream);
+                              s);
+            launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, nullptr, c->ctr(), s);
+            launch_fill_i32(L->flag.as<int>() + ub[l], 1, 0, s);
+            exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), ub[l] + 1, tmp.as<int>(), s);
+            launch_children_counted(v, L->scanned.as<int>(), c->levels[l + 1]->view(), cnt + l + 1, s);
         }
     };
     // With level_graph the launches (kernels only: fills are kernels too) are captured as a HIP
@@ -914,10 +941,10 @@ int top_levels(pisurf_ctx* c, size_t* next) {
     // instantiate anew. The update and the graph launch cost about as much host time as the 40-odd
     // launches they replace, so the launches go to the stream directly by default.
     bool graphed = false;
-    if (c->prm.level_graph && hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    if (c->prm.level_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
         issue();
         hipGraph_t gr = nullptr;
-        const hipError_t ec = hipStreamEndCapture(c->stream, &gr);
+        const hipError_t ec = hipStreamEndCapture(s, &gr);
         if (ec == hipSuccess && gr) {
             hipError_t e = hipErrorNotInitialized;
             if (c->top_exec) {
@@ -933,7 +960,7 @@ int top_levels(pisurf_ctx* c, size_t* next) {
             if (!c->top_exec) e = hipGraphInstantiate(&c->top_exec, gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
             if (e != hipSuccess) return c->hipfail(e, "octree level graph");
-            HIPCHK(c, hipGraphLaunch(c->top_exec, c->stream));
+            HIPCHK(c, hipGraphLaunch(c->top_exec, s));
             graphed = true;
         } else {
             if (gr) (void)hipGraphDestroy(gr);
@@ -942,13 +969,55 @@ int top_levels(pisurf_ctx* c, size_t* next) {
     }
     if (!graphed) issue();
     HIPCHK(c, hipGetLastError());
+    c->top_t = t;
+    return PISURF_OK;
+}
+
+// The counts of the issued levels (one read on the context's stream, which must follow their
+// launches); *next = the index of the first level built the ordinary way.
+int top_levels_finish(pisurf_ctx* c, size_t* next) {
+    *next = 0;
+    const int t = c->top_t;
+    if (t <= 0) return PISURF_OK;
     std::vector<int> n(t + 1);
-    HIPCHK(c, c->read(n.data(), cnt, (t + 1) * sizeof(int)));
+    HIPCHK(c, c->read(n.data(), c->lvcnt.as<int>(), (t + 1) * sizeof(int)));
     for (int l = 0; l <= t; l++) {
-        if (n[l] < 0 || n[l] > ub[l]) return c->fail(PISURF_E_STATE, "octree level count out of its bound");
+        if (n[l] < 0 || n[l] > c->top_ub[l]) return c->fail(PISURF_E_STATE, "octree level count out of its bound");
         c->levels[l]->n = n[l];
     }
     *next = (size_t)t;
+    return PISURF_OK;
+}
+
+// The levels above D_min take no samples: a node there is INTERNAL exactly when it is not empty
+// (afterSampleEval, src/surface_reconstructor.cpp:237-273, with depth < D_min). They are built
+// with their node counts kept in device memory and their arrays and grids sized by an upper bound,
+// so the host issues every launch without waiting; the counts are read once, together with the
+// count of the first level built the ordinary way. Level 0 (the root) is in place. Returns, in
+// *next, the index of that first ordinary level.
+int top_levels(pisurf_ctx* c, size_t* next) {
+    int rc = top_levels_issue(c, c->stream, c->tmp);
+    if (rc) return rc;
+    return top_levels_finish(c, next);
+}
+
+// A frame's root and levels above D_min, issued on a second stream as soon as the grid is built
+// (they read the grid, the bounding box and the constants, and classify by emptiness alone: no
+// samples, normals or particle records), so they run while the anisotropy and the normals run on
+// the context's stream; do_tree joins them. The second stream is the one the mesh download uses
+// at the end of the frame.
+int tree_start(pisurf_ctx* c) {
+    c->top_pending = false;
+    if (!c->dstream) HIPCHK(c, hipStreamCreateWithFlags(&c->dstream, hipStreamNonBlocking));
+    if (!c->tev_grid) HIPCHK(c, hipEventCreateWithFlags(&c->tev_grid, hipEventDisableTiming));
+    if (!c->tev_top) HIPCHK(c, hipEventCreateWithFlags(&c->tev_top, hipEventDisableTiming));
+    HIPCHK(c, hipEventRecord(c->tev_grid, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->dstream, c->tev_grid, 0));
+    int rc = tree_root(c, c->dstream);
+    if (!rc) rc = top_levels_issue(c, c->dstream, c->tmp_top);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->tev_top, c->dstream));
+    c->top_pending = true;
     return PISURF_OK;
 }
 
@@ -956,27 +1025,21 @@ int do_tree(pisurf_ctx* c) {
     int rc = frame_ready(c);
     if (rc) return rc;
     const double t0 = now_ms();
-    resize_root(c);
-    for (auto* l : c->levels) l->n = 0;
     size_t li = 0;
     auto level = [&](size_t i) {
         while (c->levels.size() <= i) c->levels.push_back(new LevelBuf());
         return c->levels[i];
     };
-    LevelBuf* L = level(0);
-    L->n = 1;
-    HIPCHK(c, L->ensure(1));
-    {
-        const float4 rc4 = make_float4(c->root_c[0], c->root_c[1], c->root_c[2], c->root_half);
-        const int8_t d0 = 0;
-        const uint64_t id0 = 0;
-        const int4 ip0 = make_int4(0, 0, 0, 0);
-        HIPCHK(c, hipMemcpyAsync(L->ip.p, &ip0, sizeof(int4), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(L->c.p, &rc4, sizeof(float4), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(L->depth.p, &d0, 1, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(L->id.p, &id0, 8, hipMemcpyHostToDevice, c->stream));
+    if (c->top_pending) {
+        // root and levels above D_min already issued on dstream (tree_start)
+        c->top_pending = false;
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->tev_top, 0));
+        if ((rc = top_levels_finish(c, &li))) return rc;
+    } else {
+        if ((rc = tree_root(c, c->stream))) return rc;
+        if ((rc = top_levels(c, &li))) return rc;
     }
-    if ((rc = top_levels(c, &li))) return rc;
+    LevelBuf* L = nullptr;
     while (true) {
         L = level(li);
         if (L->n == 0) break;  // every node above D_min empty
@@ -2492,12 +2555,26 @@ int pisurf_run_frame(pisurf_ctx* c, pisurf_tree_info* tree) {
     if (c->n <= 0) return c->fail(PISURF_E_STATE, "no particles uploaded");
     const double t0 = now_ms();
     c->bbox_done = c->grid_done = c->aniso_done = c->normals_done = c->tree_done = c->consts_done = false;
+    c->top_pending = false;
     int rc = do_grid(c, nullptr, 4.0f);
-    if (!rc) rc = do_aniso(c);
     if (!rc) {
+        // The anisotropy passes go to the GPU first; the levels above D_min are issued on their own
+        // stream while those run (issuing their 40-odd launches takes the host about 0.4 ms), and
+        // only then does the host wait for the anisotropy's count. The constants depend on the
+        // parameters only; those levels write the iso value into their empty nodes.
         compute_constants(c);
-        rc = frame_ready(c);
+        const double ta = now_ms();
+        rc = aniso_launch(c);
+        // (lanes of a multi-frame call overlap whole frames instead)
+        if (!rc && !c->lane) rc = tree_start(c);
+        if (!rc) {
+            unsigned long long nfp = 0;
+            HIPCHK(c, c->read(&nfp, aniso_nfp_dev(c), 8));
+            aniso_finish(c, nfp);
+            c->st.ms_aniso += now_ms() - ta;
+        }
     }
+    if (!rc) rc = frame_ready(c);
     if (!rc) rc = pisurf_build_tree(c, tree);
     if (!rc) rc = do_extract(c);
     if (rc) return rc;
@@ -2523,6 +2600,15 @@ static int run_frames_lanes(pisurf_ctx* c, int K, int32_t nframes, const float* 
     std::vector<pisurf_ctx*> lane(K);
     lane[0] = c;
     for (int i = 1; i < K; i++) lane[i] = c->lanes[i - 1];
+    struct LaneMode {  // the contexts mesh as lanes for this call only
+        std::vector<pisurf_ctx*>& l;
+        explicit LaneMode(std::vector<pisurf_ctx*>& v) : l(v) {
+            for (auto* x : l) x->lane = true;
+        }
+        ~LaneMode() {
+            for (auto* x : l) x->lane = false;
+        }
+    } lane_mode(lane);
     const pisurf_params base = c->prm;
     // one frame on lane L: its particles copied on the lane's stream (the frame's kernels follow
     // in stream order, and the frame's own read-backs have completed the copy before the staging
