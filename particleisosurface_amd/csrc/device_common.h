@@ -294,12 +294,15 @@ __device__ __forceinline__ int field_sum(const GridDev& g, const Recs<R>& rec, c
         for (int j = j0; j < e; j += 32) {
             const int jn = min(e, j + 32);
             unsigned m = 0;
-            for (int u = j; u < jn; u += 4) {
-                typename Vec4<R>::T a[4];
+            // (the per-row form, LEAN, requests eight records at a time: it is bound by their
+            // latency, not by registers)
+            constexpr int CS = LEAN ? 8 : 4;
+            for (int u = j; u < jn; u += CS) {
+                typename Vec4<R>::T a[CS];
 #pragma unroll
-                for (int k = 0; k < 4; k++) a[k] = rec.a[min(u + k, jn - 1)];
+                for (int k = 0; k < CS; k++) a[k] = rec.a[min(u + k, jn - 1)];
 #pragma unroll
-                for (int k = 0; k < 4; k++)
+                for (int k = 0; k < CS; k++)
                     m |= unsigned(!(dist2(px - a[k].x, py - a[k].y, pz - a[k].z) >= a[k].w)) << (u - j + k);
             }
             if (jn - j < 32) m &= (1u << (jn - j)) - 1u;
