@@ -1,6 +1,8 @@
 // Octree level sweep: node emptiness and curvature, 27-point sampling, classification and
 // child generation (SurfReconstructor::genIsoOurs state 0, src/surface_reconstructor.cpp:152-407;
 // TNode, src/iso_method_ours.cpp:16-99). One launch of each kernel covers all nodes of a level.
+#include <climits>
+
 #include "pisurf_internal.h"
 #include "qef_device.h"
 #include "run_eval.h"
@@ -240,8 +242,8 @@ __device__ __forceinline__ bool same_point(const float p[3], const float q[3]) {
            __float_as_uint(q[1]) == __float_as_uint(p[1]) && __float_as_uint(q[2]) == __float_as_uint(p[2]);
 }
 __global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* __restrict__ list, SortedKV kv,
-                                                      int64_t nent, int* __restrict__ block_heads,
-                                                      int* __restrict__ rep) {
+                                                      const int* __restrict__ ent, int64_t nent,
+                                                      int* __restrict__ block_heads, int* __restrict__ rep) {
     __shared__ int wsum[4];
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (kWave - 1);
@@ -249,7 +251,7 @@ __global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* _
     float p[3] = {0.f, 0.f, 0.f};
     int64_t s0 = j;
     if (act) {
-        entry_point(lv, list, kv.val(j), p);
+        entry_point(lv, list, ent ? ent[kv.val(j)] : kv.val(j), p);
         const uint64_t kj = kv.key(j);
         while (s0 > 0 && j - s0 < 64 && kv.key(s0 - 1) == kj) s0--;
     }
@@ -258,7 +260,7 @@ __global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* _
     int64_t r = j;
     for (int64_t k = s0; k < lo && r == j; k++) {
         float q[3];
-        entry_point(lv, list, kv.val(k), q);
+        entry_point(lv, list, ent ? ent[kv.val(k)] : kv.val(k), q);
         if (same_point(p, q)) r = k;
     }
     float q[8][3];
@@ -442,6 +444,214 @@ static int build_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist
     return nr;
 }
 
+// ---- sample points merged per tile of nodes, before the global sort -------------------------
+// The listed nodes of a level (one node size) are taken in tiles of 256 consecutive list entries
+// (children follow their parents, so a tile is mostly a compact piece of space). Sample s of node
+// ip lies on lattice point 2 ip + t(s), t(s) in {0, 1, 2}^3; the nodes that can share it are
+// ip + d with d_a in {-1, 0} when t_a = 0, {0} when t_a = 1 and {0, +1} when t_a = 2. Of those,
+// the ones with a -1 component (the others come later in Morton order) are looked up in a dense
+// LDS map over the tile's bounding box, and the entry refers to the one first in the list, if it
+// is earlier than the entry's own node and that sample's coordinates are bitwise identical to its
+// own. References point to earlier nodes, so following them ends at an entry with none, which owns
+// a record; the global heads pass, which compares coordinates, still merges identical points of
+// different records. A tile whose box (grown by one) exceeds the map gives every entry its own
+// record. Tile b writes its records to slots b * kStRecs + u, u < tcount[b] (the sort key of
+// k_sample_keys in fkey, the owning entry in rent) and every entry the slot of its record in
+// ent_rec.
+constexpr int kStT = 256;           // nodes per tile
+constexpr int kStRecs = 27 * kStT;  // records per tile at most
+constexpr int kStMap = 4096;        // dense map cells
+
+__global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, const int* __restrict__ list,
+                                                       int64_t k, float ox, float oy, float oz, int lb,
+                                                       int* __restrict__ ent_rec, uint64_t* __restrict__ fkey,
+                                                       int* __restrict__ rent, int* __restrict__ tcount) {
+    __shared__ short map[kStMap];
+    __shared__ float4 cen[kStT];
+    __shared__ short ref[kStRecs];    // per tile entry (node * 27 + sample): the entry it refers to, or -1
+    __shared__ short recid[kStRecs];  // record of an entry that owns one
+    __shared__ int red[6][kStT / kWave];
+    __shared__ int wsum[kStT / kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    const int64_t l0 = (int64_t)blockIdx.x * kStT, li = l0 + t;
+    const bool act = li < k;
+    int4 ip = make_int4(0, 0, 0, 0);
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (act) {
+        const int node = list[li];
+        c = lv.c[node];
+        ip = lv.ip[node];
+        cen[t] = c;
+    }
+    // the tile's bounding box of node coordinates
+    int mn[3] = {act ? ip.x : INT_MAX, act ? ip.y : INT_MAX, act ? ip.z : INT_MAX};
+    int mx[3] = {act ? ip.x : INT_MIN, act ? ip.y : INT_MIN, act ? ip.z : INT_MIN};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[a] = min(mn[a], __shfl_xor(mn[a], o, kWave));
+            mx[a] = max(mx[a], __shfl_xor(mx[a], o, kWave));
+        }
+    if (lane == 0)
+        for (int a = 0; a < 3; a++) {
+            red[a][w] = mn[a];
+            red[3 + a][w] = mx[a];
+        }
+    for (int q = t; q < kStMap; q += kStT) map[q] = -1;
+    __syncthreads();
+    int b0[3], dim[3];
+    bool dense = true;  // uniform over the tile
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int lo = red[a][0], hi = red[3 + a][0];
+        for (int q = 1; q < kStT / kWave; q++) {
+            lo = min(lo, red[a][q]);
+            hi = max(hi, red[3 + a][q]);
+        }
+        b0[a] = lo - 1;  // one node of margin: the neighbours looked up
+        dim[a] = hi - lo + 3;
+        if (dim[a] > kStMap) dense = false;
+    }
+    if (dense && dim[0] * dim[1] * dim[2] > kStMap) dense = false;
+    if (dense && act) map[(ip.x - b0[0]) + dim[0] * ((ip.y - b0[1]) + dim[1] * (ip.z - b0[2]))] = (short)t;
+    __syncthreads();
+    int own = 0;
+    if (act) {
+        const int m0 = (ip.x - b0[0]) + dim[0] * ((ip.y - b0[1]) + dim[1] * (ip.z - b0[2]));
+#pragma unroll
+        for (int sm = 0; sm < 27; sm++) {
+            const int ta[3] = {sm % 3, (sm / 3) % 3, sm / 9};
+            int best = t, bs = sm;
+            if (dense) {
+#pragma unroll
+                for (int d = 1; d < 8; d++) {
+                    int dd[3];
+                    bool ok = true, earlier = false;
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        const int bit = (d >> a) & 1;
+                        dd[a] = bit ? (ta[a] == 0 ? -1 : (ta[a] == 2 ? 1 : 0)) : 0;
+                        if (bit && dd[a] == 0) ok = false;
+                        if (dd[a] < 0) earlier = true;
+                    }
+                    if (!ok || !earlier) continue;
+                    const int j = map[m0 + dd[0] + dim[0] * (dd[1] + dim[1] * dd[2])];
+                    if (j >= 0 && j < best) {
+                        best = j;
+                        bs = (ta[0] - 2 * dd[0]) + 3 * (ta[1] - 2 * dd[1]) + 9 * (ta[2] - 2 * dd[2]);
+                    }
+                }
+            }
+            short r = -1;
+            if (best != t) {
+                float p[3], q[3];
+                sample_point(c, sm, p);
+                sample_point(cen[best], bs, q);
+                if (same_point(p, q)) r = (short)(best * 27 + bs);
+            }
+            ref[t * 27 + sm] = r;
+            own += r < 0 ? 1 : 0;
+        }
+    }
+    // records numbered in entry order (thread, then sample)
+    const int incl = wave_incl_scan(own, lane);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int u = incl - own, total = 0;
+#pragma unroll
+    for (int q = 0; q < kStT / kWave; q++) {
+        if (q < w) u += wsum[q];
+        total += wsum[q];
+    }
+    const int64_t b = blockIdx.x;
+    if (t == 0) tcount[b] = total;
+    if (act)
+        for (int sm = 0; sm < 27; sm++)
+            if (ref[t * 27 + sm] < 0) recid[t * 27 + sm] = (short)u++;
+    __syncthreads();
+    // entry by entry (coalesced): the record slot of every entry, the key and entry of every record
+    const int ne = (int)min<int64_t>(k - l0, kStT) * 27;
+    const float o[3] = {ox, oy, oz};
+    const uint32_t m = (1u << lb) - 1;
+    for (int e = t; e < ne; e += kStT) {
+        int r = e;
+        while (ref[r] >= 0) r = ref[r];
+        const int64_t slot = b * kStRecs + recid[r];
+        ent_rec[l0 * 27 + e] = (int)slot;
+        if (r != e) continue;
+        const float4 cc = cen[e / 27];
+        float p[3];
+        sample_point(cc, e % 27, p);
+        uint32_t qq[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float v = rintf((p[a] - o[a]) / cc.w);
+            qq[a] = (uint32_t)fminf(fmaxf(v, 0.0f), 2097151.0f);
+        }
+        fkey[slot] = (cell_group(g, p[0], p[1], p[2]) << (3 * lb)) | (qq[0] & m) | ((qq[1] & m) << lb) |
+                     ((uint64_t)(qq[2] & m) << (2 * lb));
+        rent[slot] = (int)(l0 * 27 + e);
+    }
+}
+
+// the tiles' record keys in one array (tile b from tbase[b] on), each packed with its slot when
+// vbits > 0 (else the slot in rval)
+__global__ void __launch_bounds__(256) k_sample_records_compact(const int* __restrict__ tcount,
+                                                                 const int* __restrict__ tbase,
+                                                                 const uint64_t* __restrict__ fkey, int vbits,
+                                                                 uint64_t* __restrict__ rkey, int* __restrict__ rval) {
+    const int64_t b = blockIdx.x;
+    const int n = tcount[b];
+    const int64_t d0 = tbase[b];
+    for (int u = threadIdx.x; u < n; u += 256) {
+        const int64_t r = b * kStRecs + u;
+        if (vbits) {
+            rkey[d0 + u] = (fkey[r] << vbits) | (uint64_t)r;
+        } else {
+            rkey[d0 + u] = fkey[r];
+            rval[d0 + u] = (int)r;
+        }
+    }
+}
+
+// the head (index into the evaluated points) of every record slot: rep[j] is the first equal
+// record within j's 64-record window, and a longer run can chain further back (rep[x] <= x)
+__global__ void k_record_heads(SortedKV kv, const int* __restrict__ rep, const int* __restrict__ hpos, int64_t nrec,
+                               int* __restrict__ rec_head) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nrec) return;
+    int r = rep[j];
+    while (rep[r] != r) r = rep[r];
+    rec_head[kv.val(j)] = hpos[r];
+}
+
+// every sample entry receives its point's value
+__global__ void k_entry_scatter(const int* __restrict__ list, const int* __restrict__ ent_rec,
+                                const int* __restrict__ rec_head, int64_t nent, const void* __restrict__ vals, int rsz,
+                                void* __restrict__ samples) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nent) return;
+    const int64_t dst = (int64_t)list[e / 27] * 27 + e % 27;
+    const int64_t src = rec_head[ent_rec[e]];
+    if (rsz == 8) ((double*)samples)[dst] = ((const double*)vals)[src];
+    else ((float*)samples)[dst] = ((const float*)vals)[src];
+}
+
+int64_t sample_tiles(int64_t listed) { return (listed + kStT - 1) / kStT; }
+int64_t sample_tile_records() { return kStRecs; }
+void launch_sample_tiles(GridDev g, NodeLevel lv, const int* list, int64_t k, const float origin[3], int lb,
+                         int* ent_rec, uint64_t* fkey, int* rent, int* tcount, hipStream_t s) {
+    if (k > 0)
+        k_sample_tiles<<<(unsigned)sample_tiles(k), kStT, 0, s>>>(g, lv, list, k, origin[0], origin[1], origin[2], lb,
+                                                                 ent_rec, fkey, rent, tcount);
+}
+void launch_sample_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey,
+                                   int vbits, uint64_t* rkey, int* rval, hipStream_t s) {
+    if (ntiles > 0)
+        k_sample_records_compact<<<(unsigned)ntiles, 256, 0, s>>>(tcount, tbase, fkey, vbits, rkey, rval);
+}
+
 // distinct sample points of a level (cell-major), evaluated run by run
 template <class R> struct SampleSrc {
     NodeLevel lv;
@@ -450,7 +660,11 @@ template <class R> struct SampleSrc {
     const int* hlist;
     R* out;
     R iso, sigma;
-    __device__ __forceinline__ void point(int h, float p[3]) const { entry_point(lv, list, kv.val(hlist[h]), p); }
+    const int* ent = nullptr;  // sorted item -> sample entry (nullptr: the items are the entries)
+    __device__ __forceinline__ void point(int h, float p[3]) const {
+        const int j = kv.val(hlist[h]);
+        entry_point(lv, list, ent ? ent[j] : j, p);
+    }
     __device__ __forceinline__ void store(int h, R sum, const R*) const { out[h] = iso - sigma * sum; }
 };
 
@@ -459,10 +673,10 @@ __global__ void __launch_bounds__(T) k_run_samples(GridDev g, Recs<R> rec, Field
                                                              const int* __restrict__ list, SortedKV kv,
                                                              const int* __restrict__ hlist,
                                                              const int* __restrict__ rstart, R* __restrict__ out,
-                                                             Counters* ctr) {
+                                                             const int* __restrict__ ent, Counters* ctr) {
     __shared__ RunLds<R, K && sizeof(R) == 4> lds;
     const int h0 = rstart[blockIdx.x], h1 = rstart[blockIdx.x + 1];
-    SampleSrc<R> src{lv, list, kv, hlist, out, R(fc.iso), out_sigma<R, K>(fc)};
+    SampleSrc<R> src{lv, list, kv, hlist, out, R(fc.iso), out_sigma<R, K>(fc), ent};
     const unsigned long long vis = run_eval<T, R, K, false, MODE>(g, rec, fc, src, h0, h1, lds);
     Counters* c = ctr_stripe(ctr);
     wave_count(&c->sample_candidates, vis);
@@ -704,7 +918,7 @@ void launch_heads(NodeLevel lv, const int* list, SortedKV kv, int64_t nent, int*
     if (nent <= 0) return;
     const unsigned nb = (unsigned)((nent + 255) / 256);
     int* block_off = head + (nb + 1);
-    k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, nent, head, rep);
+    k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, nullptr, nent, head, rep);
     launch_fill_i32(head + nb, 1, 0, s);
     exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
     k_head_list<<<nb, 256, 0, s>>>(rep, nent, block_off, hlist, hpos);
@@ -868,21 +1082,21 @@ void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t
 // one launch of the sample runs with a given block size (every run one workgroup)
 static void launch_run_samples(int prec, int T, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
                                const int* list, SortedKV kv, const int* hlist, const int* rstart, void* vals, int nr,
-                               Counters* ctr, hipStream_t s) {
+                               const int* ent, Counters* ctr, hipStream_t s) {
     auto go = [&](auto tag, auto mode) {
         constexpr int TT = decltype(tag)::value, M = decltype(mode)::value;
         if (prec == 2) {
             Recs<double> r{(const double4*)ra, (const double4*)rg};
             k_run_samples<double, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart, (double*)vals,
-                                                                  ctr);
+                                                                  ent, ctr);
         } else {
             Recs<float> r{(const float4*)ra, (const float4*)rg};
             if (prec == 1)
                 k_run_samples<float, true, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart, (float*)vals,
-                                                                    ctr);
+                                                                    ent, ctr);
             else
                 k_run_samples<float, false, TT, M><<<nr, TT, 0, s>>>(g, r, fc, lv, list, kv, hlist, rstart,
-                                                                     (float*)vals, ctr);
+                                                                     (float*)vals, ent, ctr);
         }
     };
     using M0 = std::integral_constant<int, kModeAny>;
@@ -893,21 +1107,22 @@ static void launch_run_samples(int prec, int T, GridDev g, const void* ra, const
 }
 
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv,
-                               const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
-                               int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
-                               int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr,
-                               hipEvent_t e0, hipEvent_t e1, HostRead* rd, hipStream_t s, hipError_t* err) {
+                               const int* list, SortedKV kv, const int* ent, int gshift, int64_t nrec,
+                               const int* ent_rec, int64_t nent, int* rec_head, int* head, int* hpos, int* rep,
+                               int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out, int* kx_out,
+                               void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr, hipEvent_t e0,
+                               hipEvent_t e1, HostRead* rd, hipStream_t s, hipError_t* err) {
     *nheads_out = 0;
     *nruns_out = 0;
     *kx_out = 0;
-    if (nent <= 0) return;
+    if (nrec <= 0) return;
     // head holds the per-block head counts (nb + 1) and their scan (nb + 1)
-    const unsigned nb = (unsigned)((nent + 255) / 256);
+    const unsigned nb = (unsigned)((nrec + 255) / 256);
     int* block_off = head + (nb + 1);
-    k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, nent, head, rep);
+    k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, ent, nrec, head, rep);
     launch_fill_i32(head + nb, 1, 0, s);
     exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
-    k_head_list<<<nb, 256, 0, s>>>(rep, nent, block_off, hlist, hpos);
+    k_head_list<<<nb, 256, 0, s>>>(rep, nrec, block_off, hlist, hpos);
     int nh = 0;
     const hipError_t e = read_device(&nh, block_off + nb, 4, s, rd);
     if (e != hipSuccess) {
@@ -927,15 +1142,20 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
                 return nr > 0;
             },
             [&](RunShape c, bool trial) {
-                launch_run_samples(prec, c.T, g, ra, rg, fc, lv, list, kv, hlist, rstart, vals, nr,
+                launch_run_samples(prec, c.T, g, ra, rg, fc, lv, list, kv, hlist, rstart, vals, nr, ent,
                                    trial ? trial_ctr : ctr, s);
             });
         *kx_out = sh.kx;
     }
     *nruns_out = nr;
     (void)hipEventRecord(e1, s);
-    k_sample_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, kv, rep, hpos, nent, vals,
-                                                                    prec == 2 ? 8 : 4, samples);
+    if (!ent_rec) {
+        k_sample_scatter<<<nb, 256, 0, s>>>(list, kv, rep, hpos, nrec, vals, prec == 2 ? 8 : 4, samples);
+        return;
+    }
+    k_record_heads<<<nb, 256, 0, s>>>(kv, rep, hpos, nrec, rec_head);
+    k_entry_scatter<<<(unsigned)((nent + 255) / 256), 256, 0, s>>>(list, ent_rec, rec_head, nent, vals,
+                                                                   prec == 2 ? 8 : 4, samples);
 }
 
 void launch_centers(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, NodeLevel lv, const int* list,

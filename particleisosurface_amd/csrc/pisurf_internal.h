@@ -180,11 +180,26 @@ void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec
 // number of cells of a row per run. Counts are read back through rd; a failed read sets *err.
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
                                int lb, int vbits, uint64_t* key, int* val, hipStream_t s);
+// Sample records merged per tile of 256 listed nodes: every entry gets the slot of its tile record
+// (ent_rec), tile b holds tcount[b] records at slots b * sample_tile_records() + u with their sort
+// keys (fkey, as launch_sample_dedupe_keys makes them) and owning entries (rent).
+int64_t sample_tiles(int64_t listed);
+int64_t sample_tile_records();
+void launch_sample_tiles(GridDev g, NodeLevel lv, const int* list, int64_t k, const float origin[3], int lb,
+                         int* ent_rec, uint64_t* fkey, int* rent, int* tcount, hipStream_t s);
+// the tiles' keys gathered from tbase[b] on (scanned tcount), packed with their slots when vbits > 0
+void launch_sample_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey,
+                                   int vbits, uint64_t* rkey, int* rval, hipStream_t s);
+// Distinct points of nrec sorted items evaluated and their values scattered to the level's samples.
+// ent NULL: the items are the sample entries (nent == nrec, ent_rec and rec_head unused); else
+// item value v is a record slot, ent[v] its entry, ent_rec the record slot of each of nent entries
+// and rec_head (one int per slot) scratch.
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
-                               const int* list, SortedKV kv, int gshift, int64_t nent, int* head,
-                               int* hpos, int* rep, int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out,
-                               int* kx_out, void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr,
-                               hipEvent_t e0, hipEvent_t e1, HostRead* rd, hipStream_t s, hipError_t* err);
+                               const int* list, SortedKV kv, const int* ent, int gshift, int64_t nrec,
+                               const int* ent_rec, int64_t nent, int* rec_head, int* head, int* hpos, int* rep,
+                               int* hlist, int* cwork, int64_t* nheads_out, int64_t* nruns_out, int* kx_out,
+                               void* vals, void* samples, int* tmp, Counters* ctr, Counters* trial_ctr, hipEvent_t e0,
+                               hipEvent_t e1, HostRead* rd, hipStream_t s, hipError_t* err);
 // Phase-split deduplicated sampling (the steps of launch_sample_dedupe_eval and
 // launch_centers_runs cut at each count the host reads), for sweeps over many frames:
 // heads of nent sorted entries, their count at heads_count(head, nent)
