@@ -775,7 +775,7 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             // entries merged per tile of nodes first (fewer keys to sort); the record slots of the
             // last, partial tile take the slot count a little past the entry count
             const int64_t ntiles = sample_tiles(k), nslots = ntiles * sample_tile_records();
-            const bool tiles = nslots < INT32_MAX;
+            const bool tiles = nslots < INT32_MAX && L->n * 27 < INT32_MAX;
             HIPCHK(c, c->sd_key.ensure(nent * 8));
             HIPCHK(c, c->sd_key2.ensure(nent * 8));
             HIPCHK(c, c->sd_val.ensure(nent * 4));

@@ -232,6 +232,15 @@ __global__ void k_sample_keys(GridDev g, NodeLevel lv, const int* __restrict__ l
 __device__ __forceinline__ void entry_point(const NodeLevel& lv, const int* list, int e, float p[3]) {
     sample_point(lv.c[list[e / 27]], e % 27, p);
 }
+// the point of sorted item value v: entry v of the list, or with ent node * 27 + sample ent[v]
+__device__ __forceinline__ void item_point(const NodeLevel& lv, const int* list, const int* ent, int v, float p[3]) {
+    if (ent) {
+        const int ns = ent[v];
+        sample_point(lv.c[ns / 27], ns % 27, p);
+    } else {
+        entry_point(lv, list, v, p);
+    }
+}
 
 // head[j] = 1 when no earlier entry of the same key run has the same coordinates; rep[j] = sorted
 // position of the first entry with j's coordinates. A run (a lattice point shared by up to 8
@@ -251,7 +260,7 @@ __global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* _
     float p[3] = {0.f, 0.f, 0.f};
     int64_t s0 = j;
     if (act) {
-        entry_point(lv, list, ent ? ent[kv.val(j)] : kv.val(j), p);
+        item_point(lv, list, ent, kv.val(j), p);
         const uint64_t kj = kv.key(j);
         while (s0 > 0 && j - s0 < 64 && kv.key(s0 - 1) == kj) s0--;
     }
@@ -260,7 +269,7 @@ __global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* _
     int64_t r = j;
     for (int64_t k = s0; k < lo && r == j; k++) {
         float q[3];
-        entry_point(lv, list, ent ? ent[kv.val(k)] : kv.val(k), q);
+        item_point(lv, list, ent, kv.val(k), q);
         if (same_point(p, q)) r = k;
     }
     float q[8][3];
@@ -456,8 +465,8 @@ static int build_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist
 // a record; the global heads pass, which compares coordinates, still merges identical points of
 // different records. A tile whose box (grown by one) exceeds the map gives every entry its own
 // record. Tile b writes its records to slots b * kStRecs + u, u < tcount[b] (the sort key of
-// k_sample_keys in fkey, the owning entry in rent) and every entry the slot of its record in
-// ent_rec.
+// k_sample_keys in fkey, node * 27 + sample of the owning entry in rent) and every entry the slot
+// of its record in ent_rec.
 constexpr int kStT = 256;           // nodes per tile
 constexpr int kStRecs = 27 * kStT;  // records per tile at most
 constexpr int kStMap = 4096;        // dense map cells
@@ -468,6 +477,7 @@ __global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, 
                                                        int* __restrict__ rent, int* __restrict__ tcount) {
     __shared__ short map[kStMap];
     __shared__ float4 cen[kStT];
+    __shared__ int nid[kStT];
     __shared__ short ref[kStRecs];    // per tile entry (node * 27 + sample): the entry it refers to, or -1
     __shared__ short recid[kStRecs];  // record of an entry that owns one
     __shared__ int red[6][kStT / kWave];
@@ -482,6 +492,7 @@ __global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, 
         c = lv.c[node];
         ip = lv.ip[node];
         cen[t] = c;
+        nid[t] = node;
     }
     // the tile's bounding box of node coordinates
     int mn[3] = {act ? ip.x : INT_MAX, act ? ip.y : INT_MAX, act ? ip.z : INT_MAX};
@@ -591,7 +602,7 @@ __global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, 
         }
         fkey[slot] = (cell_group(g, p[0], p[1], p[2]) << (3 * lb)) | (qq[0] & m) | ((qq[1] & m) << lb) |
                      ((uint64_t)(qq[2] & m) << (2 * lb));
-        rent[slot] = (int)(l0 * 27 + e);
+        rent[slot] = nid[e / 27] * 27 + e % 27;
     }
 }
 
@@ -660,11 +671,8 @@ template <class R> struct SampleSrc {
     const int* hlist;
     R* out;
     R iso, sigma;
-    const int* ent = nullptr;  // sorted item -> sample entry (nullptr: the items are the entries)
-    __device__ __forceinline__ void point(int h, float p[3]) const {
-        const int j = kv.val(hlist[h]);
-        entry_point(lv, list, ent ? ent[j] : j, p);
-    }
+    const int* ent = nullptr;  // sorted item -> node * 27 + sample (nullptr: the items are the list's entries)
+    __device__ __forceinline__ void point(int h, float p[3]) const { item_point(lv, list, ent, kv.val(hlist[h]), p); }
     __device__ __forceinline__ void store(int h, R sum, const R*) const { out[h] = iso - sigma * sum; }
 };
 

@@ -182,7 +182,8 @@ void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t
                                int lb, int vbits, uint64_t* key, int* val, hipStream_t s);
 // Sample records merged per tile of 256 listed nodes: every entry gets the slot of its tile record
 // (ent_rec), tile b holds tcount[b] records at slots b * sample_tile_records() + u with their sort
-// keys (fkey, as launch_sample_dedupe_keys makes them) and owning entries (rent).
+// keys (fkey, as launch_sample_dedupe_keys makes them) and points (rent: node * 27 + sample; the
+// level's node count times 27 must fit an int).
 int64_t sample_tiles(int64_t listed);
 int64_t sample_tile_records();
 void launch_sample_tiles(GridDev g, NodeLevel lv, const int* list, int64_t k, const float origin[3], int lb,
@@ -192,8 +193,8 @@ void launch_sample_records_compact(const int* tcount, const int* tbase, int64_t 
                                    int vbits, uint64_t* rkey, int* rval, hipStream_t s);
 // Distinct points of nrec sorted items evaluated and their values scattered to the level's samples.
 // ent NULL: the items are the sample entries (nent == nrec, ent_rec and rec_head unused); else
-// item value v is a record slot, ent[v] its entry, ent_rec the record slot of each of nent entries
-// and rec_head (one int per slot) scratch.
+// item value v is a record slot, ent[v] its node * 27 + sample, ent_rec the record slot of each of
+// nent entries and rec_head (one int per slot) scratch.
 void launch_sample_dedupe_eval(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                                const int* list, SortedKV kv, const int* ent, int gshift, int64_t nrec,
                                const int* ent_rec, int64_t nent, int* rec_head, int* head, int* hpos, int* rep,
