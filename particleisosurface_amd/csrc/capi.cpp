@@ -269,7 +269,7 @@ struct pisurf_ctx {
     // sample deduplication: keys / entries (ping-pong), heads, scanned heads, representatives,
     // head list, per-point values
     DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals, sd_cwork;
-    DBuf sd_entrec, sd_fkey, sd_rent, sd_rechead, sd_tcount, sd_tbase;  // per-tile sample records
+    DBuf sd_entrec, sd_rent, sd_rechead, sd_nrec;  // per-tile sample records
     // scratch for API calls
     DBuf q, out1, out2, out3;
     // multi-frame path: next frame's particles, their staging and copy stream
@@ -772,21 +772,17 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
         const int64_t nent = (int64_t)k * 27;
         if (nent >= INT32_MAX) return c->fail(PISURF_E_UNSUPPORTED, "octree level has too many sample points");
         if (nent > 0) {
-            // entries merged per tile of nodes first (fewer keys to sort); the record slots of the
-            // last, partial tile take the slot count a little past the entry count
-            const int64_t ntiles = sample_tiles(k), nslots = ntiles * sample_tile_records();
-            const bool tiles = nslots < INT32_MAX && L->n * 27 < INT32_MAX;
+            // entries merged per tile of nodes first (fewer keys to sort)
+            const bool tiles = L->n * 27 < INT32_MAX;
             HIPCHK(c, c->sd_key.ensure(nent * 8));
             HIPCHK(c, c->sd_key2.ensure(nent * 8));
             HIPCHK(c, c->sd_val.ensure(nent * 4));
             HIPCHK(c, c->sd_val2.ensure(nent * 4));
             if (tiles) {
                 HIPCHK(c, c->sd_entrec.ensure(nent * 4));
-                HIPCHK(c, c->sd_fkey.ensure(nslots * 8));
-                HIPCHK(c, c->sd_rent.ensure(nslots * 4));
-                HIPCHK(c, c->sd_rechead.ensure(nslots * 4));
-                HIPCHK(c, c->sd_tcount.ensure((ntiles + 1) * 4));
-                HIPCHK(c, c->sd_tbase.ensure((ntiles + 1) * 4));
+                HIPCHK(c, c->sd_rent.ensure(nent * 4));
+                HIPCHK(c, c->sd_rechead.ensure(nent * 4));
+                HIPCHK(c, c->sd_nrec.ensure(4));
             }
             HIPCHK(c, c->sd_head.ensure((2 * ((nent + 255) / 256) + 2) * 4));
             HIPCHK(c, c->sd_hpos.ensure((nent + 1) * 4));
@@ -806,23 +802,15 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             const int lb = 3;
             const int kbits = std::min(64, bits_for((int64_t)groups + 1) + 3 * lb);
             int64_t nrec = nent;
-            int vbits;
+            const int vbits = pack_vbits(kbits, nent);
             if (tiles) {
-                launch_sample_tiles(c->g, v, L->child.as<int>(), k, origin, lb, c->sd_entrec.as<int>(),
-                                    c->sd_fkey.as<uint64_t>(), c->sd_rent.as<int>(), c->sd_tcount.as<int>(),
-                                    c->stream);
-                launch_fill_i32(c->sd_tcount.as<int>() + ntiles, 1, 0, c->stream);
-                exclusive_scan_i32(c->sd_tcount.as<int>(), c->sd_tbase.as<int>(), ntiles + 1, c->tmp.as<int>(),
-                                   c->stream);
+                launch_sample_tiles(c->g, v, L->child.as<int>(), k, origin, lb, vbits, c->sd_nrec.as<int>(),
+                                    c->sd_entrec.as<int>(), c->sd_key.as<uint64_t>(), c->sd_val.as<int>(),
+                                    c->sd_rent.as<int>(), c->stream);
                 int nr = 0;
-                HIPCHK(c, c->read(&nr, c->sd_tbase.as<int>() + ntiles, 4));
+                HIPCHK(c, c->read(&nr, c->sd_nrec.as<int>(), 4));
                 nrec = nr;
-                vbits = pack_vbits(kbits, nslots);
-                launch_sample_records_compact(c->sd_tcount.as<int>(), c->sd_tbase.as<int>(), ntiles,
-                                              c->sd_fkey.as<uint64_t>(), vbits, c->sd_key.as<uint64_t>(),
-                                              c->sd_val.as<int>(), c->stream);
             } else {
-                vbits = pack_vbits(kbits, nent);
                 launch_sample_dedupe_keys(c->g, v, L->child.as<int>(), nent, origin, lb, vbits,
                                           c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->stream);
             }

@@ -464,17 +464,20 @@ static int build_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist
 // own. References point to earlier nodes, so following them ends at an entry with none, which owns
 // a record; the global heads pass, which compares coordinates, still merges identical points of
 // different records. A tile whose box (grown by one) exceeds the map gives every entry its own
-// record. Tile b writes its records to slots b * kStRecs + u, u < tcount[b] (the sort key of
-// k_sample_keys in fkey, node * 27 + sample of the owning entry in rent) and every entry the slot
-// of its record in ent_rec.
+// record. A tile takes the slots of its records from a level-wide counter (*nrec, one atomic per
+// tile) and writes per record its sort key as k_sample_keys makes it (packed with the slot when
+// vbits > 0, else the slot in rval) and node * 27 + sample of the owning entry (rent), per entry
+// the slot of its record (ent_rec). The order of the tiles' slot ranges varies from run to run;
+// what is evaluated does not: equal points get equal values whichever record is their head.
 constexpr int kStT = 256;           // nodes per tile
 constexpr int kStRecs = 27 * kStT;  // records per tile at most
 constexpr int kStMap = 4096;        // dense map cells
 
 __global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, const int* __restrict__ list,
-                                                       int64_t k, float ox, float oy, float oz, int lb,
-                                                       int* __restrict__ ent_rec, uint64_t* __restrict__ fkey,
-                                                       int* __restrict__ rent, int* __restrict__ tcount) {
+                                                       int64_t k, float ox, float oy, float oz, int lb, int vbits,
+                                                       int* __restrict__ nrec, int* __restrict__ ent_rec,
+                                                       uint64_t* __restrict__ rkey, int* __restrict__ rval,
+                                                       int* __restrict__ rent) {
     __shared__ short map[kStMap];
     __shared__ float4 cen[kStT];
     __shared__ int nid[kStT];
@@ -482,6 +485,7 @@ __global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, 
     __shared__ short recid[kStRecs];  // record of an entry that owns one
     __shared__ int red[6][kStT / kWave];
     __shared__ int wsum[kStT / kWave];
+    __shared__ int tbase;
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
     const int64_t l0 = (int64_t)blockIdx.x * kStT, li = l0 + t;
     const bool act = li < k;
@@ -575,8 +579,7 @@ __global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, 
         if (q < w) u += wsum[q];
         total += wsum[q];
     }
-    const int64_t b = blockIdx.x;
-    if (t == 0) tcount[b] = total;
+    if (t == 0) tbase = atomicAdd(nrec, total);
     if (act)
         for (int sm = 0; sm < 27; sm++)
             if (ref[t * 27 + sm] < 0) recid[t * 27 + sm] = (short)u++;
@@ -588,8 +591,8 @@ __global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, 
     for (int e = t; e < ne; e += kStT) {
         int r = e;
         while (ref[r] >= 0) r = ref[r];
-        const int64_t slot = b * kStRecs + recid[r];
-        ent_rec[l0 * 27 + e] = (int)slot;
+        const int slot = tbase + recid[r];
+        ent_rec[l0 * 27 + e] = slot;
         if (r != e) continue;
         const float4 cc = cen[e / 27];
         float p[3];
@@ -600,29 +603,15 @@ __global__ void __launch_bounds__(kStT) k_sample_tiles(GridDev g, NodeLevel lv, 
             const float v = rintf((p[a] - o[a]) / cc.w);
             qq[a] = (uint32_t)fminf(fmaxf(v, 0.0f), 2097151.0f);
         }
-        fkey[slot] = (cell_group(g, p[0], p[1], p[2]) << (3 * lb)) | (qq[0] & m) | ((qq[1] & m) << lb) |
-                     ((uint64_t)(qq[2] & m) << (2 * lb));
-        rent[slot] = nid[e / 27] * 27 + e % 27;
-    }
-}
-
-// the tiles' record keys in one array (tile b from tbase[b] on), each packed with its slot when
-// vbits > 0 (else the slot in rval)
-__global__ void __launch_bounds__(256) k_sample_records_compact(const int* __restrict__ tcount,
-                                                                 const int* __restrict__ tbase,
-                                                                 const uint64_t* __restrict__ fkey, int vbits,
-                                                                 uint64_t* __restrict__ rkey, int* __restrict__ rval) {
-    const int64_t b = blockIdx.x;
-    const int n = tcount[b];
-    const int64_t d0 = tbase[b];
-    for (int u = threadIdx.x; u < n; u += 256) {
-        const int64_t r = b * kStRecs + u;
+        const uint64_t kk = (cell_group(g, p[0], p[1], p[2]) << (3 * lb)) | (qq[0] & m) | ((qq[1] & m) << lb) |
+                            ((uint64_t)(qq[2] & m) << (2 * lb));
         if (vbits) {
-            rkey[d0 + u] = (fkey[r] << vbits) | (uint64_t)r;
+            rkey[slot] = (kk << vbits) | (uint64_t)slot;
         } else {
-            rkey[d0 + u] = fkey[r];
-            rval[d0 + u] = (int)r;
+            rkey[slot] = kk;
+            rval[slot] = slot;
         }
+        rent[slot] = nid[e / 27] * 27 + e % 27;
     }
 }
 
@@ -649,18 +638,13 @@ __global__ void k_entry_scatter(const int* __restrict__ list, const int* __restr
     else ((float*)samples)[dst] = ((const float*)vals)[src];
 }
 
-int64_t sample_tiles(int64_t listed) { return (listed + kStT - 1) / kStT; }
-int64_t sample_tile_records() { return kStRecs; }
 void launch_sample_tiles(GridDev g, NodeLevel lv, const int* list, int64_t k, const float origin[3], int lb,
-                         int* ent_rec, uint64_t* fkey, int* rent, int* tcount, hipStream_t s) {
+                         int vbits, int* nrec, int* ent_rec, uint64_t* rkey, int* rval, int* rent, hipStream_t s) {
+    launch_fill_i32(nrec, 1, 0, s);
     if (k > 0)
-        k_sample_tiles<<<(unsigned)sample_tiles(k), kStT, 0, s>>>(g, lv, list, k, origin[0], origin[1], origin[2], lb,
-                                                                 ent_rec, fkey, rent, tcount);
-}
-void launch_sample_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey,
-                                   int vbits, uint64_t* rkey, int* rval, hipStream_t s) {
-    if (ntiles > 0)
-        k_sample_records_compact<<<(unsigned)ntiles, 256, 0, s>>>(tcount, tbase, fkey, vbits, rkey, rval);
+        k_sample_tiles<<<(unsigned)((k + kStT - 1) / kStT), kStT, 0, s>>>(g, lv, list, k, origin[0], origin[1],
+                                                                        origin[2], lb, vbits, nrec, ent_rec, rkey,
+                                                                        rval, rent);
 }
 
 // distinct sample points of a level (cell-major), evaluated run by run

@@ -180,17 +180,12 @@ void launch_centers_runs(int prec, GridDev g, const void* rec_a, const void* rec
 // number of cells of a row per run. Counts are read back through rd; a failed read sets *err.
 void launch_sample_dedupe_keys(GridDev g, NodeLevel lv, const int* list, int64_t nent, const float origin[3],
                                int lb, int vbits, uint64_t* key, int* val, hipStream_t s);
-// Sample records merged per tile of 256 listed nodes: every entry gets the slot of its tile record
-// (ent_rec), tile b holds tcount[b] records at slots b * sample_tile_records() + u with their sort
-// keys (fkey, as launch_sample_dedupe_keys makes them) and points (rent: node * 27 + sample; the
-// level's node count times 27 must fit an int).
-int64_t sample_tiles(int64_t listed);
-int64_t sample_tile_records();
+// Sample records merged per tile of 256 listed nodes: the record count in *nrec (cleared here),
+// every entry's record slot (ent_rec, < k * 27), per record its sort key as
+// launch_sample_dedupe_keys makes it (packed with the slot when vbits > 0, else the slot in rval)
+// and point (rent: node * 27 + sample; the level's node count times 27 must fit an int)
 void launch_sample_tiles(GridDev g, NodeLevel lv, const int* list, int64_t k, const float origin[3], int lb,
-                         int* ent_rec, uint64_t* fkey, int* rent, int* tcount, hipStream_t s);
-// the tiles' keys gathered from tbase[b] on (scanned tcount), packed with their slots when vbits > 0
-void launch_sample_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey,
-                                   int vbits, uint64_t* rkey, int* rval, hipStream_t s);
+                         int vbits, int* nrec, int* ent_rec, uint64_t* rkey, int* rval, int* rent, hipStream_t s);
 // Distinct points of nrec sorted items evaluated and their values scattered to the level's samples.
 // ent NULL: the items are the sample entries (nent == nrec, ent_rec and rec_head unused); else
 // item value v is a record slot, ent[v] its node * 27 + sample, ent_rec the record slot of each of
