@@ -511,9 +511,12 @@ int ensure_bbox(pisurf_ctx* c) {
     if (c->bbox_done) return PISURF_OK;
     int rc = bbox_launch(c);
     if (rc) return rc;
+    // the NaN count and the box in one read: the box sits 64 bytes after the count
+    char buf[64 + 6 * sizeof(float)];
+    HIPCHK(c, c->read(buf, bbox_nan_dev(c), sizeof(buf)));
     int nans = 0;
-    HIPCHK(c, c->read(c->bbox, bbox_dev(c), 6 * sizeof(float)));
-    HIPCHK(c, c->read(&nans, bbox_nan_dev(c), sizeof(int)));
+    std::memcpy(&nans, buf, sizeof(int));
+    std::memcpy(c->bbox, buf + 64, 6 * sizeof(float));
     return bbox_finish(c, nans);
 }
 
