@@ -954,7 +954,6 @@ int top_levels_issue(pisurf_ctx* c, hipStream_t s, DBuf& tmp) {
             launch_empty_curv(c->g, c->spos.as<float4>(), c->nrm.as<float4>(), tp, v, group, kEmptyAndCurv, nullptr, 0,
                               s);
             launch_classify(prec, c->g, c->rec_a.p, c->rec_g.p, fc, tp, v, c->samples.p, nullptr, c->ctr(), s);
-            launch_fill_i32(L->flag.as<int>() + ub[l], 1, 0, s);
             exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), ub[l] + 1, tmp.as<int>(), s);
             launch_children_counted(v, L->scanned.as<int>(), c->levels[l + 1]->view(), cnt + l + 1, s);
         }
@@ -1069,7 +1068,6 @@ int do_tree(pisurf_ctx* c) {
         if (L->n == 0) break;  // every node above D_min empty
         rc = sweep(c, L, nullptr, (int)li);
         if (rc) return rc;
-        HIPCHK(c, hipMemsetAsync(L->flag.as<int>() + L->n, 0, 4, c->stream));
         exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), L->n + 1, c->tmp.as<int>(), c->stream);
         int n_int = 0;
         HIPCHK(c, c->read(&n_int, L->scanned.as<int>() + L->n, 4));
@@ -1328,7 +1326,6 @@ int do_extract_device(pisurf_ctx* c) {
     HIPCHK(c, E[28].ensure(std::max<int64_t>(nleaf, 1)));
     launch_corner_tiles(ll, lidx, leaf_node, E[28].as<uint8_t>(), E[7].as<uint64_t>(), E[29].as<int>(), E[30].as<int>(),
                         s);
-    launch_fill_i32(E[30].as<int>() + ntiles, 1, 0, s);
     if ((rc = ensure_tmp(c, ntiles + 1))) return rc;
     exclusive_scan_i32(E[30].as<int>(), E[31].as<int>(), ntiles + 1, c->tmp.as<int>(), s);
     HIPCHK(c, hipGetLastError());

@@ -152,6 +152,7 @@ __global__ void __launch_bounds__(kCtT) k_corner_tiles(LeafLevels lv, const int*
         total += wsum[k];
     }
     if (t == 0) tcount[b] = total;
+    if (b == 0 && t == 0) tcount[gridDim.x] = 0;  // closes the counts for their scan
     // each record: its key, and no leaf in any octant yet
     const int4 none = make_int4(-1, -1, -1, -1);
 #pragma unroll
@@ -339,6 +340,7 @@ __global__ void __launch_bounds__(kHeadT) k_head_count(SortedKV kv, int64_t n, i
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) tile_count[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) tile_count[gridDim.x] = 0;  // closes the counts for their scan
 }
 __global__ void __launch_bounds__(kHeadT) k_head_write(SortedKV kv, int64_t n, const int* __restrict__ tile_off,
                                                        int* __restrict__ list) {
@@ -594,6 +596,8 @@ void launch_corner_tiles(const LeafLevels& lv, const int* lidx, double4* leaf_no
     if (lv.tile0[lv.nlev] > 0)
         k_corner_tiles<<<(unsigned)lv.tile0[lv.nlev], kCtT, 0, s>>>(lv, lidx, leaf_node, leaf_pos, fkey, rleaf,
                                                                     tcount);
+    else
+        launch_fill_i32(tcount, 1, 0, s);
 }
 void launch_records_compact(const int* tcount, const int* tbase, int64_t ntiles, const uint64_t* fkey, int vbits,
                             uint64_t* rkey, int* rval, hipStream_t s) {
@@ -626,7 +630,6 @@ void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_t
         return;
     }
     k_head_count<<<(unsigned)nb, kHeadT, 0, s>>>(kv, n, tiles);
-    (void)hipMemsetAsync(tiles + nb, 0, sizeof(int), s);
     exclusive_scan_i32(tiles, tile_off, nb + 1, scan_tmp, s);
     k_head_write<<<(unsigned)nb, kHeadT, 0, s>>>(kv, n, tile_off, list);
 }

@@ -290,6 +290,7 @@ __global__ void __launch_bounds__(256) k_sample_heads(NodeLevel lv, const int* _
     if (lane == 0) wsum[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) block_heads[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) block_heads[gridDim.x] = 0;  // closes the counts for their scan
 }
 
 // hlist[h] = sorted position of the h-th head, hpos[j] = h for each head j (block_off: scanned
@@ -817,8 +818,8 @@ __global__ void __launch_bounds__(256) k_classify(GridDev g, Recs<R> rec, FieldC
         lv.node[node] = make_double4(double(vx), double(vy), double(vz), double(fval));
         lv.child[node] = -1;
         if (sc_out) sc_out[node] = sc ? 1 : 0;
-    } else if (node < lv.n) {
-        lv.flag[node] = 0;  // past a device count: nothing to scan
+    } else if (node <= lv.n) {
+        lv.flag[node] = 0;  // past a device count: nothing to scan; flag[n] closes the flags for their scan
     }
     wave_count(&ctr_stripe(ctr)->field_evals, evals);
     wave_count(&ctr_stripe(ctr)->qef_solves, qefs);
@@ -911,7 +912,6 @@ void launch_heads(NodeLevel lv, const int* list, SortedKV kv, int64_t nent, int*
     const unsigned nb = (unsigned)((nent + 255) / 256);
     int* block_off = head + (nb + 1);
     k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, nullptr, nent, head, rep);
-    launch_fill_i32(head + nb, 1, 0, s);
     exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
     k_head_list<<<nb, 256, 0, s>>>(rep, nent, block_off, hlist, hpos);
 }
@@ -1112,7 +1112,6 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     const unsigned nb = (unsigned)((nrec + 255) / 256);
     int* block_off = head + (nb + 1);
     k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, ent, nrec, head, rep);
-    launch_fill_i32(head + nb, 1, 0, s);
     exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
     k_head_list<<<nb, 256, 0, s>>>(rep, nrec, block_off, hlist, hpos);
     int nh = 0;
@@ -1228,8 +1227,11 @@ void launch_samples(int prec, GridDev g, const void* ra, const void* rg, FieldCo
 
 void launch_classify(int prec, GridDev g, const void* ra, const void* rg, FieldConst fc, TreeParams tp, NodeLevel lv,
                      const void* samples, int32_t* sc, Counters* ctr, hipStream_t s) {
-    if (lv.n <= 0) return;
-    const unsigned nb = (unsigned)((lv.n + 255) / 256);
+    if (lv.n <= 0) {
+        if (lv.n == 0) launch_fill_i32(lv.flag, 1, 0, s);  // the closing flag of an empty level
+        return;
+    }
+    const unsigned nb = (unsigned)((lv.n + 1 + 255) / 256);
     // the QEF code is compiled in only when used: it takes ~200 registers, which would cut the
     // occupancy of the common case
     auto go = [&](auto qef) {

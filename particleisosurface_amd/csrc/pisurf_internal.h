@@ -242,6 +242,8 @@ void launch_run_queue(int prec, int mode, int T, const RunFrame* frames, const i
 void launch_samples(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
                     NodeLevel lv, const int* list, const int* count, void* samples /*n*27 R*/, Counters* ctr,
                     hipStream_t s);
+// node types, vertices and values, and lv.flag (1 for an internal node; lv.flag[lv.n] = 0 closes
+// the flags for their scan)
 void launch_classify(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, TreeParams tp,
                      NodeLevel lv, const void* samples, int32_t* signchange, Counters* ctr, hipStream_t s);
 void launch_children(NodeLevel parent, const int* scanned, NodeLevel next, hipStream_t s);
@@ -282,8 +284,9 @@ void launch_leaf_flags(const LeafLevels& lv, int* flag, hipStream_t s);
 // Distinct corner points per tile of corner_tile_records() / 8 nodes of a level (lv.tile0 from
 // corner_tiles() of each level's node count), one launch for every level. Tile b writes its
 // records to slots b * corner_tile_records() + u, u < tcount[b]: the point's Morton key (3 dp
-// bits) in fkey, and in rleaf[8 slot + o] the tile's leaf in octant o of the point, or -1. Also
-// leaf_node[L] = vertex and value of leaf L, leaf_pos[L] = (value > 0).
+// bits) in fkey, and in rleaf[8 slot + o] the tile's leaf in octant o of the point, or -1;
+// tcount[ntiles] = 0 closes the counts for their scan. Also leaf_node[L] = vertex and value of
+// leaf L, leaf_pos[L] = (value > 0).
 int64_t corner_tiles(int64_t nodes);
 int64_t corner_tile_records();
 void launch_corner_tiles(const LeafLevels& lv, const int* lidx, double4* leaf_node, uint8_t* leaf_pos, uint64_t* fkey,
