@@ -305,6 +305,9 @@ struct pisurf_ctx {
     hipError_t wait() { return hr ? stream_fence(stream, hr) : stream_wait_on(stream, wev); }
     HostRead* hread() { return hr; }
     hipError_t read(void* host, const void* dptr, size_t bytes) { return read_device(host, dptr, bytes, stream, hr); }
+    // a scan total published by the scan itself (scan_read_begin / scan_read_end)
+    ScanPub scan_pub() { return scan_read_begin(hr); }
+    hipError_t read_scan(int* host, const int* dlast, ScanPub p) { return scan_read_end(host, dlast, p, stream, hr); }
 
     ~pisurf_ctx() {
         for (auto* b : sub) delete b;
@@ -763,15 +766,16 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     HIPCHK(c, c->cl_flag.ensure((L->n + 1) * 4));
     HIPCHK(c, c->cl_pos.ensure((L->n + 1) * 4));
     HIPCHK(c, c->cl_list.ensure((L->n + 1) * 4));
-    launch_sample_list(tp, v, L->flag.as<int>(), L->scanned.as<int>(), L->child.as<int>(), c->cl_flag.as<int>(),
-                       c->cl_pos.as<int>(), c->cl_list.as<int>(), c->tmp.as<int>(), c->stream);
     // Deduplicated sampling needs one node size per level (lattice keys) and 21-bit lattice
     // coordinates: 2^(depth + 1) half-widths span the root box.
     // levels above D_min sample nothing (wants_samples), so they skip the count read-back
     const bool dedupe = c->prm.sample_dedupe && level_depth >= 0 && level_depth <= 19 && level_depth >= c->dmin;
+    const ScanPub kpub = dedupe ? c->scan_pub() : ScanPub();
+    launch_sample_list(tp, v, L->flag.as<int>(), L->scanned.as<int>(), L->child.as<int>(), c->cl_flag.as<int>(),
+                       c->cl_pos.as<int>(), c->cl_list.as<int>(), c->tmp.as<int>(), c->stream, kpub);
     if (dedupe) {
         int k = 0;
-        HIPCHK(c, c->read(&k, L->scanned.as<int>() + L->n, 4));
+        HIPCHK(c, c->read_scan(&k, L->scanned.as<int>() + L->n, kpub));
         const int64_t nent = (int64_t)k * 27;
         if (nent >= INT32_MAX) return c->fail(PISURF_E_UNSUPPORTED, "octree level has too many sample points");
         if (nent > 0) {
@@ -1068,9 +1072,10 @@ int do_tree(pisurf_ctx* c) {
         if (L->n == 0) break;  // every node above D_min empty
         rc = sweep(c, L, nullptr, (int)li);
         if (rc) return rc;
-        exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), L->n + 1, c->tmp.as<int>(), c->stream);
+        const ScanPub pub = c->scan_pub();
+        exclusive_scan_i32(L->flag.as<int>(), L->scanned.as<int>(), L->n + 1, c->tmp.as<int>(), c->stream, pub);
         int n_int = 0;
-        HIPCHK(c, c->read(&n_int, L->scanned.as<int>() + L->n, 4));
+        HIPCHK(c, c->read_scan(&n_int, L->scanned.as<int>() + L->n, pub));
         c->harvest_events();
         if (n_int == 0) break;
         if ((int64_t)n_int * 8 > (int64_t)INT32_MAX / 2) return c->fail(PISURF_E_UNSUPPORTED, "octree level too large");
@@ -1253,6 +1258,13 @@ int read_int(pisurf_ctx* c, const int* dptr, int64_t& out) {
     out = v;
     return PISURF_OK;
 }
+// a scan's last element (dlast), published by the scan itself (c->scan_pub() before the scan)
+int read_scan_int(pisurf_ctx* c, const int* dlast, ScanPub p, int64_t& out) {
+    int v = 0;
+    HIPCHK(c, c->read_scan(&v, dlast, p));
+    out = v;
+    return PISURF_OK;
+}
 
 int ensure_tmp(pisurf_ctx* c, int64_t n) {
     const int64_t need = std::max(radix_tmp_ints(n), scan_tmp_ints(n + 1)) + 64;
@@ -1307,9 +1319,10 @@ int do_extract_device(pisurf_ctx* c) {
     ll.off[tv.nlev] = T;
     launch_leaf_flags(ll, E[0].as<int>(), s);
     if ((rc = ensure_tmp(c, T + 1))) return rc;
-    exclusive_scan_i32(E[0].as<int>(), E[1].as<int>(), T + 1, c->tmp.as<int>(), s);
+    ScanPub pub = c->scan_pub();
+    exclusive_scan_i32(E[0].as<int>(), E[1].as<int>(), T + 1, c->tmp.as<int>(), s, pub);
     int64_t nleaf;
-    if ((rc = read_int(c, E[1].as<int>() + T, nleaf))) return rc;
+    if ((rc = read_scan_int(c, E[1].as<int>() + T, pub, nleaf))) return rc;
     const int* lidx = E[1].as<int>();
     // distinct corner points: per tile of 128 nodes of a level, one record per point (its key and
     // the tile's leaves around it) in the tile's own region; then the keys of all records in one
@@ -1327,10 +1340,11 @@ int do_extract_device(pisurf_ctx* c) {
     launch_corner_tiles(ll, lidx, leaf_node, E[28].as<uint8_t>(), E[7].as<uint64_t>(), E[29].as<int>(), E[30].as<int>(),
                         s);
     if ((rc = ensure_tmp(c, ntiles + 1))) return rc;
-    exclusive_scan_i32(E[30].as<int>(), E[31].as<int>(), ntiles + 1, c->tmp.as<int>(), s);
+    pub = c->scan_pub();
+    exclusive_scan_i32(E[30].as<int>(), E[31].as<int>(), ntiles + 1, c->tmp.as<int>(), s, pub);
     HIPCHK(c, hipGetLastError());
     int64_t nent = 0;
-    if ((rc = read_int(c, E[31].as<int>() + ntiles, nent))) return rc;
+    if ((rc = read_scan_int(c, E[31].as<int>() + ntiles, pub, nent))) return rc;
     const int ckbits = std::max(1, 3 * tv.dp);
     const int cvbits = pack_vbits(ckbits, nslots);
     HIPCHK(c, E[3].ensure(std::max<int64_t>(nent, 1) * 8));
@@ -1347,26 +1361,29 @@ int do_extract_device(pisurf_ctx* c) {
     // distinct points: run heads of the sorted records, one thread each
     HIPCHK(c, E[25].ensure(std::max<int64_t>(nent, 1) * 4));
     HIPCHK(c, E[9].ensure(head_list_tmp_ints(nent) * 4));
-    launch_head_list(ckv, nent, E[25].as<int>(), E[9].as<int>(), c->tmp.as<int>(), s);
+    pub = nent > 0 ? c->scan_pub() : ScanPub();
+    launch_head_list(ckv, nent, E[25].as<int>(), E[9].as<int>(), c->tmp.as<int>(), s, pub);
     int64_t npts;
-    if ((rc = read_int(c, head_list_count(E[9].as<int>(), nent), npts))) return rc;
+    if ((rc = read_scan_int(c, head_list_count(E[9].as<int>(), nent), pub, npts))) return rc;
     HIPCHK(c, E[26].ensure((npts + 1) * 4));
     HIPCHK(c, E[27].ensure((npts + 1) * 4));
     HIPCHK(c, E[24].ensure(std::max<int64_t>(npts, 1) * 9 * 4));
     launch_points(tv, lidx, E[28].as<uint8_t>(), ckv, nent, E[29].as<int>(), E[25].as<int>(), npts, E[26].as<int>(),
                   E[24].as<int>(), s);
-    exclusive_scan_i32(E[26].as<int>(), E[27].as<int>(), npts + 1, c->tmp.as<int>(), s);
+    pub = c->scan_pub();
+    exclusive_scan_i32(E[26].as<int>(), E[27].as<int>(), npts + 1, c->tmp.as<int>(), s, pub);
     int64_t nd;
-    if ((rc = read_int(c, E[27].as<int>() + npts, nd))) return rc;
+    if ((rc = read_scan_int(c, E[27].as<int>() + npts, pub, nd))) return rc;
     HIPCHK(c, E[10].ensure(std::max<int64_t>(nd, 1) * 9 * 4));
     launch_points_compact(E[26].as<int>(), E[27].as<int>(), E[24].as<int>(), npts, E[10].as<int>(), s);
     // triangles
     HIPCHK(c, E[11].ensure((nd + 1) * 4));
     HIPCHK(c, E[12].ensure((nd + 1) * 4));
     launch_tri_count(E[10].as<int>(), nd, E[11].as<int>(), s);
-    exclusive_scan_i32(E[11].as<int>(), E[12].as<int>(), nd + 1, c->tmp.as<int>(), s);
+    pub = c->scan_pub();
+    exclusive_scan_i32(E[11].as<int>(), E[12].as<int>(), nd + 1, c->tmp.as<int>(), s, pub);
     int64_t nt;
-    if ((rc = read_int(c, E[12].as<int>() + nd, nt))) return rc;
+    if ((rc = read_scan_int(c, E[12].as<int>() + nd, pub, nt))) return rc;
     const int64_t nc = 3 * nt;
     HIPCHK(c, E[13].ensure(std::max<int64_t>(nc, 1) * 8));
     HIPCHK(c, E[14].ensure(std::max<int64_t>(nc, 1) * 4));
@@ -1384,9 +1401,10 @@ int do_extract_device(pisurf_ctx* c) {
     if ((rc = ensure_tmp(c, nc + 1))) return rc;
     int* vslot = E[14].as<int>();  // the corner indices written there are implicit
     launch_hash_dedupe(E[13].as<uint64_t>(), nc, E[15].as<unsigned>(), vtab, vslot, E[18].as<int>(), s);
-    exclusive_scan_i32(E[18].as<int>(), E[19].as<int>(), nc + 1, c->tmp.as<int>(), s);
+    pub = c->scan_pub();
+    exclusive_scan_i32(E[18].as<int>(), E[19].as<int>(), nc + 1, c->tmp.as<int>(), s, pub);
     int64_t nv;
-    if ((rc = read_int(c, E[19].as<int>() + nc, nv))) return rc;
+    if ((rc = read_scan_int(c, E[19].as<int>() + nc, pub, nv))) return rc;
     HIPCHK(c, E[20].ensure(std::max<int64_t>(nc, 1) * 4));
     HIPCHK(c, E[21].ensure(std::max<int64_t>(nv, 1) * 8 * 8));
     int* cvid = E[20].as<int>();
@@ -1427,9 +1445,10 @@ int do_extract_device(pisurf_ctx* c) {
                            32 + bits_for(nv), c->tmp.as<int>(), s);
         launch_tri_keep(cvid, E[6].as<int>(), nt, E[8].as<int>(), s);
     }
-    exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nt + 1, c->tmp.as<int>(), s);
+    pub = c->scan_pub();
+    exclusive_scan_i32(E[8].as<int>(), E[9].as<int>(), nt + 1, c->tmp.as<int>(), s, pub);
     int64_t ntf;
-    if ((rc = read_int(c, E[9].as<int>() + nt, ntf))) return rc;
+    if ((rc = read_scan_int(c, E[9].as<int>() + nt, pub, ntf))) return rc;
     HIPCHK(c, E[23].ensure(std::max<int64_t>(ntf, 1) * 3 * 4));
     launch_tri_out(cvid, E[8].as<int>(), E[9].as<int>(), nt, E[23].as<int>(), s);
     HIPCHK(c, hipGetLastError());

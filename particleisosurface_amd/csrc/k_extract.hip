@@ -622,7 +622,7 @@ void launch_tri_key63(const int* cv, int64_t nt, uint64_t* k, hipStream_t s) {
 }
 static_assert(kHeadTile == 2048, "head_list_count assumes tiles of 2048 entries");
 int64_t head_list_tmp_ints(int64_t n) { return 2 * ((n + kHeadTile - 1) / kHeadTile + 1); }
-void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_tmp, hipStream_t s) {
+void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_tmp, hipStream_t s, ScanPub pub) {
     const int64_t nb = (n + kHeadTile - 1) / kHeadTile;
     int* tile_off = tiles + (nb + 1);
     if (n <= 0) {
@@ -630,7 +630,7 @@ void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_t
         return;
     }
     k_head_count<<<(unsigned)nb, kHeadT, 0, s>>>(kv, n, tiles);
-    exclusive_scan_i32(tiles, tile_off, nb + 1, scan_tmp, s);
+    exclusive_scan_i32(tiles, tile_off, nb + 1, scan_tmp, s, pub);
     k_head_write<<<(unsigned)nb, kHeadT, 0, s>>>(kv, n, tile_off, list);
 }
 void launch_points(const TreeView& tv, const int* lidx, const uint8_t* leaf_pos, SortedKV kv, int64_t nent,

@@ -16,8 +16,18 @@ namespace pisurf {
 // ---------------------------------------------------------------------------------------------
 constexpr int kScanT = 256, kScanI = 8, kScanTile = kScanT * kScanI;
 
+// The thread that writes a scan's last element in its final kernel also publishes it to a
+// read-back slot (pdst, non-null only for that kernel): the value, then, once it is visible
+// system-wide, the sequence number the host waits for.
+__device__ __forceinline__ void scan_publish(char* pdst, uint32_t pseq, int v) {
+    *(volatile int*)pdst = v;
+    __threadfence_system();
+    *(volatile uint32_t*)(pdst + kReadbackBytes) = pseq;
+    __threadfence_system();
+}
+
 __global__ void __launch_bounds__(kScanT) k_scan_tile(const int* __restrict__ in, int* __restrict__ out, int64_t n,
-                                                      int* __restrict__ sums) {
+                                                      int* __restrict__ sums, char* pdst, uint32_t pseq) {
     __shared__ int wsum[kScanT / kWave];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)t * kScanI;
@@ -39,18 +49,23 @@ __global__ void __launch_bounds__(kScanT) k_scan_tile(const int* __restrict__ in
     for (int k = 0; k < kScanI; k++) {
         const int64_t i = base + k;
         if (i < n) out[i] = run;
+        if (pdst && i == n - 1) scan_publish(pdst, pseq, run);
         run += v[k];
     }
     if (t == kScanT - 1) sums[blockIdx.x] = woff + incl;
 }
 
-__global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restrict__ add) {
+__global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restrict__ add, char* pdst, uint32_t pseq) {
     const int64_t i = (int64_t)blockIdx.x * kScanTile + threadIdx.x;
     const int a = add[blockIdx.x];
 #pragma unroll
     for (int k = 0; k < kScanI; k++) {
         const int64_t j = i + (int64_t)k * kScanT;
-        if (j < n) out[j] += a;
+        if (j < n) {
+            const int v = out[j] + a;
+            out[j] = v;
+            if (pdst && j == n - 1) scan_publish(pdst, pseq, v);
+        }
     }
 }
 
@@ -58,7 +73,7 @@ __global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restri
 // reduced by the block itself, so the totals need no scan of their own (one launch fewer).
 constexpr int kScanDirect = 2048;
 __global__ void __launch_bounds__(kScanT) k_scan_add_prefix(int* __restrict__ out, int64_t n,
-                                                            const int* __restrict__ sums) {
+                                                            const int* __restrict__ sums, char* pdst, uint32_t pseq) {
     __shared__ int wsum[kScanT / kWave];
     const int t = threadIdx.x;
     if (blockIdx.x == 0) return;  // uniform over the block
@@ -73,7 +88,11 @@ __global__ void __launch_bounds__(kScanT) k_scan_add_prefix(int* __restrict__ ou
 #pragma unroll
     for (int k = 0; k < kScanI; k++) {
         const int64_t j = i + (int64_t)k * kScanT;
-        if (j < n) out[j] += a;
+        if (j < n) {
+            const int v = out[j] + a;
+            out[j] = v;
+            if (pdst && j == n - 1) scan_publish(pdst, pseq, v);
+        }
     }
 }
 
@@ -86,17 +105,17 @@ int64_t scan_tmp_ints(int64_t n) {
     return tot + 4;
 }
 
-void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_t s) {
+void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_t s, ScanPub pub) {
     if (n <= 0) return;
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     int* sums = tmp;
     int* sums_scanned = tmp + nb + 1;
-    k_scan_tile<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(in, out, n, sums);
+    k_scan_tile<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(in, out, n, sums, nb == 1 ? pub.dst : nullptr, pub.seq);
     if (nb > 1 && nb <= kScanDirect) {
-        k_scan_add_prefix<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums);
+        k_scan_add_prefix<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums, pub.dst, pub.seq);
     } else if (nb > 1) {
         exclusive_scan_i32(sums, sums_scanned, nb, tmp + 2 * nb + 2, s);
-        k_scan_add<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums_scanned);
+        k_scan_add<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums_scanned, pub.dst, pub.seq);
     }
 }
 
@@ -383,6 +402,24 @@ hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = slot_wait(rd, want, s)) != hipSuccess) return e;
     std::memcpy(host, rd->host, bytes);
+    return hipSuccess;
+}
+
+ScanPub scan_read_begin(HostRead* rd) {
+    ScanPub p;
+    if (rd) {
+        p.dst = rd->dev;
+        p.seq = ++rd->next;
+    }
+    return p;
+}
+
+hipError_t scan_read_end(int* host, const int* dlast, ScanPub p, hipStream_t s, HostRead* rd) {
+    if (!p.dst) return read_device(host, dlast, sizeof(int), s, rd);
+    hipError_t e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = slot_wait(rd, p.seq, s)) != hipSuccess) return e;
+    std::memcpy(host, rd->host, sizeof(int));
     return hipSuccess;
 }
 

@@ -419,10 +419,11 @@ static int build_groups(SortedKV kv, int gshift, const int* hlist, int nh, int* 
     int* cstart = cwork + 2 * (int64_t)(nh + 2);
     const unsigned nbh = (unsigned)((nh + 1 + 255) / 256);
     k_cell_flags<<<nbh, 256, 0, s>>>(kv, gshift, hlist, nh, flag);
-    exclusive_scan_i32(flag, pos, nh + 1, tmp, s);
+    const ScanPub pub = scan_read_begin(rd);
+    exclusive_scan_i32(flag, pos, nh + 1, tmp, s, pub);
     k_cell_starts<<<nbh, 256, 0, s>>>(flag, pos, nh, cstart);
     int ng = 0;
-    const hipError_t e = read_device(&ng, pos + nh, 4, s, rd);
+    const hipError_t e = scan_read_end(&ng, pos + nh, pub, s, rd);
     if (e != hipSuccess) {
         *err = e;
         return 0;
@@ -443,10 +444,11 @@ static int build_run_starts(GridDev g, SortedKV kv, int gshift, const int* hlist
     if (ng <= 0) return 0;
     const unsigned nbg = (unsigned)((ng + 1 + 255) / 256);
     k_run_flags<<<nbg, 256, 0, s>>>(g, kv, gshift, hlist, cstart, ng, kx, flag);
-    exclusive_scan_i32(flag, pos, ng + 1, tmp, s);
+    const ScanPub pub = scan_read_begin(rd);
+    exclusive_scan_i32(flag, pos, ng + 1, tmp, s, pub);
     k_run_starts<<<nbg, 256, 0, s>>>(flag, pos, cstart, ng, rstart);
     int nr = 0;
-    const hipError_t e = read_device(&nr, pos + ng, 4, s, rd);
+    const hipError_t e = scan_read_end(&nr, pos + ng, pub, s, rd);
     if (e != hipSuccess) {
         *err = e;
         return 0;
@@ -1055,10 +1057,10 @@ void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreePar
 }
 
 void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,
-                        int* clist, int* tmp, hipStream_t s) {
+                        int* clist, int* tmp, hipStream_t s, ScanPub pub) {
     if (lv.n <= 0) return;
     k_sample_flag<<<(unsigned)((lv.n + 1 + 255) / 256), 256, 0, s>>>(tp, lv, flag, cflag);
-    exclusive_scan_i32(flag, pos, lv.n + 1, tmp, s);
+    exclusive_scan_i32(flag, pos, lv.n + 1, tmp, s, pub);
     k_sample_list<<<(unsigned)((lv.n + 255) / 256), 256, 0, s>>>(lv, flag, pos, list);
     exclusive_scan_i32(cflag, cpos, lv.n + 1, tmp, s);
     k_sample_list<<<(unsigned)((lv.n + 255) / 256), 256, 0, s>>>(lv, cflag, cpos, clist);
@@ -1112,10 +1114,11 @@ void launch_sample_dedupe_eval(int prec, GridDev g, const void* ra, const void* 
     const unsigned nb = (unsigned)((nrec + 255) / 256);
     int* block_off = head + (nb + 1);
     k_sample_heads<<<nb, 256, 0, s>>>(lv, list, kv, ent, nrec, head, rep);
-    exclusive_scan_i32(head, block_off, nb + 1, tmp, s);
+    const ScanPub pub = scan_read_begin(rd);
+    exclusive_scan_i32(head, block_off, nb + 1, tmp, s, pub);
     k_head_list<<<nb, 256, 0, s>>>(rep, nrec, block_off, hlist, hpos);
     int nh = 0;
-    const hipError_t e = read_device(&nh, block_off + nb, 4, s, rd);
+    const hipError_t e = scan_read_end(&nh, block_off + nb, pub, s, rd);
     if (e != hipSuccess) {
         *err = e;
         return;

@@ -8,9 +8,14 @@
 namespace pisurf {
 
 // ---- scan / sort workspace ----
+// where a scan publishes its last element for the host (scan_read_begin); dst NULL: nowhere
+struct ScanPub {
+    char* dst = nullptr;
+    uint32_t seq = 0;
+};
 // device exclusive scan of n int32 values; tmp must hold scan_tmp_ints(n) ints
 int64_t scan_tmp_ints(int64_t n);
-void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
+void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_t s, ScanPub pub = ScanPub());
 // stable LSD radix sort of (key, value) by the low `bits` bits of key; results in k0/v0.
 // k1/v1 are ping-pong buffers; tmp must hold radix_tmp_ints(n) ints.
 int64_t radix_tmp_ints(int64_t n);
@@ -56,6 +61,12 @@ void host_read_release(HostRead* rd);
 hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s, HostRead* rd);
 // wait for the stream's earlier work
 hipError_t stream_fence(hipStream_t s, HostRead* rd);
+// A scan total that the scan's final kernel publishes itself, with no copy launch after it:
+// scan_read_begin takes the slot's next sequence number (dst NULL when rd is NULL), the scan
+// writes its last element and then that number to the slot, and scan_read_end waits for it (or,
+// without a slot, reads dlast). No other read may use the slot in between.
+ScanPub scan_read_begin(HostRead* rd);
+hipError_t scan_read_end(int* host, const int* dlast, ScanPub p, hipStream_t s, HostRead* rd);
 
 // ---- grid ----
 void launch_bbox(const float* xyz, int64_t n, float* partial /*>= 6*1024*/, float* out6, int* nan_count,
@@ -159,10 +170,10 @@ struct TreeParams {
 constexpr int kEmptyAndCurv = 0, kEmptyOnly = 1, kCurvOnly = 2;
 void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, int group,
                        int pass, const void* samples, int rsz, hipStream_t s);
-// flag[n+1], pos[n+1] (pos[n] = number of listed nodes), list[n]: nodes to sample; cflag / cpos /
-// clist: leaf-depth nodes that need only their centre value
+// flag[n+1], pos[n+1] (pos[n] = number of listed nodes, published to pub), list[n]: nodes to
+// sample; cflag / cpos / clist: leaf-depth nodes that need only their centre value
 void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,
-                        int* clist, int* tmp, hipStream_t s);
+                        int* clist, int* tmp, hipStream_t s, ScanPub pub = ScanPub());
 void launch_centers(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                     const int* list, const int* count, void* samples, Counters* ctr, hipStream_t s);
 // the same for k listed centres (count known on the host), grouped by hash cell and evaluated in
@@ -318,7 +329,9 @@ void launch_tri_corners(const int* duals, int64_t nd, const int* toff, uint64_t*
 // n / 2048 + 2 ints.
 int64_t head_list_tmp_ints(int64_t n);
 inline int* head_list_count(int* tiles, int64_t n) { return tiles + 2 * ((n + 2047) / 2048) + 1; }
-void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_tmp, hipStream_t s);
+// (n > 0: the head count is published to pub)
+void launch_head_list(SortedKV kv, int64_t n, int* list, int* tiles, int* scan_tmp, hipStream_t s,
+                      ScanPub pub = ScanPub());
 void launch_tri_key_hi(const int* cv, int64_t nt, uint32_t* k, int* v, hipStream_t s);
 void launch_tri_key_lo(const int* cv, const int* perm, int64_t nt, uint64_t* k, int* v, hipStream_t s);
 void launch_tri_keep(const int* cv, const int* order, int64_t nt, int* keep, hipStream_t s);
