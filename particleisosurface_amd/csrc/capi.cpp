@@ -798,7 +798,11 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             HIPCHK(c, c->sd_vals.ensure(nent * rsz));
             HIPCHK(c, c->sd_cwork.ensure(4 * (nent + 2) * 4));
             const int64_t need = std::max(radix_tmp_ints(nent), scan_tmp_ints(nent + 1)) + 64;
-            HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, need * 4)));
+            if ((size_t)need * 4 > c->tmp.cap) {
+                // kernels queued before the count read may still use the workspace
+                HIPCHK(c, c->wait());
+                HIPCHK(c, c->tmp.ensure(need * 4));
+            }
             const float origin[3] = {c->root_c[0] - c->root_half, c->root_c[1] - c->root_half,
                                      c->root_c[2] - c->root_half};
             // distinct points grouped by hash cell (cell-major keys), evaluated in runs of cells
@@ -864,7 +868,11 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
             HIPCHK(c, c->sd_val2.ensure((size_t)k * 4));
             HIPCHK(c, c->sd_cwork.ensure(4 * ((size_t)k + 2) * 4));
             const int64_t need = std::max(radix_tmp_ints(k), scan_tmp_ints((int64_t)k + 1)) + 64;
-            HIPCHK(c, c->tmp.ensure(std::max<size_t>(c->tmp.cap, need * 4)));
+            if ((size_t)need * 4 > c->tmp.cap) {
+                // kernels queued before the count read may still use the workspace
+                HIPCHK(c, c->wait());
+                HIPCHK(c, c->tmp.ensure(need * 4));
+            }
             hipError_t herr = hipSuccess;
             launch_centers_runs(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), k,
                                 c->sd_key.as<uint64_t>(), c->sd_val.as<int>(), c->sd_key2.as<uint64_t>(),
@@ -1092,7 +1100,8 @@ int do_tree(pisurf_ctx* c) {
         HIPCHK(c, hipGetLastError());
         li++;
     }
-    HIPCHK(c, c->wait());
+    // the last count read ended the loop: the stream holds only the tail of that scan, which the
+    // extraction's launches follow in stream order
     c->tree_done = true;
     c->st.n_nodes = 0;
     for (auto* l : c->levels) c->st.n_nodes += l->n;
