@@ -306,7 +306,7 @@ struct pisurf_ctx {
     HostRead* hread() { return hr; }
     hipError_t read(void* host, const void* dptr, size_t bytes) { return read_device(host, dptr, bytes, stream, hr); }
     // a scan total published by the scan itself (scan_read_begin / scan_read_end)
-    ScanPub scan_pub() { return scan_read_begin(hr); }
+    ScanPub scan_pub(int off = 0) { return scan_read_begin(hr, off); }
     hipError_t read_scan(int* host, const int* dlast, ScanPub p) { return scan_read_end(host, dlast, p, stream, hr); }
 
     ~pisurf_ctx() {
@@ -770,9 +770,13 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     // coordinates: 2^(depth + 1) half-widths span the root box.
     // levels above D_min sample nothing (wants_samples), so they skip the count read-back
     const bool dedupe = c->prm.sample_dedupe && level_depth >= 0 && level_depth <= 19 && level_depth >= c->dmin;
+    // centre values of the leaf-depth level by runs (QEF off); that count is read after the
+    // sampling's reads, so it is published far from the bytes they copy
+    const bool centre_runs = level_depth >= 0 && leaf_level && c->prm.qef_mode == 0 && level_depth >= c->dmin;
     const ScanPub kpub = dedupe ? c->scan_pub() : ScanPub();
+    const ScanPub cpub = centre_runs ? c->scan_pub(kScanPubFar) : ScanPub();
     launch_sample_list(tp, v, L->flag.as<int>(), L->scanned.as<int>(), L->child.as<int>(), c->cl_flag.as<int>(),
-                       c->cl_pos.as<int>(), c->cl_list.as<int>(), c->tmp.as<int>(), c->stream, kpub);
+                       c->cl_pos.as<int>(), c->cl_list.as<int>(), c->tmp.as<int>(), c->stream, kpub, cpub);
     if (dedupe) {
         int k = 0;
         HIPCHK(c, c->read_scan(&k, L->scanned.as<int>() + L->n, kpub));
@@ -858,9 +862,9 @@ int sweep(pisurf_ctx* c, LevelBuf* L, int32_t* sc_dev, int level_depth) {
     if (level_depth < 0) {
         launch_centers(prec, c->g, c->rec_a.p, c->rec_g.p, fc, v, c->cl_list.as<int>(), c->cl_pos.as<int>() + L->n,
                        c->samples.p, c->ctr(), c->stream);
-    } else if (leaf_level && c->prm.qef_mode == 0 && level_depth >= c->dmin) {
+    } else if (centre_runs) {
         int k = 0;
-        HIPCHK(c, c->read(&k, c->cl_pos.as<int>() + L->n, 4));
+        HIPCHK(c, c->read_scan(&k, c->cl_pos.as<int>() + L->n, cpub));
         if (k > 0) {
             HIPCHK(c, c->sd_key.ensure((size_t)k * 8));
             HIPCHK(c, c->sd_key2.ensure((size_t)k * 8));

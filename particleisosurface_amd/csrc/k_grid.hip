@@ -17,17 +17,17 @@ namespace pisurf {
 constexpr int kScanT = 256, kScanI = 8, kScanTile = kScanT * kScanI;
 
 // The thread that writes a scan's last element in its final kernel also publishes it to a
-// read-back slot (pdst, non-null only for that kernel): the value, then, once it is visible
-// system-wide, the sequence number the host waits for.
-__device__ __forceinline__ void scan_publish(char* pdst, uint32_t pseq, int v) {
-    *(volatile int*)pdst = v;
+// read-back slot (pdst, non-null only for that kernel): the value at byte poff, then, once it is
+// visible system-wide, the sequence number the host waits for.
+__device__ __forceinline__ void scan_publish(char* pdst, int poff, uint32_t pseq, int v) {
+    *(volatile int*)(pdst + poff) = v;
     __threadfence_system();
     *(volatile uint32_t*)(pdst + kReadbackBytes) = pseq;
     __threadfence_system();
 }
 
 __global__ void __launch_bounds__(kScanT) k_scan_tile(const int* __restrict__ in, int* __restrict__ out, int64_t n,
-                                                      int* __restrict__ sums, char* pdst, uint32_t pseq) {
+                                                      int* __restrict__ sums, char* pdst, int poff, uint32_t pseq) {
     __shared__ int wsum[kScanT / kWave];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)t * kScanI;
@@ -49,13 +49,14 @@ __global__ void __launch_bounds__(kScanT) k_scan_tile(const int* __restrict__ in
     for (int k = 0; k < kScanI; k++) {
         const int64_t i = base + k;
         if (i < n) out[i] = run;
-        if (pdst && i == n - 1) scan_publish(pdst, pseq, run);
+        if (pdst && i == n - 1) scan_publish(pdst, poff, pseq, run);
         run += v[k];
     }
     if (t == kScanT - 1) sums[blockIdx.x] = woff + incl;
 }
 
-__global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restrict__ add, char* pdst, uint32_t pseq) {
+__global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restrict__ add, char* pdst, int poff,
+                           uint32_t pseq) {
     const int64_t i = (int64_t)blockIdx.x * kScanTile + threadIdx.x;
     const int a = add[blockIdx.x];
 #pragma unroll
@@ -64,7 +65,7 @@ __global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restri
         if (j < n) {
             const int v = out[j] + a;
             out[j] = v;
-            if (pdst && j == n - 1) scan_publish(pdst, pseq, v);
+            if (pdst && j == n - 1) scan_publish(pdst, poff, pseq, v);
         }
     }
 }
@@ -73,7 +74,8 @@ __global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restri
 // reduced by the block itself, so the totals need no scan of their own (one launch fewer).
 constexpr int kScanDirect = 2048;
 __global__ void __launch_bounds__(kScanT) k_scan_add_prefix(int* __restrict__ out, int64_t n,
-                                                            const int* __restrict__ sums, char* pdst, uint32_t pseq) {
+                                                            const int* __restrict__ sums, char* pdst, int poff,
+                                                            uint32_t pseq) {
     __shared__ int wsum[kScanT / kWave];
     const int t = threadIdx.x;
     if (blockIdx.x == 0) return;  // uniform over the block
@@ -91,7 +93,7 @@ __global__ void __launch_bounds__(kScanT) k_scan_add_prefix(int* __restrict__ ou
         if (j < n) {
             const int v = out[j] + a;
             out[j] = v;
-            if (pdst && j == n - 1) scan_publish(pdst, pseq, v);
+            if (pdst && j == n - 1) scan_publish(pdst, poff, pseq, v);
         }
     }
 }
@@ -110,12 +112,13 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* tmp, hipStream_
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     int* sums = tmp;
     int* sums_scanned = tmp + nb + 1;
-    k_scan_tile<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(in, out, n, sums, nb == 1 ? pub.dst : nullptr, pub.seq);
+    k_scan_tile<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(in, out, n, sums, nb == 1 ? pub.dst : nullptr, pub.off,
+                                                            pub.seq);
     if (nb > 1 && nb <= kScanDirect) {
-        k_scan_add_prefix<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums, pub.dst, pub.seq);
+        k_scan_add_prefix<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums, pub.dst, pub.off, pub.seq);
     } else if (nb > 1) {
         exclusive_scan_i32(sums, sums_scanned, nb, tmp + 2 * nb + 2, s);
-        k_scan_add<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums_scanned, pub.dst, pub.seq);
+        k_scan_add<<<dim3((unsigned)nb), dim3(kScanT), 0, s>>>(out, n, sums_scanned, pub.dst, pub.off, pub.seq);
     }
 }
 
@@ -336,12 +339,15 @@ __global__ void __launch_bounds__(256) k_publish(const void* __restrict__ src, i
     }
 }
 
-// Poll the sequence word. Every few thousand polls the stream is queried, so that a device error
-// ends the wait, and a stream that has finished without the word showing ends it too.
+// Poll the sequence word until it reaches want (or a later number: a scan's published value can
+// be read after later reads of the slot). Every few thousand polls the stream is queried, so that
+// a device error ends the wait, and a stream that has finished without the word showing ends it
+// too.
 hipError_t slot_wait(HostRead* rd, uint32_t want, hipStream_t s) {
     volatile uint32_t* w = (volatile uint32_t*)(rd->host + kReadbackBytes);
+    auto reached = [&] { return (int32_t)(*w - want) >= 0; };
     for (uint32_t i = 1;; i++) {
-        if (*w == want) {
+        if (reached()) {
             std::atomic_thread_fence(std::memory_order_acquire);
             return hipSuccess;
         }
@@ -351,7 +357,7 @@ hipError_t slot_wait(HostRead* rd, uint32_t want, hipStream_t s) {
             if (e == hipErrorNotReady) continue;
             if (e != hipSuccess) return e;
             for (int k = 0; k < (1 << 16); k++)
-                if (*w == want) {
+                if (reached()) {
                     std::atomic_thread_fence(std::memory_order_acquire);
                     return hipSuccess;
                 }
@@ -405,10 +411,11 @@ hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s
     return hipSuccess;
 }
 
-ScanPub scan_read_begin(HostRead* rd) {
+ScanPub scan_read_begin(HostRead* rd, int off) {
     ScanPub p;
-    if (rd) {
+    if (rd && off >= 0 && off + 4 <= (int)kReadbackBytes) {
         p.dst = rd->dev;
+        p.off = off;
         p.seq = ++rd->next;
     }
     return p;
@@ -419,7 +426,7 @@ hipError_t scan_read_end(int* host, const int* dlast, ScanPub p, hipStream_t s, 
     hipError_t e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = slot_wait(rd, p.seq, s)) != hipSuccess) return e;
-    std::memcpy(host, rd->host, sizeof(int));
+    std::memcpy(host, rd->host + p.off, sizeof(int));
     return hipSuccess;
 }
 

@@ -1057,12 +1057,12 @@ void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreePar
 }
 
 void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,
-                        int* clist, int* tmp, hipStream_t s, ScanPub pub) {
+                        int* clist, int* tmp, hipStream_t s, ScanPub pub, ScanPub cpub) {
     if (lv.n <= 0) return;
     k_sample_flag<<<(unsigned)((lv.n + 1 + 255) / 256), 256, 0, s>>>(tp, lv, flag, cflag);
     exclusive_scan_i32(flag, pos, lv.n + 1, tmp, s, pub);
     k_sample_list<<<(unsigned)((lv.n + 255) / 256), 256, 0, s>>>(lv, flag, pos, list);
-    exclusive_scan_i32(cflag, cpos, lv.n + 1, tmp, s);
+    exclusive_scan_i32(cflag, cpos, lv.n + 1, tmp, s, cpub);
     k_sample_list<<<(unsigned)((lv.n + 255) / 256), 256, 0, s>>>(lv, cflag, cpos, clist);
 }
 

@@ -8,9 +8,11 @@
 namespace pisurf {
 
 // ---- scan / sort workspace ----
-// where a scan publishes its last element for the host (scan_read_begin); dst NULL: nowhere
+// where a scan publishes its last element for the host (scan_read_begin): the slot's device
+// address, the value's offset in it and the sequence number; dst NULL: nowhere
 struct ScanPub {
     char* dst = nullptr;
+    int off = 0;
     uint32_t seq = 0;
 };
 // device exclusive scan of n int32 values; tmp must hold scan_tmp_ints(n) ints
@@ -63,9 +65,12 @@ hipError_t read_device(void* host, const void* dptr, size_t bytes, hipStream_t s
 hipError_t stream_fence(hipStream_t s, HostRead* rd);
 // A scan total that the scan's final kernel publishes itself, with no copy launch after it:
 // scan_read_begin takes the slot's next sequence number (dst NULL when rd is NULL), the scan
-// writes its last element and then that number to the slot, and scan_read_end waits for it (or,
-// without a slot, reads dlast). No other read may use the slot in between.
-ScanPub scan_read_begin(HostRead* rd);
+// writes its last element at byte off of the slot and then that number, and scan_read_end waits
+// for it (or, without a slot, reads dlast). Reads in between may use the slot (the host waits
+// for a sequence number or a later one) as long as they leave the value's bytes alone: copies
+// fill the slot from byte 0, so a value kept past other reads takes kScanPubFar.
+constexpr int kScanPubFar = (int)kReadbackBytes - 64;
+ScanPub scan_read_begin(HostRead* rd, int off = 0);
 hipError_t scan_read_end(int* host, const int* dlast, ScanPub p, hipStream_t s, HostRead* rd);
 
 // ---- grid ----
@@ -171,9 +176,10 @@ constexpr int kEmptyAndCurv = 0, kEmptyOnly = 1, kCurvOnly = 2;
 void launch_empty_curv(GridDev g, const float4* spos, const float4* nrm, TreeParams tp, NodeLevel lv, int group,
                        int pass, const void* samples, int rsz, hipStream_t s);
 // flag[n+1], pos[n+1] (pos[n] = number of listed nodes, published to pub), list[n]: nodes to
-// sample; cflag / cpos / clist: leaf-depth nodes that need only their centre value
+// sample; cflag / cpos / clist: leaf-depth nodes that need only their centre value (cpos[n]
+// published to cpub)
 void launch_sample_list(TreeParams tp, NodeLevel lv, int* flag, int* pos, int* list, int* cflag, int* cpos,
-                        int* clist, int* tmp, hipStream_t s, ScanPub pub = ScanPub());
+                        int* clist, int* tmp, hipStream_t s, ScanPub pub = ScanPub(), ScanPub cpub = ScanPub());
 void launch_centers(int prec, GridDev g, const void* rec_a, const void* rec_g, FieldConst fc, NodeLevel lv,
                     const int* list, const int* count, void* samples, Counters* ctr, hipStream_t s);
 // the same for k listed centres (count known on the host), grouped by hash cell and evaluated in
