@@ -266,6 +266,7 @@ struct pisurf_ctx {
     bool lane = false;         // meshing as a lane of pisurf_run_frames (whole frames overlap instead)
     int top_t = 0;             // levels 0 .. top_t - 1 built there
     std::vector<int64_t> top_ub;
+    std::vector<int> top_n;    // their counts once read (empty: not read yet)
     // sample deduplication: keys / entries (ping-pong), heads, scanned heads, representatives,
     // head list, per-point values
     DBuf sd_key, sd_val, sd_key2, sd_val2, sd_head, sd_hpos, sd_rep, sd_hlist, sd_vals, sd_cwork;
@@ -662,6 +663,8 @@ int normals_launch(pisurf_ctx* c) {
     return PISURF_OK;
 }
 
+int top_levels_read(pisurf_ctx* c);
+
 int do_normals(pisurf_ctx* c) {
     int rc = need_aniso(c);
     if (rc) return rc;
@@ -669,7 +672,14 @@ int do_normals(pisurf_ctx* c) {
     const double t0 = now_ms();
     rc = normals_launch(c);
     if (rc) return rc;
-    HIPCHK(c, c->wait());
+    if (c->top_pending) {
+        // the counts of the levels above D_min (issued on the second stream, long done by now)
+        // are read as the fence that ends the normals: one round trip instead of two
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->tev_top, 0));
+        if ((rc = top_levels_read(c))) return rc;
+    } else {
+        HIPCHK(c, c->wait());
+    }
     c->st.ms_normals += now_ms() - t0;
     return PISURF_OK;
 }
@@ -919,6 +929,7 @@ int tree_root(pisurf_ctx* c, hipStream_t s) {
 // device until top_levels_finish reads them).
 int top_levels_issue(pisurf_ctx* c, hipStream_t s, DBuf& tmp) {
     c->top_t = 0;
+    c->top_n.clear();
     c->top_ub.assign(1, 1);
     const int top = c->dmin;
     if (top <= 0) return PISURF_OK;
@@ -1009,17 +1020,25 @@ int top_levels_issue(pisurf_ctx* c, hipStream_t s, DBuf& tmp) {
     if (!graphed) issue();
     HIPCHK(c, hipGetLastError());
     c->top_t = t;
+    c->top_n.clear();
     return PISURF_OK;
 }
 
 // The counts of the issued levels (one read on the context's stream, which must follow their
 // launches); *next = the index of the first level built the ordinary way.
+int top_levels_read(pisurf_ctx* c) {
+    const int t = c->top_t;
+    c->top_n.assign(t > 0 ? t + 1 : 0, 0);
+    if (t > 0) HIPCHK(c, c->read(c->top_n.data(), c->lvcnt.as<int>(), (t + 1) * sizeof(int)));
+    return PISURF_OK;
+}
 int top_levels_finish(pisurf_ctx* c, size_t* next) {
     *next = 0;
     const int t = c->top_t;
     if (t <= 0) return PISURF_OK;
-    std::vector<int> n(t + 1);
-    HIPCHK(c, c->read(n.data(), c->lvcnt.as<int>(), (t + 1) * sizeof(int)));
+    int rc;
+    if ((int)c->top_n.size() != t + 1 && (rc = top_levels_read(c))) return rc;
+    const std::vector<int>& n = c->top_n;
     for (int l = 0; l <= t; l++) {
         if (n[l] < 0 || n[l] > c->top_ub[l]) return c->fail(PISURF_E_STATE, "octree level count out of its bound");
         c->levels[l]->n = n[l];
