@@ -1356,7 +1356,7 @@ int do_extract_device(pisurf_ctx* c) {
     int64_t nleaf;
     if ((rc = read_scan_int(c, E[1].as<int>() + T, pub, nleaf))) return rc;
     const int* lidx = E[1].as<int>();
-    // distinct corner points: per tile of 128 nodes of a level, one record per point (its key and
+    // distinct corner points: per tile of 256 nodes of a level, one record per point (its key and
     // the tile's leaves around it) in the tile's own region; then the keys of all records in one
     // array, sorted
     const int64_t ntiles = ll.tile0[tv.nlev];

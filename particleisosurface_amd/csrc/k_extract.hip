@@ -49,7 +49,7 @@ __global__ void k_leaf_flags(LeafLevels lv, int* __restrict__ flag) {
     flag[i] = is_leaf_type(lv.type[L][i - lv.off[L]]) ? 1 : 0;
 }
 
-// The distinct corner points of the leaves, in three steps. Each workgroup takes a tile of 128
+// The distinct corner points of the leaves, in three steps. Each workgroup takes a tile of 256
 // consecutive nodes of one level (a level's nodes are in Morton order, so a tile covers a compact
 // piece of space and its leaves share most corners) and merges the tile's corner keys in an LDS
 // hash table: one record per distinct point of the tile, holding its Morton key and, for each of
@@ -64,10 +64,10 @@ __global__ void k_leaf_flags(LeafLevels lv, int* __restrict__ flag) {
 //
 // Keys: Morton interleave of the finest-level corner coordinates (3 dp bits); corners on the root
 // boundary are not points of the traversal and are skipped.
-constexpr int kCtT = 128;               // nodes per tile
+constexpr int kCtT = 256;               // nodes per tile
 constexpr int kCtRecs = 8 * kCtT;       // corner keys, hence records, per tile at most
 constexpr int kCtSlots = 2 * kCtRecs;   // hash slots: load factor at most 1/2
-constexpr int kCtSlotBits = 11;
+constexpr int kCtSlotBits = 12;
 static_assert((1 << kCtSlotBits) == kCtSlots, "slot bits");
 constexpr uint64_t kCtEmpty = ~0ull;     // above every key (3 dp <= 63 bits)
 
