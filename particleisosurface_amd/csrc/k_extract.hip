@@ -224,7 +224,7 @@ __global__ void k_tree_index(TreeView tv, int* __restrict__ gidx) {
     }
     gidx[c] = (L << 27) | (int)i;
 }
-int tree_index_level(int dp) { return min(dp, 7); }
+int tree_index_level(int dp) { return min(dp, 8); }
 void launch_tree_index(TreeView tv, int* gidx, hipStream_t s) {
     const int64_t n = (int64_t)1 << (3 * tv.glev);
     k_tree_index<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tv, gidx);
