@@ -283,7 +283,7 @@ struct pisurf_ctx {
     hipEvent_t eb0 = nullptr, eb1 = nullptr;
     std::vector<hipEvent_t> ev_chunk;  // ends of the bisection chunks
     // device extraction workspace
-    DBuf eb[37];
+    DBuf eb[38];
     // mesh
     PinnedVec<double> mesh_v;
     PinnedVec<int32_t> mesh_t;
@@ -1330,8 +1330,13 @@ int do_extract_device(pisurf_ctx* c) {
     // the tree index from which the point pass's descents start
     tv.glev = tree_index_level(tv.dp);
     HIPCHK(c, E[36].ensure(((size_t)1 << (3 * tv.glev)) * 4));
+    int* coarse = nullptr;
+    if (tv.glev >= 1) {
+        HIPCHK(c, E[37].ensure(((size_t)1 << (3 * (tv.glev - 1))) * 4));
+        coarse = E[37].as<int>();
+    }
     tv.gidx = E[36].as<int>();
-    launch_tree_index(tv, E[36].as<int>(), s);
+    launch_tree_index(tv, E[36].as<int>(), coarse, s);
     int rc;
     // leaves
     HIPCHK(c, E[0].ensure((T + 1) * 4));

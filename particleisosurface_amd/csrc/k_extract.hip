@@ -224,9 +224,33 @@ __global__ void k_tree_index(TreeView tv, int* __restrict__ gidx) {
     }
     gidx[c] = (L << 27) | (int)i;
 }
+// the index at level glev from the one at glev - 1 (coarse): a cell keeps its parent cell's entry
+// when that is a leaf, and otherwise takes the child of the parent cell's node that holds it
+__global__ void k_tree_index_refine(TreeView tv, const int* __restrict__ coarse, int* __restrict__ gidx) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = tv.glev, gc = g - 1;
+    if (c >= ((int64_t)1 << (3 * g))) return;
+    const int x = (int)(c & ((1 << g) - 1)), y = (int)((c >> g) & ((1 << g) - 1)), z = (int)(c >> (2 * g));
+    const int e = coarse[(int64_t)(x >> 1) | ((int64_t)(y >> 1) << gc) | ((int64_t)(z >> 1) << (2 * gc))];
+    const int L = e >> 27, i = e & ((1 << 27) - 1);
+    if (L < gc || is_leaf_type(tv.type[gc][i])) {
+        gidx[c] = e;
+        return;
+    }
+    const int b = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
+    gidx[c] = (g << 27) | (tv.child[gc][i] + b);
+}
 int tree_index_level(int dp) { return min(dp, 8); }
-void launch_tree_index(TreeView tv, int* gidx, hipStream_t s) {
+void launch_tree_index(TreeView tv, int* gidx, int* coarse, hipStream_t s) {
     const int64_t n = (int64_t)1 << (3 * tv.glev);
+    if (tv.glev >= 1 && coarse) {
+        TreeView tc = tv;
+        tc.glev = tv.glev - 1;
+        const int64_t nc = (int64_t)1 << (3 * tc.glev);
+        k_tree_index<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(tc, coarse);
+        k_tree_index_refine<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tv, coarse, gidx);
+        return;
+    }
     k_tree_index<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tv, gidx);
 }
 // where the descent to finest cell (qx, qy, qz) resumes: its index cell's node

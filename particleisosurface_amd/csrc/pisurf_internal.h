@@ -283,9 +283,10 @@ struct TreeView {
     const int* gidx = nullptr;
     int glev = 0;
 };
-// levels of the tree index of a tree of deepest level dp, and its cell count
+// levels of the tree index of a tree of deepest level dp; the index (8^glev cells), built from
+// the one a level above in coarse (8^(glev - 1) cells) when coarse is not NULL
 int tree_index_level(int dp);
-void launch_tree_index(TreeView tv, int* gidx, hipStream_t s);
+void launch_tree_index(TreeView tv, int* gidx, int* coarse, hipStream_t s);
 // the levels of a built tree as the extraction's per-node passes read them
 struct LeafLevels {
     int nlev;                          // levels present
